@@ -1,0 +1,240 @@
+/*
+ * raycer_hip.h — C ABI of the MI355X (gfx950) triangle path tracer.
+ *
+ * This is the device boundary that replaces raycer's OpenCL GPU tracer
+ * (src/OpenCL/CLTracer.h:23-90, CLScene.h:23-33).  Conventions:
+ *   - every function returns int status, RC_OK == 0; rc_last_error() gives the
+ *     message of the last failure on the calling thread (the counterpart of
+ *     CLManager::checkError, src/OpenCL/CLManager.cpp:220-226, which throws);
+ *   - handles are opaque; host pointers passed in are only read during the call;
+ *   - one context per host thread, or externally synchronised; calls block until
+ *     their GPU work has finished unless stated otherwise;
+ *   - there is no CPU fallback: without a usable gfx950 device rc_create fails.
+ */
+#ifndef RAYCER_HIP_H
+#define RAYCER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RC_OK = 0,
+    RC_ERR_INVALID = 1,     /* bad argument or inconsistent input */
+    RC_ERR_HIP = 2,         /* a HIP runtime call failed */
+    RC_ERR_ALLOC = 3,       /* device allocation failed */
+    RC_ERR_INTERRUPTED = 4, /* *interrupt became nonzero; the film holds the passes done */
+    RC_ERR_NO_DEVICE = 5    /* no gfx950 device */
+};
+
+typedef struct rc_ctx rc_ctx;
+typedef struct rc_scene rc_scene;
+typedef struct rc_film rc_film;
+typedef struct rc_cache rc_cache;
+
+/* Ray (32 bytes): Ray {origin, direction, minDistance, maxDistance}
+ * (src/Raytracing/Ray.h:12-29).  tmax may be +inf. */
+typedef struct rc_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} rc_ray;
+
+/* Closest hit (16 bytes): distance, barycentrics and the ORIGINAL triangle
+ * index (-1 = miss).  Replaces Intersection {wasFound, distance, primitive}
+ * (src/Raytracing/Intersection.h:17-27). */
+typedef struct rc_hit {
+    float t;
+    float u;
+    float v;
+    int32_t tri;
+} rc_hit;
+
+/* FlatBVHNode exactly as the reference lays it out in host memory
+ * (src/Raytracing/Primitives/FlatBVH.h:38-53): preorder, left child at i + 1,
+ * right child at i + right_offset, right_offset == 0 marks a leaf whose
+ * triangles are ordered_ids[start_offset .. start_offset + primitive_count). */
+typedef struct rc_flat_node {
+    double aabb_min[3];
+    double aabb_max[3];
+    int64_t right_offset;
+    uint64_t start_offset;
+    uint64_t primitive_count;
+} rc_flat_node;
+
+/* Material (Material.h:16-120 subset): Lambert reflectance, emittance,
+ * Phong ambient/specular terms; flags bit0 isEmissive, bit1 nonShadowing. */
+typedef struct rc_material {
+    float diffuse[3];
+    float emittance[3];
+    float ambient[3];
+    float specular[3];
+    float shininess;
+    uint32_t flags;
+    uint32_t pad[2];
+} rc_material;
+
+enum { RC_MAT_EMISSIVE = 1u, RC_MAT_NON_SHADOWING = 2u };
+
+/* Camera: position, EulerAngle (pitch, yaw, roll in degrees) and the
+ * horizontal field of view in degrees (src/Raytracing/Camera.cpp:29-34,160,308-331). */
+typedef struct rc_camera {
+    float position[3];
+    float pitch, yaw, roll;
+    float fov;
+} rc_camera;
+
+enum { RC_FILTER_BOX = 0, RC_FILTER_TENT = 1 };
+enum { RC_SAMPLER_JITTERED = 0, RC_SAMPLER_RANDOM = 1 };
+enum { RC_EST_REFERENCE = 0, RC_EST_LAMBERT = 1 };
+
+/* Render settings: Scene::General names where they exist (Scene.h:89-131). */
+typedef struct rc_render_cfg {
+    uint32_t width, height;
+    uint32_t spp;               /* samples per pixel; jittered needs spp = n*n (multiSampleCountSqrt = n) */
+    uint32_t sampler;           /* RC_SAMPLER_JITTERED (1 sample = pixel centre, weight 1) or RC_SAMPLER_RANDOM */
+    uint32_t max_path_length;   /* maxPathLength: rays per path including the camera ray */
+    uint32_t rr_start;          /* Russian roulette at path vertex k >= rr_start; 0 = off */
+    uint32_t filter;            /* RC_FILTER_* (multiSamplerFilterType) */
+    uint32_t estimator;         /* RC_EST_REFERENCE: throughput *= 2 rho |n.w|; RC_EST_LAMBERT: *= rho */
+    uint32_t cache_vertex;      /* with a radiance cache: path vertex that ends in the cache (0 = off) */
+    uint32_t pad0;
+    float filter_radius;
+    float ray_start_offset;     /* rayStartOffset */
+    float sky[3];               /* radiance on a miss (the reference returns black) */
+    float sun_dir[3];           /* unit vector towards a directional sun */
+    float sun_irradiance[3];    /* 0 = no sun */
+    uint32_t frame;             /* RNG stream for progressive frames */
+    uint64_t seed;
+} rc_render_cfg;
+
+/* Tone-mapper settings (Scene::ToneMapper, Scene.h:137-160), Reinhard only. */
+typedef struct rc_tonemap {
+    float key;
+    float gamma;
+    uint32_t clamp;
+    uint32_t apply_gamma;
+    uint32_t fast_pow;          /* 1 = MathUtils::fastPow (reference), 0 = pow */
+    uint32_t pad;
+} rc_tonemap;
+
+/* Point light + ambient for the Phong raycast (Lights.h PointLight/AmbientLight). */
+typedef struct rc_point_light {
+    float position[3];
+    float color[3];
+    float intensity;
+    float max_distance;
+    float attenuation;
+    float ambient_color[3];
+    float ambient_intensity;
+} rc_point_light;
+
+/* Launch tuning (autotuned): threads per block of the traversal kernels, pass
+ * size in samples per pixel, persistent grid size in blocks per CU. */
+typedef struct rc_tuning {
+    uint32_t block;
+    uint32_t spp_per_pass;
+    uint32_t blocks_per_cu;
+    uint32_t pad;
+} rc_tuning;
+
+typedef struct rc_stats {
+    uint64_t extension_rays;     /* closest-hit rays traced by the last render */
+    uint64_t shadow_rays;        /* any-hit rays traced by the last render */
+    uint64_t samples;            /* camera samples of the last render */
+    uint64_t node_records;       /* 32-byte node records fetched (counting runs only) */
+    uint64_t tri_tests;          /* 48-byte triangle records tested (counting runs only) */
+    uint64_t overflow_rays;      /* rays re-traced with the deep-stack kernel */
+    uint64_t extend_launches;
+    double extend_ms;            /* sum of closest-hit kernel durations (profiling on) */
+    double shadow_ms;
+    double shade_ms;
+    double other_ms;
+    double render_ms;            /* wall time of the last render call */
+    uint64_t shadow_launches;
+} rc_stats;
+
+/* ------------------------------------------------------------ context */
+int rc_last_error(char* buf, size_t len);
+/* CLManager::initialize + CLTracer::initializeKernels (CLTracer.cpp:45-71). */
+int rc_create(int device, rc_ctx** out);
+int rc_destroy(rc_ctx* ctx);
+int rc_device_name(rc_ctx* ctx, char* buf, size_t len, int* cu_count);
+/* profiling: 1 = time every launch with HIP events on the context's stream */
+int rc_set_profiling(rc_ctx* ctx, int enable);
+/* counting: 1 = use the traversal variant that counts node/triangle fetches */
+int rc_set_counting(rc_ctx* ctx, int enable);
+int rc_set_tuning(rc_ctx* ctx, const rc_tuning* t);
+int rc_get_stats(rc_ctx* ctx, rc_stats* out);
+int rc_synchronize(rc_ctx* ctx);
+
+/* ------------------------------------------------------------ scene */
+/* CLTracer::initializeBuffers / CLScene::readSceneFull (CLScene.cpp:72-263):
+ * the host FlatBVH (nodes + orderedPrimitiveIds) is repacked into 32-byte
+ * child-pair records and 48-byte triangle records and uploaded.
+ * vertices: ntri * 9 floats in original triangle order; tri_material: ntri. */
+int rc_scene_upload(rc_ctx* ctx, const float* vertices, size_t ntri, const uint32_t* tri_material,
+                    const rc_material* materials, size_t nmat, const rc_flat_node* nodes, size_t nnodes,
+                    const uint64_t* ordered_ids, rc_scene** out);
+int rc_scene_free(rc_scene* scene);
+int rc_scene_info(const rc_scene* scene, size_t* ntri, size_t* nrecords, size_t* device_bytes);
+
+/* ------------------------------------------------------------ ray queries */
+/* FlatBVH::intersect + Triangle::intersect for a ray batch (closest hit).
+ * Host arrays in, host arrays out. */
+int rc_trace_closest(rc_ctx* ctx, const rc_scene* scene, const rc_ray* rays, size_t n, rc_hit* hits);
+/* Any hit (fastOcclusion, FlatBVH.cpp:49-50): occluded[i] = 1 if a triangle
+ * whose material is not nonShadowing lies at t in [tmin, tmax]. */
+int rc_trace_any(rc_ctx* ctx, const rc_scene* scene, const rc_ray* rays, size_t n, uint8_t* occluded);
+/* Same queries on device-resident buffers (no copies, asynchronous on the
+ * context's stream). */
+int rc_trace_closest_device(rc_ctx* ctx, const rc_scene* scene, const rc_ray* d_rays, size_t n, rc_hit* d_hits);
+int rc_trace_any_device(rc_ctx* ctx, const rc_scene* scene, const rc_ray* d_rays, size_t n, uint8_t* d_occluded);
+/* Device buffers owned by the context's allocator, for the *_device calls. */
+int rc_device_alloc(rc_ctx* ctx, size_t bytes, void** dptr);
+int rc_device_free(rc_ctx* ctx, void* dptr);
+int rc_memcpy_h2d(rc_ctx* ctx, void* dst, const void* src, size_t bytes);
+int rc_memcpy_d2h(rc_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ------------------------------------------------------------ film */
+/* Film (src/Rendering/Film.h:23-52): per pixel FP32 Kahan-compensated sums of
+ * w*color and w.  Accumulates across rc_render calls until rc_film_clear. */
+int rc_film_create(rc_ctx* ctx, uint32_t width, uint32_t height, rc_film** out);
+int rc_film_free(rc_film* film);
+int rc_film_clear(rc_ctx* ctx, rc_film* film);
+/* linear image sum/weight as RGBA float (Film::generateToneMappedImage first half) */
+int rc_film_download_linear(rc_ctx* ctx, const rc_film* film, float* rgba);
+/* raw accumulators: per pixel sum.rgb, weight (4 floats) */
+int rc_film_download_sums(rc_ctx* ctx, const rc_film* film, float* sums);
+/* ReinhardToneMapper::apply on the device (ReinhardToneMapper.cpp:18-86);
+ * log_avg / max_lum may be NULL. */
+int rc_film_tonemap(rc_ctx* ctx, const rc_film* film, const rc_tonemap* tm, float* rgba, double* log_avg,
+                    double* max_lum);
+
+/* ------------------------------------------------------------ rendering */
+/* Tracer::run + PathTracer (Tracer.cpp:58-113, PathTracer.cpp:13-75) as a
+ * wavefront path tracer.  interrupt (may be NULL) is polled between passes;
+ * pixels_processed (may be NULL) is advanced as passes finish. */
+int rc_render(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam, const rc_render_cfg* cfg,
+              volatile const int* interrupt, uint64_t* pixels_processed);
+/* Primary ray + Phong + one point-light hard shadow (Raytracer.cpp:312-407,
+ * 524-551).  The camera ray of pixel (x, y) passes through (x + u0, y + u1),
+ * (u0, u1) the jitter draw of that pixel.  hit_ids / t / rgba are host arrays
+ * of width*height (rgba: 4 floats per pixel); any may be NULL. */
+int rc_raycast_phong(rc_ctx* ctx, const rc_scene* scene, const rc_camera* cam, uint32_t width, uint32_t height,
+                     uint64_t seed, float ray_start_offset, const rc_point_light* light, int32_t* hit_ids,
+                     float* t, float* rgba);
+
+/* Many scenes in one captured HIP graph (persistent launch): scenes[i] is
+ * rendered into films[i] with cams[i] and cfgs[i]. */
+int rc_render_batch(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* const* films, const rc_camera* cams,
+                    const rc_render_cfg* cfgs, size_t nscenes, const rc_tuning* tuning);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

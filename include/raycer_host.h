@@ -1,0 +1,74 @@
+/*
+ * raycer_host.h — C ABI of the host half of the renderer: the FP64 SAH BVH
+ * builder (Scene::initialize -> FlatBVH::build) and the seeded scene
+ * generators used by the benchmarks and tests (the counterpart of raycer's
+ * Scene::createTestScene, src/Raytracing/Scene.h:60-87).
+ *
+ * Results are written into caller-owned buffers after a size query.  Same
+ * status and error conventions as raycer_hip.h.
+ */
+#ifndef RAYCER_HOST_H
+#define RAYCER_HOST_H
+
+#include "raycer_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* BVHBuildInfo (src/Raytracing/Primitives/FlatBVH.h:19-36); only the SAH path
+ * (useSAH = true, regularSAHSplits = 0) is provided. */
+typedef struct rch_bvh_build_info {
+    uint64_t max_leaf_size;  /* maxLeafSize, default 5 */
+    uint32_t threads;        /* 0 = all hardware threads */
+    uint32_t pad;
+} rch_bvh_build_info;
+
+typedef struct rch_bvh rch_bvh;
+
+/* FlatBVH::build (src/Raytracing/Primitives/FlatBVH.cpp:89-204) over the
+ * triangles' FP64 bounding boxes, with the reference's 6-candidate SAH split
+ * (FlatBVH.cpp:241-342).  Independent subtrees are built on separate threads;
+ * the node order is the reference's preorder regardless of thread count. */
+int rch_bvh_build(const float* vertices, size_t ntri, const rch_bvh_build_info* info, rch_bvh** out);
+size_t rch_bvh_node_count(const rch_bvh* bvh);
+int rch_bvh_get(const rch_bvh* bvh, rc_flat_node* nodes, uint64_t* ordered_ids);
+double rch_bvh_build_ms(const rch_bvh* bvh);
+uint32_t rch_bvh_depth(const rch_bvh* bvh);
+int rch_bvh_free(rch_bvh* bvh);
+
+/* ------------------------------------------------------------ scenes */
+enum {
+    RCH_SCENE_BOX = 1,          /* closed box + two rotated inner boxes + ceiling light, 36 triangles */
+    RCH_SCENE_SPHERE_GROUND = 2,/* displaced icosphere on a jittered ground grid (1,179,648 triangles at level 7 / grid 512) */
+    RCH_SCENE_TERRAIN = 3,      /* ridged-fBm heightfield, 2 (n-1)^2 triangles (2,093,058 at n = 1024) */
+    RCH_SCENE_SOUP = 4          /* seeded random triangle soup (batched multi-scene workload) */
+};
+
+typedef struct rch_scene_params {
+    uint32_t kind;
+    uint32_t seed;
+    uint32_t level;      /* SPHERE_GROUND: icosphere subdivision level (7) */
+    uint32_t grid;       /* SPHERE_GROUND: ground quads per side (512); TERRAIN: vertices per side (1024) */
+    uint64_t tri_count;  /* SOUP: 0 = draw log-uniformly in [1e3, 5e5] from the seed */
+} rch_scene_params;
+
+typedef struct rch_scene_desc {
+    size_t ntri;
+    size_t nmat;
+    rc_camera camera;
+    rc_render_cfg cfg;        /* the workload's render settings */
+    rc_point_light light;     /* SPHERE_GROUND only */
+    float bounds_min[3];
+    float bounds_max[3];
+} rch_scene_desc;
+
+/* Fills desc (sizes, camera, settings) for the given parameters. */
+int rch_scene_describe(const rch_scene_params* p, rch_scene_desc* desc);
+/* Writes ntri * 9 vertices, ntri material indices and nmat materials. */
+int rch_scene_generate(const rch_scene_params* p, float* vertices, uint32_t* tri_material, rc_material* materials);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,412 @@
+// FP64 restatement of raycer's camera, sampling, path tracer, film, Phong
+// raycast and Reinhard tone mapper.  TEST INFRASTRUCTURE ONLY.
+#include <omp.h>
+
+#include <vector>
+
+#include "core.hpp"
+#include "oracle.h"
+
+using namespace orc;
+
+extern "C" size_t or_bvh_ordered_count(const or_bvh* b);
+
+// ---------------------------------------------------------------- small pins
+// MathUtils::fastPow (src/Math/MathUtils.cpp:56-68): the high 32-bit word of the
+// double is treated as an integer, scaled about 1072632447, the low word zeroed.
+extern "C" double or_fastpow(double a, double b) {
+    uint64_t bits;
+    std::memcpy(&bits, &a, 8);
+    int32_t hi = int32_t(uint32_t(bits >> 32));
+    int32_t nhi = static_cast<int32_t>(b * (hi - 1072632447) + 1072632447);
+    uint64_t ob = uint64_t(uint32_t(nhi)) << 32;
+    double r;
+    std::memcpy(&r, &ob, 8);
+    return r;
+}
+extern "C" void or_normalize(const double in[3], double out[3]) {
+    V3 r = normalized(V3(in[0], in[1], in[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+extern "C" void or_cross(const double a[3], const double b[3], double out[3]) {
+    V3 r = cross(V3(a[0], a[1], a[2]), V3(b[0], b[1], b[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+extern "C" void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox4x32_10(ctr, key, out);
+}
+extern "C" double or_uniform(uint32_t x) { return u01(x); }
+
+// BoxFilter (BoxFilter.cpp:15-37) and TentFilter (TentFilter.cpp:12-44); the
+// 2-D weight is the product wx * wy (Filter.cpp:31-39).
+static double filter_1d(uint32_t f, double r, double x) {
+    if (f == 0) return (x >= -r && x < r) ? 1.0 / (2.0 * r) : 0.0;
+    double rinv = 1.0 / r;
+    double s = std::abs(x * rinv);
+    return (s < 1.0 ? 1.0 - s : 0.0) * rinv;
+}
+extern "C" double or_filter_weight(uint32_t f, double r, double x, double y) {
+    return filter_1d(f, r, x) * filter_1d(f, r, y);
+}
+extern "C" void or_hemisphere(const double n[3], double u1, double u2, double out[3]) {
+    V3 r = hemisphere(V3(n[0], n[1], n[2]), u1, u2);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+// ---------------------------------------------------------------- camera
+namespace {
+struct Cam {
+    V3 pos, fwd, right, up, center;
+    double ipw, iph, aspect;
+};
+// EulerAngle::getDirection (EulerAngle.cpp:77-87), ONB::fromNormal (ONB.cpp:32-39),
+// Camera::setImagePlaneSize (Camera.cpp:29-34), imagePlaneDistance (Camera.cpp:160),
+// cameraState (Camera.cpp:279-283).
+Cam make_cam(const double c[7], uint32_t w, uint32_t h) {
+    Cam k;
+    k.pos = V3(c[0], c[1], c[2]);
+    double pitch = c[3], yaw = c[4];
+    double fov = std::max(1.0, std::min(c[6], 180.0));
+    V3 d(-std::sin(deg2rad(yaw)) * std::cos(deg2rad(pitch)), std::sin(deg2rad(pitch)),
+         -std::cos(deg2rad(yaw)) * std::cos(deg2rad(pitch)));
+    d = normalized(d);
+    k.right = normalized(cross(d, ALMOST_UP));
+    k.up = normalized(cross(k.right, d));
+    k.fwd = d;
+    double dist = 0.5 / std::tan(deg2rad(fov / 2.0));
+    k.center = k.pos + k.fwd * dist;
+    k.ipw = double(w - 1);
+    k.iph = double(h - 1);
+    k.aspect = double(h) / double(w);
+    return k;
+}
+// Camera::getRay, perspective branch (Camera.cpp:308-331)
+void cam_ray(const Cam& k, double px, double py, V3& o, V3& d) {
+    double dx = (px / k.ipw) - 0.5;
+    double dy = (py / k.iph) - 0.5;
+    V3 p = k.center + (dx * k.right) + ((dy * k.aspect) * k.up);
+    o = k.pos;
+    d = normalized(p - k.pos);
+}
+RayD make_ray(V3 o, V3 d, double tmin, double tmax) {
+    RayD r;
+    r.o = o; r.d = d; r.tmin = tmin; r.tmax = tmax;
+    r.inv = V3(1.0 / d.x, 1.0 / d.y, 1.0 / d.z);
+    return r;
+}
+}  // namespace
+
+extern "C" void or_camera_ray(const double cam[7], uint32_t w, uint32_t h, double px, double py,
+                              double org[3], double dir[3]) {
+    Cam k = make_cam(cam, w, h);
+    V3 o, d;
+    cam_ray(k, px, py, o, d);
+    org[0] = o.x; org[1] = o.y; org[2] = o.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+// ---------------------------------------------------------------- path tracer
+namespace {
+struct Scene {
+    const or_bvh* bvh;
+    const float* verts;
+    const uint32_t* tri_mat;
+    const double* mats;
+    std::vector<uint32_t> tri_flags;  // material flags per triangle
+};
+
+bool closest(const Scene& s, const RayD& r, uint32_t traversal, double& t, int64_t& id) {
+    if (traversal == 0) return trace_ref(s.bvh, s.verts, r, false, t, id, nullptr, nullptr);
+    Diag D;
+    trace_ordered(s.bvh, s.verts, r, 0.0, 0.0, D);
+    t = D.t;
+    id = D.id;
+    return id >= 0;
+}
+bool occluded(const Scene& s, const RayD& r) {
+    double t;
+    int64_t id;
+    return trace_ref(s.bvh, s.verts, r, true, t, id, s.tri_flags.data(), nullptr);
+}
+
+struct PathOut {
+    V3 L;
+    int rays = 0;
+};
+
+// PathTracer::traceRecursive (PathTracer.cpp:25-75) unrolled into a loop, plus
+// the terms the reference lacks: sky on miss, sun next-event estimation,
+// Russian roulette, and the optional unbiased Lambert weight.
+//   reference estimator: throughput *= 2 * rho * |n . w|   (PathTracer.cpp:70-71)
+//   lambert estimator:   throughput *= rho
+// Sun NEE uses the BRDF each estimator implies: (2 rho / pi) cos_i (reference)
+// or rho / pi (lambert), times E cos_s; it happens at vertex k only when a
+// continuation ray k + 1 would still be allowed.
+PathOut trace_path(const Scene& s, const or_render_cfg& c, uint32_t pixel, uint32_t sample, V3 o, V3 d,
+                   double* trace_out) {
+    PathOut P;
+    V3 thr(1, 1, 1);
+    V3 L(0, 0, 0);
+    V3 sky(c.sky[0], c.sky[1], c.sky[2]);
+    V3 sun(c.sun_dir[0], c.sun_dir[1], c.sun_dir[2]);
+    V3 E(c.sun_irradiance[0], c.sun_irradiance[1], c.sun_irradiance[2]);
+    bool has_sun = maxc(E) > 0.0;
+    for (uint32_t k = 0; k < c.max_path_length; ++k) {
+        RayD r = make_ray(o, d, 0.0, std::numeric_limits<double>::max());
+        double t;
+        int64_t id;
+        bool hit = closest(s, r, c.traversal, t, id);
+        P.rays++;
+        if (trace_out) {
+            double* q = trace_out + 8 * k;
+            q[0] = hit ? t : -1.0; q[1] = double(id);
+        }
+        if (!hit) {
+            L = L + mul(thr, sky);
+            if (trace_out) { double* q = trace_out + 8 * k; q[2] = thr.x; q[3] = thr.y; q[4] = thr.z; q[5] = L.x; q[6] = L.y; q[7] = L.z; }
+            break;
+        }
+        const double* m = s.mats + OR_MAT_STRIDE * s.tri_mat[id];
+        uint32_t flags = uint32_t(m[13]);
+        if (flags & 1u) {
+            L = L + mul(thr, V3(m[3], m[4], m[5]));
+            if (trace_out) { double* q = trace_out + 8 * k; q[2] = thr.x; q[3] = thr.y; q[4] = thr.z; q[5] = L.x; q[6] = L.y; q[7] = L.z; }
+            break;
+        }
+        Tri tr = load_tri(s.verts, uint64_t(id));
+        V3 n = tri_normal(tr);
+        V3 p = r.o + (t * r.d);
+        V3 rho(m[0], m[1], m[2]);
+        double u[4];
+        draw4(c.seed, pixel, sample, c.frame, k, u);
+        bool more = (k + 1) < c.max_path_length;
+        if (has_sun && more) {
+            double cs = dot(n, sun);
+            if (cs > 0.0) {
+                RayD sr = make_ray(p + sun * c.ray_start_offset, sun, 0.0, std::numeric_limits<double>::max());
+                if (!occluded(s, sr)) {
+                    double f = c.estimator == 0 ? (2.0 / M_PI) * cs * cs : (1.0 / M_PI) * cs;
+                    L = L + mul(thr, mul(rho, E)) * f;
+                }
+            }
+        }
+        if (trace_out) { double* q = trace_out + 8 * k; q[2] = thr.x; q[3] = thr.y; q[4] = thr.z; q[5] = L.x; q[6] = L.y; q[7] = L.z; }
+        if (!more) break;
+        V3 w = hemisphere(n, u[0], u[1]);
+        if (c.estimator == 0) thr = mul(thr, rho * (2.0 * std::abs(dot(w, n))));
+        else thr = mul(thr, rho);
+        if (c.rr_start > 0 && k >= c.rr_start) {
+            double q = std::min(1.0, maxc(thr));
+            if (u[2] >= q) break;
+            thr = thr * (1.0 / q);
+        }
+        o = p + w * c.ray_start_offset;
+        d = w;
+    }
+    P.L = L;
+    return P;
+}
+
+uint32_t isqrt_exact(uint32_t v) {
+    uint32_t r = uint32_t(std::sqrt(double(v)));
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+void sample_camera(const Cam& k, const or_render_cfg& c, uint32_t pixel, uint32_t s, V3& o, V3& d, double& w) {
+    double px = double(pixel % c.width), py = double(pixel / c.width);
+    // Tracer::generateMultiSamples (Tracer.cpp:115-144): one jittered sample is
+    // the pixel centre with weight 1; otherwise the sample point (x + r) / n of
+    // stratum (x, y) of an n x n grid (JitteredSampler.cpp:21-34) or a uniform
+    // point (RandomSampler.cpp:23-35), offset (s - 0.5) * 2 * radius, weighted by
+    // the filter.
+    if (c.sampler == 0 && c.spp == 1) {
+        w = 1.0;
+    } else {
+        double u[4];
+        draw4(c.seed, pixel, s, c.frame, CAM_DIM, u);
+        double ox, oy;
+        if (c.sampler == 0) {
+            uint32_t n = isqrt_exact(c.spp);
+            uint32_t sx = s % n, sy = s / n;
+            ox = (double(sx) + u[0]) / double(n);
+            oy = (double(sy) + u[1]) / double(n);
+        } else {
+            ox = u[0];
+            oy = u[1];
+        }
+        ox = (ox - 0.5) * 2.0 * c.filter_radius;
+        oy = (oy - 0.5) * 2.0 * c.filter_radius;
+        w = filter_1d(c.filter, c.filter_radius, ox) * filter_1d(c.filter, c.filter_radius, oy);
+        px += ox;
+        py += oy;
+    }
+    cam_ray(k, px, py, o, d);
+}
+}  // namespace
+
+extern "C" int or_render(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,
+                         const double* mats, size_t nmat, const double cam[7], const or_render_cfg* cfg,
+                         const uint32_t* pixel_ids, size_t npix, double* stats, int nthreads,
+                         volatile int* interrupt) {
+    Scene s{b, vertices, tri_mat, mats, {}};
+    s.tri_flags.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) s.tri_flags[i] = uint32_t(mats[OR_MAT_STRIDE * tri_mat[i] + 13]);
+    (void)nmat;
+    const or_render_cfg c = *cfg;
+    Cam k = make_cam(cam, c.width, c.height);
+    size_t count = pixel_ids ? npix : size_t(c.width) * c.height;
+    uint32_t spp = c.spp;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    int stopped = 0;
+    // Tracer::run (Tracer.cpp:58-113): dynamic schedule in chunks of 1000 pixels.
+#pragma omp parallel for schedule(dynamic, 1000)
+    for (int64_t i = 0; i < int64_t(count); ++i) {
+        if (interrupt && *interrupt) { stopped = 1; continue; }
+        uint32_t pixel = pixel_ids ? pixel_ids[i] : uint32_t(i);
+        double* st = stats + 12 * i;
+        for (int q = 0; q < 12; ++q) st[q] = 0.0;
+        for (uint32_t smp = 0; smp < spp; ++smp) {
+            V3 o, d;
+            double w;
+            sample_camera(k, c, pixel, smp, o, d, w);
+            PathOut P = trace_path(s, c, pixel, smp, o, d, nullptr);
+            double L[3] = {P.L.x, P.L.y, P.L.z};
+            st[0] += w;
+            st[1] += w * w;
+            for (int ch = 0; ch < 3; ++ch) {
+                st[2 + ch] += w * L[ch];
+                st[5 + ch] += w * w * L[ch];
+                st[8 + ch] += w * w * L[ch] * L[ch];
+            }
+            st[11] += P.rays;
+        }
+    }
+    return stopped;
+}
+
+extern "C" int or_trace_path(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,
+                             const double* mats, const double cam[7], const or_render_cfg* cfg, uint32_t pixel,
+                             uint32_t sample, double L_out[3], double* weight_out, double* vertices_out) {
+    Scene s{b, vertices, tri_mat, mats, {}};
+    s.tri_flags.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) s.tri_flags[i] = uint32_t(mats[OR_MAT_STRIDE * tri_mat[i] + 13]);
+    Cam k = make_cam(cam, cfg->width, cfg->height);
+    V3 o, d;
+    double w;
+    sample_camera(k, *cfg, pixel, sample, o, d, w);
+    PathOut P = trace_path(s, *cfg, pixel, sample, o, d, vertices_out);
+    L_out[0] = P.L.x; L_out[1] = P.L.y; L_out[2] = P.L.z;
+    *weight_out = w;
+    return P.rays;
+}
+
+// ---------------------------------------------------------------- Phong raycast
+// Raytracer::calculateLightColor for one point light (Raytracer.cpp:312-384),
+// calculatePhongShadingColor (:386-407) and the point-light hard shadow
+// (:524-551).  The primary ray goes through pixel (x + u0, y + u1) with
+// (u0, u1) the CAM_DIM draw of (pixel, sample 0, frame 0).
+extern "C" void or_raycast_phong(const or_bvh* b, const float* vertices, const uint32_t* tri_mat,
+                                 const double* mats, const double cam[7], uint32_t w, uint32_t h, uint64_t seed,
+                                 double offset, const double light[9], const double amb[4],
+                                 const uint32_t* pixel_ids, size_t npix, int64_t* hit_out, double* t_out,
+                                 double* rgb_out, int nthreads) {
+    Cam k = make_cam(cam, w, h);
+    size_t count = pixel_ids ? npix : size_t(w) * h;
+    V3 lp(light[0], light[1], light[2]);
+    V3 lc = V3(light[3], light[4], light[5]) * light[6];
+    double maxd = light[7], att = light[8];
+    V3 ambient = V3(amb[0], amb[1], amb[2]) * amb[3];
+    size_t ntri = or_bvh_ordered_count(b);
+    std::vector<uint32_t> tri_flags(ntri);
+    for (size_t i = 0; i < ntri; ++i) tri_flags[i] = uint32_t(mats[OR_MAT_STRIDE * tri_mat[i] + 13]);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1000)
+    for (int64_t i = 0; i < int64_t(count); ++i) {
+        uint32_t pixel = pixel_ids ? pixel_ids[i] : uint32_t(i);
+        double u[4];
+        draw4(seed, pixel, 0, 0, CAM_DIM, u);
+        V3 o, d;
+        cam_ray(k, double(pixel % w) + u[0], double(pixel / w) + u[1], o, d);
+        RayD r = make_ray(o, d, 0.0, std::numeric_limits<double>::max());
+        Diag D;
+        trace_ordered(b, vertices, r, 0.0, 0.0, D);
+        hit_out[i] = D.id;
+        t_out[i] = D.t;
+        V3 col(0, 0, 0);
+        if (D.id >= 0) {
+            const double* m = mats + OR_MAT_STRIDE * tri_mat[D.id];
+            V3 kd(m[0], m[1], m[2]), ka(m[6], m[7], m[8]), ks(m[9], m[10], m[11]);
+            double shin = m[12];
+            Tri tr = load_tri(vertices, uint64_t(D.id));
+            V3 n = tri_normal(tr);
+            V3 p = o + (D.t * d);
+            col = mul(ambient, ka);
+            V3 tl = lp - p;
+            double dist = length(tl);
+            V3 l = normalized(tl);
+            V3 ph(0, 0, 0);
+            double diff = dot(l, n);
+            if (diff > 0.0) {
+                ph = mul(lc * diff, kd);
+                if (!(ks.x == 0.0 && ks.y == 0.0 && ks.z == 0.0)) {
+                    V3 refl = normalized((2.0 * diff * n) - l);
+                    double sa = dot(refl, V3(-d.x, -d.y, -d.z));
+                    if (sa > 0.0) ph = ph + mul(lc * std::pow(sa, shin), ks);
+                }
+            }
+            V3 ldir = normalized(lp - p);
+            RayD sr = make_ray(p + ldir * offset, ldir, std::numeric_limits<double>::lowest(), length(lp - p));
+            double t2;
+            int64_t id2;
+            // shadow rays skip nonShadowing materials (Triangle.cpp:55-56) and
+            // stop at the first hit (fastOcclusion, FlatBVH.cpp:49-50)
+            bool shadow = trace_ref(b, vertices, sr, true, t2, id2, tri_flags.data(), nullptr);
+            double da = std::min(1.0, dist / maxd);
+            da = 1.0 - std::pow(da, att);
+            col = col + ph * (da * (shadow ? 0.0 : 1.0));
+        }
+        rgb_out[3 * i] = col.x; rgb_out[3 * i + 1] = col.y; rgb_out[3 * i + 2] = col.z;
+    }
+}
+
+// ---------------------------------------------------------------- Reinhard
+// ReinhardToneMapper::apply (ReinhardToneMapper.cpp:18-86) without the moving
+// average (enableAveraging = false, Scene.h default).  Luminance weights from
+// Color::getLuminance (Color.cpp:130-133); clamp as Color::clamped
+// (Color.cpp:166-176, std::max(0, std::min(c, 1)) so NaN becomes 0); gamma via
+// fastPow (Color.cpp:260-270) or std::pow.
+extern "C" void or_reinhard(const float* lin, size_t npix, const double prm[5], float* out,
+                            double* log_avg_out, double* max_lum_out) {
+    const double eps = 0.01;
+    double key = prm[0], gamma = prm[1];
+    bool do_clamp = prm[2] != 0.0, do_gamma = prm[3] != 0.0, fast = prm[4] != 0.0;
+    double logsum = 0.0, maxl = 1.0;
+    for (size_t i = 0; i < npix; ++i) {
+        double r = lin[4 * i], g = lin[4 * i + 1], b = lin[4 * i + 2];
+        double l = 0.2126 * r + 0.7152 * g + 0.0722 * b;
+        logsum += std::log(eps + l);
+        if (l > maxl) maxl = l;
+    }
+    double logavg = std::exp(logsum / double(npix));
+    double scale = key / logavg;
+    double inv2 = 1.0 / (maxl * maxl);
+    double ig = 1.0 / gamma;
+    for (size_t i = 0; i < npix; ++i) {
+        double c[3] = {lin[4 * i], lin[4 * i + 1], lin[4 * i + 2]};
+        double l = 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2];
+        double sl = scale * l;
+        double ml = (sl * (1.0 + (sl * inv2))) / (1.0 + sl);
+        double cs = ml / l;
+        for (int k = 0; k < 3; ++k) {
+            double v = c[k] * cs;
+            if (do_clamp) v = std::max(0.0, std::min(v, 1.0));
+            if (do_gamma) v = fast ? or_fastpow(v, ig) : std::pow(v, ig);
+            out[4 * i + k] = float(v);
+        }
+        out[4 * i + 3] = 1.0f;
+    }
+    if (log_avg_out) *log_avg_out = logavg;
+    if (max_lum_out) *max_lum_out = maxl;
+}

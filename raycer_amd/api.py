@@ -1,0 +1,365 @@
+"""ctypes binding of libraycer.so (include/raycer_hip.h, include/raycer_host.h).
+
+The structures below mirror the C headers field for field.  Every call goes
+through the C ABI; a nonzero status raises RaycerError with the library's
+message (the counterpart of CLManager::checkError throwing
+std::runtime_error, src/OpenCL/CLManager.cpp:220-226).  There is no fallback:
+if libraycer.so is missing or no gfx950 device is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libraycer.so"
+
+RC_OK, RC_ERR_INVALID, RC_ERR_HIP, RC_ERR_ALLOC, RC_ERR_INTERRUPTED, RC_ERR_NO_DEVICE = range(6)
+RC_MAT_EMISSIVE, RC_MAT_NON_SHADOWING = 1, 2
+RC_FILTER_BOX, RC_FILTER_TENT = 0, 1
+RC_SAMPLER_JITTERED, RC_SAMPLER_RANDOM = 0, 1
+RC_EST_REFERENCE, RC_EST_LAMBERT = 0, 1
+RCH_SCENE_BOX, RCH_SCENE_SPHERE_GROUND, RCH_SCENE_TERRAIN, RCH_SCENE_SOUP = 1, 2, 3, 4
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("tri", "<i4")])
+NODE_DTYPE = np.dtype([("aabb_min", "<f8", 3), ("aabb_max", "<f8", 3), ("right_offset", "<i8"),
+                       ("start_offset", "<u8"), ("primitive_count", "<u8")])
+MATERIAL_DTYPE = np.dtype([("diffuse", "<f4", 3), ("emittance", "<f4", 3), ("ambient", "<f4", 3),
+                           ("specular", "<f4", 3), ("shininess", "<f4"), ("flags", "<u4"), ("pad", "<u4", 2)])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and NODE_DTYPE.itemsize == 72
+assert MATERIAL_DTYPE.itemsize == 64
+
+
+class RaycerError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"raycer error {code}: {msg}")
+        self.code = code
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("pitch", C.c_float), ("yaw", C.c_float), ("roll", C.c_float),
+                ("fov", C.c_float)]
+
+
+class RenderCfg(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sampler", C.c_uint32),
+                ("max_path_length", C.c_uint32), ("rr_start", C.c_uint32), ("filter", C.c_uint32),
+                ("estimator", C.c_uint32), ("cache_vertex", C.c_uint32), ("pad0", C.c_uint32),
+                ("filter_radius", C.c_float), ("ray_start_offset", C.c_float), ("sky", C.c_float * 3),
+                ("sun_dir", C.c_float * 3), ("sun_irradiance", C.c_float * 3), ("frame", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+class ToneMap(C.Structure):
+    _fields_ = [("key", C.c_float), ("gamma", C.c_float), ("clamp", C.c_uint32), ("apply_gamma", C.c_uint32),
+                ("fast_pow", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class PointLight(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("color", C.c_float * 3), ("intensity", C.c_float),
+                ("max_distance", C.c_float), ("attenuation", C.c_float), ("ambient_color", C.c_float * 3),
+                ("ambient_intensity", C.c_float)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("spp_per_pass", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("samples", C.c_uint64),
+                ("node_records", C.c_uint64), ("tri_tests", C.c_uint64), ("overflow_rays", C.c_uint64),
+                ("extend_launches", C.c_uint64), ("extend_ms", C.c_double), ("shadow_ms", C.c_double),
+                ("shade_ms", C.c_double), ("other_ms", C.c_double), ("render_ms", C.c_double),
+                ("shadow_launches", C.c_uint64)]
+
+
+class BVHBuildInfo(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_uint64), ("threads", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class SceneParams(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("seed", C.c_uint32), ("level", C.c_uint32), ("grid", C.c_uint32),
+                ("tri_count", C.c_uint64)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("ntri", C.c_size_t), ("nmat", C.c_size_t), ("camera", Camera), ("cfg", RenderCfg),
+                ("light", PointLight), ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libraycer.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RaycerError(RC_ERR_NO_DEVICE, f"{LIB_PATH} is missing: run `python -m raycer_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        vp, sz, u32, u64, i32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
+        sig = {
+            "rc_last_error": [C.c_char_p, sz],
+            "rc_create": [i32, C.POINTER(vp)],
+            "rc_destroy": [vp],
+            "rc_device_name": [vp, C.c_char_p, sz, C.POINTER(i32)],
+            "rc_set_profiling": [vp, i32],
+            "rc_set_counting": [vp, i32],
+            "rc_set_tuning": [vp, C.POINTER(Tuning)],
+            "rc_get_stats": [vp, C.POINTER(Stats)],
+            "rc_synchronize": [vp],
+            "rc_scene_upload": [vp, vp, sz, vp, vp, sz, vp, sz, vp, C.POINTER(vp)],
+            "rc_scene_free": [vp],
+            "rc_scene_info": [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)],
+            "rc_trace_closest": [vp, vp, vp, sz, vp],
+            "rc_trace_any": [vp, vp, vp, sz, vp],
+            "rc_trace_closest_device": [vp, vp, vp, sz, vp],
+            "rc_trace_any_device": [vp, vp, vp, sz, vp],
+            "rc_device_alloc": [vp, sz, C.POINTER(vp)],
+            "rc_device_free": [vp, vp],
+            "rc_memcpy_h2d": [vp, vp, vp, sz],
+            "rc_memcpy_d2h": [vp, vp, vp, sz],
+            "rc_film_create": [vp, u32, u32, C.POINTER(vp)],
+            "rc_film_free": [vp],
+            "rc_film_clear": [vp, vp],
+            "rc_film_download_linear": [vp, vp, vp],
+            "rc_film_download_sums": [vp, vp, vp],
+            "rc_film_tonemap": [vp, vp, C.POINTER(ToneMap), vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+            "rc_render": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), C.POINTER(C.c_int),
+                          C.POINTER(C.c_uint64)],
+            "rc_raycast_phong": [vp, vp, C.POINTER(Camera), u32, u32, u64, C.c_float, C.POINTER(PointLight), vp, vp,
+                                 vp],
+            "rc_render_batch": [vp, vp, vp, vp, vp, sz, C.POINTER(Tuning)],
+            "rch_bvh_build": [vp, sz, C.POINTER(BVHBuildInfo), C.POINTER(vp)],
+            "rch_bvh_node_count": [vp],
+            "rch_bvh_get": [vp, vp, vp],
+            "rch_bvh_build_ms": [vp],
+            "rch_bvh_depth": [vp],
+            "rch_bvh_free": [vp],
+            "rch_scene_describe": [C.POINTER(SceneParams), C.POINTER(SceneDesc)],
+            "rch_scene_generate": [C.POINTER(SceneParams), vp, vp, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.rch_bvh_node_count.restype = C.c_size_t
+        L.rch_bvh_build_ms.restype = C.c_double
+        L.rch_bvh_depth.restype = C.c_uint32
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(4096)
+    lib().rc_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc != RC_OK:
+        raise RaycerError(rc, last_error())
+
+
+def _ptr(a: np.ndarray | None) -> C.c_void_p | None:
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------------ host services
+@dataclass
+class SceneData:
+    vertices: np.ndarray      # (ntri, 9) float32
+    tri_material: np.ndarray  # (ntri,) uint32
+    materials: np.ndarray     # (nmat,) MATERIAL_DTYPE
+    desc: SceneDesc
+
+    @property
+    def ntri(self) -> int:
+        return int(self.vertices.shape[0])
+
+
+def scene_params(kind: int, seed: int = 0, level: int = 0, grid: int = 0, tri_count: int = 0) -> SceneParams:
+    return SceneParams(kind, seed, level, grid, tri_count)
+
+
+def describe_scene(p: SceneParams) -> SceneDesc:
+    d = SceneDesc()
+    check(lib().rch_scene_describe(C.byref(p), C.byref(d)))
+    return d
+
+
+def generate_scene(p: SceneParams) -> SceneData:
+    d = describe_scene(p)
+    v = np.empty((d.ntri, 9), np.float32)
+    m = np.empty(d.ntri, np.uint32)
+    mats = np.zeros(d.nmat, MATERIAL_DTYPE)
+    check(lib().rch_scene_generate(C.byref(p), _ptr(v), _ptr(m), _ptr(mats)))
+    return SceneData(v, m, mats, d)
+
+
+def build_bvh(vertices: np.ndarray, max_leaf: int = 5, threads: int = 0):
+    """FlatBVH::build through rch_bvh_build; returns (nodes, ordered_ids, build_ms, depth)."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 9)
+    h = C.c_void_p()
+    info = BVHBuildInfo(max_leaf, threads, 0)
+    check(lib().rch_bvh_build(_ptr(v), v.shape[0], C.byref(info), C.byref(h)))
+    try:
+        n = lib().rch_bvh_node_count(h)
+        nodes = np.empty(n, NODE_DTYPE)
+        ordered = np.empty(v.shape[0], np.uint64)
+        check(lib().rch_bvh_get(h, _ptr(nodes), _ptr(ordered)))
+        return nodes, ordered, lib().rch_bvh_build_ms(h), lib().rch_bvh_depth(h)
+    finally:
+        lib().rch_bvh_free(h)
+
+
+# ------------------------------------------------------------------ device objects
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().rc_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().rc_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_name(self) -> tuple[str, int]:
+        buf = C.create_string_buffer(256)
+        cu = C.c_int()
+        check(lib().rc_device_name(self.h, buf, len(buf), C.byref(cu)))
+        return buf.value.decode(), cu.value
+
+    def set_profiling(self, on: bool) -> None:
+        check(lib().rc_set_profiling(self.h, int(on)))
+
+    def set_counting(self, on: bool) -> None:
+        check(lib().rc_set_counting(self.h, int(on)))
+
+    def set_tuning(self, block=128, spp_per_pass=1, blocks_per_cu=8) -> None:
+        t = Tuning(block, spp_per_pass, blocks_per_cu, 0)
+        check(lib().rc_set_tuning(self.h, C.byref(t)))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(lib().rc_get_stats(self.h, C.byref(s)))
+        return s
+
+    def synchronize(self) -> None:
+        check(lib().rc_synchronize(self.h))
+
+    def upload(self, sd: SceneData, nodes=None, ordered=None) -> "Scene":
+        if nodes is None:
+            nodes, ordered, _, _ = build_bvh(sd.vertices)
+        return Scene(self, sd, nodes, ordered)
+
+    def trace_closest(self, scene: "Scene", rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE).reshape(-1)
+        hits = np.empty(rays.shape[0], HIT_DTYPE)
+        check(lib().rc_trace_closest(self.h, scene.h, _ptr(rays), rays.shape[0], _ptr(hits)))
+        return hits
+
+    def trace_any(self, scene: "Scene", rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE).reshape(-1)
+        occ = np.empty(rays.shape[0], np.uint8)
+        check(lib().rc_trace_any(self.h, scene.h, _ptr(rays), rays.shape[0], _ptr(occ)))
+        return occ
+
+    def film(self, width: int, height: int) -> "Film":
+        return Film(self, width, height)
+
+    def render(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, interrupt=None,
+               pixels_processed=None) -> None:
+        check(lib().rc_render(self.h, scene.h, film.h, C.byref(cam), C.byref(cfg), interrupt, pixels_processed))
+
+    def raycast_phong(self, scene: "Scene", cam: Camera, width: int, height: int, seed: int, offset: float,
+                      light: PointLight):
+        n = width * height
+        ids = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        rgba = np.empty((n, 4), np.float32)
+        check(lib().rc_raycast_phong(self.h, scene.h, C.byref(cam), width, height, seed, offset, C.byref(light),
+                                     _ptr(ids), _ptr(t), _ptr(rgba)))
+        return ids, t, rgba
+
+
+class Scene:
+    def __init__(self, ctx: Context, sd: SceneData, nodes: np.ndarray, ordered: np.ndarray):
+        self.ctx = ctx
+        h = C.c_void_p()
+        v = np.ascontiguousarray(sd.vertices, np.float32)
+        m = np.ascontiguousarray(sd.tri_material, np.uint32)
+        mats = np.ascontiguousarray(sd.materials)
+        nodes = np.ascontiguousarray(nodes)
+        ordered = np.ascontiguousarray(ordered, np.uint64)
+        check(lib().rc_scene_upload(ctx.h, _ptr(v), v.shape[0], _ptr(m), _ptr(mats), mats.shape[0], _ptr(nodes),
+                                    nodes.shape[0], _ptr(ordered), C.byref(h)))
+        self.h = h
+
+    def info(self) -> tuple[int, int, int]:
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        check(lib().rc_scene_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().rc_scene_free(self.h)
+            self.h = None
+
+
+class Film:
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx, self.width, self.height = ctx, width, height
+        h = C.c_void_p()
+        check(lib().rc_film_create(ctx.h, width, height, C.byref(h)))
+        self.h = h
+
+    def clear(self) -> None:
+        check(lib().rc_film_clear(self.ctx.h, self.h))
+
+    def linear(self) -> np.ndarray:
+        out = np.empty((self.width * self.height, 4), np.float32)
+        check(lib().rc_film_download_linear(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def sums(self) -> np.ndarray:
+        out = np.empty((self.width * self.height, 4), np.float32)
+        check(lib().rc_film_download_sums(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def tonemap(self, key=0.18, gamma=2.2, clamp=True, apply_gamma=True, fast_pow=True):
+        out = np.empty((self.width * self.height, 4), np.float32)
+        tm = ToneMap(key, gamma, int(clamp), int(apply_gamma), int(fast_pow), 0)
+        la, ml = C.c_double(), C.c_double()
+        check(lib().rc_film_tonemap(self.ctx.h, self.h, C.byref(tm), _ptr(out), C.byref(la), C.byref(ml)))
+        return out, la.value, ml.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().rc_film_free(self.h)
+            self.h = None
+
+
+def make_rays(org: np.ndarray, dirs: np.ndarray, tmin=0.0, tmax=np.inf) -> np.ndarray:
+    n = dirs.shape[0]
+    r = np.empty(n, RAY_DTYPE)
+    r["org"] = np.broadcast_to(org, (n, 3))
+    r["dir"] = dirs
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r

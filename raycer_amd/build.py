@@ -1,0 +1,106 @@
+"""Builds libraycer.so (gfx950 HIP kernels + C ABI + host C++) in-tree.
+
+    python -m raycer_amd.build            # library + console runner
+    python -m raycer_amd.build --oracle   # also the FP64 oracle (test infrastructure)
+
+Every HIP file is compiled by hipcc for gfx950 only, with packed-FP32
+instruction selection disabled.  Objects go to raycer_amd/_build/, the
+library to raycer_amd/libraycer.so and the C++ console runner to
+raycer_amd/raycer_render.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libraycer.so"
+RUNNER = PKG / "raycer_render"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+HIPCC = str(ROCM / "bin" / "hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-variable",
+    "-Wno-unused-but-set-variable",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{ROOT / 'include'}",
+    f"-I{CSRC}",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+
+def _run(cmd: list[str]) -> None:
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"command failed ({p.returncode}): {' '.join(cmd)}\n{out}")
+    if out.strip():
+        print(out)
+
+
+def _objects() -> list[tuple[list[str], Path]]:
+    jobs = []
+    for src in sorted(CSRC.glob("*.hip")):
+        obj = BUILD / (src.stem + ".hip.o")
+        jobs.append(([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)], obj))
+    for src in sorted((CSRC / "host").glob("*.cpp")):
+        obj = BUILD / (src.stem + ".cpp.o")
+        jobs.append((["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)], obj))
+    return jobs
+
+
+def build_library(jobs: int = 8) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    work = _objects()
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda j: _run(j[0]), work))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *[str(o) for _, o in work]])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_runner() -> Path:
+    src = CSRC / "cpp" / "console_runner.cpp"
+    if not src.exists():
+        return RUNNER
+    _run(["g++", *CXX_FLAGS, str(src), "-o", str(RUNNER), f"-L{PKG}", "-lraycer", "-Wl,-rpath,$ORIGIN"])
+    return RUNNER
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j4"])
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("-j", type=int, default=8)
+    a = ap.parse_args(argv)
+    if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
+        print("hipcc not found", file=sys.stderr)
+        return 1
+    build_library(a.j)
+    build_runner()
+    if a.oracle:
+        build_oracle()
+    print(f"built {LIB}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,198 @@
+// Host FP64 SAH BVH builder: the algorithm of raycer's FlatBVH::build
+// (src/Raytracing/Primitives/FlatBVH.cpp:89-204) and its SAH split selection
+// (calculateSAHSplit / calculateSAHScore / calculateMedianPoint, FlatBVH.cpp:241-342),
+// producing the same preorder node array and orderedPrimitiveIds.
+//
+// The reference builds with an explicit stack on one thread and sorts the
+// centroids of every node to find medians.  Here subtrees are built
+// recursively; the two children of a large node are built on separate threads
+// into private node arrays that are concatenated in preorder (rightOffset is
+// relative, so concatenation needs no fix-up), and the median comes from
+// nth_element, which yields the same values as the sorted array.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "raycer_host.h"
+#include "rc_host_internal.hpp"
+
+namespace {
+
+struct Prim {
+    double c[3];   // AABB::getCenter of the triangle box
+    double lo[3];
+    double hi[3];
+    uint64_t id;
+};
+
+struct Box {
+    double lo[3] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max(),
+                    std::numeric_limits<double>::max()};
+    double hi[3] = {std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest(),
+                    std::numeric_limits<double>::lowest()};
+    void add(const Prim& p) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p.lo[k]);
+            hi[k] = std::max(hi[k], p.hi[k]);
+        }
+    }
+    double area() const {
+        double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+        return 2.0 * (ex * ey + ez * ey + ex * ez);
+    }
+    double center(int a) const { return (lo[a] + hi[a]) * 0.5; }
+};
+
+struct Builder {
+    std::vector<Prim> prims;
+    uint64_t max_leaf = 5;
+    unsigned spawn_depth = 0;
+    static constexpr uint64_t kParallelMin = 20000;
+
+    double score(uint64_t s, uint64_t e, int axis, double split, double node_area) const {
+        Box L, R;
+        uint64_t nl = 0, nr = 0;
+        for (uint64_t i = s; i < e; ++i) {
+            if (prims[i].c[axis] <= split) { L.add(prims[i]); ++nl; }
+            else { R.add(prims[i]); ++nr; }
+        }
+        double sc = 0.0;
+        if (nl > 0) sc += (L.area() / node_area) * double(nl);
+        if (nr > 0) sc += (R.area() / node_area) * double(nr);
+        return sc;
+    }
+
+    double median(uint64_t s, uint64_t e, int axis, std::vector<double>& tmp) const {
+        uint64_t n = e - s;
+        tmp.resize(n);
+        for (uint64_t i = 0; i < n; ++i) tmp[i] = prims[s + i].c[axis];
+        uint64_t k = n / 2;
+        std::nth_element(tmp.begin(), tmp.begin() + k, tmp.end());
+        double hi = tmp[k];
+        if (n % 2 == 0) {
+            double lo = *std::max_element(tmp.begin(), tmp.begin() + k);
+            return (lo + hi) / 2.0;
+        }
+        return hi;
+    }
+
+    // Appends the subtree over prims[s, e) to out in preorder; returns its depth.
+    uint32_t build(uint64_t s, uint64_t e, std::vector<rc_flat_node>& out, unsigned depth) {
+        Box box;
+        for (uint64_t i = s; i < e; ++i) box.add(prims[i]);
+        rc_flat_node n;
+        for (int k = 0; k < 3; ++k) { n.aabb_min[k] = box.lo[k]; n.aabb_max[k] = box.hi[k]; }
+        n.start_offset = s;
+        n.primitive_count = e - s;
+        n.right_offset = 0;
+        size_t self = out.size();
+        out.push_back(n);
+        if (e - s <= max_leaf) return 1;
+
+        // 6 candidates in the reference order: for each axis the box centre,
+        // then the centroid median; strict '<' keeps the first best.  A node box
+        // of zero area makes every score NaN: the reference then leaves
+        // axis/split uninitialised, here it is axis 0 at the box centre.
+        std::vector<double> tmp;
+        double node_area = box.area();
+        double best = std::numeric_limits<double>::max();
+        int axis = 0;
+        double split = box.center(0);
+        for (int a = 0; a <= 2; ++a) {
+            double c = box.center(a);
+            double sc = score(s, e, a, c, node_area);
+            if (sc < best) { best = sc; axis = a; split = c; }
+            double m = median(s, e, a, tmp);
+            sc = score(s, e, a, m, node_area);
+            if (sc < best) { best = sc; axis = a; split = m; }
+        }
+        // In-place partition with the reference's swap sequence (FlatBVH.cpp:166-173).
+        uint64_t mid = s;
+        for (uint64_t i = s; i < e; ++i) {
+            if (prims[i].c[axis] <= split) {
+                std::swap(prims[i], prims[mid]);
+                ++mid;
+            }
+        }
+        if (mid == s || mid == e) mid = s + (e - s) / 2;  // FlatBVH.cpp:175-177
+
+        uint32_t dl, dr;
+        if (depth < spawn_depth && (e - s) >= kParallelMin) {
+            std::vector<rc_flat_node> left, right;
+            left.reserve(2 * (mid - s) / max_leaf + 8);
+            right.reserve(2 * (e - mid) / max_leaf + 8);
+            std::thread th([&] { dl = build(s, mid, left, depth + 1); });
+            dr = build(mid, e, right, depth + 1);
+            th.join();
+            out[self].right_offset = int64_t(1 + left.size());
+            out.insert(out.end(), left.begin(), left.end());
+            out.insert(out.end(), right.begin(), right.end());
+        } else {
+            dl = build(s, mid, out, depth + 1);
+            out[self].right_offset = int64_t(out.size() - self);
+            dr = build(mid, e, out, depth + 1);
+        }
+        return 1 + std::max(dl, dr);
+    }
+};
+
+}  // namespace
+
+struct rch_bvh {
+    std::vector<rc_flat_node> nodes;
+    std::vector<uint64_t> ordered;
+    double build_ms = 0.0;
+    uint32_t depth = 0;
+};
+
+extern "C" int rch_bvh_build(const float* v, size_t ntri, const rch_bvh_build_info* info, rch_bvh** out) {
+    if (!out || !v || ntri == 0) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_build: empty input");
+    if (ntri > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_build: more than 2^31-1 triangles");
+    auto t0 = std::chrono::steady_clock::now();
+    rch_bvh* b = new (std::nothrow) rch_bvh();
+    if (!b) return rc_host_fail(RC_ERR_ALLOC, "rch_bvh_build: out of memory");
+    Builder B;
+    B.max_leaf = info && info->max_leaf_size ? info->max_leaf_size : 5;
+    unsigned threads = info && info->threads ? info->threads : std::max(1u, std::thread::hardware_concurrency());
+    while ((1u << B.spawn_depth) < threads * 2 && B.spawn_depth < 12) ++B.spawn_depth;
+    if (threads == 1) B.spawn_depth = 0;
+    B.prims.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) {
+        const float* p = v + 9 * i;
+        Prim& q = B.prims[i];
+        for (int k = 0; k < 3; ++k) {
+            double a = p[k], bb = p[3 + k], c = p[6 + k];
+            q.lo[k] = std::min(a, std::min(bb, c));
+            q.hi[k] = std::max(a, std::max(bb, c));
+            q.c[k] = (q.lo[k] + q.hi[k]) * 0.5;
+        }
+        q.id = i;
+    }
+    b->nodes.reserve(2 * ntri / B.max_leaf + 16);
+    b->depth = B.build(0, ntri, b->nodes, 0);
+    b->ordered.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) b->ordered[i] = B.prims[i].id;
+    b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = b;
+    return RC_OK;
+}
+
+extern "C" size_t rch_bvh_node_count(const rch_bvh* b) { return b ? b->nodes.size() : 0; }
+extern "C" int rch_bvh_get(const rch_bvh* b, rc_flat_node* nodes, uint64_t* ordered) {
+    if (!b) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_get: null bvh");
+    if (nodes) std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(rc_flat_node));
+    if (ordered) std::memcpy(ordered, b->ordered.data(), b->ordered.size() * sizeof(uint64_t));
+    return RC_OK;
+}
+extern "C" double rch_bvh_build_ms(const rch_bvh* b) { return b ? b->build_ms : 0.0; }
+extern "C" uint32_t rch_bvh_depth(const rch_bvh* b) { return b ? b->depth : 0; }
+extern "C" int rch_bvh_free(rch_bvh* b) {
+    delete b;
+    return RC_OK;
+}

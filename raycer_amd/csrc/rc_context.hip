@@ -1,0 +1,437 @@
+// Context, error reporting, scene upload (FlatBVH repack) and device memory
+// entry points.  Counterpart of CLManager + CLTracer::initializeBuffers +
+// CLScene::readSceneFull (src/OpenCL/CLManager.cpp, CLTracer.cpp:173-238,
+// CLScene.cpp:72-263).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "rc_internal.hpp"
+
+using namespace rcd;
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int rc_host_fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+extern "C" int rc_last_error(char* buf, size_t len) {
+    if (!buf || len == 0) return RC_ERR_INVALID;
+    size_t n = std::min(len - 1, g_last_error.size());
+    std::memcpy(buf, g_last_error.data(), n);
+    buf[n] = 0;
+    return RC_OK;
+}
+
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes) return RC_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t alloc = std::max<size_t>(need, 256);
+    hipError_t e = hipMalloc(&p, alloc);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return rc_host_fail(RC_ERR_ALLOC, std::string("hipMalloc(") + std::to_string(alloc) + "): " + hipGetErrorString(e));
+    }
+    bytes = alloc;
+    return RC_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int rc_check_ctx(const rc_ctx* ctx, const char* fn) {
+    if (!ctx) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": null context");
+    return RC_OK;
+}
+
+int rc_check_scene(const rc_ctx* ctx, const rc_scene* s, const char* fn) {
+    RC_TRY(rc_check_ctx(ctx, fn));
+    if (!s) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": null scene");
+    if (s->ctx != ctx) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": scene belongs to another context");
+    return RC_OK;
+}
+
+int rc_ctx::prof_begin(hipEvent_t* a) {
+    *a = nullptr;
+    if (!profiling) return RC_OK;
+    if (ev_free.empty()) {
+        hipEvent_t e;
+        RC_TRY_HIP(hipEventCreate(&e));
+        ev_free.push_back(e);
+    }
+    *a = ev_free.back();
+    ev_free.pop_back();
+    RC_TRY_HIP(hipEventRecord(*a, stream));
+    return RC_OK;
+}
+
+int rc_ctx::prof_end(hipEvent_t a, int kind) {
+    if (!profiling || !a) return RC_OK;
+    if (ev_free.empty()) {
+        hipEvent_t e;
+        RC_TRY_HIP(hipEventCreate(&e));
+        ev_free.push_back(e);
+    }
+    hipEvent_t b = ev_free.back();
+    ev_free.pop_back();
+    RC_TRY_HIP(hipEventRecord(b, stream));
+    pending.push_back({a, b, kind});
+    return RC_OK;
+}
+
+int rc_ctx::prof_collect() {
+    RC_TRY_HIP(hipStreamSynchronize(stream));
+    for (const Pending& p : pending) {
+        float ms = 0.0f;
+        RC_TRY_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        switch (p.kind) {
+            case PK_EXTEND: stats.extend_ms += ms; stats.extend_launches++; break;
+            case PK_SHADOW: stats.shadow_ms += ms; stats.shadow_launches++; break;
+            case PK_SHADE: stats.shade_ms += ms; break;
+            default: stats.other_ms += ms; break;
+        }
+        ev_free.push_back(p.a);
+        ev_free.push_back(p.b);
+    }
+    pending.clear();
+    return RC_OK;
+}
+
+extern "C" int rc_create(int device, rc_ctx** out) {
+    if (!out) return rc_host_fail(RC_ERR_INVALID, "rc_create: null out");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return rc_host_fail(RC_ERR_NO_DEVICE, std::string("rc_create: no HIP device (") + hipGetErrorString(e) + ")");
+    if (device < 0 || device >= n) return rc_host_fail(RC_ERR_INVALID, "rc_create: device index out of range");
+    hipDeviceProp_t prop;
+    RC_TRY_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return rc_host_fail(RC_ERR_NO_DEVICE, std::string("rc_create: device is ") + prop.gcnArchName +
+                                                  ", this library is built for gfx950 only");
+    RC_TRY_HIP(hipSetDevice(device));
+    rc_ctx* c = new (std::nothrow) rc_ctx();
+    if (!c) return rc_host_fail(RC_ERR_ALLOC, "rc_create: out of memory");
+    c->device = device;
+    c->cu_count = prop.multiProcessorCount;
+    c->name = prop.name;
+    hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (se != hipSuccess) {
+        delete c;
+        return rc_host_fail(RC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(se));
+    }
+    hipError_t me = hipMalloc(&c->counters, sizeof(DevCounters));
+    if (me != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc_host_fail(RC_ERR_ALLOC, std::string("hipMalloc(counters): ") + hipGetErrorString(me));
+    }
+    (void)hipMemset(c->counters, 0, sizeof(DevCounters));
+    *out = c;
+    return RC_OK;
+}
+
+extern "C" int rc_destroy(rc_ctx* c) {
+    if (!c) return RC_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->batch_graph) (void)hipGraphExecDestroy(c->batch_graph);
+    DevBuf* bufs[] = {&c->trace_rays, &c->trace_hits, &c->trace_occ, &c->ovf_list, &c->deep_stack, &c->ray_a,
+                      &c->ray_b, &c->st_a, &c->st_b, &c->hits, &c->srays, &c->scontrib, &c->socc, &c->slotL,
+                      &c->slotW};
+    for (DevBuf* b : bufs) b->release();
+    for (auto& p : c->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
+    if (c->counters) (void)hipFree(c->counters);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RC_OK;
+}
+
+extern "C" int rc_device_name(rc_ctx* c, char* buf, size_t len, int* cu) {
+    RC_TRY(rc_check_ctx(c, "rc_device_name"));
+    if (buf && len) {
+        size_t n = std::min(len - 1, c->name.size());
+        std::memcpy(buf, c->name.data(), n);
+        buf[n] = 0;
+    }
+    if (cu) *cu = c->cu_count;
+    return RC_OK;
+}
+
+extern "C" int rc_set_profiling(rc_ctx* c, int enable) {
+    RC_TRY(rc_check_ctx(c, "rc_set_profiling"));
+    c->profiling = enable != 0;
+    return RC_OK;
+}
+
+extern "C" int rc_set_counting(rc_ctx* c, int enable) {
+    RC_TRY(rc_check_ctx(c, "rc_set_counting"));
+    c->counting = enable != 0;
+    return RC_OK;
+}
+
+extern "C" int rc_set_tuning(rc_ctx* c, const rc_tuning* t) {
+    RC_TRY(rc_check_ctx(c, "rc_set_tuning"));
+    if (!t) return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: null tuning");
+    if (t->block != 64 && t->block != 128 && t->block != 256)
+        return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: block must be 64, 128 or 256");
+    if (t->spp_per_pass == 0 || t->spp_per_pass > 64)
+        return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: spp_per_pass must be in [1, 64]");
+    if (t->blocks_per_cu == 0 || t->blocks_per_cu > 64)
+        return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: blocks_per_cu must be in [1, 64]");
+    c->tuning = *t;
+    return RC_OK;
+}
+
+extern "C" int rc_get_stats(rc_ctx* c, rc_stats* out) {
+    RC_TRY(rc_check_ctx(c, "rc_get_stats"));
+    if (!out) return rc_host_fail(RC_ERR_INVALID, "rc_get_stats: null out");
+    RC_TRY(c->prof_collect());
+    *out = c->stats;
+    return RC_OK;
+}
+
+extern "C" int rc_synchronize(rc_ctx* c) {
+    RC_TRY(rc_check_ctx(c, "rc_synchronize"));
+    RC_TRY_HIP(hipStreamSynchronize(c->stream));
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------- scene upload
+namespace {
+
+float round_down(double x) {
+    float f = float(x);
+    if (double(f) > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+float round_up(double x) {
+    float f = float(x);
+    if (double(f) < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+}  // namespace
+
+extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uint32_t* tri_mat,
+                               const rc_material* mats, size_t nmat, const rc_flat_node* nodes, size_t nnodes,
+                               const uint64_t* ordered, rc_scene** out) {
+    RC_TRY(rc_check_ctx(c, "rc_scene_upload"));
+    if (!out || !v || !tri_mat || !mats || !nodes || !ordered || ntri == 0 || nnodes == 0 || nmat == 0)
+        return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: null or empty input");
+    if (ntri >= 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: too many triangles");
+    *out = nullptr;
+    // Validate the FlatBVH: a permutation of the triangles, leaves inside the
+    // ordered range, right offsets pointing forward inside the array.
+    {
+        std::vector<uint8_t> seen(ntri, 0);
+        for (size_t i = 0; i < ntri; ++i) {
+            if (ordered[i] >= ntri || seen[ordered[i]])
+                return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: orderedPrimitiveIds is not a permutation");
+            seen[ordered[i]] = 1;
+        }
+        for (size_t i = 0; i < ntri; ++i)
+            if (tri_mat[i] >= nmat) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: material index out of range");
+        for (size_t i = 0; i < nnodes; ++i) {
+            const rc_flat_node& n = nodes[i];
+            if (n.right_offset == 0) {
+                if (n.primitive_count == 0 || n.start_offset + n.primitive_count > ntri)
+                    return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: leaf range outside the triangles");
+            } else if (n.right_offset < 2 || i + uint64_t(n.right_offset) >= nnodes) {
+                return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: bad rightOffset");
+            }
+        }
+    }
+    // Child-pair records in preorder of the interior nodes.
+    std::vector<uint32_t> rec_of(nnodes, 0xffffffffu);
+    size_t ninterior = 0;
+    for (size_t i = 0; i < nnodes; ++i)
+        if (nodes[i].right_offset != 0) rec_of[i] = uint32_t(2 * ninterior++);
+    size_t nrec = ninterior ? 2 * ninterior : 2;
+    if (nrec >= 0xffffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: too many nodes");
+    std::vector<NodeRec> recs(nrec);
+    auto fill = [&](NodeRec& r, size_t child) {
+        const rc_flat_node& n = nodes[child];
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k] = round_down(n.aabb_min[k]);
+            r.hi[k] = round_up(n.aabb_max[k]);
+        }
+        if (n.right_offset == 0) {
+            r.link = uint32_t(n.start_offset);
+            r.count = uint32_t(n.primitive_count);
+        } else {
+            r.link = rec_of[child];
+            r.count = 0;
+        }
+    };
+    if (ninterior == 0) {
+        // a single leaf: both records carry it (testing it twice is harmless)
+        fill(recs[0], 0);
+        fill(recs[1], 0);
+    } else {
+        for (size_t i = 0; i < nnodes; ++i) {
+            if (nodes[i].right_offset == 0) continue;
+            fill(recs[rec_of[i]], i + 1);
+            fill(recs[rec_of[i] + 1], i + size_t(nodes[i].right_offset));
+        }
+    }
+    // depth of the pair tree: the traversal stack never exceeds it
+    uint32_t depth = 0;
+    {
+        std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 1u}};
+        while (!st.empty()) {
+            auto [r, d] = st.back();
+            st.pop_back();
+            depth = std::max(depth, d);
+            for (int k = 0; k < 2; ++k) {
+                const NodeRec& x = recs[r + k];
+                if (x.count == 0) {
+                    if (x.link <= r || x.link + 1 >= nrec)
+                        return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: node link does not point forward");
+                    st.push_back({x.link, d + 1});
+                }
+            }
+        }
+    }
+    if (depth >= uint32_t(kDeepStack))
+        return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: BVH deeper than " + std::to_string(kDeepStack));
+    std::vector<TriRec> tris(ntri);
+    std::vector<float4> shade(ntri);
+    std::vector<DevMaterial> dm(nmat);
+    for (size_t i = 0; i < nmat; ++i) {
+        const rc_material& m = mats[i];
+        uint32_t fl = m.flags;
+        float flf;
+        std::memcpy(&flf, &fl, 4);
+        dm[i].diffuse = make_float4(m.diffuse[0], m.diffuse[1], m.diffuse[2], flf);
+        dm[i].emittance = make_float4(m.emittance[0], m.emittance[1], m.emittance[2], m.shininess);
+        dm[i].ambient = make_float4(m.ambient[0], m.ambient[1], m.ambient[2], 0.0f);
+        dm[i].specular = make_float4(m.specular[0], m.specular[1], m.specular[2], 0.0f);
+    }
+    float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < ntri; ++i) {
+        const float* p = v + 9 * i;
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = double(p[3 + k]) - double(p[k]);
+            e2[k] = double(p[6 + k]) - double(p[k]);
+            for (int q = 0; q < 3; ++q) {
+                bmin[k] = std::min(bmin[k], p[3 * q + k]);
+                bmax[k] = std::max(bmax[k], p[3 * q + k]);
+            }
+        }
+        // Triangle::initialize normal = normalize(e1 x e2) in FP64 (Triangle.cpp:27)
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        double inv = 1.0 / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        uint32_t mi = tri_mat[i];
+        float mf;
+        std::memcpy(&mf, &mi, 4);
+        shade[i] = make_float4(float(n[0] * inv), float(n[1] * inv), float(n[2] * inv), mf);
+    }
+    for (size_t j = 0; j < ntri; ++j) {
+        uint32_t id = uint32_t(ordered[j]);
+        const float* p = v + 9 * size_t(id);
+        uint32_t fl = mats[tri_mat[id]].flags;
+        float idf, flf;
+        std::memcpy(&idf, &id, 4);
+        std::memcpy(&flf, &fl, 4);
+        tris[j].v0 = make_float4(p[0], p[1], p[2], idf);
+        tris[j].e1 = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], flf);
+        tris[j].e2 = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.0f);
+    }
+    rc_scene* s = new (std::nothrow) rc_scene();
+    if (!s) return rc_host_fail(RC_ERR_ALLOC, "rc_scene_upload: out of memory");
+    s->ctx = c;
+    s->nrec = nrec;
+    s->ntri = ntri;
+    s->nmat = nmat;
+    s->depth = depth;
+    std::memcpy(s->bmin, bmin, sizeof(bmin));
+    std::memcpy(s->bmax, bmax, sizeof(bmax));
+    size_t b_nodes = nrec * sizeof(NodeRec), b_tris = ntri * sizeof(TriRec), b_shade = ntri * sizeof(float4),
+           b_mats = nmat * sizeof(DevMaterial);
+    s->bytes = b_nodes + b_tris + b_shade + b_mats;
+    auto cleanup = [&](int code, const std::string& msg) {
+        if (s->nodes) (void)hipFree(s->nodes);
+        if (s->tris) (void)hipFree(s->tris);
+        if (s->shade) (void)hipFree(s->shade);
+        if (s->mats) (void)hipFree(s->mats);
+        delete s;
+        return rc_host_fail(code, msg);
+    };
+    if (hipMalloc(&s->nodes, b_nodes) != hipSuccess || hipMalloc(&s->tris, b_tris) != hipSuccess ||
+        hipMalloc(&s->shade, b_shade) != hipSuccess || hipMalloc(&s->mats, b_mats) != hipSuccess)
+        return cleanup(RC_ERR_ALLOC, "rc_scene_upload: device allocation failed");
+    if (hipMemcpy(s->nodes, recs.data(), b_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->tris, tris.data(), b_tris, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->shade, shade.data(), b_shade, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->mats, dm.data(), b_mats, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(RC_ERR_HIP, "rc_scene_upload: copy to device failed");
+    *out = s;
+    return RC_OK;
+}
+
+extern "C" int rc_scene_free(rc_scene* s) {
+    if (!s) return RC_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(s->nodes);
+    (void)hipFree(s->tris);
+    (void)hipFree(s->shade);
+    (void)hipFree(s->mats);
+    delete s;
+    return RC_OK;
+}
+
+extern "C" int rc_scene_info(const rc_scene* s, size_t* ntri, size_t* nrec, size_t* bytes) {
+    if (!s) return rc_host_fail(RC_ERR_INVALID, "rc_scene_info: null scene");
+    if (ntri) *ntri = s->ntri;
+    if (nrec) *nrec = s->nrec;
+    if (bytes) *bytes = s->bytes;
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------- device memory
+extern "C" int rc_device_alloc(rc_ctx* c, size_t bytes, void** p) {
+    RC_TRY(rc_check_ctx(c, "rc_device_alloc"));
+    if (!p || bytes == 0) return rc_host_fail(RC_ERR_INVALID, "rc_device_alloc: bad argument");
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return rc_host_fail(RC_ERR_ALLOC, std::string("rc_device_alloc: ") + hipGetErrorString(e));
+    return RC_OK;
+}
+extern "C" int rc_device_free(rc_ctx* c, void* p) {
+    RC_TRY(rc_check_ctx(c, "rc_device_free"));
+    if (p) {
+        RC_TRY_HIP(hipStreamSynchronize(c->stream));
+        RC_TRY_HIP(hipFree(p));
+    }
+    return RC_OK;
+}
+extern "C" int rc_memcpy_h2d(rc_ctx* c, void* dst, const void* src, size_t bytes) {
+    RC_TRY(rc_check_ctx(c, "rc_memcpy_h2d"));
+    RC_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    RC_TRY_HIP(hipStreamSynchronize(c->stream));
+    return RC_OK;
+}
+extern "C" int rc_memcpy_d2h(rc_ctx* c, void* dst, const void* src, size_t bytes) {
+    RC_TRY(rc_check_ctx(c, "rc_memcpy_d2h"));
+    RC_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    RC_TRY_HIP(hipStreamSynchronize(c->stream));
+    return RC_OK;
+}

@@ -1,0 +1,104 @@
+// Host-side internals shared by the .hip translation units of libraycer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "host/rc_host_internal.hpp"
+#include "raycer_hip.h"
+#include "rc_device.hpp"
+
+#define RC_TRY_HIP(expr)                                                                               \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return rc_host_fail(RC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+#define RC_TRY(expr)                 \
+    do {                             \
+        int r_ = (expr);             \
+        if (r_ != RC_OK) return r_;  \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Device counters written by the kernels (one allocation per context).
+struct DevCounters {
+    unsigned long long node_records;
+    unsigned long long tri_tests;
+    unsigned long long ext_rays;
+    unsigned long long shadow_rays;
+    uint32_t ovf_count;
+    uint32_t queue_count[4];   // wavefront queue sizes (ping, pong, shadow, scratch)
+    uint32_t error;
+};
+
+enum ProfKind { PK_EXTEND = 0, PK_SHADOW = 1, PK_SHADE = 2, PK_OTHER = 3 };
+
+struct rc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    std::string name;
+    bool profiling = false;
+    bool counting = false;
+    rc_tuning tuning{128, 1, 8, 0};
+    rc_stats stats{};
+    DevCounters* counters = nullptr;  // device
+    DevBuf trace_rays, trace_hits, trace_occ, ovf_list, deep_stack;
+    // wavefront state
+    DevBuf ray_a, ray_b, st_a, st_b, hits, srays, scontrib, socc, slotL, slotW;
+    // profiling events
+    struct Pending {
+        hipEvent_t a, b;
+        int kind;
+    };
+    std::vector<hipEvent_t> ev_free;
+    std::vector<Pending> pending;
+    hipGraphExec_t batch_graph = nullptr;
+
+    int prof_begin(hipEvent_t* a);
+    int prof_end(hipEvent_t a, int kind);
+    int prof_collect();  // synchronises and accumulates pending timings
+};
+
+struct rc_scene {
+    rc_ctx* ctx = nullptr;
+    rcd::NodeRec* nodes = nullptr;
+    size_t nrec = 0;
+    rcd::TriRec* tris = nullptr;
+    size_t ntri = 0;
+    float4* shade = nullptr;  // normal.xyz, material index bits
+    rcd::DevMaterial* mats = nullptr;
+    size_t nmat = 0;
+    float bmin[3], bmax[3];
+    uint32_t depth = 0;
+    size_t bytes = 0;
+};
+
+struct rc_film {
+    rc_ctx* ctx = nullptr;
+    uint32_t width = 0, height = 0;
+    float4* sum = nullptr;   // w*rgb, w
+    float4* comp = nullptr;  // Kahan compensation terms
+    DevBuf linear, partial;
+};
+
+// Launches the closest-hit (ANY = false) or any-hit traversal over a device
+// ray array.  n_dev (may be null) is a device count read by the kernel; n_bound
+// bounds it and sizes the grid.
+int rc_launch_trace(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, const uint32_t* n_dev,
+                    uint32_t n_bound, rc_hit* hits, uint8_t* occ);
+int rc_check_ctx(const rc_ctx* ctx, const char* fn);
+int rc_check_scene(const rc_ctx* ctx, const rc_scene* s, const char* fn);

@@ -1,0 +1,610 @@
+// Wavefront path tracer and Phong raycast: the device counterpart of
+// Tracer::run / generateMultiSamples (src/Raytracing/Tracers/Tracer.cpp:58-144),
+// PathTracer::traceRecursive (PathTracer.cpp:25-75) and the Phong + point-light
+// shadow part of Raytracer (Raytracer.cpp:312-407, 524-551).
+//
+// A render is a sequence of passes; a pass traces spp_per_pass samples of
+// every pixel.  Slot s = j * npix + pixel holds sample j of the pass.  Per
+// bounce: extend (closest hit over the live queue) -> shade (emission, sky,
+// sun next-event ray, cosine bounce, Russian roulette; survivors compacted
+// into the next queue with one atomic per wave) -> shadow (any hit) -> apply.
+// After the last bounce every pixel adds its spp_per_pass slot results to the
+// film in a fixed order, with Kahan compensation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "rc_internal.hpp"
+
+using namespace rcd;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr float kPi = 3.14159265358979323846f;
+
+struct CamBasis {
+    float3 pos, right, up, center;
+    float ipw, iph, aspect;
+};
+
+// Camera::update / getRay basis in FP64 (EulerAngle.cpp:77-87, ONB.cpp:32-39,
+// Camera.cpp:29-34,160,279-283), rounded to FP32 for the kernels.
+CamBasis make_basis(const rc_camera& c, uint32_t w, uint32_t h) {
+    const double d2r = M_PI / 180.0;
+    double pitch = c.pitch, yaw = c.yaw;
+    double fov = std::max(1.0, std::min(double(c.fov), 180.0));
+    double f[3] = {-std::sin(yaw * d2r) * std::cos(pitch * d2r), std::sin(pitch * d2r),
+                   -std::cos(yaw * d2r) * std::cos(pitch * d2r)};
+    auto norm = [](double* v) {
+        double inv = 1.0 / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int k = 0; k < 3; ++k) v[k] *= inv;
+    };
+    auto cross = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    norm(f);
+    const double up0[3] = {0.0001, 1.0, 0.0001};  // Vector3::ALMOST_UP
+    double r[3], u[3];
+    cross(f, up0, r);
+    norm(r);
+    cross(r, f, u);
+    norm(u);
+    double dist = 0.5 / std::tan(fov / 2.0 * d2r);
+    CamBasis b;
+    b.pos = make_float3(c.position[0], c.position[1], c.position[2]);
+    b.right = make_float3(float(r[0]), float(r[1]), float(r[2]));
+    b.up = make_float3(float(u[0]), float(u[1]), float(u[2]));
+    b.center = make_float3(float(c.position[0] + f[0] * dist), float(c.position[1] + f[1] * dist),
+                           float(c.position[2] + f[2] * dist));
+    b.ipw = float(w - 1);
+    b.iph = float(h - 1);
+    b.aspect = float(double(h) / double(w));
+    return b;
+}
+
+__device__ __forceinline__ float3 cam_dir(const CamBasis& b, float px, float py) {
+    float dx = (px / b.ipw) - 0.5f;
+    float dy = (py / b.iph) - 0.5f;
+    float3 p = b.center + (dx * b.right) + ((dy * b.aspect) * b.up);
+    return normalize3(p - b.pos);
+}
+
+struct PassParams {
+    CamBasis cam;
+    uint32_t width, npix, spp_pass, sample0;
+    uint32_t spp, sampler, jitter_n, filter;
+    float radius, offset;
+    uint32_t max_len, rr_start, estimator, frame;
+    float3 sky, sun, sun_e;
+    uint64_t seed;
+    uint32_t has_sun, pad;
+};
+
+__device__ __forceinline__ float filter_1d(uint32_t f, float r, float x) {
+    if (f == RC_FILTER_BOX) return (x >= -r && x < r) ? 1.0f / (2.0f * r) : 0.0f;
+    float rinv = 1.0f / r;
+    float s = fabsf(x * rinv);
+    return (s < 1.0f ? 1.0f - s : 0.0f) * rinv;
+}
+
+// Tracer::generateMultiSamples (Tracer.cpp:115-144) for one slot.
+__global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restrict__ rays, float4* __restrict__ st,
+                                                   float4* __restrict__ L, float* __restrict__ W,
+                                                   DevCounters* __restrict__ ctr) {
+    uint32_t n = P.npix * P.spp_pass;
+    uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot == 0) {
+        ctr->queue_count[0] = n;
+        ctr->queue_count[1] = 0;
+        ctr->queue_count[2] = 0;
+    }
+    if (slot >= n) return;
+    uint32_t j = slot / P.npix, pixel = slot - j * P.npix;
+    uint32_t s = P.sample0 + j;
+    float px = float(pixel % P.width), py = float(pixel / P.width);
+    float w = 1.0f;
+    if (!(P.sampler == RC_SAMPLER_JITTERED && P.spp == 1)) {
+        float4 u = draw4(P.seed, pixel, s, P.frame, kCamDim);
+        float ox, oy;
+        if (P.sampler == RC_SAMPLER_JITTERED) {
+            uint32_t sy = s / P.jitter_n, sx = s - sy * P.jitter_n;
+            ox = (float(sx) + u.x) / float(P.jitter_n);
+            oy = (float(sy) + u.y) / float(P.jitter_n);
+        } else {
+            ox = u.x;
+            oy = u.y;
+        }
+        ox = (ox - 0.5f) * 2.0f * P.radius;
+        oy = (oy - 0.5f) * 2.0f * P.radius;
+        w = filter_1d(P.filter, P.radius, ox) * filter_1d(P.filter, P.radius, oy);
+        px += ox;
+        py += oy;
+    }
+    float3 d = cam_dir(P.cam, px, py);
+    rc_ray r;
+    r.org[0] = P.cam.pos.x; r.org[1] = P.cam.pos.y; r.org[2] = P.cam.pos.z;
+    r.tmin = 0.0f;
+    r.dir[0] = d.x; r.dir[1] = d.y; r.dir[2] = d.z;
+    r.tmax = __builtin_inff();
+    rays[slot] = r;
+    st[slot] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(slot));
+    L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W[slot] = w;
+}
+
+// Reserves `pred ? 1 : 0` entries of a queue: one atomic per wave, positions
+// from the ballot prefix.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
+    uint64_t mask = __ballot(pred);
+    uint64_t active = __ballot(1);
+    uint32_t lane = __lane_id();
+    uint32_t prefix = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t total = __popcll(mask);
+    int leader = __ffsll((unsigned long long)active) - 1;
+    uint32_t base = 0;
+    if (int(lane) == leader && total) base = atomicAdd(counter, total);
+    base = __shfl(base, leader, 64);
+    return base + prefix;
+}
+
+// mapToHemisphere with distribution 1 (Sampler.cpp:50-64) in the ONB of a
+// triangle without texcoords (Triangle.cpp:40-44: T = n x ALMOST_UP, B = T x n).
+__device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
+    float3 T = normalize3(cross3(n, make_float3(0.0001f, 1.0f, 0.0001f)));
+    float3 B = normalize3(cross3(T, n));
+    float phi = 2.0f * kPi * u1;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    float ct = sqrtf(1.0f - u2);
+    float stt = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    return ((stt * cp) * T + (stt * sp) * B) + ct * n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(PassParams P, uint32_t k, const rc_ray* __restrict__ rays,
+                                                  const float4* __restrict__ st, const rc_hit* __restrict__ hits,
+                                                  const float4* __restrict__ shade, const DevMaterial* __restrict__ mats,
+                                                  rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
+                                                  rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
+                                                  float4* __restrict__ L, DevCounters* __restrict__ ctr,
+                                                  uint32_t cin, uint32_t cout) {
+    const uint32_t n = ctr->queue_count[cin];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+    if (gtid == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
+    const bool more = (k + 1) < P.max_len;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        float4 r0 = rp[0], r1 = rp[1];
+        float4 s = st[i];
+        float4 h = reinterpret_cast<const float4*>(hits)[i];
+        uint32_t slot = __float_as_uint(s.w);
+        float3 thr = f3(s);
+        int tri = __float_as_int(h.w);
+        bool alive = false, shadow = false;
+        float3 newo, newd, contrib, so;
+        if (tri < 0) {
+            float4 l = L[slot];
+            L[slot] = make_float4(l.x + thr.x * P.sky.x, l.y + thr.y * P.sky.y, l.z + thr.z * P.sky.z, l.w);
+        } else {
+            float4 sh = shade[tri];
+            const DevMaterial& m = mats[__float_as_uint(sh.w)];
+            float4 dif = m.diffuse;
+            uint32_t flags = __float_as_uint(dif.w);
+            if (flags & RC_MAT_EMISSIVE) {
+                float4 e = m.emittance;
+                float4 l = L[slot];
+                L[slot] = make_float4(l.x + thr.x * e.x, l.y + thr.y * e.y, l.z + thr.z * e.z, l.w);
+            } else {
+                float3 nrm = f3(sh);
+                float3 o = f3(r0), d = f3(r1);
+                float3 p = o + (h.x * d);
+                float3 rho = f3(dif);
+                float4 u = draw4(P.seed, slot % P.npix, P.sample0 + slot / P.npix, P.frame, k);
+                if (P.has_sun && more) {
+                    float cs = dot3(nrm, P.sun);
+                    if (cs > 0.0f) {
+                        float f = P.estimator == RC_EST_REFERENCE ? (2.0f / kPi) * cs * cs : (1.0f / kPi) * cs;
+                        contrib = mul3(thr, mul3(rho, P.sun_e)) * f;
+                        so = p + P.sun * P.offset;
+                        shadow = true;
+                    }
+                }
+                if (more) {
+                    float3 w = hemisphere(nrm, u.x, u.y);
+                    if (P.estimator == RC_EST_REFERENCE) thr = mul3(thr, rho * (2.0f * fabsf(dot3(w, nrm))));
+                    else thr = mul3(thr, rho);
+                    alive = true;
+                    if (P.rr_start > 0 && k >= P.rr_start) {
+                        float q = fminf(1.0f, fmaxf(thr.x, fmaxf(thr.y, thr.z)));
+                        if (u.z >= q) alive = false;
+                        else thr = thr * (1.0f / q);
+                    }
+                    newo = p + w * P.offset;
+                    newd = w;
+                }
+            }
+        }
+        uint32_t pos = wave_append(alive, &ctr->queue_count[cout]);
+        if (alive) {
+            float4* ro = reinterpret_cast<float4*>(rays_out + pos);
+            ro[0] = make_float4(newo.x, newo.y, newo.z, 0.0f);
+            ro[1] = make_float4(newd.x, newd.y, newd.z, __builtin_inff());
+            st_out[pos] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(slot));
+        }
+        uint32_t spos = wave_append(shadow, &ctr->queue_count[2]);
+        if (shadow) {
+            float4* ro = reinterpret_cast<float4*>(srays + spos);
+            ro[0] = make_float4(so.x, so.y, so.z, 0.0f);
+            ro[1] = make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff());
+            scontrib[spos] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(slot));
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shadow_apply(const float4* __restrict__ scontrib,
+                                                         const uint8_t* __restrict__ occ, float4* __restrict__ L,
+                                                         DevCounters* __restrict__ ctr) {
+    const uint32_t n = ctr->queue_count[2];
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+    if (gtid == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
+    for (uint32_t i = gtid; i < n; i += stride) {
+        if (occ[i]) continue;
+        float4 c = scontrib[i];
+        uint32_t slot = __float_as_uint(c.w);
+        float4 l = L[slot];
+        L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+    }
+}
+
+// Zeroes the queue the next shade writes and the shadow queue.
+__global__ void k_prep(DevCounters* ctr, uint32_t cout) {
+    ctr->queue_count[cout] = 0;
+    ctr->queue_count[2] = 0;
+}
+
+__device__ __forceinline__ void kahan(float& s, float& c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Film::addSample (Film.cpp:54-59) for the pass's samples of each pixel, in
+// slot order, with FP32 Kahan compensation of the running sums.
+__global__ __launch_bounds__(kBlock) void k_film_add(uint32_t npix, uint32_t spp_pass, const float4* __restrict__ L,
+                                                     const float* __restrict__ W, float4* __restrict__ sum,
+                                                     float4* __restrict__ comp) {
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    float4 s = sum[p], c = comp[p];
+    for (uint32_t j = 0; j < spp_pass; ++j) {
+        uint32_t slot = j * npix + p;
+        float w = W[slot];
+        float4 l = L[slot];
+        kahan(s.x, c.x, w * l.x);
+        kahan(s.y, c.y, w * l.y);
+        kahan(s.z, c.z, w * l.z);
+        kahan(s.w, c.w, w);
+    }
+    sum[p] = s;
+    comp[p] = c;
+}
+
+int alloc_pass(rc_ctx* c, size_t npaths) {
+    RC_TRY(c->ray_a.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(c->ray_b.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(c->st_a.ensure(npaths * sizeof(float4)));
+    RC_TRY(c->st_b.ensure(npaths * sizeof(float4)));
+    RC_TRY(c->hits.ensure(npaths * sizeof(rc_hit)));
+    RC_TRY(c->srays.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(c->scontrib.ensure(npaths * sizeof(float4)));
+    RC_TRY(c->socc.ensure(npaths));
+    RC_TRY(c->slotL.ensure(npaths * sizeof(float4)));
+    RC_TRY(c->slotW.ensure(npaths * sizeof(float)));
+    return RC_OK;
+}
+
+uint32_t isqrt_exact(uint32_t v) {
+    uint32_t r = uint32_t(std::sqrt(double(v)));
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+__global__ void k_zero_counters(DevCounters* ctr) {
+    ctr->node_records = 0;
+    ctr->tri_tests = 0;
+    ctr->ext_rays = 0;
+    ctr->shadow_rays = 0;
+    ctr->ovf_count = 0;
+    ctr->error = 0;
+}
+
+}  // namespace
+
+int rc_validate_cfg(const rc_render_cfg* cfg, const char* fn) {
+    if (!cfg) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": null config");
+    if (cfg->width < 2 || cfg->height < 2 || cfg->width > 16384 || cfg->height > 16384)
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": film size out of range");
+    if (cfg->spp == 0 || cfg->spp > 65536) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": spp out of range");
+    if (cfg->sampler == RC_SAMPLER_JITTERED) {
+        uint32_t n = isqrt_exact(cfg->spp);
+        if (n * n != cfg->spp)
+            return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": jittered sampling needs spp = n*n");
+    } else if (cfg->sampler != RC_SAMPLER_RANDOM) {
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": unknown sampler");
+    }
+    if (cfg->max_path_length == 0 || cfg->max_path_length > 64)
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": max_path_length out of range");
+    if (cfg->filter > RC_FILTER_TENT || !(cfg->filter_radius > 0.0f))
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": bad filter");
+    if (cfg->estimator > RC_EST_LAMBERT) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": bad estimator");
+    return RC_OK;
+}
+
+// Sizes every buffer a frame of this config needs (outside any capture).
+int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass) {
+    spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
+    const size_t npaths = size_t(cfg->width) * cfg->height * spp_pass;
+    if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
+    RC_TRY(alloc_pass(c, npaths));
+    RC_TRY(c->ovf_list.ensure(npaths * sizeof(uint32_t)));
+    RC_TRY(c->deep_stack.ensure(size_t(2048) * kDeepStack * sizeof(uint32_t)));
+    return RC_OK;
+}
+
+// One frame of a scene into a film: the passes of rc_render, enqueued on the
+// context stream.  Used by rc_render and the batched graph.
+int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                     uint32_t spp_pass, volatile const int* interrupt, uint64_t* pixels_processed, bool sync_passes) {
+    const uint32_t npix = cfg->width * cfg->height;
+    spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
+    const size_t npaths = size_t(npix) * spp_pass;
+    if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
+    RC_TRY(alloc_pass(c, npaths));
+    PassParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.cam = make_basis(*cam, cfg->width, cfg->height);
+    P.width = cfg->width;
+    P.npix = npix;
+    P.spp = cfg->spp;
+    P.sampler = cfg->sampler;
+    P.jitter_n = isqrt_exact(cfg->spp);
+    P.filter = cfg->filter;
+    P.radius = cfg->filter_radius;
+    P.offset = cfg->ray_start_offset;
+    P.max_len = cfg->max_path_length;
+    P.rr_start = cfg->rr_start;
+    P.estimator = cfg->estimator;
+    P.frame = cfg->frame;
+    P.sky = make_float3(cfg->sky[0], cfg->sky[1], cfg->sky[2]);
+    P.sun = make_float3(cfg->sun_dir[0], cfg->sun_dir[1], cfg->sun_dir[2]);
+    P.sun_e = make_float3(cfg->sun_irradiance[0], cfg->sun_irradiance[1], cfg->sun_irradiance[2]);
+    P.has_sun = (cfg->sun_irradiance[0] > 0.0f || cfg->sun_irradiance[1] > 0.0f || cfg->sun_irradiance[2] > 0.0f);
+    P.seed = cfg->seed;
+    const uint32_t grid_all = uint32_t((npaths + kBlock - 1) / kBlock);
+    const uint32_t grid_persist = std::min(grid_all, uint32_t(c->cu_count) * 8u);
+    rc_ray* ra = c->ray_a.as<rc_ray>();
+    rc_ray* rb = c->ray_b.as<rc_ray>();
+    float4* sa = c->st_a.as<float4>();
+    float4* sb = c->st_b.as<float4>();
+    for (uint32_t s0 = 0; s0 < cfg->spp; s0 += spp_pass) {
+        if (interrupt && *interrupt) return rc_host_fail(RC_ERR_INTERRUPTED, "rc_render: interrupted");
+        uint32_t this_pass = std::min(spp_pass, cfg->spp - s0);
+        P.spp_pass = this_pass;
+        P.sample0 = s0;
+        const uint32_t n = npix * this_pass;
+        const uint32_t grid_n = (n + kBlock - 1) / kBlock;
+        hipEvent_t ev;
+        RC_TRY(c->prof_begin(&ev));
+        k_camera<<<grid_n, kBlock, 0, c->stream>>>(P, ra, sa, c->slotL.as<float4>(), c->slotW.as<float>(), c->counters);
+        RC_TRY(c->prof_end(ev, PK_OTHER));
+        rc_ray* rin = ra;
+        rc_ray* rout = rb;
+        float4* sin_ = sa;
+        float4* sout = sb;
+        for (uint32_t k = 0; k < cfg->max_path_length; ++k) {
+            uint32_t cin = k & 1u, cout = cin ^ 1u;
+            RC_TRY(rc_launch_trace(c, s, false, rin, &c->counters->queue_count[cin], n, c->hits.as<rc_hit>(), nullptr));
+            RC_TRY(c->prof_begin(&ev));
+            k_prep<<<1, 1, 0, c->stream>>>(c->counters, cout);
+            k_shade<<<std::min(grid_persist, grid_n), kBlock, 0, c->stream>>>(
+                P, k, rin, sin_, c->hits.as<rc_hit>(), s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
+                c->scontrib.as<float4>(), c->slotL.as<float4>(), c->counters, cin, cout);
+            RC_TRY(c->prof_end(ev, PK_SHADE));
+            if (P.has_sun && k + 1 < cfg->max_path_length) {
+                RC_TRY(rc_launch_trace(c, s, true, c->srays.as<rc_ray>(), &c->counters->queue_count[2], n, nullptr,
+                                       c->socc.as<uint8_t>()));
+                RC_TRY(c->prof_begin(&ev));
+                k_shadow_apply<<<std::min(grid_persist, grid_n), kBlock, 0, c->stream>>>(
+                    c->scontrib.as<float4>(), c->socc.as<uint8_t>(), c->slotL.as<float4>(), c->counters);
+                RC_TRY(c->prof_end(ev, PK_OTHER));
+            }
+            std::swap(rin, rout);
+            std::swap(sin_, sout);
+        }
+        RC_TRY(c->prof_begin(&ev));
+        k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(npix, this_pass, c->slotL.as<float4>(),
+                                                                          c->slotW.as<float>(), f->sum, f->comp);
+        RC_TRY(c->prof_end(ev, PK_OTHER));
+        RC_TRY_HIP(hipGetLastError());
+        if (sync_passes) {
+            RC_TRY_HIP(hipStreamSynchronize(c->stream));
+            if (pixels_processed)
+                *pixels_processed = uint64_t(double(npix) * double(s0 + this_pass) / double(cfg->spp));
+        }
+    }
+    return RC_OK;
+}
+
+extern "C" int rc_render(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                         volatile const int* interrupt, uint64_t* pixels_processed) {
+    RC_TRY(rc_check_scene(c, s, "rc_render"));
+    RC_TRY(rc_validate_cfg(cfg, "rc_render"));
+    if (!f || f->ctx != c) return rc_host_fail(RC_ERR_INVALID, "rc_render: bad film");
+    if (!cam) return rc_host_fail(RC_ERR_INVALID, "rc_render: null camera");
+    if (f->width != cfg->width || f->height != cfg->height)
+        return rc_host_fail(RC_ERR_INVALID, "rc_render: film size differs from the config");
+    RC_TRY_HIP(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    c->stats.extend_ms = c->stats.shadow_ms = c->stats.shade_ms = c->stats.other_ms = 0.0;
+    c->stats.extend_launches = c->stats.shadow_launches = 0;
+    k_zero_counters<<<1, 1, 0, c->stream>>>(c->counters);
+    int rc = rc_enqueue_frame(c, s, f, cam, cfg, c->tuning.spp_per_pass, interrupt, pixels_processed,
+                              interrupt != nullptr || pixels_processed != nullptr);
+    RC_TRY_HIP(hipStreamSynchronize(c->stream));
+    if (rc != RC_OK) return rc;
+    DevCounters h;
+    RC_TRY_HIP(hipMemcpy(&h, c->counters, sizeof(DevCounters), hipMemcpyDeviceToHost));
+    if (h.error) return rc_host_fail(RC_ERR_HIP, "rc_render: traversal stack exceeded the deep-stack limit");
+    c->stats.render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    RC_TRY(c->prof_collect());
+    c->stats.extension_rays = h.ext_rays;
+    c->stats.shadow_rays = h.shadow_rays;
+    c->stats.samples = uint64_t(cfg->width) * cfg->height * cfg->spp;
+    c->stats.node_records = h.node_records;
+    c->stats.tri_tests = h.tri_tests;
+    c->stats.overflow_rays = h.ovf_count;
+    if (pixels_processed) *pixels_processed = uint64_t(cfg->width) * cfg->height;
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------- Phong raycast
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_cast_camera(CamBasis cam, uint32_t width, uint32_t npix, uint64_t seed,
+                                                        rc_ray* __restrict__ rays) {
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    float4 u = draw4(seed, p, 0, 0, kCamDim);
+    float3 d = cam_dir(cam, float(p % width) + u.x, float(p / width) + u.y);
+    float4* r = reinterpret_cast<float4*>(rays + p);
+    r[0] = make_float4(cam.pos.x, cam.pos.y, cam.pos.z, 0.0f);
+    r[1] = make_float4(d.x, d.y, d.z, __builtin_inff());
+}
+
+struct LightParams {
+    float3 pos, color, ambient;
+    float max_d, att, offset;
+};
+
+// Raytracer::calculateLightColor for one point light + calculatePhongShadingColor
+// (Raytracer.cpp:312-407); the shadow ray of calculateShadowAmount (:524-551)
+// starts rayStartOffset along the light direction and ends at the light.
+__global__ __launch_bounds__(kBlock) void k_phong_prepare(uint32_t npix, LightParams Lp, const rc_ray* __restrict__ rays,
+                                                          const rc_hit* __restrict__ hits,
+                                                          const float4* __restrict__ shade,
+                                                          const DevMaterial* __restrict__ mats,
+                                                          rc_ray* __restrict__ srays, float4* __restrict__ direct,
+                                                          float4* __restrict__ rgba) {
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    float4 h = reinterpret_cast<const float4*>(hits)[p];
+    int tri = __float_as_int(h.w);
+    float4* sr = reinterpret_cast<float4*>(srays + p);
+    if (tri < 0) {
+        sr[0] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // empty interval: never occluded
+        sr[1] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+        direct[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rgba[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        return;
+    }
+    const float4* rp = reinterpret_cast<const float4*>(rays + p);
+    float3 o = f3(rp[0]), d = f3(rp[1]);
+    float3 pos = o + (h.x * d);
+    float4 sh = shade[tri];
+    const DevMaterial& m = mats[__float_as_uint(sh.w)];
+    float3 n = f3(sh), kd = f3(m.diffuse), ka = f3(m.ambient), ks = f3(m.specular);
+    float shin = m.emittance.w;
+    float3 col = mul3(Lp.ambient, ka);
+    float3 tl = Lp.pos - pos;
+    float dist = sqrtf(dot3(tl, tl));
+    float3 l = normalize3(tl);
+    float3 ph = make_float3(0.0f, 0.0f, 0.0f);
+    float diff = dot3(l, n);
+    if (diff > 0.0f) {
+        ph = mul3(Lp.color * diff, kd);
+        if (!(ks.x == 0.0f && ks.y == 0.0f && ks.z == 0.0f)) {
+            float3 refl = normalize3((2.0f * diff * n) - l);
+            float sa = dot3(refl, make_float3(-d.x, -d.y, -d.z));
+            if (sa > 0.0f) ph = ph + mul3(Lp.color * powf(sa, shin), ks);
+        }
+    }
+    float da = fminf(1.0f, dist / Lp.max_d);
+    da = 1.0f - powf(da, Lp.att);
+    float3 so = pos + l * Lp.offset;
+    sr[0] = make_float4(so.x, so.y, so.z, 0.0f);
+    sr[1] = make_float4(l.x, l.y, l.z, dist);
+    direct[p] = make_float4(ph.x * da, ph.y * da, ph.z * da, 0.0f);
+    rgba[p] = make_float4(col.x, col.y, col.z, 1.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_phong_finish(uint32_t npix, const uint8_t* __restrict__ occ,
+                                                         const float4* __restrict__ direct, float4* __restrict__ rgba) {
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    if (occ[p]) return;
+    float4 c = rgba[p], dd = direct[p];
+    rgba[p] = make_float4(c.x + dd.x, c.y + dd.y, c.z + dd.z, 1.0f);
+}
+
+}  // namespace
+
+extern "C" int rc_raycast_phong(rc_ctx* c, const rc_scene* s, const rc_camera* cam, uint32_t w, uint32_t h,
+                                uint64_t seed, float offset, const rc_point_light* light, int32_t* hit_ids, float* t,
+                                float* rgba) {
+    RC_TRY(rc_check_scene(c, s, "rc_raycast_phong"));
+    if (!cam || !light) return rc_host_fail(RC_ERR_INVALID, "rc_raycast_phong: null camera or light");
+    if (w < 2 || h < 2 || size_t(w) * h > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_raycast_phong: bad size");
+    RC_TRY_HIP(hipSetDevice(c->device));
+    const uint32_t npix = w * h;
+    RC_TRY(alloc_pass(c, npix));
+    CamBasis B = make_basis(*cam, w, h);
+    LightParams Lp;
+    Lp.pos = make_float3(light->position[0], light->position[1], light->position[2]);
+    Lp.color = make_float3(light->color[0] * light->intensity, light->color[1] * light->intensity,
+                           light->color[2] * light->intensity);
+    Lp.ambient = make_float3(light->ambient_color[0] * light->ambient_intensity,
+                             light->ambient_color[1] * light->ambient_intensity,
+                             light->ambient_color[2] * light->ambient_intensity);
+    Lp.max_d = light->max_distance;
+    Lp.att = light->attenuation;
+    Lp.offset = offset;
+    const uint32_t grid = (npix + kBlock - 1) / kBlock;
+    k_zero_counters<<<1, 1, 0, c->stream>>>(c->counters);
+    hipEvent_t ev;
+    RC_TRY(c->prof_begin(&ev));
+    k_cast_camera<<<grid, kBlock, 0, c->stream>>>(B, w, npix, seed, c->ray_a.as<rc_ray>());
+    RC_TRY(c->prof_end(ev, PK_OTHER));
+    RC_TRY(rc_launch_trace(c, s, false, c->ray_a.as<rc_ray>(), nullptr, npix, c->hits.as<rc_hit>(), nullptr));
+    float4* rgba_d = c->slotL.as<float4>();
+    RC_TRY(c->prof_begin(&ev));
+    k_phong_prepare<<<grid, kBlock, 0, c->stream>>>(npix, Lp, c->ray_a.as<rc_ray>(), c->hits.as<rc_hit>(), s->shade,
+                                                    s->mats, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), rgba_d);
+    RC_TRY(c->prof_end(ev, PK_SHADE));
+    RC_TRY(rc_launch_trace(c, s, true, c->srays.as<rc_ray>(), nullptr, npix, nullptr, c->socc.as<uint8_t>()));
+    RC_TRY(c->prof_begin(&ev));
+    k_phong_finish<<<grid, kBlock, 0, c->stream>>>(npix, c->socc.as<uint8_t>(), c->scontrib.as<float4>(), rgba_d);
+    RC_TRY(c->prof_end(ev, PK_OTHER));
+    RC_TRY_HIP(hipGetLastError());
+    std::vector<rc_hit> hh;
+    if (hit_ids || t) {
+        hh.resize(npix);
+        RC_TRY_HIP(hipMemcpyAsync(hh.data(), c->hits.p, npix * sizeof(rc_hit), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (rgba) RC_TRY_HIP(hipMemcpyAsync(rgba, rgba_d, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    RC_TRY_HIP(hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < npix && !hh.empty(); ++i) {
+        if (hit_ids) hit_ids[i] = hh[i].tri;
+        if (t) t[i] = hh[i].t;
+    }
+    RC_TRY(c->prof_collect());
+    c->stats.extension_rays = npix;
+    c->stats.shadow_rays = npix;
+    return RC_OK;
+}

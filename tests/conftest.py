@@ -1,0 +1,27 @@
+import os
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+# Worker threads for the oracle: the GPU box grants 16 cores to a job.
+ORACLE_THREADS = int(os.environ.get("RAYCER_ORACLE_THREADS", "16"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (run on the MI355X box)")
+    config.addinivalue_line("markers", "slow: long-running (full-size workloads)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """One HIP context for the whole GPU session (fails loudly without a GPU)."""
+    import raycer_amd as ra
+
+    c = ra.Context(0)
+    yield c
+    c.close()

@@ -1,0 +1,86 @@
+"""Input generators and FP64 recomputations shared by the tests."""
+import math
+
+import numpy as np
+
+from raycer_amd import api
+
+
+def camera_rays(cam, w, h, jitter=None, pixels=None):
+    """Camera::getRay for pixel coordinates (x + jx, y + jy) in FP64 numpy,
+    rounded to the rc_ray layout.  jitter: (npix, 2) offsets or None."""
+    pitch, yaw = math.radians(cam.pitch), math.radians(cam.yaw)
+    f = np.array([-math.sin(yaw) * math.cos(pitch), math.sin(pitch), -math.cos(yaw) * math.cos(pitch)])
+    f /= np.linalg.norm(f)
+    r = np.cross(f, [0.0001, 1.0, 0.0001])
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    u /= np.linalg.norm(u)
+    dist = 0.5 / math.tan(math.radians(min(max(cam.fov, 1.0), 180.0)) / 2.0)
+    pos = np.array(cam.position, np.float64)
+    centre = pos + f * dist
+    idx = np.arange(w * h) if pixels is None else np.asarray(pixels)
+    px = (idx % w).astype(np.float64)
+    py = (idx // w).astype(np.float64)
+    if jitter is not None:
+        px = px + jitter[:, 0]
+        py = py + jitter[:, 1]
+    dx = px / (w - 1) - 0.5
+    dy = py / (h - 1) - 0.5
+    p = centre + dx[:, None] * r + (dy * (h / w))[:, None] * u
+    d = p - pos
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return api.make_rays(pos.astype(np.float32), d.astype(np.float32))
+
+
+def mt_fp64(vertices, tri_ids, rays):
+    """FP64 Möller–Trumbore distance of each ray to the given triangle."""
+    v = vertices[tri_ids].astype(np.float64).reshape(-1, 3, 3)
+    o = rays["org"].astype(np.float64)
+    d = rays["dir"].astype(np.float64)
+    e1, e2 = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+    p = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, p)
+    tv = o - v[:, 0]
+    q = np.cross(tv, e1)
+    return np.einsum("ij,ij->i", e2, q) / det
+
+
+def random_rays(n, seed, lo, hi, tmax=np.inf):
+    rng = np.random.default_rng(seed)
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    ext = hi - lo
+    org = lo - 0.5 * ext + rng.random((n, 3)) * 2 * ext
+    tgt = lo + rng.random((n, 3)) * ext
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty(n, api.RAY_DTYPE)
+    r["org"] = org
+    r["dir"] = d
+    r["tmin"] = 0.0
+    r["tmax"] = tmax
+    return r
+
+
+def check_closest(gpu_hits, vertices, rays, diag, max_listed=0.01, rel_t=1e-5):
+    """Hit index bit-exact against the oracle except listed near-ties; the GPU's
+    t within rel_t of the FP64 distance to the triangle it reports."""
+    t_o, id_o, flags = diag
+    gid = gpu_hits["tri"].astype(np.int64)
+    mismatch = gid != id_o
+    listed = flags != 0
+    unlisted = mismatch & ~listed
+    assert listed.mean() <= max_listed, f"{listed.mean():.4%} of rays listed as near ties"
+    assert not unlisted.any(), (
+        f"{unlisted.sum()} unlisted mismatches, e.g. rays {np.nonzero(unlisted)[0][:8].tolist()} "
+        f"gpu {gid[unlisted][:8].tolist()} oracle {id_o[unlisted][:8].tolist()}")
+    hit = gid >= 0
+    if hit.any():
+        t64 = mt_fp64(vertices, gid[hit], rays[hit])
+        err = np.abs(gpu_hits["t"][hit].astype(np.float64) - t64) / np.maximum(np.abs(t64), 1e-30)
+        assert err.max() <= rel_t, f"max relative t error {err.max():.3g}"
+    agree = hit & ~mismatch
+    if agree.any():
+        err = np.abs(gpu_hits["t"][agree] - t_o[agree]) / t_o[agree]
+        assert err.max() <= rel_t
+    return mismatch.sum(), listed.sum()

@@ -1,0 +1,394 @@
+"""Pins of the FP64 oracle against the reference's own test vectors, published
+known answers and closed forms (CPU only).
+
+Each check is chosen so a plausible mistake in the restatement (dropped term,
+wrong sign or index, transposed operand) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+
+# ------------------------------------------------------------------ math KATs
+def test_vector3_reference_vectors():
+    # src/Tests/Vector3Test.cpp:49-56: normalize(1,2,3), cross((1,2,3),(3,2,1))
+    n = orc.normalize([1.0, 2.0, 3.0])
+    assert np.allclose(n, [0.26726124191242440, 0.53452248382484879, 0.80178372573727319], rtol=0, atol=1e-16)
+    assert np.array_equal(orc.cross([1, 2, 3], [3, 2, 1]), [-4.0, 8.0, -4.0])
+    # RIGHT x UP == FORWARD (RIGHT=(1,0,0), UP=(0,1,0), FORWARD=(0,0,1)); a swapped
+    # operand order would give (0,0,-1)
+    assert np.array_equal(orc.cross([1, 0, 0], [0, 1, 0]), [0, 0, 1])
+
+
+@pytest.mark.parametrize("a,b,expected", [
+    (1.0, 1 / 2.2, 0.98418569564819336),
+    (0.5, 1 / 2.2, 0.75691318511962891),
+    (0.18, 1 / 2.2, 0.45118379592895508),
+    (2.0, 0.0, 0.97100782394409180),
+])
+def test_fastpow_known_answers(a, b, expected):
+    # MathUtils::fastPow (src/Math/MathUtils.cpp:56-68): closed forms computed
+    # by hand from the high-word formula
+    assert orc.fastpow(a, b) == pytest.approx(expected, abs=5e-17)
+
+
+def test_fastpow_high_word_formula():
+    # independent restatement on the bit pattern
+    import struct
+    for a in [1e-3, 0.25, 0.7, 1.0, 3.5]:
+        hi = struct.unpack("<Q", struct.pack("<d", a))[0] >> 32
+        hi = hi - (1 << 32) if hi >= (1 << 31) else hi
+        nhi = int((1 / 2.2) * (hi - 1072632447) + 1072632447)
+        exp = struct.unpack("<d", struct.pack("<Q", (nhi & 0xFFFFFFFF) << 32))[0]
+        assert orc.fastpow(a, 1 / 2.2) == exp
+
+
+@pytest.mark.parametrize("ctr,key,expected", [
+    # Random123 known-answer vectors for philox4x32_10
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(ctr, key, expected):
+    assert orc.philox(ctr, key) == expected
+
+
+def test_uniform_mapping_is_exact_fp32():
+    L = orc.lib()
+    for x in [0, 255, 256, 0x12345678, 0xFFFFFFFF]:
+        u = L.or_uniform(x)
+        assert u == (x >> 8) / 16777216.0
+        assert np.float32(u) == u
+    assert L.or_uniform(0xFFFFFFFF) < 1.0
+
+
+# ------------------------------------------------------------------ triangle
+TRI = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float64)  # z = 0 plane, normal +z
+
+
+def test_mt_known_barycentrics():
+    hit, t, u, v = orc.intersect_triangle(TRI, [0.25, 0.5, 2.0], [0, 0, -1])
+    assert hit and t == 2.0 and u == 0.25 and v == 0.5
+
+
+def test_mt_double_sided_and_behind():
+    hit, t, _, _ = orc.intersect_triangle(TRI, [0.2, 0.2, -3.0], [0, 0, 1])
+    assert hit and t == 3.0                                   # back face hits (double-sided)
+    assert not orc.intersect_triangle(TRI, [0.2, 0.2, 1.0], [0, 0, 1])[0]   # t < 0
+
+
+@pytest.mark.parametrize("p", [[0, 0], [1, 0], [0, 1], [0.5, 0.5], [0.5, 0], [0, 0.5]])
+def test_mt_inclusive_edges_and_vertices(p):
+    assert orc.intersect_triangle(TRI, [p[0], p[1], 1.0], [0, 0, -1])[0]
+
+
+def test_mt_outside_and_parallel():
+    assert not orc.intersect_triangle(TRI, [0.6, 0.6, 1.0], [0, 0, -1])[0]    # u + v > 1
+    assert not orc.intersect_triangle(TRI, [-1e-9, 0.5, 1.0], [0, 0, -1])[0]  # u < 0
+    assert not orc.intersect_triangle(TRI, [0.2, 0.2, 1.0], [1, 0, 0])[0]     # det = 0
+
+
+def test_mt_distance_window():
+    assert not orc.intersect_triangle(TRI, [0.2, 0.2, 2.0], [0, 0, -1], tmax=1.5)[0]
+    assert orc.intersect_triangle(TRI, [0.2, 0.2, 2.0], [0, 0, -1], tmax=2.0)[0]      # inclusive maxDistance
+    assert not orc.intersect_triangle(TRI, [0.2, 0.2, 2.0], [0, 0, -1], tmin=2.5)[0]
+
+
+def test_mt_matches_plane_intersection_fp64():
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        tri = rng.normal(size=9)
+        o = rng.normal(size=3) * 3
+        bary = rng.dirichlet([1, 1, 1])
+        target = bary[0] * tri[0:3] + bary[1] * tri[3:6] + bary[2] * tri[6:9]
+        d = target - o
+        d /= np.linalg.norm(d)
+        hit, t, u, v = orc.intersect_triangle(tri, o, d)
+        n = np.cross(tri[3:6] - tri[0:3], tri[6:9] - tri[0:3])
+        t_plane = np.dot(n, tri[0:3] - o) / np.dot(n, d)
+        assert hit
+        assert t == pytest.approx(t_plane, rel=1e-9)
+        assert (u, v) == pytest.approx((bary[1], bary[2]), abs=1e-9)
+
+
+# ------------------------------------------------------------------ BVH traversal vs brute force
+def _random_soup(n, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 1, 3))
+    return (c + rng.normal(scale=0.03, size=(n, 3, 3))).reshape(n, 9).astype(np.float32)
+
+
+def _random_rays(n, seed, box=1.0):
+    # origins around the unit cube, aimed at random points inside it
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = rng.random((n, 3)) * box * 3 - box
+    d = rng.random((n, 3)) - r[:, 0:3]
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 7] = np.inf
+    return r
+
+
+def test_reference_traversal_equals_brute_force():
+    v = _random_soup(800, 1)
+    rays = _random_rays(3000, 2)
+    b = orc.BVH(v)
+    t_ref, id_ref = b.trace_ref(rays, threads=4)
+    t_bf, id_bf = orc.trace_brute(v, rays, threads=4)
+    t_dg, id_dg, _ = b.trace_diag(rays, threads=4)
+    assert np.array_equal(np.isfinite(t_bf), id_bf >= 0)
+    hit = id_bf >= 0
+    assert hit.mean() > 0.2
+    assert np.array_equal(id_ref >= 0, hit) and np.array_equal(id_dg >= 0, hit)
+    assert np.array_equal(t_ref[hit], t_bf[hit]) and np.array_equal(t_dg[hit], t_bf[hit])
+    # ids agree except exact ties (none expected in a random soup)
+    assert np.array_equal(id_ref, id_bf) and np.array_equal(id_dg, id_bf)
+
+
+def test_bvh_invariants_and_tiny_hand_worked():
+    # two far-apart triangle clusters, 6 triangles -> root splits on x,
+    # leaves of 3 (maxLeafSize 5): preorder [root, left leaf, right leaf]
+    tris = []
+    for x0 in (0.0, 10.0):
+        for k in range(3):
+            tris.append([x0 + k * 0.1, 0, 0, x0 + k * 0.1 + 0.05, 0, 0, x0 + k * 0.1, 0.05, 0])
+    v = np.array(tris, np.float32)
+    b = orc.BVH(v)
+    nodes, ordered = b.nodes(), b.ordered()
+    assert len(nodes) == 3
+    assert nodes[0]["right_offset"] == 2 and nodes[1]["right_offset"] == 0 and nodes[2]["right_offset"] == 0
+    assert nodes[1]["primitive_count"] == 3 and nodes[2]["start_offset"] == 3
+    assert sorted(ordered[:3]) == [0, 1, 2] and sorted(ordered[3:]) == [3, 4, 5]
+    # invariants on a larger soup
+    v = _random_soup(3000, 7)
+    b = orc.BVH(v)
+    nodes, ordered = b.nodes(), b.ordered()
+    assert sorted(ordered.tolist()) == list(range(3000))
+    for i, n in enumerate(nodes):
+        if n["right_offset"] == 0:
+            assert 1 <= n["primitive_count"] <= 5
+            tri = v[ordered[n["start_offset"]:n["start_offset"] + n["primitive_count"]]].reshape(-1, 3)
+            assert np.all(tri >= n["aabb_min"] - 0) and np.all(tri <= n["aabb_max"])
+        else:
+            for c in (i + 1, i + n["right_offset"]):
+                assert np.all(nodes[c]["aabb_min"] >= n["aabb_min"]) and np.all(nodes[c]["aabb_max"] <= n["aabb_max"])
+            # rightOffset points past the whole left subtree
+            assert nodes[i + 1]["start_offset"] == n["start_offset"]
+            assert nodes[i + n["right_offset"]]["start_offset"] == n["start_offset"] + nodes[i + 1]["primitive_count"]
+
+
+def test_sah_choice_matches_brute_force_evaluation():
+    # root split of a small input equals the best of the 6 candidates
+    # (centre/median per axis, first best wins), evaluated independently here
+    rng = np.random.default_rng(11)
+    v = (rng.random((9, 1, 3)) * [4, 1, 2] + rng.normal(scale=0.05, size=(9, 3, 3))).reshape(9, 9).astype(np.float32)
+    tri = v.reshape(9, 3, 3).astype(np.float64)
+    lo, hi = tri.min(1), tri.max(1)
+    c = (lo + hi) * 0.5
+    blo, bhi = lo.min(0), hi.max(0)
+
+    def area(a, b):
+        e = b - a
+        return 2 * (e[0] * e[1] + e[2] * e[1] + e[0] * e[2])
+
+    best, choice = np.inf, None
+    for ax in range(3):
+        for s in ((blo[ax] + bhi[ax]) * 0.5, np.median(c[:, ax])):
+            L = c[:, ax] <= s
+            sc = 0.0
+            if L.any():
+                sc += area(lo[L].min(0), hi[L].max(0)) / area(blo, bhi) * L.sum()
+            if (~L).any():
+                sc += area(lo[~L].min(0), hi[~L].max(0)) / area(blo, bhi) * (~L).sum()
+            if sc < best:
+                best, choice = sc, (ax, s)
+    b = orc.BVH(v)
+    nodes, ordered = b.nodes(), b.ordered()
+    ax, s = choice
+    left = set(np.nonzero(c[:, ax] <= s)[0].tolist())
+    nl = nodes[1]["primitive_count"] if nodes[0]["right_offset"] else 0
+    if 0 < len(left) < 9:
+        assert nl == len(left) and set(ordered[:nl].tolist()) == left
+
+
+# ------------------------------------------------------------------ camera, sampling, filters
+def test_camera_closed_forms():
+    cam = [0.5, 0.5, -1.4, 0.0, 180.0, 0.0, 40.0]
+    # centre of an odd image is the forward direction (0, 0, 1) for yaw 180
+    o, d = orc.camera_ray(cam, 257, 257, 128.0, 128.0)
+    assert np.allclose(o, [0.5, 0.5, -1.4]) and np.allclose(d, [0, 0, 1], atol=1e-12)
+    # pixel (0, 0) goes through the image-plane corner: dx = dy = -0.5, right = -x
+    o, d = orc.camera_ray(cam, 256, 256, 0.0, 0.0)
+    dist = 0.5 / math.tan(math.radians(20.0))
+    corner = np.array([0.5, -0.5, dist])  # -0.5 * right(-x) and -0.5 * aspect * up(+y)
+    assert np.allclose(d, corner / np.linalg.norm(corner), atol=1e-4)
+    # horizontal fov: the left/right edge rays span exactly 40 degrees
+    _, dl = orc.camera_ray(cam, 256, 256, 0.0, 127.5)
+    _, dr = orc.camera_ray(cam, 256, 256, 255.0, 127.5)
+    assert math.degrees(math.acos(np.clip(np.dot(dl, dr), -1, 1))) == pytest.approx(40.0, abs=0.01)
+
+
+def test_filters_closed_forms():
+    assert orc.filter_weight(0, 0.5, 0.3, -0.2) == 1.0
+    assert orc.filter_weight(0, 0.5, 0.5, 0.0) == 0.0       # [-r, r): right edge excluded
+    assert orc.filter_weight(0, 0.5, -0.5, 0.0) == 1.0
+    assert orc.filter_weight(1, 1.0, 0.5, 0.0) == 0.5
+    assert orc.filter_weight(1, 1.0, 0.5, 0.5) == 0.25
+    assert orc.filter_weight(1, 2.0, 1.0, 0.0) == pytest.approx(0.25 * 0.5)
+
+
+def test_hemisphere_moments():
+    rng = np.random.default_rng(3)
+    n = np.array([0.3, 0.9, -0.2])
+    n /= np.linalg.norm(n)
+    u = rng.random((40000, 2))
+    cs = np.array([np.dot(orc.hemisphere(n, a, b), n) for a, b in u])
+    assert np.all(cs >= -1e-12)
+    assert cs.mean() == pytest.approx(2 / 3, abs=0.01)        # E[cos] = 2/3
+    assert (cs ** 2).mean() == pytest.approx(1 / 2, abs=0.01)  # E[cos^2] = 1/2
+    lens = [np.linalg.norm(orc.hemisphere(n, a, b)) for a, b in u[:100]]
+    assert np.allclose(lens, 1.0)
+
+
+# ------------------------------------------------------------------ tone mapping
+def test_reinhard_closed_form_constant_image():
+    c = np.float32(0.4)
+    img = np.tile(np.array([c, c, c, 1], np.float32), (64, 1))
+    out, la, ml = orc.reinhard(img)
+    L = 0.2126 * float(c) + 0.7152 * float(c) + 0.0722 * float(c)
+    assert la == pytest.approx(math.exp(math.log(0.01 + L)), rel=1e-12)
+    assert ml == 1.0
+    s = 0.18 / la * L
+    m = s * (1 + s) / (1 + s)
+    v = min(max(float(c) * m / L, 0.0), 1.0)
+    assert out[0, 0] == np.float32(orc.fastpow(v, 1 / 2.2))
+    assert out[0, 3] == 1.0
+
+
+def test_reinhard_black_pixel_clamps_nan_to_zero():
+    img = np.array([[0, 0, 0, 1], [0.5, 0.5, 0.5, 1]], np.float32)
+    out, _, _ = orc.reinhard(img)
+    # luminance 0 -> 0/0 = NaN scale -> clamp (std::max(0, std::min(NaN, 1))) gives 0
+    assert out[0, 0] == np.float32(orc.fastpow(0.0, 1 / 2.2))
+    assert np.isfinite(out).all()
+
+
+# ------------------------------------------------------------------ estimator closed forms
+def _plane_scene(albedo):
+    # one big horizontal plane (two triangles) under a sky; camera looks down
+    v = np.array([[-50, 0, -50, -50, 0, 50, 50, 0, 50], [-50, 0, -50, 50, 0, 50, 50, 0, -50]], np.float32)
+    mats = np.zeros((1, 16))
+    mats[0, 0:3] = albedo
+    return v, np.zeros(2, np.uint32), mats
+
+
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_plane_under_sky_and_sun_closed_form(estimator):
+    rho = 0.6
+    v, tm, mats = _plane_scene(rho)
+    b = orc.BVH(v)
+    cfg = orc.RenderCfg()
+    cfg.width, cfg.height, cfg.spp, cfg.sampler = 4, 4, 1024, 1
+    cfg.max_path_length, cfg.rr_start, cfg.filter, cfg.estimator, cfg.traversal = 4, 0, 0, estimator, 1
+    cfg.filter_radius, cfg.ray_start_offset = 0.5, 1e-5
+    sky, E = 0.5, 2.0
+    s = np.array([0.3, 0.8, 0.2])
+    s /= np.linalg.norm(s)
+    for k in range(3):
+        cfg.sky[k] = sky
+        cfg.sun_dir[k] = s[k]
+        cfg.sun_irradiance[k] = E
+    cfg.seed = 9
+    cam = [0, 1, 0, -90.0, 0.0, 0.0, 30.0]  # straight down
+    st = orc.render(b, tm, mats, cam, cfg, threads=4)
+    Lhat = st[:, 2].sum() / st[:, 0].sum()
+    cs = s[1]
+    if estimator == 0:   # reference: E[2 rho cos] = 4 rho / 3; NEE (2 rho / pi) cos^2 E
+        expect = sky * 4 * rho / 3 + (2 * rho / math.pi) * cs * cs * E
+    else:                # lambert: rho; NEE rho / pi cos E
+        expect = sky * rho + rho / math.pi * cs * E
+    assert Lhat == pytest.approx(expect, rel=0.01)
+
+
+def test_russian_roulette_keeps_the_expectation():
+    v, tm, mats = _plane_scene(0.9)
+    # add a ceiling so paths bounce several times before escaping
+    ceil = np.array([[-50, 2, -50, 50, 2, 50, -50, 2, 50], [-50, 2, -50, 50, 2, -50, 50, 2, 50]], np.float32)
+    hole = np.array([[-0.5, 1.0, -0.5, -0.5, 1.0, 0.5, 0.5, 1.0, 0.5]], np.float32)  # small occluder
+    v = np.concatenate([v, ceil, hole])
+    tm = np.zeros(len(v), np.uint32)
+    mats = np.zeros((1, 16))
+    mats[0, 0:3] = 0.5
+    mats_e = np.zeros((2, 16))
+    mats_e[0, 0:3] = 0.5
+    mats_e[1, 3:6] = 3.0
+    mats_e[1, 13] = 1
+    tm[2:4] = 1   # ceiling is the emitter
+    b = orc.BVH(v)
+    res = []
+    for rr in (0, 1):
+        cfg = orc.RenderCfg()
+        cfg.width, cfg.height, cfg.spp, cfg.sampler = 8, 8, 256, 1
+        cfg.max_path_length, cfg.rr_start, cfg.estimator, cfg.traversal = 6, rr, 1, 1
+        cfg.filter_radius, cfg.ray_start_offset = 0.5, 1e-5
+        cfg.seed = 100 + rr
+        st = orc.render(b, tm, mats_e, [0, 0.5, 0, -90.0, 0.0, 0.0, 60.0], cfg, threads=4)
+        res.append((st[:, 2].sum() / st[:, 0].sum(), st[:, 11].sum()))
+    (m0, rays0), (m1, rays1) = res
+    assert m1 == pytest.approx(m0, rel=0.02)
+    assert rays1 < rays0  # roulette ends paths early
+
+
+# ------------------------------------------------------------------ MLP
+def test_mlp_param_count_matches_cache_shape():
+    assert orc.mlp_param_count(81, 64, 4, 3) == 17923
+
+
+def test_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(0)
+    i, h, nh, o = 7, 6, 2, 3
+    n = orc.mlp_param_count(i, h, nh, o)
+    p = rng.normal(scale=0.4, size=n)
+    x = rng.normal(size=(5, i))
+    t = rng.random((5, o)) + 0.2
+    for rel in (False, True):
+        loss, g = orc.mlp_loss_grad(p, x, t, i, h, nh, o, rel=rel)
+        num = np.empty(n)
+        for k in range(n):
+            e = np.zeros(n)
+            e[k] = 1e-6
+            lp, _ = orc.mlp_loss_grad(p + e, x, t, i, h, nh, o, rel=False) if not rel else (None, None)
+            if rel:
+                # relative loss with stop-gradient denominator: finite differences of
+                # the loss with the denominator frozen at the current prediction
+                y0 = orc.mlp_forward(p, x, i, h, nh, o)
+                den = y0 ** 2 + 1e-2
+                f = lambda q: (((orc.mlp_forward(q, x, i, h, nh, o) - t) ** 2) / den).mean()
+                num[k] = (f(p + e) - f(p - e)) / 2e-6
+            else:
+                lm, _ = orc.mlp_loss_grad(p - e, x, t, i, h, nh, o, rel=False)
+                num[k] = (lp - lm) / 2e-6
+        assert np.allclose(g, num, rtol=1e-5, atol=1e-8)
+
+
+def test_adam_first_step_closed_form():
+    p = np.array([1.0, -2.0, 0.5])
+    g = np.array([0.3, -0.1, 0.0])
+    p1, m, v = orc.adam_step(p, np.zeros(3), np.zeros(3), g, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8, t=1)
+    # bias-corrected first step is lr * g / (|g| + eps)
+    assert np.allclose(p1, p - 1e-3 * g / (np.abs(g) + 1e-8))
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.01 * g * g)
+
+
+def test_cache_encoding_layout():
+    e = orc.cache_encode([0.25, 0.5, 0.0], [0, 1, 0], [1, 0, 0], [0.2, 0.3, 0.4])
+    assert e.shape == (81,)
+    assert e[0] == pytest.approx(math.sin(math.pi * 0.25)) and e[1] == pytest.approx(math.cos(math.pi * 0.25))
+    assert e[2] == pytest.approx(math.sin(2 * math.pi * 0.25))
+    assert e[24] == pytest.approx(math.sin(math.pi * 0.5))
+    assert np.allclose(e[48:72:2], 0.0) and np.allclose(e[49:72:2], 1.0)
+    assert np.array_equal(e[72:], [0, 1, 0, 1, 0, 0, 0.2, 0.3, 0.4])
