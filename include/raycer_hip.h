@@ -171,6 +171,9 @@ int rc_set_counting(rc_ctx* ctx, int enable);
 int rc_set_tuning(rc_ctx* ctx, const rc_tuning* t);
 int rc_get_stats(rc_ctx* ctx, rc_stats* out);
 int rc_synchronize(rc_ctx* ctx);
+/* Achievable-bandwidth probe (no reference counterpart): a float4 streaming
+ * copy of `bytes` bytes, `iters` times; returns read+write GB/s. */
+int rc_copy_bandwidth(rc_ctx* ctx, size_t bytes, int iters, double* gbps);
 
 /* ------------------------------------------------------------ scene */
 /* CLTracer::initializeBuffers / CLScene::readSceneFull (CLScene.cpp:72-263):

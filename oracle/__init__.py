@@ -141,15 +141,19 @@ def materials_array(mats) -> np.ndarray:
 
 
 class BVH:
-    def __init__(self, vertices: np.ndarray, max_leaf: int = 5):
+    """FlatBVH restatement; fast=True builds it in the -ffast-math library
+    (used only for CPU-baseline timing)."""
+
+    def __init__(self, vertices: np.ndarray, max_leaf: int = 5, fast: bool = False):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 9)
+        self.fast = fast
         self.h = C.c_void_p()
-        if lib().or_bvh_build(_p(self.vertices), self.vertices.shape[0], max_leaf, C.byref(self.h)) != 0:
+        if lib(fast).or_bvh_build(_p(self.vertices), self.vertices.shape[0], max_leaf, C.byref(self.h)) != 0:
             raise ValueError("or_bvh_build failed")
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_bvh_free(self.h)
+            lib(self.fast).or_bvh_free(self.h)
             self.h = None
 
     def nodes(self) -> np.ndarray:
@@ -210,9 +214,10 @@ def render_cfg(cfg, traversal: int = 1) -> RenderCfg:
     return o
 
 
-def render(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, pixel_ids=None, threads: int = 0, fast: bool = False):
-    """Per-pixel stats (npix, 12): sum_w, sum_w2, sum_wL[3], sum_w2L[3], sum_w2L2[3], rays."""
-    L = lib(fast)
+def render(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, pixel_ids=None, threads: int = 0):
+    """Per-pixel stats (npix, 12): sum_w, sum_w2, sum_wL[3], sum_w2L[3], sum_w2L2[3], rays.
+    Runs in the library the BVH was built with."""
+    L = lib(bvh.fast)
     tm = np.ascontiguousarray(tri_mat, np.uint32)
     ma = np.ascontiguousarray(materials_array(mats) if mats.dtype.names else mats, np.float64)
     ids = None if pixel_ids is None else np.ascontiguousarray(pixel_ids, np.uint32)
@@ -220,18 +225,8 @@ def render(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, pixel_ids=None, threads
     out = np.empty((npix, 12))
     stop = C.c_int(0)
     ca = camera_array(cam)
-    if fast:
-        # the fast build has its own BVH handle type; build it there
-        h = C.c_void_p()
-        L.or_bvh_build(_p(bvh.vertices), bvh.vertices.shape[0], 5, C.byref(h))
-        try:
-            L.or_render(h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), ma.shape[0],
-                        _p(ca), C.byref(cfg), _p(ids), npix, _p(out), threads, C.byref(stop))
-        finally:
-            L.or_bvh_free(h)
-    else:
-        L.or_render(bvh.h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), ma.shape[0],
-                    _p(ca), C.byref(cfg), _p(ids), npix, _p(out), threads, C.byref(stop))
+    L.or_render(bvh.h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), ma.shape[0],
+                _p(ca), C.byref(cfg), _p(ids), npix, _p(out), threads, C.byref(stop))
     return out
 
 

@@ -135,6 +135,7 @@ def lib() -> C.CDLL:
             "rc_raycast_phong": [vp, vp, C.POINTER(Camera), u32, u32, u64, C.c_float, C.POINTER(PointLight), vp, vp,
                                  vp],
             "rc_render_batch": [vp, vp, vp, vp, vp, sz, C.POINTER(Tuning)],
+            "rc_copy_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
             "rch_bvh_build": [vp, sz, C.POINTER(BVHBuildInfo), C.POINTER(vp)],
             "rch_bvh_node_count": [vp],
             "rch_bvh_get": [vp, vp, vp],
@@ -262,6 +263,11 @@ class Context:
 
     def synchronize(self) -> None:
         check(lib().rc_synchronize(self.h))
+
+    def copy_bandwidth(self, nbytes: int = 1 << 30, iters: int = 20) -> float:
+        g = C.c_double()
+        check(lib().rc_copy_bandwidth(self.h, nbytes, iters, C.byref(g)))
+        return g.value
 
     def upload(self, sd: SceneData, nodes=None, ordered=None) -> "Scene":
         if nodes is None:

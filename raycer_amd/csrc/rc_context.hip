@@ -127,7 +127,7 @@ extern "C" int rc_create(int device, rc_ctx** out) {
     if (!c) return rc_host_fail(RC_ERR_ALLOC, "rc_create: out of memory");
     c->device = device;
     c->cu_count = prop.multiProcessorCount;
-    c->name = prop.name;
+    c->name = prop.name[0] ? prop.name : prop.gcnArchName;
     hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (se != hipSuccess) {
         delete c;
@@ -353,8 +353,8 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
         std::memcpy(&idf, &id, 4);
         std::memcpy(&flf, &fl, 4);
         tris[j].v0 = make_float4(p[0], p[1], p[2], idf);
-        tris[j].e1 = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], flf);
-        tris[j].e2 = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.0f);
+        tris[j].v1 = make_float4(p[3], p[4], p[5], flf);
+        tris[j].v2 = make_float4(p[6], p[7], p[8], 0.0f);
     }
     rc_scene* s = new (std::nothrow) rc_scene();
     if (!s) return rc_host_fail(RC_ERR_ALLOC, "rc_scene_upload: out of memory");
@@ -433,5 +433,40 @@ extern "C" int rc_memcpy_d2h(rc_ctx* c, void* dst, const void* src, size_t bytes
     RC_TRY(rc_check_ctx(c, "rc_memcpy_d2h"));
     RC_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------- bandwidth probe
+namespace {
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int rc_copy_bandwidth(rc_ctx* c, size_t bytes, int iters, double* gbps) {
+    RC_TRY(rc_check_ctx(c, "rc_copy_bandwidth"));
+    if (!gbps || bytes < 4096 || iters <= 0) return rc_host_fail(RC_ERR_INVALID, "rc_copy_bandwidth: bad argument");
+    RC_TRY_HIP(hipSetDevice(c->device));
+    size_t n = bytes / sizeof(float4);
+    DevBuf a, b;
+    RC_TRY(a.ensure(n * sizeof(float4)));
+    int rc = b.ensure(n * sizeof(float4));
+    if (rc != RC_OK) { a.release(); return rc; }
+    hipEvent_t e0, e1;
+    RC_TRY_HIP(hipEventCreate(&e0));
+    RC_TRY_HIP(hipEventCreate(&e1));
+    unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, size_t(c->cu_count) * 16));
+    k_copy<<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n);  // warm-up
+    RC_TRY_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i) k_copy<<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n);
+    RC_TRY_HIP(hipEventRecord(e1, c->stream));
+    RC_TRY_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    RC_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    a.release();
+    b.release();
+    *gbps = 2.0 * double(n * sizeof(float4)) * iters / (double(ms) * 1e-3) / 1e9;
     return RC_OK;
 }

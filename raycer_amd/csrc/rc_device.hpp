@@ -10,8 +10,9 @@
 //                  interior child whose own pair starts at record `link`.
 //                  Pairs are laid out in the host tree's preorder, so links
 //                  always point forward.
-//   TriRec   48 B  v0.xyz + original triangle id, e1.xyz + flags, e2.xyz
-//                  (e1 = v1 - v0, e2 = v2 - v0), in BVH leaf order.
+//   TriRec   48 B  v0.xyz + original triangle id, v1.xyz + flags, v2.xyz,
+//                  in BVH leaf order (the vertices themselves, so the FP64
+//                  refinement sees exactly the input edges).
 //   ShadeRec 16 B  geometric normal + material index, in original order.
 #pragma once
 
@@ -32,8 +33,8 @@ static_assert(sizeof(NodeRec) == 32, "NodeRec must be 32 bytes");
 
 struct alignas(16) TriRec {
     float4 v0;  // w: original triangle id (bits)
-    float4 e1;  // w: flags (bit1 nonShadowing)
-    float4 e2;  // w: unused
+    float4 v1;  // w: flags (bit1 nonShadowing)
+    float4 v2;  // w: unused
 };
 static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
 
@@ -102,27 +103,57 @@ __device__ __forceinline__ float slab(float4 r0, float4 r1, float3 o, float3 inv
     float tz0 = (r0.z - o.z) * inv.z, tz1 = (r1.y - o.z) * inv.z;
     float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    return tn <= tf * 1.0000005f ? tn : __builtin_inff();
+    return tn <= tf * 1.000002f ? tn : __builtin_inff();
 }
 
 // FP32 Möller–Trumbore with the reference's structure (Triangle.cpp:61-91):
-// double-sided, inclusive barycentric bounds; the determinant test is the
-// scale-free det == 0 instead of |det| < epsilon.
-__device__ __forceinline__ bool mt_hit(const TriRec& T, float3 o, float3 d, float& t, float& u, float& v) {
-    float3 e1 = f3(T.e1), e2 = f3(T.e2);
+// double-sided; the determinant test is the scale-free det == 0 instead of
+// |det| < epsilon.  This is a conservative filter: the barycentric bounds are
+// widened by kFilterBand so FP32 rounding never rejects a triangle the FP64
+// decision (mt_hit64, inclusive bounds) would accept.
+constexpr float kFilterBand = 1e-3f;
+__device__ __forceinline__ bool mt_hit(const TriRec& T, float3 o, float3 d, float& t) {
+    float3 v0 = f3(T.v0);
+    float3 e1 = f3(T.v1) - v0, e2 = f3(T.v2) - v0;
     float3 pv = cross3(d, e2);
     float det = dot3(e1, pv);
     if (det == 0.0f) return false;
     float inv = 1.0f / det;
-    float3 tv = o - f3(T.v0);
-    u = dot3(tv, pv) * inv;
-    if (u < 0.0f || u > 1.0f) return false;
+    float3 tv = o - v0;
+    float u = dot3(tv, pv) * inv;
+    if (u < -kFilterBand || u > 1.0f + kFilterBand) return false;
     float3 qv = cross3(tv, e1);
-    v = dot3(d, qv) * inv;
-    if (v < 0.0f || u + v > 1.0f) return false;
+    float v = dot3(d, qv) * inv;
+    if (v < -kFilterBand || u + v > 1.0f + kFilterBand) return false;
     t = dot3(e2, qv) * inv;
     return true;
 }
+
+// The same test in FP64 on the exact FP32 inputs, for candidates the filter
+// accepted: it decides the winner and gives t to FP64 accuracy, so sliver
+// triangles at grazing angles (where FP32 t loses ~1e-4) still resolve like
+// the FP64 reference.
+__device__ __forceinline__ bool mt_hit64(const TriRec& T, float3 of, float3 df, double& t, double& u, double& v) {
+    const double v0x = T.v0.x, v0y = T.v0.y, v0z = T.v0.z;
+    const double e1x = double(T.v1.x) - v0x, e1y = double(T.v1.y) - v0y, e1z = double(T.v1.z) - v0z;
+    const double e2x = double(T.v2.x) - v0x, e2y = double(T.v2.y) - v0y, e2z = double(T.v2.z) - v0z;
+    const double dx = df.x, dy = df.y, dz = df.z;
+    const double px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
+    const double det = e1x * px + e1y * py + e1z * pz;
+    if (det == 0.0) return false;
+    const double inv = 1.0 / det;
+    const double tx = double(of.x) - v0x, ty = double(of.y) - v0y, tz = double(of.z) - v0z;
+    u = (tx * px + ty * py + tz * pz) * inv;
+    if (u < 0.0 || u > 1.0) return false;
+    const double qx = ty * e1z - tz * e1y, qy = tz * e1x - tx * e1z, qz = tx * e1y - ty * e1x;
+    v = (dx * qx + dy * qy + dz * qz) * inv;
+    if (v < 0.0 || (u + v) > 1.0) return false;
+    t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    return true;
+}
+
+// Relative window inside which an FP32 distance defers to the FP64 one.
+constexpr float kRefineWindow = 1e-3f;
 
 struct TraceResult {
     float t, u, v;
@@ -134,15 +165,20 @@ struct TraceResult {
 // visited first, the farther one pushed on a per-thread stack whose entries
 // live at stack[k * stride] (LDS for the main kernels, global memory for the
 // deep-stack kernel).  Leaves are tested as soon as their box is hit.
-// Closest hit keeps the smallest t; an exact tie goes to the lower original
-// triangle id, so the answer does not depend on traversal order.
-// ANY: stop at the first triangle (not nonShadowing) with t in [tmin, tmax].
+// Candidates pass the FP32 filter within kRefineWindow of the current best
+// and are then decided in FP64.  Closest hit keeps the smallest t; an exact
+// tie goes to the lower original triangle id, so the answer does not depend
+// on traversal order.  ANY: stop at the first triangle (not nonShadowing)
+// with t in [tmin, tmax].
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ TraceResult traverse(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                 float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                 int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
     TraceResult R{tmax, 0.0f, 0.0f, kNoHit, false};
     tmin = fmaxf(tmin, 0.0f);
+    const double tmin64 = tmin, tmax64 = tmax;
+    double best = tmax64;
+    float cull = tmax * (1.0f + kRefineWindow);
     float3 inv = safe_inv(d);
     uint32_t cur = 0;
     int sp = 0;
@@ -150,8 +186,8 @@ __device__ __forceinline__ TraceResult traverse(const NodeRec* __restrict__ node
         const float4* p = reinterpret_cast<const float4*>(nodes + cur);
         float4 a0 = p[0], a1 = p[1], b0 = p[2], b1 = p[3];
         if (COUNT) n_rec += 2;
-        float ta = slab(a0, a1, o, inv, tmin, R.t);
-        float tb = slab(b0, b1, o, inv, tmin, R.t);
+        float ta = slab(a0, a1, o, inv, tmin, cull);
+        float tb = slab(b0, b1, o, inv, tmin, cull);
         uint32_t la = __float_as_uint(a1.z), ca = __float_as_uint(a1.w);
         uint32_t lb = __float_as_uint(b1.z), cb = __float_as_uint(b1.w);
         if (tb < ta) {  // make `a` the nearer child
@@ -167,30 +203,39 @@ __device__ __forceinline__ TraceResult traverse(const NodeRec* __restrict__ node
             float te = leaf == 0 ? ta : tb;
             if (!(h && c > 0u)) continue;
             if (leaf == 0) ha = false; else hb = false;
-            if (te > R.t) continue;
+            if (te > cull) continue;
             for (uint32_t j = l; j < l + c; ++j) {
                 const float4* q = reinterpret_cast<const float4*>(tris + j);
                 TriRec T;
                 T.v0 = q[0];
-                T.e1 = q[1];
-                T.e2 = q[2];
+                T.v1 = q[1];
+                T.v2 = q[2];
                 if (COUNT) n_tri += 1;
-                float t, u, v;
-                if (!mt_hit(T, o, d, t, u, v)) continue;
-                if (t < tmin) continue;
+                float t32;
+                if (!mt_hit(T, o, d, t32)) continue;
+                if (t32 > cull || t32 < tmin * (1.0f - kRefineWindow) - 1e-30f) continue;
+                double t64, u64, v64;
+                if (!mt_hit64(T, o, d, t64, u64, v64)) continue;
+                if (t64 < tmin64) continue;
                 uint32_t id = __float_as_uint(T.v0.w);
                 if (ANY) {
-                    if (t <= tmax && !(__float_as_uint(T.e1.w) & RC_MAT_NON_SHADOWING)) {
-                        R.t = t; R.id = id;
+                    if (t64 <= tmax64 && !(__float_as_uint(T.v1.w) & RC_MAT_NON_SHADOWING)) {
+                        R.t = float(t64);
+                        R.id = id;
                         return R;
                     }
-                } else if (t < R.t || (t == R.t && id < R.id)) {
-                    R.t = t; R.u = u; R.v = v; R.id = id;
+                } else if (t64 < best || (t64 == best && id < R.id)) {
+                    best = t64;
+                    R.t = float(t64);
+                    R.u = float(u64);
+                    R.v = float(v64);
+                    R.id = id;
+                    cull = R.t * (1.0f + kRefineWindow);
                 }
             }
         }
-        if (ha && ta > R.t) ha = false;
-        if (hb && tb > R.t) hb = false;
+        if (ha && ta > cull) ha = false;
+        if (hb && tb > cull) hb = false;
         if (ha && hb) {
             if (sp >= max_depth) { R.overflow = true; return R; }
             stack[sp * stride] = lb;

@@ -105,30 +105,36 @@ int rc_launch_trace(rc_ctx* c, const rc_scene* s, bool any, const rc_ray* rays, 
     RC_TRY(c->deep_stack.ensure(size_t(kDeepThreads) * kDeepStack * sizeof(uint32_t)));
     uint32_t max_blocks = uint32_t(c->cu_count) * c->tuning.blocks_per_cu;
     uint32_t blocks = std::max(1u, std::min((n_bound + kBlock - 1) / kBlock, max_blocks));
-    hipEvent_t ev;
-    RC_TRY(c->prof_begin(&ev));
     k_reset_overflow<<<1, 1, 0, c->stream>>>(c->counters);
     uint32_t* ovf = c->ovf_list.as<uint32_t>();
     uint32_t* gst = c->deep_stack.as<uint32_t>();
+    hipEvent_t ev;
+    RC_TRY(c->prof_begin(&ev));
     if (any) {
-        if (c->counting) {
+        if (c->counting)
             k_trace<true, true><<<blocks, kBlock, 0, c->stream>>>(s->nodes, s->tris, rays, n_dev, n_bound, hits, occ, ovf, c->counters);
-            k_trace_deep<true, true><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
-        } else {
+        else
             k_trace<true, false><<<blocks, kBlock, 0, c->stream>>>(s->nodes, s->tris, rays, n_dev, n_bound, hits, occ, ovf, c->counters);
-            k_trace_deep<true, false><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
-        }
     } else {
-        if (c->counting) {
+        if (c->counting)
             k_trace<false, true><<<blocks, kBlock, 0, c->stream>>>(s->nodes, s->tris, rays, n_dev, n_bound, hits, occ, ovf, c->counters);
-            k_trace_deep<false, true><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
-        } else {
+        else
             k_trace<false, false><<<blocks, kBlock, 0, c->stream>>>(s->nodes, s->tris, rays, n_dev, n_bound, hits, occ, ovf, c->counters);
+    }
+    RC_TRY(c->prof_end(ev, any ? PK_SHADOW : PK_EXTEND));
+    // rays whose stack outgrew LDS are walked again with the deep stack
+    if (any) {
+        if (c->counting)
+            k_trace_deep<true, true><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
+        else
+            k_trace_deep<true, false><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
+    } else {
+        if (c->counting)
+            k_trace_deep<false, true><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
+        else
             k_trace_deep<false, false><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
-        }
     }
     RC_TRY_HIP(hipGetLastError());
-    RC_TRY(c->prof_end(ev, any ? PK_SHADOW : PK_EXTEND));
     return RC_OK;
 }
 

@@ -53,7 +53,10 @@ def test_closest_hit_random_rays(ctx, name):
     v = sd.vertices.reshape(-1, 3)
     rays = random_rays(60000, 7, v.min(0), v.max(0))
     hits = ctx.trace_closest(s, rays)
-    check_closest(hits, sd.vertices, rays, ob.trace_diag(rays, threads=ORACLE_THREADS))
+    # in the box the inner boxes' bottom faces are coplanar with the floor:
+    # rays from below hit both at the same t, exact ties the oracle lists
+    check_closest(hits, sd.vertices, rays, ob.trace_diag(rays, threads=ORACLE_THREADS),
+                  max_listed=0.05 if name == "box" else 0.01)
 
 
 def test_closest_hit_matches_brute_force_without_bvh(ctx):
@@ -79,7 +82,9 @@ def test_tmin_tmax_window(ctx):
     h2 = ctx.trace_closest(s, win2)
     hit = (full["tri"] >= 0) & (h2["tri"] >= 0)
     assert np.all(h2["t"][hit] >= full["t"][hit] * 1.001)
-    check_closest(h2, sd.vertices, win2, ob.trace_diag(win2, threads=ORACLE_THREADS))
+    # past the first hit many rays run along the floor under the inner boxes,
+    # whose bottom faces are coplanar with it (listed exact ties)
+    check_closest(h2, sd.vertices, win2, ob.trace_diag(win2, threads=ORACLE_THREADS), max_listed=0.05)
 
 
 def test_axis_aligned_rays_and_miss(ctx):
@@ -89,8 +94,11 @@ def test_axis_aligned_rays_and_miss(ctx):
     rays = api.make_rays(org, np.tile(np.float32([0, 0, 1]), (len(org), 1)))
     rays["org"] = org
     hits = ctx.trace_closest(s, rays)
-    check_closest(hits, sd.vertices, rays, ob.trace_diag(rays, threads=ORACLE_THREADS))
-    up = api.make_rays(np.float32([0.5, 2.0, 0.5]), np.float32([[0, 1, 0]]))
+    # the 17 rays with x == y hit the back wall exactly on the diagonal shared
+    # by its two triangles: exact ties, resolved to the lower id
+    t_o, id_o, flags = ob.trace_diag(rays, threads=ORACLE_THREADS)
+    check_closest(hits, sd.vertices, rays, (t_o, id_o, flags), max_listed=0.1)
+    up =api.make_rays(np.float32([0.5, 2.0, 0.5]), np.float32([[0, 1, 0]]))
     assert ctx.trace_closest(s, up)["tri"][0] == -1
     assert np.isinf(ctx.trace_closest(s, up)["t"][0])
 
