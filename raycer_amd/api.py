@@ -9,13 +9,14 @@ if libraycer.so is missing or no gfx950 device is present, the calls raise.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from pathlib import Path
 
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "libraycer.so"
+LIB_PATH = Path(os.environ.get("RAYCER_LIBRARY", str(PKG / "libraycer.so")))
 
 RC_OK, RC_ERR_INVALID, RC_ERR_HIP, RC_ERR_ALLOC, RC_ERR_INTERRUPTED, RC_ERR_NO_DEVICE = range(6)
 RC_MAT_EMISSIVE, RC_MAT_NON_SHADOWING = 1, 2

@@ -259,40 +259,82 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
             }
         }
     }
-    // Child-pair records in preorder of the interior nodes.
-    std::vector<uint32_t> rec_of(nnodes, 0xffffffffu);
-    size_t ninterior = 0;
-    for (size_t i = 0; i < nnodes; ++i)
-        if (nodes[i].right_offset != 0) rec_of[i] = uint32_t(2 * ninterior++);
-    size_t nrec = ninterior ? 2 * ninterior : 2;
-    if (nrec >= 0xffffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: too many nodes");
-    std::vector<NodeRec> recs(nrec);
-    auto fill = [&](NodeRec& r, size_t child) {
+    // Collapse the binary FlatBVH into 4-wide nodes: starting from an interior
+    // node's two children, the interior child with the largest surface area is
+    // replaced by its own two children until there are four.  Each wide node is
+    // stored as kWidth consecutive child records in preorder; empty slots carry
+    // count = kEmptyChild and are never entered.
+    auto area = [&](size_t i) {
+        const rc_flat_node& n = nodes[i];
+        double ex = n.aabb_max[0] - n.aabb_min[0], ey = n.aabb_max[1] - n.aabb_min[1],
+               ez = n.aabb_max[2] - n.aabb_min[2];
+        return ex * ey + ey * ez + ez * ex;
+    };
+    auto is_leaf = [&](size_t i) { return nodes[i].right_offset == 0; };
+    std::vector<NodeRec> recs;
+    recs.reserve(2 * nnodes / 3 + kWidth);
+    auto fill = [&](size_t at, size_t child, uint32_t link) {
+        NodeRec& r = recs[at];
         const rc_flat_node& n = nodes[child];
         for (int k = 0; k < 3; ++k) {
             r.lo[k] = round_down(n.aabb_min[k]);
             r.hi[k] = round_up(n.aabb_max[k]);
         }
-        if (n.right_offset == 0) {
+        if (is_leaf(child)) {
             r.link = uint32_t(n.start_offset);
             r.count = uint32_t(n.primitive_count);
         } else {
-            r.link = rec_of[child];
+            r.link = link;
             r.count = 0;
         }
     };
-    if (ninterior == 0) {
-        // a single leaf: both records carry it (testing it twice is harmless)
-        fill(recs[0], 0);
-        fill(recs[1], 0);
-    } else {
-        for (size_t i = 0; i < nnodes; ++i) {
-            if (nodes[i].right_offset == 0) continue;
-            fill(recs[rec_of[i]], i + 1);
-            fill(recs[rec_of[i] + 1], i + size_t(nodes[i].right_offset));
+    // explicit stack of (binary node, record slot that links to its wide node)
+    struct Pending { size_t node; size_t slot; };
+    std::vector<Pending> todo;
+    auto emit = [&](size_t n) -> uint32_t {  // returns the first record of n's wide node
+        size_t kids[kWidth];
+        int nk = 0;
+        if (is_leaf(n)) {
+            kids[nk++] = n;  // a single-leaf tree: one leaf child
+        } else {
+            kids[nk++] = n + 1;
+            kids[nk++] = n + size_t(nodes[n].right_offset);
+            while (nk < kWidth) {
+                int best = -1;
+                double ba = -1.0;
+                for (int k = 0; k < nk; ++k)
+                    if (!is_leaf(kids[k]) && area(kids[k]) > ba) { ba = area(kids[k]); best = k; }
+                if (best < 0) break;
+                size_t c = kids[best];
+                kids[best] = c + 1;
+                kids[nk++] = c + size_t(nodes[c].right_offset);
+            }
         }
+        uint32_t base = uint32_t(recs.size());
+        recs.resize(recs.size() + kWidth);
+        for (int k = 0; k < kWidth; ++k) {
+            NodeRec& r = recs[base + k];
+            if (k >= nk) {
+                for (int q = 0; q < 3; ++q) { r.lo[q] = 0.0f; r.hi[q] = 0.0f; }
+                r.link = 0;
+                r.count = kEmptyChild;
+                continue;
+            }
+            fill(base + k, kids[k], 0);
+            if (!is_leaf(kids[k])) todo.push_back({kids[k], base + size_t(k)});
+        }
+        return base;
+    };
+    emit(0);
+    while (!todo.empty()) {
+        Pending p = todo.back();
+        todo.pop_back();
+        uint32_t b = emit(p.node);
+        recs[p.slot].link = b;
     }
-    // depth of the pair tree: the traversal stack never exceeds it
+    size_t nrec = recs.size();
+    if (nrec >= 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: too many nodes");
+    // depth of the wide tree; links must point forward so traversal terminates
     uint32_t depth = 0;
     {
         std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 1u}};
@@ -300,18 +342,19 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
             auto [r, d] = st.back();
             st.pop_back();
             depth = std::max(depth, d);
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < kWidth; ++k) {
                 const NodeRec& x = recs[r + k];
                 if (x.count == 0) {
-                    if (x.link <= r || x.link + 1 >= nrec)
+                    if (x.link <= r || x.link + kWidth > nrec || x.link % kWidth != 0)
                         return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: node link does not point forward");
                     st.push_back({x.link, d + 1});
                 }
             }
         }
     }
-    if (depth >= uint32_t(kDeepStack))
-        return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: BVH deeper than " + std::to_string(kDeepStack));
+    // a traversal stack holds at most (kWidth - 1) entries per level
+    if (depth * (kWidth - 1) >= uint32_t(kDeepStack))
+        return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: BVH too deep for the traversal stack");
     std::vector<TriRec> tris(ntri);
     std::vector<float4> shade(ntri);
     std::vector<DevMaterial> dm(nmat);

@@ -1,14 +1,15 @@
 // Device-side data layouts and inline helpers of the gfx950 path tracer.
 //
 // Scene layout in HBM (all read-only during a render):
-//   NodeRec  32 B  child-pair records: an interior node of the host FlatBVH is
-//                  stored as the two records of its children, side by side
-//                  (64 B, 64-B aligned), so one visit reads both child boxes
-//                  with four 128-bit loads.  Record fields: child box
-//                  lo.xyz/hi.xyz, link, count.  count > 0: leaf child whose
-//                  triangles are tris[link, link + count); count == 0:
-//                  interior child whose own pair starts at record `link`.
-//                  Pairs are laid out in the host tree's preorder, so links
+//   NodeRec  32 B  child records.  The host FlatBVH is collapsed into 4-wide
+//                  nodes; a wide node is stored as the 4 records of its
+//                  children, side by side (128 B, one cache line), so one visit
+//                  reads all child boxes with eight 128-bit loads.  Record
+//                  fields: child box lo.xyz/hi.xyz, link, count.  count in
+//                  1..kEmptyChild-1: leaf child whose triangles are
+//                  tris[link, link + count); count == 0: interior child whose
+//                  own wide node starts at record `link`; count == kEmptyChild:
+//                  unused slot.  Wide nodes are laid out in preorder, so links
 //                  always point forward.
 //   TriRec   48 B  v0.xyz + original triangle id, v1.xyz + flags, v2.xyz,
 //                  in BVH leaf order (the vertices themselves, so the FP64
@@ -46,7 +47,9 @@ struct alignas(16) DevMaterial {
 };
 
 constexpr uint32_t kNoHit = 0x7fffffffu;
-constexpr int kDeepStack = 256;  // deepest tree the overflow kernel can walk
+constexpr int kDeepStack = 256;  // stack entries of the overflow kernel
+constexpr int kWidth = 4;        // children per wide node
+constexpr uint32_t kEmptyChild = 0xffffffffu;
 
 // ---------------------------------------------------------------- RNG
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64 products.
@@ -161,19 +164,75 @@ struct TraceResult {
     bool overflow;
 };
 
-// Ordered, t-culled traversal over child-pair records.  The nearer child is
-// visited first, the farther one pushed on a per-thread stack whose entries
-// live at stack[k * stride] (LDS for the main kernels, global memory for the
-// deep-stack kernel).  Leaves are tested as soon as their box is hit.
-// Candidates pass the FP32 filter within kRefineWindow of the current best
-// and are then decided in FP64.  Closest hit keeps the smallest t; an exact
-// tie goes to the lower original triangle id, so the answer does not depend
-// on traversal order.  ANY: stop at the first triangle (not nonShadowing)
-// with t in [tmin, tmax].
+// Wave-wide sum, one 64-bit atomic per wave (every lane must call it).
+__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
+    unsigned long long x = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
+}
+
+// Tests the triangles tris[first, first + count).  Candidates pass the FP32
+// filter within kRefineWindow of the current best and are then decided in
+// FP64.  Closest hit keeps the smallest t; an exact tie goes to the lower
+// original triangle id, so the answer does not depend on traversal order.
+// Returns true when an any-hit query is answered.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool test_leaf(const TriRec* __restrict__ tris, uint32_t first, uint32_t count, float3 o,
+                                          float3 d, float tmin, double tmin64, double tmax64, double& best,
+                                          float& cull, TraceResult& R, uint32_t& n_tri) {
+    for (uint32_t j = first; j < first + count; ++j) {
+        const float4* q = reinterpret_cast<const float4*>(tris + j);
+        TriRec T;
+        T.v0 = q[0];
+        T.v1 = q[1];
+        T.v2 = q[2];
+        if (COUNT) n_tri += 1;
+        float t32;
+        if (!mt_hit(T, o, d, t32)) continue;
+        if (t32 > cull || t32 < tmin * (1.0f - kRefineWindow) - 1e-30f) continue;
+        double t64, u64, v64;
+        if (!mt_hit64(T, o, d, t64, u64, v64)) continue;
+        if (t64 < tmin64) continue;
+        uint32_t id = __float_as_uint(T.v0.w);
+        if (ANY) {
+            if (t64 <= tmax64 && !(__float_as_uint(T.v1.w) & RC_MAT_NON_SHADOWING)) {
+                R.t = float(t64);
+                R.id = id;
+                return true;
+            }
+        } else if (t64 < best || (t64 == best && id < R.id)) {
+            best = t64;
+            R.t = float(t64);
+            R.u = float(u64);
+            R.v = float(v64);
+            R.id = id;
+            cull = R.t * (1.0f + kRefineWindow);
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, float& tb, uint32_t& lb, uint32_t& cb) {
+    if (tb < ta) {
+        float t = ta; ta = tb; tb = t;
+        uint32_t x = la; la = lb; lb = x;
+        x = ca; ca = cb; cb = x;
+    }
+}
+
+// Ordered, t-culled traversal over 4-wide nodes.  The four child boxes are
+// tested and sorted by entry distance (a 5-comparator network); leaf children
+// are tested in that order at once, the nearest interior child is visited
+// next and the others are pushed far-to-near on a per-thread stack whose
+// entries live at stack[k * stride] (LDS for the main kernels, global memory
+// for the deep-stack kernel).  ANY: stop at the first triangle (not
+// nonShadowing) with t in [tmin, tmax].
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ TraceResult traverse(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                 float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                 int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
+    const float INF = __builtin_inff();
     TraceResult R{tmax, 0.0f, 0.0f, kNoHit, false};
     tmin = fmaxf(tmin, 0.0f);
     const double tmin64 = tmin, tmax64 = tmax;
@@ -184,67 +243,53 @@ __device__ __forceinline__ TraceResult traverse(const NodeRec* __restrict__ node
     int sp = 0;
     while (true) {
         const float4* p = reinterpret_cast<const float4*>(nodes + cur);
-        float4 a0 = p[0], a1 = p[1], b0 = p[2], b1 = p[3];
-        if (COUNT) n_rec += 2;
-        float ta = slab(a0, a1, o, inv, tmin, cull);
-        float tb = slab(b0, b1, o, inv, tmin, cull);
-        uint32_t la = __float_as_uint(a1.z), ca = __float_as_uint(a1.w);
-        uint32_t lb = __float_as_uint(b1.z), cb = __float_as_uint(b1.w);
-        if (tb < ta) {  // make `a` the nearer child
-            float tt = ta; ta = tb; tb = tt;
-            uint32_t x = la; la = lb; lb = x;
-            x = ca; ca = cb; cb = x;
+        float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6], q7 = p[7];
+        if (COUNT) n_rec += kWidth;
+        uint32_t l0 = __float_as_uint(q1.z), c0 = __float_as_uint(q1.w);
+        uint32_t l1 = __float_as_uint(q3.z), c1 = __float_as_uint(q3.w);
+        uint32_t l2 = __float_as_uint(q5.z), c2 = __float_as_uint(q5.w);
+        uint32_t l3 = __float_as_uint(q7.z), c3 = __float_as_uint(q7.w);
+        float t0 = c0 == kEmptyChild ? INF : slab(q0, q1, o, inv, tmin, cull);
+        float t1 = c1 == kEmptyChild ? INF : slab(q2, q3, o, inv, tmin, cull);
+        float t2 = c2 == kEmptyChild ? INF : slab(q4, q5, o, inv, tmin, cull);
+        float t3 = c3 == kEmptyChild ? INF : slab(q6, q7, o, inv, tmin, cull);
+        order2(t0, l0, c0, t1, l1, c1);
+        order2(t2, l2, c2, t3, l3, c3);
+        order2(t0, l0, c0, t2, l2, c2);
+        order2(t1, l1, c1, t3, l3, c3);
+        order2(t1, l1, c1, t2, l2, c2);
+        // leaf children, nearest first
+        if (c0 != 0 && t0 != INF && t0 <= cull)
+            if (test_leaf<ANY, COUNT>(tris, l0, c0, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+        if (c1 != 0 && t1 != INF && t1 <= cull)
+            if (test_leaf<ANY, COUNT>(tris, l1, c1, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+        if (c2 != 0 && t2 != INF && t2 <= cull)
+            if (test_leaf<ANY, COUNT>(tris, l2, c2, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+        if (c3 != 0 && t3 != INF && t3 <= cull)
+            if (test_leaf<ANY, COUNT>(tris, l3, c3, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+        // interior children: farthest pushed first, nearest visited next
+        bool have = false;
+        uint32_t next = 0;
+#define RC_INTERIOR(t, l, c)                                   \
+        if (c == 0 && t != INF && t <= cull) {                 \
+            if (have) {                                        \
+                if (sp >= max_depth) {                         \
+                    R.overflow = true;                         \
+                    return R;                                  \
+                }                                              \
+                stack[sp * stride] = next;                     \
+                ++sp;                                          \
+            }                                                  \
+            next = l;                                          \
+            have = true;                                       \
         }
-        bool ha = ta != __builtin_inff(), hb = tb != __builtin_inff();
-#pragma unroll 1
-        for (int leaf = 0; leaf < 2; ++leaf) {
-            bool h = leaf == 0 ? ha : hb;
-            uint32_t l = leaf == 0 ? la : lb, c = leaf == 0 ? ca : cb;
-            float te = leaf == 0 ? ta : tb;
-            if (!(h && c > 0u)) continue;
-            if (leaf == 0) ha = false; else hb = false;
-            if (te > cull) continue;
-            for (uint32_t j = l; j < l + c; ++j) {
-                const float4* q = reinterpret_cast<const float4*>(tris + j);
-                TriRec T;
-                T.v0 = q[0];
-                T.v1 = q[1];
-                T.v2 = q[2];
-                if (COUNT) n_tri += 1;
-                float t32;
-                if (!mt_hit(T, o, d, t32)) continue;
-                if (t32 > cull || t32 < tmin * (1.0f - kRefineWindow) - 1e-30f) continue;
-                double t64, u64, v64;
-                if (!mt_hit64(T, o, d, t64, u64, v64)) continue;
-                if (t64 < tmin64) continue;
-                uint32_t id = __float_as_uint(T.v0.w);
-                if (ANY) {
-                    if (t64 <= tmax64 && !(__float_as_uint(T.v1.w) & RC_MAT_NON_SHADOWING)) {
-                        R.t = float(t64);
-                        R.id = id;
-                        return R;
-                    }
-                } else if (t64 < best || (t64 == best && id < R.id)) {
-                    best = t64;
-                    R.t = float(t64);
-                    R.u = float(u64);
-                    R.v = float(v64);
-                    R.id = id;
-                    cull = R.t * (1.0f + kRefineWindow);
-                }
-            }
-        }
-        if (ha && ta > cull) ha = false;
-        if (hb && tb > cull) hb = false;
-        if (ha && hb) {
-            if (sp >= max_depth) { R.overflow = true; return R; }
-            stack[sp * stride] = lb;
-            ++sp;
-            cur = la;
-        } else if (ha) {
-            cur = la;
-        } else if (hb) {
-            cur = lb;
+        RC_INTERIOR(t3, l3, c3)
+        RC_INTERIOR(t2, l2, c2)
+        RC_INTERIOR(t1, l1, c1)
+        RC_INTERIOR(t0, l0, c0)
+#undef RC_INTERIOR
+        if (have) {
+            cur = next;
         } else {
             if (sp == 0) break;
             --sp;

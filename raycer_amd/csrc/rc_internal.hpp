@@ -53,7 +53,7 @@ struct rc_ctx {
     std::string name;
     bool profiling = false;
     bool counting = false;
-    rc_tuning tuning{128, 1, 8, 0};
+    rc_tuning tuning{256, 1, 64, 0};
     rc_stats stats{};
     DevCounters* counters = nullptr;  // device
     DevBuf trace_rays, trace_hits, trace_occ, ovf_list, deep_stack;
@@ -100,5 +100,6 @@ struct rc_film {
 // bounds it and sizes the grid.
 int rc_launch_trace(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, const uint32_t* n_dev,
                     uint32_t n_bound, rc_hit* hits, uint8_t* occ);
+int rc_launch_deep(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, rc_hit* hits, uint8_t* occ);
 int rc_check_ctx(const rc_ctx* ctx, const char* fn);
 int rc_check_scene(const rc_ctx* ctx, const rc_scene* s, const char* fn);

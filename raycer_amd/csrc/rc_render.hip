@@ -25,6 +25,17 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr float kPi = 3.14159265358979323846f;
+#ifndef RC_TRACE_BLOCK
+#define RC_TRACE_BLOCK 128
+#endif
+#ifndef RC_LDS_DEPTH
+#define RC_LDS_DEPTH 16
+#endif
+constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the traversal kernels
+constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
+#ifndef RC_MIN_WAVES
+#define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
+#endif
 
 struct CamBasis {
     float3 pos, right, up, center;
@@ -166,93 +177,190 @@ __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
     return ((stt * cp) * T + (stt * sp) * B) + ct * n;
 }
 
-__global__ __launch_bounds__(kBlock) void k_shade(PassParams P, uint32_t k, const rc_ray* __restrict__ rays,
-                                                  const float4* __restrict__ st, const rc_hit* __restrict__ hits,
-                                                  const float4* __restrict__ shade, const DevMaterial* __restrict__ mats,
-                                                  rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
-                                                  rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
-                                                  float4* __restrict__ L, DevCounters* __restrict__ ctr,
-                                                  uint32_t cin, uint32_t cout) {
-    const uint32_t n = ctr->queue_count[cin];
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
-    if (gtid == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
+struct ShadeOut {
+    bool alive, shadow;
+    float3 thr, newo, newd, contrib, so;
+};
+
+// One path vertex: PathTracer::traceRecursive (PathTracer.cpp:25-75) for the
+// hit (t, tri) of ray (r0, r1), plus sky on a miss, sun next-event estimation
+// and Russian roulette.  Radiance goes to L[slot]; the continuation and the
+// shadow ray are returned for the caller to append.
+__device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k, float4 r0, float4 r1, float4 s,
+                                                 float t, int tri, const float4* __restrict__ shade,
+                                                 const DevMaterial* __restrict__ mats, float4* __restrict__ L) {
+    ShadeOut O;
+    O.alive = O.shadow = false;
     const bool more = (k + 1) < P.max_len;
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const float4* rp = reinterpret_cast<const float4*>(rays + i);
-        float4 r0 = rp[0], r1 = rp[1];
-        float4 s = st[i];
-        float4 h = reinterpret_cast<const float4*>(hits)[i];
-        uint32_t slot = __float_as_uint(s.w);
-        float3 thr = f3(s);
-        int tri = __float_as_int(h.w);
-        bool alive = false, shadow = false;
-        float3 newo, newd, contrib, so;
-        if (tri < 0) {
-            float4 l = L[slot];
-            L[slot] = make_float4(l.x + thr.x * P.sky.x, l.y + thr.y * P.sky.y, l.z + thr.z * P.sky.z, l.w);
-        } else {
-            float4 sh = shade[tri];
-            const DevMaterial& m = mats[__float_as_uint(sh.w)];
-            float4 dif = m.diffuse;
-            uint32_t flags = __float_as_uint(dif.w);
-            if (flags & RC_MAT_EMISSIVE) {
-                float4 e = m.emittance;
-                float4 l = L[slot];
-                L[slot] = make_float4(l.x + thr.x * e.x, l.y + thr.y * e.y, l.z + thr.z * e.z, l.w);
-            } else {
-                float3 nrm = f3(sh);
-                float3 o = f3(r0), d = f3(r1);
-                float3 p = o + (h.x * d);
-                float3 rho = f3(dif);
-                float4 u = draw4(P.seed, slot % P.npix, P.sample0 + slot / P.npix, P.frame, k);
-                if (P.has_sun && more) {
-                    float cs = dot3(nrm, P.sun);
-                    if (cs > 0.0f) {
-                        float f = P.estimator == RC_EST_REFERENCE ? (2.0f / kPi) * cs * cs : (1.0f / kPi) * cs;
-                        contrib = mul3(thr, mul3(rho, P.sun_e)) * f;
-                        so = p + P.sun * P.offset;
-                        shadow = true;
-                    }
-                }
-                if (more) {
-                    float3 w = hemisphere(nrm, u.x, u.y);
-                    if (P.estimator == RC_EST_REFERENCE) thr = mul3(thr, rho * (2.0f * fabsf(dot3(w, nrm))));
-                    else thr = mul3(thr, rho);
-                    alive = true;
-                    if (P.rr_start > 0 && k >= P.rr_start) {
-                        float q = fminf(1.0f, fmaxf(thr.x, fmaxf(thr.y, thr.z)));
-                        if (u.z >= q) alive = false;
-                        else thr = thr * (1.0f / q);
-                    }
-                    newo = p + w * P.offset;
-                    newd = w;
-                }
-            }
+    uint32_t slot = __float_as_uint(s.w);
+    float3 thr = f3(s);
+    if (tri < 0) {
+        float4 l = L[slot];
+        L[slot] = make_float4(l.x + thr.x * P.sky.x, l.y + thr.y * P.sky.y, l.z + thr.z * P.sky.z, l.w);
+        return O;
+    }
+    float4 sh = shade[tri];
+    const DevMaterial& m = mats[__float_as_uint(sh.w)];
+    float4 dif = m.diffuse;
+    if (__float_as_uint(dif.w) & RC_MAT_EMISSIVE) {
+        float4 e = m.emittance;
+        float4 l = L[slot];
+        L[slot] = make_float4(l.x + thr.x * e.x, l.y + thr.y * e.y, l.z + thr.z * e.z, l.w);
+        return O;
+    }
+    if (!more) return O;
+    float3 nrm = f3(sh);
+    float3 p = f3(r0) + (t * f3(r1));
+    float3 rho = f3(dif);
+    float4 u = draw4(P.seed, slot % P.npix, P.sample0 + slot / P.npix, P.frame, k);
+    if (P.has_sun) {
+        float cs = dot3(nrm, P.sun);
+        if (cs > 0.0f) {
+            float f = P.estimator == RC_EST_REFERENCE ? (2.0f / kPi) * cs * cs : (1.0f / kPi) * cs;
+            O.contrib = mul3(thr, mul3(rho, P.sun_e)) * f;
+            O.so = p + P.sun * P.offset;
+            O.shadow = true;
         }
-        uint32_t pos = wave_append(alive, &ctr->queue_count[cout]);
-        if (alive) {
-            float4* ro = reinterpret_cast<float4*>(rays_out + pos);
-            ro[0] = make_float4(newo.x, newo.y, newo.z, 0.0f);
-            ro[1] = make_float4(newd.x, newd.y, newd.z, __builtin_inff());
-            st_out[pos] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(slot));
-        }
-        uint32_t spos = wave_append(shadow, &ctr->queue_count[2]);
-        if (shadow) {
-            float4* ro = reinterpret_cast<float4*>(srays + spos);
-            ro[0] = make_float4(so.x, so.y, so.z, 0.0f);
-            ro[1] = make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff());
-            scontrib[spos] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(slot));
-        }
+    }
+    float3 w = hemisphere(nrm, u.x, u.y);
+    if (P.estimator == RC_EST_REFERENCE) thr = mul3(thr, rho * (2.0f * fabsf(dot3(w, nrm))));
+    else thr = mul3(thr, rho);
+    O.alive = true;
+    if (P.rr_start > 0 && k >= P.rr_start) {
+        float q = fminf(1.0f, fmaxf(thr.x, fmaxf(thr.y, thr.z)));
+        if (u.z >= q) O.alive = false;
+        else thr = thr * (1.0f / q);
+    }
+    O.thr = thr;
+    O.newo = p + w * P.offset;
+    O.newd = w;
+    return O;
+}
+
+// Appends the continuation and shadow ray of a shaded vertex (every lane of
+// the wave calls this at the same point).
+__device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut& O, uint32_t slot,
+                                            rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
+                                            rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
+                                            DevCounters* __restrict__ ctr, uint32_t cout) {
+    uint32_t pos = wave_append(O.alive, &ctr->queue_count[cout]);
+    if (O.alive) {
+        float4* ro = reinterpret_cast<float4*>(rays_out + pos);
+        ro[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f);
+        ro[1] = make_float4(O.newd.x, O.newd.y, O.newd.z, __builtin_inff());
+        st_out[pos] = make_float4(O.thr.x, O.thr.y, O.thr.z, __uint_as_float(slot));
+    }
+    uint32_t spos = wave_append(O.shadow, &ctr->queue_count[2]);
+    if (O.shadow) {
+        float4* ro = reinterpret_cast<float4*>(srays + spos);
+        ro[0] = make_float4(O.so.x, O.so.y, O.so.z, 0.0f);
+        ro[1] = make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff());
+        scontrib[spos] = make_float4(O.contrib.x, O.contrib.y, O.contrib.z, __uint_as_float(slot));
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_shadow_apply(const float4* __restrict__ scontrib,
-                                                         const uint8_t* __restrict__ occ, float4* __restrict__ L,
-                                                         DevCounters* __restrict__ ctr) {
+// Extend + shade: closest-hit traversal of queue entry i and, in the same
+// thread, the shading of its hit.  A ray whose traversal stack outgrows LDS is
+// listed for the deep-stack kernel and shaded afterwards by k_shade_listed.
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
+    PassParams P, uint32_t k, const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+    const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
+    const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
+    rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
+    DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout) {
+    __shared__ uint32_t stack[kTraceLds * kTraceBlock];
+    const uint32_t n = ctr->queue_count[cin];
+    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
+    uint32_t n_rec = 0, n_tri = 0;
+    ShadeOut O;
+    O.alive = O.shadow = false;
+    uint32_t slot = 0;
+    if (i < n) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        float4 r0 = rp[0], r1 = rp[1];
+        float4 s = st[i];
+        slot = __float_as_uint(s.w);
+        TraceResult R = traverse<false, COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
+                                               kTraceBlock, kTraceLds, n_rec, n_tri);
+        if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+        else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L);
+    }
+    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout);
+    if (COUNT) {
+        wave_add(&ctr->node_records, n_rec);
+        wave_add(&ctr->tri_tests, n_tri);
+    }
+}
+
+// Shading of the listed queue entries (rays the deep-stack kernel traced).
+__global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t k, const rc_ray* __restrict__ rays,
+                                                         const float4* __restrict__ st, const rc_hit* __restrict__ hits,
+                                                         const float4* __restrict__ shade,
+                                                         const DevMaterial* __restrict__ mats,
+                                                         rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
+                                                         rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
+                                                         float4* __restrict__ L, const uint32_t* __restrict__ list,
+                                                         DevCounters* __restrict__ ctr, uint32_t cout) {
+    const uint32_t n = ctr->ovf_count;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        uint32_t q = base + threadIdx.x;
+        ShadeOut O;
+        O.alive = O.shadow = false;
+        uint32_t slot = 0;
+        if (q < n) {
+            uint32_t i = list[q];
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            float4 s = st[i];
+            float4 h = reinterpret_cast<const float4*>(hits)[i];
+            slot = __float_as_uint(s.w);
+            O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L);
+        }
+        emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout);
+    }
+}
+
+// Any hit of a shadow ray; an unoccluded ray adds its contribution to its
+// slot.  Overflowing rays are listed for the deep-stack kernel and applied by
+// k_shadow_apply_listed.
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const NodeRec* __restrict__ nodes,
+                                                       const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
+                                                       const float4* __restrict__ scontrib, float4* __restrict__ L,
+                                                       uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr) {
+    __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[2];
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
-    if (gtid == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
-    for (uint32_t i = gtid; i < n; i += stride) {
+    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
+    uint32_t n_rec = 0, n_tri = 0;
+    if (i < n) {
+        const float4* rp = reinterpret_cast<const float4*>(srays + i);
+        float4 r0 = rp[0], r1 = rp[1];
+        TraceResult R = traverse<true, COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
+                                              kTraceBlock, kTraceLds, n_rec, n_tri);
+        if (R.overflow) {
+            ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+        } else if (R.id == kNoHit) {
+            float4 c = scontrib[i];
+            uint32_t slot = __float_as_uint(c.w);
+            float4 l = L[slot];
+            L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+        }
+    }
+    if (COUNT) {
+        wave_add(&ctr->node_records, n_rec);
+        wave_add(&ctr->tri_tests, n_tri);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shadow_apply_listed(const float4* __restrict__ scontrib,
+                                                                const uint8_t* __restrict__ occ,
+                                                                const uint32_t* __restrict__ list,
+                                                                float4* __restrict__ L, DevCounters* __restrict__ ctr) {
+    const uint32_t n = ctr->ovf_count;
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+        uint32_t i = list[q];
         if (occ[i]) continue;
         float4 c = scontrib[i];
         uint32_t slot = __float_as_uint(c.w);
@@ -261,11 +369,14 @@ __global__ __launch_bounds__(kBlock) void k_shadow_apply(const float4* __restric
     }
 }
 
-// Zeroes the queue the next shade writes and the shadow queue.
+// Zeroes the queue the next shade writes, the shadow queue and the overflow list.
 __global__ void k_prep(DevCounters* ctr, uint32_t cout) {
     ctr->queue_count[cout] = 0;
     ctr->queue_count[2] = 0;
+    ctr->ovf_count = 0;
 }
+
+__global__ void k_reset_ovf(DevCounters* ctr) { ctr->ovf_count = 0; }
 
 __device__ __forceinline__ void kahan(float& s, float& c, float v) {
     float y = v - c;
@@ -306,6 +417,8 @@ int alloc_pass(rc_ctx* c, size_t npaths) {
     RC_TRY(c->socc.ensure(npaths));
     RC_TRY(c->slotL.ensure(npaths * sizeof(float4)));
     RC_TRY(c->slotW.ensure(npaths * sizeof(float)));
+    RC_TRY(c->ovf_list.ensure(npaths * sizeof(uint32_t)));
+    RC_TRY(c->deep_stack.ensure(size_t(2048) * kDeepStack * sizeof(uint32_t)));
     return RC_OK;
 }
 
@@ -387,8 +500,7 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     P.sun_e = make_float3(cfg->sun_irradiance[0], cfg->sun_irradiance[1], cfg->sun_irradiance[2]);
     P.has_sun = (cfg->sun_irradiance[0] > 0.0f || cfg->sun_irradiance[1] > 0.0f || cfg->sun_irradiance[2] > 0.0f);
     P.seed = cfg->seed;
-    const uint32_t grid_all = uint32_t((npaths + kBlock - 1) / kBlock);
-    const uint32_t grid_persist = std::min(grid_all, uint32_t(c->cu_count) * 8u);
+    const uint32_t grid_small = uint32_t(c->cu_count) * 2u;  // list-driven cold-path kernels
     rc_ray* ra = c->ray_a.as<rc_ray>();
     rc_ray* rb = c->ray_b.as<rc_ray>();
     float4* sa = c->st_a.as<float4>();
@@ -408,22 +520,42 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
         rc_ray* rout = rb;
         float4* sin_ = sa;
         float4* sout = sb;
+        const uint32_t grid_t = (n + kTraceBlock - 1) / kTraceBlock;
+        uint32_t* ovf = c->ovf_list.as<uint32_t>();
         for (uint32_t k = 0; k < cfg->max_path_length; ++k) {
             uint32_t cin = k & 1u, cout = cin ^ 1u;
-            RC_TRY(rc_launch_trace(c, s, false, rin, &c->counters->queue_count[cin], n, c->hits.as<rc_hit>(), nullptr));
-            RC_TRY(c->prof_begin(&ev));
             k_prep<<<1, 1, 0, c->stream>>>(c->counters, cout);
-            k_shade<<<std::min(grid_persist, grid_n), kBlock, 0, c->stream>>>(
-                P, k, rin, sin_, c->hits.as<rc_hit>(), s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
-                c->scontrib.as<float4>(), c->slotL.as<float4>(), c->counters, cin, cout);
-            RC_TRY(c->prof_end(ev, PK_SHADE));
+            RC_TRY(c->prof_begin(&ev));
+            if (c->counting)
+                k_extend_shade<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
+                    P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
+                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout);
+            else
+                k_extend_shade<false><<<grid_t, kTraceBlock, 0, c->stream>>>(
+                    P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
+                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout);
+            RC_TRY(c->prof_end(ev, PK_EXTEND));
+            // cold path: rays whose stack outgrew LDS
+            RC_TRY(rc_launch_deep(c, s, false, rin, c->hits.as<rc_hit>(), nullptr));
+            k_shade_listed<<<grid_small, kBlock, 0, c->stream>>>(P, k, rin, sin_, c->hits.as<rc_hit>(), s->shade,
+                                                                 s->mats, rout, sout, c->srays.as<rc_ray>(),
+                                                                 c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
+                                                                 c->counters, cout);
             if (P.has_sun && k + 1 < cfg->max_path_length) {
-                RC_TRY(rc_launch_trace(c, s, true, c->srays.as<rc_ray>(), &c->counters->queue_count[2], n, nullptr,
-                                       c->socc.as<uint8_t>()));
+                k_reset_ovf<<<1, 1, 0, c->stream>>>(c->counters);
                 RC_TRY(c->prof_begin(&ev));
-                k_shadow_apply<<<std::min(grid_persist, grid_n), kBlock, 0, c->stream>>>(
-                    c->scontrib.as<float4>(), c->socc.as<uint8_t>(), c->slotL.as<float4>(), c->counters);
-                RC_TRY(c->prof_end(ev, PK_OTHER));
+                if (c->counting)
+                    k_shadow<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
+                        s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
+                        c->counters);
+                else
+                    k_shadow<false><<<grid_t, kTraceBlock, 0, c->stream>>>(
+                        s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
+                        c->counters);
+                RC_TRY(c->prof_end(ev, PK_SHADOW));
+                RC_TRY(rc_launch_deep(c, s, true, c->srays.as<rc_ray>(), nullptr, c->socc.as<uint8_t>()));
+                k_shadow_apply_listed<<<grid_small, kBlock, 0, c->stream>>>(
+                    c->scontrib.as<float4>(), c->socc.as<uint8_t>(), ovf, c->slotL.as<float4>(), c->counters);
             }
             std::swap(rin, rout);
             std::swap(sin_, sout);

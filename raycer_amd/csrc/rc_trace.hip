@@ -18,16 +18,15 @@ using namespace rcd;
 
 namespace {
 
-constexpr int kBlock = 128;
-constexpr int kLdsDepth = 32;
+#ifndef RC_TRACE_BLOCK
+#define RC_TRACE_BLOCK 256
+#endif
+#ifndef RC_LDS_DEPTH
+#define RC_LDS_DEPTH 16
+#endif
+constexpr int kBlock = RC_TRACE_BLOCK;
+constexpr int kLdsDepth = RC_LDS_DEPTH;
 constexpr int kDeepThreads = 2048;
-
-__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
-    unsigned long long x = v;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
-}
 
 __device__ __forceinline__ void store_result(bool any, const TraceResult& R, uint32_t i, rc_hit* hits, uint8_t* occ) {
     bool hit = R.id != kNoHit;
@@ -103,6 +102,8 @@ int rc_launch_trace(rc_ctx* c, const rc_scene* s, bool any, const rc_ray* rays, 
     if (n_bound == 0) return RC_OK;
     RC_TRY(c->ovf_list.ensure(size_t(n_bound) * sizeof(uint32_t)));
     RC_TRY(c->deep_stack.ensure(size_t(kDeepThreads) * kDeepStack * sizeof(uint32_t)));
+    // one lane per ray up to cu_count * blocks_per_cu blocks; beyond that the
+    // kernel's grid-stride loop takes over
     uint32_t max_blocks = uint32_t(c->cu_count) * c->tuning.blocks_per_cu;
     uint32_t blocks = std::max(1u, std::min((n_bound + kBlock - 1) / kBlock, max_blocks));
     k_reset_overflow<<<1, 1, 0, c->stream>>>(c->counters);
@@ -123,6 +124,27 @@ int rc_launch_trace(rc_ctx* c, const rc_scene* s, bool any, const rc_ray* rays, 
     }
     RC_TRY(c->prof_end(ev, any ? PK_SHADOW : PK_EXTEND));
     // rays whose stack outgrew LDS are walked again with the deep stack
+    if (any) {
+        if (c->counting)
+            k_trace_deep<true, true><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
+        else
+            k_trace_deep<true, false><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
+    } else {
+        if (c->counting)
+            k_trace_deep<false, true><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
+        else
+            k_trace_deep<false, false><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);
+    }
+    RC_TRY_HIP(hipGetLastError());
+    return RC_OK;
+}
+
+// Deep-stack traversal of the rays listed in the overflow list (count in the
+// device counters); hits (closest) or occluded flags (any) are written at the
+// listed ray indices.
+int rc_launch_deep(rc_ctx* c, const rc_scene* s, bool any, const rc_ray* rays, rc_hit* hits, uint8_t* occ) {
+    uint32_t* ovf = c->ovf_list.as<uint32_t>();
+    uint32_t* gst = c->deep_stack.as<uint32_t>();
     if (any) {
         if (c->counting)
             k_trace_deep<true, true><<<kDeepThreads / 64, 64, 0, c->stream>>>(s->nodes, s->tris, rays, hits, occ, ovf, gst, c->counters);

@@ -121,10 +121,27 @@ def mean_and_var(st):
     return mu, np.maximum(var, 0)
 
 
-def three_sigma(gpu, mu, var, p):
-    # GPU and oracle estimate the same pixel with the same estimator and sample
-    # count, so the difference has twice the oracle's variance
-    z = np.abs(gpu - mu) / (np.sqrt(2 * var) + 1e-7)
+def gpu_mean_and_var(ctx, s, cam, cfg, frames):
+    """GPU estimate from `frames` independent frames (RNG frame index 0..K-1),
+    each in its own film: pooled mean and the variance of that mean."""
+    sums = []
+    for f in range(frames):
+        film = ctx.film(cfg.width, cfg.height)
+        cfg.frame = f
+        ctx.render(s, film, cam, cfg)
+        sums.append(film.sums())
+        film.close()
+    sums = np.stack(sums)                      # (K, npix, 4): sum wL, sum w
+    means = sums[:, :, :3] / sums[:, :, 3:4]
+    mean = sums[:, :, :3].sum(0) / sums[:, :, 3:4].sum(0)
+    var = means.var(axis=0, ddof=1) / frames
+    return mean, var
+
+
+def three_sigma(a, va, b, vb, p):
+    """Pixels where |a - b| > 3 sigma in any channel, sigma^2 = va + vb (the
+    pooled variance of two independent estimates), and the binomial bound."""
+    z = np.abs(a - b) / (np.sqrt(va + vb) + 1e-7)
     outside = int((z > 3).any(axis=1).sum())
     bound = binomial_bound(3 * len(z), p)
     return outside, bound
@@ -134,33 +151,54 @@ def test_terrain_three_sigma_independent_seeds(ctx):
     # sky + sun next-event estimation: a bounded estimator, so the pixel means
     # are close to Gaussian at 256 samples and p = 0.0027 applies as stated
     sd, s, ob = scene(ctx, "terrain")
-    lin, _ = render_gpu(ctx, s, sd.desc.camera, cfg_for(sd, 64, 36, 256, seed=1234))
+    g, vg = gpu_mean_and_var(ctx, s, sd.desc.camera, cfg_for(sd, 64, 36, 16, seed=1234), 16)
     mu, var = mean_and_var(oracle_stats(ob, sd, sd.desc.camera, cfg_for(sd, 64, 36, 256, seed=98765)))
-    outside, bound = three_sigma(lin[:, :3], mu, var, 0.0027)
+    outside, bound = three_sigma(g, vg, mu, var, 0.0027)
     print(f"terrain 3-sigma: {outside} of {len(mu)} pixels outside (bound {bound:.0f})")
     assert outside <= bound
-    mse = ((lin[:, :3] - mu) ** 2).mean()
-    assert mse <= 1.5 * 2 * var.mean()
+    mse = ((g - mu) ** 2).mean()
+    assert mse <= 1.5 * (vg + var).mean()
 
 
 def test_box_three_sigma_and_blocks_independent_seeds(ctx):
     # The box is lit only by a small emitter found by BSDF sampling (as in the
     # reference): per-sample radiance is 0 or a large value, a Poisson-like
-    # estimator whose 3-sigma tail at ~10 expected light hits is about 0.4%
-    # rather than the Gaussian 0.27%; the per-pixel bound uses p = 0.005, and
-    # 8x8-pixel block means (65,536 samples each) must all lie within 4.5 sigma.
+    # estimator whose 3-sigma tail is heavier than the Gaussian 0.27%; the
+    # per-pixel bound uses p = 0.005, and 8x8-pixel block means (65,536 samples
+    # each) must all lie within 4.5 sigma.
     sd, s, ob = scene(ctx, "box")
     w = h = 32
-    lin, _ = render_gpu(ctx, s, sd.desc.camera, cfg_for(sd, w, h, 1024, seed=1234))
+    g, vg = gpu_mean_and_var(ctx, s, sd.desc.camera, cfg_for(sd, w, h, 64, seed=1234), 16)
     st = oracle_stats(ob, sd, sd.desc.camera, cfg_for(sd, w, h, 1024, seed=98765))
     mu, var = mean_and_var(st)
-    outside, bound = three_sigma(lin[:, :3], mu, var, 0.005)
+    outside, bound = three_sigma(g, vg, mu, var, 0.005)
     print(f"box 3-sigma: {outside} of {len(mu)} pixels outside (bound {bound:.0f})")
     assert outside <= bound
     blk = lambda a: a.reshape(h // 8, 8, w // 8, 8, 3).mean(axis=(1, 3))
-    bvar = var.reshape(h // 8, 8, w // 8, 8, 3).sum(axis=(1, 3)) / 64 ** 2
-    zb = np.abs(blk(lin[:, :3]) - blk(mu)) / np.sqrt(2 * bvar + 1e-12)
+    bvar = lambda v: v.reshape(h // 8, 8, w // 8, 8, 3).sum(axis=(1, 3)) / 64 ** 2
+    zb = np.abs(blk(g) - blk(mu)) / np.sqrt(bvar(vg) + bvar(var) + 1e-12)
     assert zb.max() < 4.5, zb.max()
+
+
+def test_three_sigma_test_rejects_a_biased_estimator():
+    """The statistical test has power: the oracle against itself passes, and
+    against a copy whose light is 10% brighter it fails (CPU only, oracle)."""
+    sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_BOX))
+    ob = orc.BVH(sd.vertices)
+    cfg = cfg_for(sd, 32, 32, 1024)
+    a = mean_and_var(oracle_stats(ob, sd, sd.desc.camera, cfg_for(sd, 32, 32, 1024, seed=1)))
+    b = mean_and_var(oracle_stats(ob, sd, sd.desc.camera, cfg_for(sd, 32, 32, 1024, seed=2)))
+    out, bound = three_sigma(a[0], a[1], b[0], b[1], 0.005)
+    assert out <= bound
+    mats = sd.materials.copy()
+    mats["emittance"] *= 1.10
+    sd2 = ra.SceneData(sd.vertices, sd.tri_material, mats, sd.desc)
+    c = mean_and_var(oracle_stats(ob, sd2, sd.desc.camera, cfg_for(sd, 32, 32, 1024, seed=3)))
+    h, w = 32, 32
+    blk = lambda x: x.reshape(h // 8, 8, w // 8, 8, 3).mean(axis=(1, 3))
+    bvar = lambda v: v.reshape(h // 8, 8, w // 8, 8, 3).sum(axis=(1, 3)) / 64 ** 2
+    zb = np.abs(blk(a[0]) - blk(c[0])) / np.sqrt(bvar(a[1]) + bvar(c[1]) + 1e-12)
+    assert zb.max() >= 4.5
 
 
 def test_progressive_accumulation_and_determinism(ctx):
