@@ -1,0 +1,461 @@
+// raycer.hpp — C++ interface of the MI355X path tracer, modelled on raycer's
+// own classes so a raycer caller can switch renderers:
+//   Scene, Scene::General/ToneMapper/RootBVH   (src/Raytracing/Scene.h:89-217)
+//   FlatBVH, BVHBuildInfo, FlatBVHNode          (src/Raytracing/Primitives/FlatBVH.h:19-106)
+//   Film                                        (src/Rendering/Film.h:23-52)
+//   TracerState                                 (src/Raytracing/Tracers/TracerState.h:13-26)
+//   HipTracer with CLTracer's lifecycle         (src/OpenCL/CLTracer.h:23-90)
+// Everything calls the C ABI of libraycer (raycer_hip.h, raycer_host.h);
+// failures throw RaycerError (std::runtime_error), like CLManager::checkError.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "raycer_hip.h"
+#include "raycer_host.h"
+
+namespace raycer {
+
+class RaycerError : public std::runtime_error {
+public:
+    RaycerError(int code, const std::string& msg) : std::runtime_error(msg), code(code) {}
+    int code;
+};
+
+inline void check(int rc) {
+    if (rc != RC_OK) {
+        char buf[2048];
+        rc_last_error(buf, sizeof(buf));
+        throw RaycerError(rc, std::string("raycer error ") + std::to_string(rc) + ": " + buf);
+    }
+}
+
+enum class TracerType { RAY, PATH };
+enum class FilterType { BOX, TENT };
+enum class SamplerType { JITTERED, RANDOM };
+enum class ToneMapperType { REINHARD };
+
+struct Color {
+    double r = 0.0, g = 0.0, b = 0.0;
+};
+
+struct Vector3 {
+    double x = 0.0, y = 0.0, z = 0.0;
+};
+
+struct EulerAngle {
+    double pitch = 0.0, yaw = 0.0, roll = 0.0;
+};
+
+struct Material {
+    Color diffuseReflectance;
+    Color emittance;
+    Color ambientReflectance;
+    Color specularReflectance;
+    double specularShininess = 2.0;
+    bool isEmissive = false;
+    bool nonShadowing = false;
+};
+
+struct Triangle {
+    Vector3 vertices[3];
+    uint64_t materialId = 0;  // index into Scene::materials
+};
+
+// BVHBuildInfo (FlatBVH.h:19-36): only the SAH path is provided.
+struct BVHBuildInfo {
+    uint64_t maxLeafSize = 5;
+    bool useSAH = true;
+    uint32_t threads = 0;  // 0 = all hardware threads
+};
+
+using FlatBVHNode = rc_flat_node;
+
+class FlatBVH {
+public:
+    // FlatBVH::build (FlatBVH.cpp:89-204) over the triangles' boxes.
+    void build(const std::vector<float>& vertices, const BVHBuildInfo& info) {
+        if (!info.useSAH) throw RaycerError(RC_ERR_INVALID, "only the SAH build is provided");
+        rch_bvh_build_info bi{info.maxLeafSize, info.threads, 0};
+        rch_bvh* h = nullptr;
+        check(rch_bvh_build(vertices.data(), vertices.size() / 9, &bi, &h));
+        flatNodes.resize(rch_bvh_node_count(h));
+        orderedPrimitiveIds.resize(vertices.size() / 9);
+        int rc = rch_bvh_get(h, flatNodes.data(), orderedPrimitiveIds.data());
+        buildMilliseconds = rch_bvh_build_ms(h);
+        rch_bvh_free(h);
+        check(rc);
+        hasBeenBuilt = true;
+    }
+
+    bool hasBeenBuilt = false;
+    std::vector<FlatBVHNode> flatNodes;
+    std::vector<uint64_t> orderedPrimitiveIds;
+    double buildMilliseconds = 0.0;
+};
+
+struct Camera {
+    Vector3 position;
+    EulerAngle orientation;  // pitch, yaw, roll in degrees
+    double fov = 75.0;       // horizontal, degrees
+};
+
+class Scene {
+public:
+    // Scene::General (Scene.h:89-131) subset plus the estimator terms this
+    // renderer adds; defaults are the reference's.
+    struct General {
+        TracerType tracerType = TracerType::PATH;
+        uint64_t maxPathLength = 3;
+        uint64_t multiSampleCountSqrt = 1;
+        SamplerType multiSamplerType = SamplerType::JITTERED;
+        uint64_t randomSampleCount = 0;  // RANDOM sampler: samples per pixel
+        FilterType multiSamplerFilterType = FilterType::BOX;
+        double filterRadius = 0.5;
+        double rayStartOffset = 0.00001;
+        uint64_t russianRouletteStart = 0;  // path vertex where roulette starts; 0 = off
+        bool lambertEstimator = false;      // false: the reference's 2 rho |cos| weight
+        Color skyRadiance;                  // the reference returns black on a miss
+        Vector3 sunDirection;               // towards the sun
+        Color sunIrradiance;
+        uint64_t seed = 0;
+    } general;
+
+    Camera camera;
+
+    // Scene::ToneMapper (Scene.h:137-160), Reinhard.
+    struct ToneMapper {
+        ToneMapperType type = ToneMapperType::REINHARD;
+        bool applyGamma = true;
+        bool shouldClamp = true;
+        double gamma = 2.2;
+        double key = 0.18;
+        bool fastPow = true;  // MathUtils::fastPow, as the reference
+    } toneMapper;
+
+    struct RootBVH {
+        bool enabled = true;
+        BVHBuildInfo buildInfo;
+        FlatBVH bvh;
+    } rootBVH;
+
+    struct Primitives {
+        std::vector<Triangle> triangles;
+    } primitives;
+
+    std::vector<Material> materials;
+    rc_point_light pointLight{};  // used by the RAY tracer
+
+    // Scene::initialize (Scene.cpp:203-500): flattens the triangles and builds
+    // the root BVH unless one was restored.
+    void initialize() {
+        vertices_.resize(primitives.triangles.size() * 9);
+        materialIds_.resize(primitives.triangles.size());
+        for (size_t i = 0; i < primitives.triangles.size(); ++i) {
+            const Triangle& t = primitives.triangles[i];
+            if (t.materialId >= materials.size()) throw RaycerError(RC_ERR_INVALID, "a triangle has a non-existent material id");
+            for (int v = 0; v < 3; ++v) {
+                vertices_[9 * i + 3 * v + 0] = float(t.vertices[v].x);
+                vertices_[9 * i + 3 * v + 1] = float(t.vertices[v].y);
+                vertices_[9 * i + 3 * v + 2] = float(t.vertices[v].z);
+            }
+            materialIds_[i] = uint32_t(t.materialId);
+        }
+        if (!rootBVH.bvh.hasBeenBuilt) rootBVH.bvh.build(vertices_, rootBVH.buildInfo);
+    }
+
+    // Scene::createTestScene counterpart: the seeded benchmark scenes.
+    static Scene createTestScene(uint32_t kind, uint32_t seed = 0, uint32_t level = 0, uint32_t grid = 0,
+                                 uint64_t triCount = 0) {
+        rch_scene_params p{kind, seed, level, grid, triCount};
+        rch_scene_desc d;
+        check(rch_scene_describe(&p, &d));
+        std::vector<float> v(d.ntri * 9);
+        std::vector<uint32_t> m(d.ntri);
+        std::vector<rc_material> mats(d.nmat);
+        check(rch_scene_generate(&p, v.data(), m.data(), mats.data()));
+        Scene s;
+        s.primitives.triangles.resize(d.ntri);
+        for (size_t i = 0; i < d.ntri; ++i) {
+            for (int k = 0; k < 3; ++k)
+                s.primitives.triangles[i].vertices[k] = Vector3{v[9 * i + 3 * k], v[9 * i + 3 * k + 1], v[9 * i + 3 * k + 2]};
+            s.primitives.triangles[i].materialId = m[i];
+        }
+        for (const rc_material& rm : mats) {
+            Material mt;
+            mt.diffuseReflectance = Color{rm.diffuse[0], rm.diffuse[1], rm.diffuse[2]};
+            mt.emittance = Color{rm.emittance[0], rm.emittance[1], rm.emittance[2]};
+            mt.ambientReflectance = Color{rm.ambient[0], rm.ambient[1], rm.ambient[2]};
+            mt.specularReflectance = Color{rm.specular[0], rm.specular[1], rm.specular[2]};
+            mt.specularShininess = rm.shininess;
+            mt.isEmissive = rm.flags & RC_MAT_EMISSIVE;
+            mt.nonShadowing = rm.flags & RC_MAT_NON_SHADOWING;
+            s.materials.push_back(mt);
+        }
+        const rc_render_cfg& c = d.cfg;
+        s.camera.position = Vector3{d.camera.position[0], d.camera.position[1], d.camera.position[2]};
+        s.camera.orientation = EulerAngle{d.camera.pitch, d.camera.yaw, d.camera.roll};
+        s.camera.fov = d.camera.fov;
+        s.general.maxPathLength = c.max_path_length;
+        if (c.sampler == RC_SAMPLER_JITTERED) {
+            uint64_t n = 1;
+            while ((n + 1) * (n + 1) <= c.spp) ++n;
+            s.general.multiSampleCountSqrt = n;
+        } else {
+            s.general.multiSamplerType = SamplerType::RANDOM;
+            s.general.randomSampleCount = c.spp;
+        }
+        s.general.multiSamplerFilterType = c.filter == RC_FILTER_TENT ? FilterType::TENT : FilterType::BOX;
+        s.general.filterRadius = c.filter_radius;
+        s.general.rayStartOffset = c.ray_start_offset;
+        s.general.russianRouletteStart = c.rr_start;
+        s.general.lambertEstimator = c.estimator == RC_EST_LAMBERT;
+        s.general.skyRadiance = Color{c.sky[0], c.sky[1], c.sky[2]};
+        s.general.sunDirection = Vector3{c.sun_dir[0], c.sun_dir[1], c.sun_dir[2]};
+        s.general.sunIrradiance = Color{c.sun_irradiance[0], c.sun_irradiance[1], c.sun_irradiance[2]};
+        s.general.seed = c.seed;
+        s.pointLight = d.light;
+        s.defaultWidth = c.width;
+        s.defaultHeight = c.height;
+        return s;
+    }
+
+    const std::vector<float>& flatVertices() const { return vertices_; }
+    const std::vector<uint32_t>& materialIds() const { return materialIds_; }
+    uint32_t defaultWidth = 0, defaultHeight = 0;
+
+private:
+    std::vector<float> vertices_;
+    std::vector<uint32_t> materialIds_;
+};
+
+// RGBA float image, row 0 at the bottom (src/Rendering/Image.h:10-14).
+struct Image {
+    uint64_t width = 0, height = 0;
+    std::vector<float> pixels;  // width * height * 4
+
+    // Portable float map: stored bottom row first, like this image.
+    bool savePFM(const std::string& path) const {
+        FILE* f = std::fopen(path.c_str(), "wb");
+        if (!f) return false;
+        std::fprintf(f, "PF\n%llu %llu\n-1.0\n", (unsigned long long)width, (unsigned long long)height);
+        std::vector<float> rgb(width * 3);
+        for (uint64_t y = 0; y < height; ++y) {
+            for (uint64_t x = 0; x < width; ++x)
+                for (int c = 0; c < 3; ++c) rgb[3 * x + c] = pixels[4 * (y * width + x) + c];
+            std::fwrite(rgb.data(), sizeof(float), rgb.size(), f);
+        }
+        return std::fclose(f) == 0;
+    }
+
+    // 8-bit binary PPM, top row first (Image::save flips rows, Image.cpp:118-124).
+    bool savePPM(const std::string& path) const {
+        FILE* f = std::fopen(path.c_str(), "wb");
+        if (!f) return false;
+        std::fprintf(f, "P6\n%llu %llu\n255\n", (unsigned long long)width, (unsigned long long)height);
+        std::vector<unsigned char> row(width * 3);
+        for (uint64_t yy = 0; yy < height; ++yy) {
+            uint64_t y = height - 1 - yy;
+            for (uint64_t x = 0; x < width; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    float v = pixels[4 * (y * width + x) + c];
+                    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+                    row[3 * x + c] = (unsigned char)(v * 255.0f + 0.5f);
+                }
+            std::fwrite(row.data(), 1, row.size(), f);
+        }
+        return std::fclose(f) == 0;
+    }
+};
+
+// Film (Film.h:23-52): the host side holds the images; the accumulators live
+// on the device inside HipTracer, as CLTracer keeps them in its own buffers.
+class Film {
+public:
+    void resize(uint64_t w, uint64_t h) {
+        width_ = w;
+        height_ = h;
+    }
+    void setToneMappedImage(const Image& img) { toneMapped_ = img; }
+    void setLinearImage(const Image& img) { linear_ = img; }
+    const Image& getToneMappedImage() const { return toneMapped_; }
+    const Image& getLinearImage() const { return linear_; }
+    uint64_t getWidth() const { return width_; }
+    uint64_t getHeight() const { return height_; }
+
+private:
+    uint64_t width_ = 0, height_ = 0;
+    Image linear_, toneMapped_;
+};
+
+// TracerState (TracerState.h:13-26)
+struct TracerState {
+    TracerState() : pixelsProcessed(0) {}
+    Scene* scene = nullptr;
+    Film* film = nullptr;
+    uint64_t filmWidth = 0;
+    uint64_t filmHeight = 0;
+    uint64_t pixelStartOffset = 0;
+    uint64_t pixelCount = 0;
+    std::atomic<uint64_t> pixelsProcessed;
+};
+
+// The GPU tracer with CLTracer's lifecycle (CLTracer.h:23-90): the tracer owns
+// every device buffer; run() accumulates into the device film until clear().
+class HipTracer {
+public:
+    explicit HipTracer(int device = 0) { check(rc_create(device, &ctx_)); }
+    ~HipTracer() {
+        releaseImageBuffers();
+        releaseBuffers();
+        rc_destroy(ctx_);
+    }
+    HipTracer(const HipTracer&) = delete;
+    HipTracer& operator=(const HipTracer&) = delete;
+
+    // Kernels are compiled ahead of time for gfx950; nothing to do.
+    void initializeKernels() {}
+    void releaseKernels() {}
+
+    void initializeImageBuffers(uint64_t width, uint64_t height) {
+        releaseImageBuffers();
+        check(rc_film_create(ctx_, uint32_t(width), uint32_t(height), &film_));
+        width_ = width;
+        height_ = height;
+        frame_ = 0;
+    }
+    void releaseImageBuffers() {
+        if (film_) rc_film_free(film_);
+        film_ = nullptr;
+    }
+
+    // CLTracer::initializeBuffers: reads the scene (built root BVH required).
+    void initializeBuffers(const Scene& scene) {
+        if (!scene.rootBVH.bvh.hasBeenBuilt) throw RaycerError(RC_ERR_INVALID, "scene is not initialized");
+        releaseBuffers();
+        std::vector<rc_material> mats(scene.materials.size());
+        for (size_t i = 0; i < mats.size(); ++i) {
+            const Material& m = scene.materials[i];
+            rc_material& r = mats[i];
+            r = rc_material{};
+            const Color* src[4] = {&m.diffuseReflectance, &m.emittance, &m.ambientReflectance, &m.specularReflectance};
+            float* dst[4] = {r.diffuse, r.emittance, r.ambient, r.specular};
+            for (int k = 0; k < 4; ++k) {
+                dst[k][0] = float(src[k]->r);
+                dst[k][1] = float(src[k]->g);
+                dst[k][2] = float(src[k]->b);
+            }
+            r.shininess = float(m.specularShininess);
+            r.flags = (m.isEmissive ? RC_MAT_EMISSIVE : 0u) | (m.nonShadowing ? RC_MAT_NON_SHADOWING : 0u);
+        }
+        const auto& bvh = scene.rootBVH.bvh;
+        check(rc_scene_upload(ctx_, scene.flatVertices().data(), scene.flatVertices().size() / 9,
+                              scene.materialIds().data(), mats.data(), mats.size(), bvh.flatNodes.data(),
+                              bvh.flatNodes.size(), bvh.orderedPrimitiveIds.data(), &scene_));
+    }
+    void releaseBuffers() {
+        if (scene_) rc_scene_free(scene_);
+        scene_ = nullptr;
+    }
+
+    // Tracer::run: renders one frame of the scene's samples into the device
+    // film, polling `interrupted` between passes and publishing progress in
+    // state.pixelsProcessed.
+    void run(TracerState& state, std::atomic<bool>& interrupted) {
+        if (!film_ || !scene_) throw RaycerError(RC_ERR_INVALID, "HipTracer buffers are not initialized");
+        if (state.filmWidth != width_ || state.filmHeight != height_)
+            throw RaycerError(RC_ERR_INVALID, "TracerState film size differs from the image buffers");
+        const Scene& sc = *state.scene;
+        rc_render_cfg c{};
+        c.width = uint32_t(width_);
+        c.height = uint32_t(height_);
+        if (sc.general.multiSamplerType == SamplerType::JITTERED) {
+            c.sampler = RC_SAMPLER_JITTERED;
+            c.spp = uint32_t(sc.general.multiSampleCountSqrt * sc.general.multiSampleCountSqrt);
+        } else {
+            c.sampler = RC_SAMPLER_RANDOM;
+            c.spp = uint32_t(sc.general.randomSampleCount);
+        }
+        c.max_path_length = uint32_t(sc.general.maxPathLength);
+        c.rr_start = uint32_t(sc.general.russianRouletteStart);
+        c.filter = sc.general.multiSamplerFilterType == FilterType::TENT ? RC_FILTER_TENT : RC_FILTER_BOX;
+        c.filter_radius = float(sc.general.filterRadius);
+        c.estimator = sc.general.lambertEstimator ? RC_EST_LAMBERT : RC_EST_REFERENCE;
+        c.ray_start_offset = float(sc.general.rayStartOffset);
+        c.sky[0] = float(sc.general.skyRadiance.r);
+        c.sky[1] = float(sc.general.skyRadiance.g);
+        c.sky[2] = float(sc.general.skyRadiance.b);
+        c.sun_dir[0] = float(sc.general.sunDirection.x);
+        c.sun_dir[1] = float(sc.general.sunDirection.y);
+        c.sun_dir[2] = float(sc.general.sunDirection.z);
+        c.sun_irradiance[0] = float(sc.general.sunIrradiance.r);
+        c.sun_irradiance[1] = float(sc.general.sunIrradiance.g);
+        c.sun_irradiance[2] = float(sc.general.sunIrradiance.b);
+        c.seed = sc.general.seed;
+        c.frame = frame_++;
+        rc_camera cam{{float(sc.camera.position.x), float(sc.camera.position.y), float(sc.camera.position.z)},
+                      float(sc.camera.orientation.pitch), float(sc.camera.orientation.yaw),
+                      float(sc.camera.orientation.roll), float(sc.camera.fov)};
+        struct Poll {
+            TracerState* state;
+            std::atomic<bool>* interrupted;
+        } poll{&state, &interrupted};
+        auto cb = [](void* u, uint64_t done) -> int {
+            auto* p = static_cast<Poll*>(u);
+            p->state->pixelsProcessed = done;
+            return p->interrupted->load() ? 1 : 0;
+        };
+        int rc = rc_render_progress(ctx_, scene_, film_, &cam, &c, cb, &poll);
+        if (rc == RC_ERR_INTERRUPTED) return;  // the film keeps the passes done, as Tracer::run
+        check(rc);
+        state.pixelsProcessed = state.pixelCount ? state.pixelCount : width_ * height_;
+    }
+
+    // ReinhardToneMapper on the device, then download (CLTracer::downloadImage).
+    Image downloadImage(const Scene::ToneMapper& tm = Scene::ToneMapper()) {
+        Image img = blank();
+        rc_tonemap t{float(tm.key), float(tm.gamma), tm.shouldClamp ? 1u : 0u, tm.applyGamma ? 1u : 0u,
+                     tm.fastPow ? 1u : 0u, 0u};
+        check(rc_film_tonemap(ctx_, film_, &t, img.pixels.data(), nullptr, nullptr));
+        return img;
+    }
+    Image downloadLinearImage() {
+        Image img = blank();
+        check(rc_film_download_linear(ctx_, film_, img.pixels.data()));
+        return img;
+    }
+    // CLTracer::clear: restart progressive accumulation.
+    void clear() {
+        check(rc_film_clear(ctx_, film_));
+        frame_ = 0;
+    }
+
+    rc_ctx* context() { return ctx_; }
+    rc_stats stats() {
+        rc_stats s;
+        check(rc_get_stats(ctx_, &s));
+        return s;
+    }
+
+private:
+    Image blank() const {
+        Image img;
+        img.width = width_;
+        img.height = height_;
+        img.pixels.resize(width_ * height_ * 4);
+        return img;
+    }
+    rc_ctx* ctx_ = nullptr;
+    rc_scene* scene_ = nullptr;
+    rc_film* film_ = nullptr;
+    uint64_t width_ = 0, height_ = 0;
+    uint32_t frame_ = 0;
+};
+
+}  // namespace raycer

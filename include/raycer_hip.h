@@ -224,6 +224,13 @@ int rc_film_tonemap(rc_ctx* ctx, const rc_film* film, const rc_tonemap* tm, floa
  * pixels_processed (may be NULL) is advanced as passes finish. */
 int rc_render(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam, const rc_render_cfg* cfg,
               volatile const int* interrupt, uint64_t* pixels_processed);
+/* Progress callback, called on the calling thread after every pass with the
+ * pixels finished so far; a nonzero return interrupts the render
+ * (TracerState::pixelsProcessed and the interrupted flag of Tracer::run,
+ * Tracer.cpp:82-95). */
+typedef int (*rc_progress_fn)(void* user, uint64_t pixels_processed);
+int rc_render_progress(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam,
+                       const rc_render_cfg* cfg, rc_progress_fn progress, void* user);
 /* Primary ray + Phong + one point-light hard shadow (Raytracer.cpp:312-407,
  * 524-551).  The camera ray of pixel (x, y) passes through (x + u0, y + u1),
  * (u0, u1) the jitter draw of that pixel.  hit_ids / t / rgba are host arrays

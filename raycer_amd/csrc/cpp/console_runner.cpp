@@ -1,0 +1,139 @@
+// raycer_render: non-interactive renderer, the counterpart of raycer's
+// ConsoleRunner (src/Runners/ConsoleRunner.cpp:22-175).  Builds a seeded test
+// scene, initialises it (root BVH), renders on a render thread while the main
+// thread polls TracerState::pixelsProcessed every 100 ms, tone maps and saves.
+// SIGINT interrupts the render; the passes done are still saved, as the
+// reference saves partial_result.png (ConsoleRunner.cpp:63-65).
+//
+//   raycer_render --scene box|sphere|terrain|soup [--seed N] [--level N] [--grid N]
+//                 [--tris N] [--width W] [--height H] [--spp N] [--frames F]
+//                 [--linear out.pfm] [--tonemapped out.ppm]
+#include <atomic>
+#include <chrono>
+#include <csignal>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <thread>
+
+#include "raycer.hpp"
+
+namespace {
+std::atomic<bool> g_interrupted(false);
+void on_sigint(int) { g_interrupted = true; }
+
+uint32_t scene_kind(const std::string& s) {
+    if (s == "box") return RCH_SCENE_BOX;
+    if (s == "sphere") return RCH_SCENE_SPHERE_GROUND;
+    if (s == "terrain") return RCH_SCENE_TERRAIN;
+    if (s == "soup") return RCH_SCENE_SOUP;
+    throw raycer::RaycerError(RC_ERR_INVALID, "unknown scene " + s);
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string scene_name = "box", linear_out, tm_out;
+    uint32_t seed = 0, level = 0, grid = 0, width = 0, height = 0, spp = 0, frames = 1;
+    uint64_t tris = 0;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&]() -> std::string {
+            if (i + 1 >= argc) throw raycer::RaycerError(RC_ERR_INVALID, "missing value for " + a);
+            return argv[++i];
+        };
+        try {
+            if (a == "--scene") scene_name = next();
+            else if (a == "--seed") seed = std::stoul(next());
+            else if (a == "--level") level = std::stoul(next());
+            else if (a == "--grid") grid = std::stoul(next());
+            else if (a == "--tris") tris = std::stoull(next());
+            else if (a == "--width") width = std::stoul(next());
+            else if (a == "--height") height = std::stoul(next());
+            else if (a == "--spp") spp = std::stoul(next());
+            else if (a == "--frames") frames = std::stoul(next());
+            else if (a == "--linear") linear_out = next();
+            else if (a == "--tonemapped") tm_out = next();
+            else {
+                std::fprintf(stderr, "unknown option %s\n", a.c_str());
+                return 2;
+            }
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 2;
+        }
+    }
+    std::signal(SIGINT, on_sigint);
+    try {
+        auto t0 = std::chrono::steady_clock::now();
+        raycer::Scene scene = raycer::Scene::createTestScene(scene_kind(scene_name), seed, level, grid, tris);
+        if (spp) {
+            if (scene.general.multiSamplerType == raycer::SamplerType::RANDOM) {
+                scene.general.randomSampleCount = spp;
+            } else {
+                uint64_t n = 1;
+                while ((n + 1) * (n + 1) <= spp) ++n;
+                scene.general.multiSampleCountSqrt = n;
+            }
+        }
+        scene.initialize();
+        uint32_t w = width ? width : scene.defaultWidth, h = height ? height : scene.defaultHeight;
+        raycer::Film film;
+        film.resize(w, h);
+        raycer::HipTracer tracer(0);
+        tracer.initializeKernels();
+        tracer.initializeImageBuffers(w, h);
+        tracer.initializeBuffers(scene);
+        raycer::TracerState state;
+        state.scene = &scene;
+        state.film = &film;
+        state.filmWidth = w;
+        state.filmHeight = h;
+        state.pixelCount = uint64_t(w) * h;
+        double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+
+        std::exception_ptr render_error = nullptr;
+        auto r0 = std::chrono::steady_clock::now();
+        std::thread render([&]() {
+            try {
+                for (uint32_t f = 0; f < frames && !g_interrupted; ++f) tracer.run(state, g_interrupted);
+            } catch (...) {
+                render_error = std::current_exception();
+                g_interrupted = true;
+            }
+        });
+        uint64_t last = ~0ull;
+        while (true) {
+            std::this_thread::sleep_for(std::chrono::milliseconds(100));
+            uint64_t p = state.pixelsProcessed.load();
+            if (p != last) {
+                std::fprintf(stderr, "\rpixels %llu / %llu", (unsigned long long)p, (unsigned long long)state.pixelCount);
+                last = p;
+            }
+            if (p >= state.pixelCount || g_interrupted) break;
+        }
+        render.join();
+        std::fprintf(stderr, "\n");
+        if (render_error) std::rethrow_exception(render_error);
+        double render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
+        film.setLinearImage(tracer.downloadLinearImage());
+        film.setToneMappedImage(tracer.downloadImage(scene.toneMapper));
+        bool ok = true;
+        if (!linear_out.empty()) ok &= film.getLinearImage().savePFM(linear_out);
+        if (!tm_out.empty()) ok &= film.getToneMappedImage().savePPM(tm_out);
+        uint64_t samples_per_pixel = scene.general.multiSamplerType == raycer::SamplerType::RANDOM
+                                         ? scene.general.randomSampleCount
+                                         : scene.general.multiSampleCountSqrt * scene.general.multiSampleCountSqrt;
+        double msamples = double(w) * h * samples_per_pixel * frames / (render_ms * 1e3);
+        std::printf("{\"scene\": \"%s\", \"triangles\": %zu, \"bvh_ms\": %.1f, \"setup_ms\": %.1f, "
+                    "\"render_ms\": %.1f, \"frames\": %u, \"spp\": %llu, \"msamples_per_s\": %.2f, "
+                    "\"interrupted\": %s}\n",
+                    scene_name.c_str(), scene.primitives.triangles.size(), scene.rootBVH.bvh.buildMilliseconds,
+                    setup_ms, render_ms, frames, (unsigned long long)samples_per_pixel, msamples,
+                    g_interrupted ? "true" : "false");
+        return ok ? 0 : 1;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}

@@ -16,7 +16,7 @@
 
 int rc_validate_cfg(const rc_render_cfg* cfg, const char* fn);
 int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                     uint32_t spp_pass, volatile const int* interrupt, uint64_t* pixels_processed, bool sync_passes);
+                     uint32_t spp_pass, rc_progress_fn progress, void* user);
 int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass);
 
 namespace {
@@ -54,7 +54,7 @@ extern "C" int rc_render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film
     k_batch_zero<<<1, 1, 0, c->stream>>>(c->counters);
     int rc = RC_OK;
     for (size_t i = 0; i < n && rc == RC_OK; ++i)
-        rc = rc_enqueue_frame(c, scenes[i], films[i], &cams[i], &cfgs[i], t.spp_per_pass, nullptr, nullptr, false);
+        rc = rc_enqueue_frame(c, scenes[i], films[i], &cams[i], &cfgs[i], t.spp_per_pass, nullptr, nullptr);
     hipError_t ec = hipStreamEndCapture(c->stream, &graph);
     c->profiling = was_profiling;
     if (rc != RC_OK) {

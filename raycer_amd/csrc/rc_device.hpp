@@ -164,6 +164,17 @@ struct TraceResult {
     bool overflow;
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup w runs on XCD
+// w % 8).  This bijection on [0, n) gives XCD x the contiguous range of
+// logical blocks [x n / 8, (x + 1) n / 8), so each XCD's private L2 sees a
+// contiguous, spatially coherent slice of the ray queue.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t w, uint32_t n) {
+    uint32_t x = w % kXcds, local = w / kXcds;
+    uint32_t per = n / kXcds, rem = n % kXcds;
+    return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+}
+
 // Wave-wide sum, one 64-bit atomic per wave (every lane must call it).
 __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
     unsigned long long x = v;

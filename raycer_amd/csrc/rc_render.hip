@@ -36,6 +36,16 @@ constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
 #endif
+// Giving each XCD a contiguous slice of the grid halves throughput here: the
+// grid is sized for the queue's capacity while only a prefix is live after
+// the first bounce, so the live rays land on one or two XCDs.  Off by default.
+#ifndef RC_XCD_REMAP
+#define RC_XCD_REMAP 0
+#endif
+
+__device__ __forceinline__ uint32_t trace_block() {
+    return RC_XCD_REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+}
 
 struct CamBasis {
     float3 pos, right, up, center;
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
     DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[cin];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    const uint32_t i = trace_block() * kTraceBlock + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
@@ -331,7 +341,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const Node
                                                        uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[2];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    const uint32_t i = trace_block() * kTraceBlock + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     if (i < n) {
@@ -474,7 +484,7 @@ int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass) {
 // One frame of a scene into a film: the passes of rc_render, enqueued on the
 // context stream.  Used by rc_render and the batched graph.
 int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                     uint32_t spp_pass, volatile const int* interrupt, uint64_t* pixels_processed, bool sync_passes) {
+                     uint32_t spp_pass, rc_progress_fn progress, void* user) {
     const uint32_t npix = cfg->width * cfg->height;
     spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
     const size_t npaths = size_t(npix) * spp_pass;
@@ -506,7 +516,11 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     float4* sa = c->st_a.as<float4>();
     float4* sb = c->st_b.as<float4>();
     for (uint32_t s0 = 0; s0 < cfg->spp; s0 += spp_pass) {
-        if (interrupt && *interrupt) return rc_host_fail(RC_ERR_INTERRUPTED, "rc_render: interrupted");
+        if (progress) {
+            RC_TRY_HIP(hipStreamSynchronize(c->stream));
+            uint64_t done = uint64_t(double(npix) * double(s0) / double(cfg->spp));
+            if (progress(user, done)) return rc_host_fail(RC_ERR_INTERRUPTED, "rc_render: interrupted");
+        }
         uint32_t this_pass = std::min(spp_pass, cfg->spp - s0);
         P.spp_pass = this_pass;
         P.sample0 = s0;
@@ -565,17 +579,33 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
                                                                           c->slotW.as<float>(), f->sum, f->comp);
         RC_TRY(c->prof_end(ev, PK_OTHER));
         RC_TRY_HIP(hipGetLastError());
-        if (sync_passes) {
-            RC_TRY_HIP(hipStreamSynchronize(c->stream));
-            if (pixels_processed)
-                *pixels_processed = uint64_t(double(npix) * double(s0 + this_pass) / double(cfg->spp));
-        }
     }
     return RC_OK;
 }
 
+namespace {
+struct PointerProgress {
+    volatile const int* interrupt;
+    uint64_t* pixels;
+};
+int pointer_progress(void* user, uint64_t done) {
+    auto* p = static_cast<PointerProgress*>(user);
+    if (p->pixels) *p->pixels = done;
+    return p->interrupt && *p->interrupt;
+}
+}  // namespace
+
 extern "C" int rc_render(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
                          volatile const int* interrupt, uint64_t* pixels_processed) {
+    PointerProgress pp{interrupt, pixels_processed};
+    bool poll = interrupt != nullptr || pixels_processed != nullptr;
+    int rc = rc_render_progress(c, s, f, cam, cfg, poll ? pointer_progress : nullptr, &pp);
+    if (rc == RC_OK && pixels_processed) *pixels_processed = uint64_t(cfg->width) * cfg->height;
+    return rc;
+}
+
+extern "C" int rc_render_progress(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
+                                  const rc_render_cfg* cfg, rc_progress_fn progress, void* user) {
     RC_TRY(rc_check_scene(c, s, "rc_render"));
     RC_TRY(rc_validate_cfg(cfg, "rc_render"));
     if (!f || f->ctx != c) return rc_host_fail(RC_ERR_INVALID, "rc_render: bad film");
@@ -587,8 +617,7 @@ extern "C" int rc_render(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_came
     c->stats.extend_ms = c->stats.shadow_ms = c->stats.shade_ms = c->stats.other_ms = 0.0;
     c->stats.extend_launches = c->stats.shadow_launches = 0;
     k_zero_counters<<<1, 1, 0, c->stream>>>(c->counters);
-    int rc = rc_enqueue_frame(c, s, f, cam, cfg, c->tuning.spp_per_pass, interrupt, pixels_processed,
-                              interrupt != nullptr || pixels_processed != nullptr);
+    int rc = rc_enqueue_frame(c, s, f, cam, cfg, c->tuning.spp_per_pass, progress, user);
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     if (rc != RC_OK) return rc;
     DevCounters h;
@@ -602,7 +631,7 @@ extern "C" int rc_render(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_came
     c->stats.node_records = h.node_records;
     c->stats.tri_tests = h.tri_tests;
     c->stats.overflow_rays = h.ovf_count;
-    if (pixels_processed) *pixels_processed = uint64_t(cfg->width) * cfg->height;
+    if (progress) progress(user, uint64_t(cfg->width) * cfg->height);
     return RC_OK;
 }
 
