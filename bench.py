@@ -9,13 +9,18 @@ built and uploaded before timing, so the timed region holds only GPU work on
 device-resident inputs.
 
 Prints one JSON line.  Besides the headline Msamples/s it reports the
-roofline of the dominant kernel (closest-hit traversal): algorithmic bytes per
-extension ray = 32 * node records fetched + 48 * triangle records tested + 32
-(ray in) + 16 (hit out), counted by the traversal's counting variant on one
-untimed frame and divided by the kernel time measured with HIP events on the
-renderer's stream during the timed frames.  The CPU baseline is the FP64
-oracle (restatement of the reference's OpenMP CPU path, built -O3
--ffast-math) on a seeded 1/16 sample of the pixels.
+roofline of the dominant kernel, k_extend_shade (closest-hit traversal fused
+with shading).  Its algorithmic bytes per frame are
+  32 * node records fetched + 48 * triangle records tested
+  + 96 per extension ray (ray 32 + path state 16 + shading record 16 in,
+    radiance accumulator 16 read + 16 written)
+  + 48 per continuation ray written (ray 32 + state 16)
+  + 48 per shadow ray written (ray 32 + contribution 16),
+counted by the counting variant on one untimed frame (materials, a few
+64-byte records, are left out) and divided by the kernel time measured with HIP
+events on the renderer's stream during the timed frames.  The CPU baseline is
+the FP64 oracle (restatement of the reference's OpenMP CPU path, built -O3
+-ffast-math) on a seeded half of the pixels.
 """
 from __future__ import annotations
 
@@ -40,7 +45,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sd, cam, cfg, frac: int = 16, seed: int = 7) -> dict:
+def cpu_baseline(sd, cam, cfg, frac: int = 2, seed: int = 7) -> dict:
     """FP64 oracle (-O3 -ffast-math, OpenMP) on a seeded 1/frac pixel sample."""
     import oracle as orc
 
@@ -104,7 +109,9 @@ def main() -> int:
     cst = ctx.stats()
     ctx.set_counting(False)
     ext_rays = cst.extension_rays
-    alg_bytes = 32 * cst.node_records + 48 * cst.tri_tests + 48 * ext_rays
+    camera_rays = cfg.width * cfg.height * cfg.spp
+    alg_bytes = (32 * cst.node_records + 48 * cst.tri_tests + 96 * ext_rays + 48 * (ext_rays - camera_rays)
+                 + 48 * cst.shadow_rays)
     log(f"counting frame: {ext_rays} extension rays, {cst.node_records / ext_rays:.1f} records/ray, "
         f"{cst.tri_tests / ext_rays:.2f} triangles/ray, {cst.shadow_rays} shadow rays")
 
@@ -169,7 +176,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "closest-hit traversal (k_trace<false,false>)",
+            "kernel": "k_extend_shade<false> (closest-hit traversal fused with shading)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -193,7 +200,7 @@ def main() -> int:
     scene.close()
     ctx.close()
     if not a.no_cpu_baseline:
-        log("cpu baseline (oracle, 1/16 of the pixels)")
+        log("cpu baseline (oracle, half of the pixels)")
         out["cpu_baseline"] = cpu_baseline(sd, cam, cfg)
     else:
         out["cpu_baseline"] = None

@@ -156,6 +156,10 @@ typedef struct rc_stats {
     double other_ms;
     double render_ms;            /* wall time of the last render call */
     uint64_t shadow_launches;
+    double cache_ms;             /* radiance-cache inference kernels (profiling on) */
+    double train_ms;             /* radiance-cache training step of the last cached render */
+    uint64_t cache_queries;      /* paths that ended into the cache */
+    uint64_t train_samples;      /* training samples of the last cached render */
 } rc_stats;
 
 /* ------------------------------------------------------------ context */
@@ -238,6 +242,49 @@ int rc_render_progress(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const 
 int rc_raycast_phong(rc_ctx* ctx, const rc_scene* scene, const rc_camera* cam, uint32_t width, uint32_t height,
                      uint64_t seed, float ray_start_offset, const rc_point_light* light, int32_t* hit_ids,
                      float* t, float* rgba);
+
+/* ------------------------------------------------------------ radiance cache
+ * No counterpart in the reference.  An MLP (81 -> 64 x 4 ReLU -> 3 exp,
+ * 17,923 parameters) with BF16 weights on MFMA and FP32 accumulation, trained
+ * online with Adam on radiance samples the path tracer produces; a render with
+ * cfg.cache_vertex = k ends paths at vertex k into the cache.
+ * Parameters are FP32 masters, packed per layer as W (out x in, row-major) then b.
+ * A query is 12 floats: position (scene units), direction, normal, albedo;
+ * the position is scaled to [0,1]^3 by the cache's bounds before the 12-frequency
+ * sin/cos encoding. */
+enum { RC_CACHE_PARAMS = 17923, RC_CACHE_QUERY_FLOATS = 12 };
+typedef struct rc_cache_cfg {
+    float lr;            /* Adam learning rate (1e-3) */
+    float beta1, beta2;  /* 0.9, 0.99 */
+    float adam_eps;      /* 1e-8 */
+    float loss_eps;      /* relative L2 denominator epsilon (1e-2) */
+    uint32_t seed;       /* weight initialisation */
+    uint32_t max_batch;  /* largest training batch (65,536) */
+    uint32_t train_stride;  /* renderer: one training path per train_stride slots (0 = about max_batch a frame) */
+} rc_cache_cfg;
+
+int rc_cache_create(rc_ctx* ctx, const rc_cache_cfg* cfg, const float bounds_min[3], const float bounds_max[3],
+                    rc_cache** out);
+int rc_cache_free(rc_cache* cache);
+int rc_cache_set_params(rc_cache* cache, const float* params);
+int rc_cache_get_params(rc_cache* cache, float* params);
+/* queries: n * 12 floats (host); radiance: n * 3 floats (host) */
+int rc_cache_infer(rc_ctx* ctx, rc_cache* cache, const float* queries, size_t n, float* radiance);
+/* One Adam step on a batch (host arrays); loss (may be NULL) gets the batch loss. */
+int rc_cache_train_step(rc_ctx* ctx, rc_cache* cache, const float* queries, const float* targets, size_t n,
+                        float* loss);
+/* Gradient of the batch loss without an update (for checks): grad has 17,923 floats. */
+int rc_cache_gradient(rc_ctx* ctx, rc_cache* cache, const float* queries, const float* targets, size_t n,
+                      float* grad, float* loss);
+/* rc_render with paths ending into the cache at cfg->cache_vertex (1 <=
+ * cache_vertex < max_path_length - 1): a path that reaches that vertex without
+ * emission adds throughput x cache(query) and stops.  Paths of one slot in
+ * train_stride run to the end instead and give up to max_batch training
+ * samples (target = radiance gathered after the vertex / throughput at it);
+ * with train != 0 one Adam step runs on them after the frame and loss (may be
+ * NULL) gets that batch's loss. */
+int rc_render_cached(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam,
+                     const rc_render_cfg* cfg, rc_cache* cache, int train, float* loss);
 
 /* Many scenes in one captured HIP graph (persistent launch): scenes[i] is
  * rendered into films[i] with cams[i] and cfgs[i]. */

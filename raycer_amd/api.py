@@ -76,7 +76,17 @@ class Stats(C.Structure):
                 ("node_records", C.c_uint64), ("tri_tests", C.c_uint64), ("overflow_rays", C.c_uint64),
                 ("extend_launches", C.c_uint64), ("extend_ms", C.c_double), ("shadow_ms", C.c_double),
                 ("shade_ms", C.c_double), ("other_ms", C.c_double), ("render_ms", C.c_double),
-                ("shadow_launches", C.c_uint64)]
+                ("shadow_launches", C.c_uint64), ("cache_ms", C.c_double), ("train_ms", C.c_double),
+                ("cache_queries", C.c_uint64), ("train_samples", C.c_uint64)]
+
+
+class CacheCfg(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("adam_eps", C.c_float),
+                ("loss_eps", C.c_float), ("seed", C.c_uint32), ("max_batch", C.c_uint32),
+                ("train_stride", C.c_uint32)]
+
+
+CACHE_PARAMS = 17923
 
 
 class BVHBuildInfo(C.Structure):
@@ -137,6 +147,15 @@ def lib() -> C.CDLL:
                                  vp],
             "rc_render_batch": [vp, vp, vp, vp, vp, sz, C.POINTER(Tuning)],
             "rc_copy_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
+            "rc_render_progress": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), vp, vp],
+            "rc_cache_create": [vp, C.POINTER(CacheCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(vp)],
+            "rc_cache_free": [vp],
+            "rc_cache_set_params": [vp, vp],
+            "rc_cache_get_params": [vp, vp],
+            "rc_cache_infer": [vp, vp, vp, sz, vp],
+            "rc_cache_train_step": [vp, vp, vp, vp, sz, C.POINTER(C.c_float)],
+            "rc_cache_gradient": [vp, vp, vp, vp, sz, vp, C.POINTER(C.c_float)],
+            "rc_render_cached": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), vp, i32, C.POINTER(C.c_float)],
             "rch_bvh_build": [vp, sz, C.POINTER(BVHBuildInfo), C.POINTER(vp)],
             "rch_bvh_node_count": [vp],
             "rch_bvh_get": [vp, vp, vp],
@@ -303,6 +322,63 @@ class Context:
         check(lib().rc_raycast_phong(self.h, scene.h, C.byref(cam), width, height, seed, offset, C.byref(light),
                                      _ptr(ids), _ptr(t), _ptr(rgba)))
         return ids, t, rgba
+
+
+class RadianceCache:
+    """The MLP radiance cache (rc_cache_*)."""
+
+    def __init__(self, ctx: Context, bounds_min, bounds_max, lr=1e-3, seed=1, max_batch=65536, train_stride=0,
+                 loss_eps=1e-2):
+        self.ctx = ctx
+        cfg = CacheCfg(lr, 0.9, 0.99, 1e-8, loss_eps, seed, max_batch, train_stride)
+        lo = (C.c_float * 3)(*bounds_min)
+        hi = (C.c_float * 3)(*bounds_max)
+        h = C.c_void_p()
+        check(lib().rc_cache_create(ctx.h, C.byref(cfg), lo, hi, C.byref(h)))
+        self.h = h
+
+    def params(self) -> np.ndarray:
+        p = np.empty(CACHE_PARAMS, np.float32)
+        check(lib().rc_cache_get_params(self.h, _ptr(p)))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib().rc_cache_set_params(self.h, _ptr(p)))
+
+    def infer(self, queries) -> np.ndarray:
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 12)
+        out = np.empty((q.shape[0], 3), np.float32)
+        check(lib().rc_cache_infer(self.ctx.h, self.h, _ptr(q), q.shape[0], _ptr(out)))
+        return out
+
+    def gradient(self, queries, targets):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 12)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        g = np.empty(CACHE_PARAMS, np.float32)
+        loss = C.c_float()
+        check(lib().rc_cache_gradient(self.ctx.h, self.h, _ptr(q), _ptr(t), q.shape[0], _ptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, queries, targets) -> float:
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 12)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        check(lib().rc_cache_train_step(self.ctx.h, self.h, _ptr(q), _ptr(t), q.shape[0], C.byref(loss)))
+        return loss.value
+
+    def render(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, train: bool = True) -> float:
+        """One frame with paths ending into the cache at cfg.cache_vertex; with
+        train, one Adam step on the frame's training samples (returns its loss)."""
+        loss = C.c_float()
+        check(lib().rc_render_cached(self.ctx.h, scene.h, film.h, C.byref(cam), C.byref(cfg), self.h, int(train),
+                                     C.byref(loss)))
+        return loss.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().rc_cache_free(self.h)
+            self.h = None
 
 
 class Scene:

@@ -14,11 +14,6 @@
 
 #include "rc_internal.hpp"
 
-int rc_validate_cfg(const rc_render_cfg* cfg, const char* fn);
-int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                     uint32_t spp_pass, rc_progress_fn progress, void* user);
-int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass);
-
 namespace {
 __global__ void k_batch_zero(DevCounters* ctr) {
     ctr->node_records = 0;

@@ -100,6 +100,7 @@ int rc_ctx::prof_collect() {
             case PK_EXTEND: stats.extend_ms += ms; stats.extend_launches++; break;
             case PK_SHADOW: stats.shadow_ms += ms; stats.shadow_launches++; break;
             case PK_SHADE: stats.shade_ms += ms; break;
+            case PK_CACHE: stats.cache_ms += ms; break;
             default: stats.other_ms += ms; break;
         }
         ev_free.push_back(p.a);

@@ -44,7 +44,7 @@ struct DevCounters {
     uint32_t error;
 };
 
-enum ProfKind { PK_EXTEND = 0, PK_SHADOW = 1, PK_SHADE = 2, PK_OTHER = 3 };
+enum ProfKind { PK_EXTEND = 0, PK_SHADOW = 1, PK_SHADE = 2, PK_OTHER = 3, PK_CACHE = 4 };
 
 struct rc_ctx {
     int device = 0;
@@ -101,5 +101,39 @@ struct rc_film {
 int rc_launch_trace(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, const uint32_t* n_dev,
                     uint32_t n_bound, rc_hit* hits, uint8_t* occ);
 int rc_launch_deep(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, rc_hit* hits, uint8_t* occ);
+
+// Radiance-cache plumbing of a frame (all null / 0 when the cache is off).
+struct CacheIO {
+    float4* queries;        // 4 float4 per query: (p, slot), (w_o, thr.r), (n, thr.g), (albedo, thr.b)
+    float4* records;        // 5 float4 per training vertex: the query layout, then (L before, 0)
+    uint32_t* counters;     // [0] queries, [1] records of the pass; [2] training samples, [3] queries of the frame
+    float* batch_q;         // 12 floats per training sample
+    float* batch_t;         // 3 floats per training sample
+    uint32_t batch_cap;
+    uint32_t cache_vertex;  // 0 = off
+    uint32_t train_stride;  // one training path per train_stride slots
+    uint32_t cap;           // record / query capacity of a pass
+};
+
+int rc_validate_cfg(const rc_render_cfg* cfg, const char* fn);
+// Sizes every buffer a frame of this config needs (outside any capture).
+int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass);
+// Enqueues the passes of one frame on the context stream; cache_io / cache are
+// null for the plain renderer.
+int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                     uint32_t spp_pass, rc_progress_fn progress, void* user, const CacheIO* cache_io = nullptr,
+                     rc_cache* cache = nullptr);
+// Radiance cache, device side (rc_mlp.hip).
+// L[slot] += thr * cache(query) for the *n_dev (<= n_bound) queries.
+int rc_cache_apply_device(rc_ctx* c, rc_cache* k, const float4* queries, const uint32_t* n_dev, uint32_t n_bound,
+                          float4* L);
+// Sizes the cache's frame buffers for passes of pass_paths slots and a frame
+// of frame_samples samples, fills io and enqueues the clearing of its counters.
+int rc_cache_frame_io(rc_ctx* c, rc_cache* k, uint32_t cache_vertex, size_t pass_paths, uint64_t frame_samples,
+                      CacheIO* io);
+// After a frame: the frame's query and training-sample counts and, if train,
+// one optimizer step over the training samples it gathered.
+int rc_cache_train_frame(rc_ctx* c, rc_cache* k, bool train, float* loss, uint32_t* samples, uint32_t* queries);
+rc_ctx* rc_cache_owner(const rc_cache* k);
 int rc_check_ctx(const rc_ctx* ctx, const char* fn);
 int rc_check_scene(const rc_ctx* ctx, const rc_scene* s, const char* fn);

@@ -192,13 +192,25 @@ struct ShadeOut {
     float3 thr, newo, newd, contrib, so;
 };
 
+// Training paths: a fixed-rate, frame-varying subset of the slots that run
+// to the end instead of stopping into the cache.
+__device__ __forceinline__ bool training_path(const PassParams& P, uint32_t slot, uint32_t stride) {
+    uint32_t pixel = slot % P.npix, sample = P.sample0 + slot / P.npix;
+    uint32_t h = pixel * 2654435761u ^ (sample * 40503u + P.frame * 2246822519u);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h % stride == 0u;
+}
+
 // One path vertex: PathTracer::traceRecursive (PathTracer.cpp:25-75) for the
 // hit (t, tri) of ray (r0, r1), plus sky on a miss, sun next-event estimation
 // and Russian roulette.  Radiance goes to L[slot]; the continuation and the
 // shadow ray are returned for the caller to append.
 __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k, float4 r0, float4 r1, float4 s,
                                                  float t, int tri, const float4* __restrict__ shade,
-                                                 const DevMaterial* __restrict__ mats, float4* __restrict__ L) {
+                                                 const DevMaterial* __restrict__ mats, float4* __restrict__ L,
+                                                 const CacheIO& CI) {
     ShadeOut O;
     O.alive = O.shadow = false;
     const bool more = (k + 1) < P.max_len;
@@ -218,10 +230,40 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
         L[slot] = make_float4(l.x + thr.x * e.x, l.y + thr.y * e.y, l.z + thr.z * e.z, l.w);
         return O;
     }
-    if (!more) return O;
     float3 nrm = f3(sh);
     float3 p = f3(r0) + (t * f3(r1));
     float3 rho = f3(dif);
+    if (CI.cache_vertex > 0 && k == CI.cache_vertex) {
+        const float3 wo = make_float3(-r1.x, -r1.y, -r1.z);
+        bool train = training_path(P, slot, CI.train_stride);
+        if (!train) {
+            // the path ends here: its remaining radiance comes from the cache
+            uint32_t q = atomicAdd(&CI.counters[0], 1u);
+            if (q < CI.cap) {
+                float4* Q = CI.queries + 4 * size_t(q);
+                Q[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+                Q[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
+                Q[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.y);
+                Q[3] = make_float4(rho.x, rho.y, rho.z, thr.z);
+            }
+            return O;
+        }
+        {
+            // training vertex: the radiance the full path gathers after it,
+            // over the throughput arriving at it, is the target for this query
+            uint32_t r = atomicAdd(&CI.counters[1], 1u);
+            if (r < CI.cap) {
+                float4 lb = L[slot];
+                float4* R = CI.records + 5 * size_t(r);
+                R[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+                R[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
+                R[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.y);
+                R[3] = make_float4(rho.x, rho.y, rho.z, thr.z);
+                R[4] = make_float4(lb.x, lb.y, lb.z, 0.0f);
+            }
+        }
+    }
+    if (!more) return O;
     float4 u = draw4(P.seed, slot % P.npix, P.sample0 + slot / P.npix, P.frame, k);
     if (P.has_sun) {
         float cs = dot3(nrm, P.sun);
@@ -278,7 +320,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
     const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
     const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
-    DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout) {
+    DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[cin];
     const uint32_t i = trace_block() * kTraceBlock + threadIdx.x;
@@ -295,7 +337,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
         TraceResult R = traverse<false, COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
                                                kTraceBlock, kTraceLds, n_rec, n_tri);
         if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
-        else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L);
+        else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
     }
     emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout);
     if (COUNT) {
@@ -312,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
                                                          rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                                          rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
                                                          float4* __restrict__ L, const uint32_t* __restrict__ list,
-                                                         DevCounters* __restrict__ ctr, uint32_t cout) {
+                                                         DevCounters* __restrict__ ctr, uint32_t cout, CacheIO CI) {
     const uint32_t n = ctr->ovf_count;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t q = base + threadIdx.x;
@@ -325,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
             float4 s = st[i];
             float4 h = reinterpret_cast<const float4*>(hits)[i];
             slot = __float_as_uint(s.w);
-            O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L);
+            O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L, CI);
         }
         emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout);
     }
@@ -377,6 +419,38 @@ __global__ __launch_bounds__(kBlock) void k_shadow_apply_listed(const float4* __
         float4 l = L[slot];
         L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
     }
+}
+
+// Training samples of a pass: query + target (L_final - L_before) / T for the
+// recorded training vertices, appended to the frame's batch (capped); also
+// adds the pass's query count to the frame total counters[3].
+__global__ __launch_bounds__(kBlock) void k_cache_targets(const float4* __restrict__ records,
+                                                          uint32_t* __restrict__ counters, uint32_t pass_cap,
+                                                          const float4* __restrict__ L, float* __restrict__ batch_q,
+                                                          float* __restrict__ batch_t, uint32_t batch_cap) {
+    const uint32_t n = min(counters[1], pass_cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[3] += min(counters[0], pass_cap);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4* R = records + 5 * size_t(i);
+        float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], r4 = R[4];
+        float4 lf = L[__float_as_uint(r0.w)];
+        float T[3] = {r1.w, r2.w, r3.w};
+        float d[3] = {lf.x - r4.x, lf.y - r4.y, lf.z - r4.z};
+        uint32_t pos = atomicAdd(&counters[2], 1u);
+        if (pos >= batch_cap) continue;
+        float* q = batch_q + size_t(pos) * 12;
+        q[0] = r0.x; q[1] = r0.y; q[2] = r0.z;
+        q[3] = r1.x; q[4] = r1.y; q[5] = r1.z;
+        q[6] = r2.x; q[7] = r2.y; q[8] = r2.z;
+        q[9] = r3.x; q[10] = r3.y; q[11] = r3.z;
+        for (int c = 0; c < 3; ++c) batch_t[size_t(pos) * 3 + c] = T[c] > 1e-8f ? fmaxf(d[c], 0.0f) / T[c] : 0.0f;
+    }
+}
+
+__global__ void k_cache_reset(uint32_t* counters, uint32_t which) {
+    counters[0] = 0;
+    counters[1] = 0;
+    if (which) counters[2] = 0;
 }
 
 // Zeroes the queue the next shade writes, the shadow queue and the overflow list.
@@ -484,7 +558,10 @@ int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass) {
 // One frame of a scene into a film: the passes of rc_render, enqueued on the
 // context stream.  Used by rc_render and the batched graph.
 int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                     uint32_t spp_pass, rc_progress_fn progress, void* user) {
+                     uint32_t spp_pass, rc_progress_fn progress, void* user, const CacheIO* cache_io,
+                     rc_cache* cache) {
+    CacheIO CI{};
+    if (cache_io) CI = *cache_io;
     const uint32_t npix = cfg->width * cfg->height;
     spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
     const size_t npaths = size_t(npix) * spp_pass;
@@ -527,6 +604,7 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
         const uint32_t n = npix * this_pass;
         const uint32_t grid_n = (n + kBlock - 1) / kBlock;
         hipEvent_t ev;
+        if (CI.cache_vertex) k_cache_reset<<<1, 1, 0, c->stream>>>(CI.counters, 0);
         RC_TRY(c->prof_begin(&ev));
         k_camera<<<grid_n, kBlock, 0, c->stream>>>(P, ra, sa, c->slotL.as<float4>(), c->slotW.as<float>(), c->counters);
         RC_TRY(c->prof_end(ev, PK_OTHER));
@@ -543,18 +621,18 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
             if (c->counting)
                 k_extend_shade<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
                     P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
-                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout);
+                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
             else
                 k_extend_shade<false><<<grid_t, kTraceBlock, 0, c->stream>>>(
                     P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
-                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout);
+                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
             RC_TRY(c->prof_end(ev, PK_EXTEND));
             // cold path: rays whose stack outgrew LDS
             RC_TRY(rc_launch_deep(c, s, false, rin, c->hits.as<rc_hit>(), nullptr));
             k_shade_listed<<<grid_small, kBlock, 0, c->stream>>>(P, k, rin, sin_, c->hits.as<rc_hit>(), s->shade,
                                                                  s->mats, rout, sout, c->srays.as<rc_ray>(),
                                                                  c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
-                                                                 c->counters, cout);
+                                                                 c->counters, cout, CI);
             if (P.has_sun && k + 1 < cfg->max_path_length) {
                 k_reset_ovf<<<1, 1, 0, c->stream>>>(c->counters);
                 RC_TRY(c->prof_begin(&ev));
@@ -573,6 +651,15 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
             }
             std::swap(rin, rout);
             std::swap(sin_, sout);
+        }
+        if (CI.cache_vertex) {
+            // paths that stopped at the cache vertex: add thr * cache(x, w_o)
+            RC_TRY(c->prof_begin(&ev));
+            RC_TRY(rc_cache_apply_device(c, cache, CI.queries, &CI.counters[0], CI.cap, c->slotL.as<float4>()));
+            RC_TRY(c->prof_end(ev, PK_CACHE));
+            k_cache_targets<<<grid_small, kBlock, 0, c->stream>>>(CI.records, CI.counters, CI.cap,
+                                                                  c->slotL.as<float4>(), CI.batch_q, CI.batch_t,
+                                                                  CI.batch_cap);
         }
         RC_TRY(c->prof_begin(&ev));
         k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(npix, this_pass, c->slotL.as<float4>(),
@@ -604,20 +691,36 @@ extern "C" int rc_render(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_came
     return rc;
 }
 
-extern "C" int rc_render_progress(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
-                                  const rc_render_cfg* cfg, rc_progress_fn progress, void* user) {
-    RC_TRY(rc_check_scene(c, s, "rc_render"));
-    RC_TRY(rc_validate_cfg(cfg, "rc_render"));
-    if (!f || f->ctx != c) return rc_host_fail(RC_ERR_INVALID, "rc_render: bad film");
-    if (!cam) return rc_host_fail(RC_ERR_INVALID, "rc_render: null camera");
+namespace {
+int check_frame_args(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                     const char* fn) {
+    RC_TRY(rc_check_scene(c, s, fn));
+    RC_TRY(rc_validate_cfg(cfg, fn));
+    if (!f || f->ctx != c) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": bad film");
+    if (!cam) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": null camera");
     if (f->width != cfg->width || f->height != cfg->height)
-        return rc_host_fail(RC_ERR_INVALID, "rc_render: film size differs from the config");
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": film size differs from the config");
+    return RC_OK;
+}
+
+// One frame, synchronously, with the statistics of rc_get_stats.
+int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                 rc_progress_fn progress, void* user, rc_cache* cache) {
     RC_TRY_HIP(hipSetDevice(c->device));
     auto t0 = std::chrono::steady_clock::now();
     c->stats.extend_ms = c->stats.shadow_ms = c->stats.shade_ms = c->stats.other_ms = 0.0;
+    c->stats.cache_ms = c->stats.train_ms = 0.0;
     c->stats.extend_launches = c->stats.shadow_launches = 0;
+    c->stats.cache_queries = c->stats.train_samples = 0;
     k_zero_counters<<<1, 1, 0, c->stream>>>(c->counters);
-    int rc = rc_enqueue_frame(c, s, f, cam, cfg, c->tuning.spp_per_pass, progress, user);
+    CacheIO io{};
+    if (cache) {
+        const uint32_t spp_pass = std::max(1u, std::min(c->tuning.spp_per_pass, cfg->spp));
+        RC_TRY(rc_cache_frame_io(c, cache, cfg->cache_vertex, size_t(cfg->width) * cfg->height * spp_pass,
+                                 uint64_t(cfg->width) * cfg->height * cfg->spp, &io));
+    }
+    int rc = rc_enqueue_frame(c, s, f, cam, cfg, c->tuning.spp_per_pass, progress, user, cache ? &io : nullptr,
+                              cache);
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     if (rc != RC_OK) return rc;
     DevCounters h;
@@ -632,6 +735,32 @@ extern "C" int rc_render_progress(rc_ctx* c, const rc_scene* s, rc_film* f, cons
     c->stats.tri_tests = h.tri_tests;
     c->stats.overflow_rays = h.ovf_count;
     if (progress) progress(user, uint64_t(cfg->width) * cfg->height);
+    return RC_OK;
+}
+}  // namespace
+
+extern "C" int rc_render_progress(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
+                                  const rc_render_cfg* cfg, rc_progress_fn progress, void* user) {
+    RC_TRY(check_frame_args(c, s, f, cam, cfg, "rc_render"));
+    return render_frame(c, s, f, cam, cfg, progress, user, nullptr);
+}
+
+extern "C" int rc_render_cached(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
+                                const rc_render_cfg* cfg, rc_cache* cache, int train, float* loss) {
+    RC_TRY(check_frame_args(c, s, f, cam, cfg, "rc_render_cached"));
+    if (!cache || rc_cache_owner(cache) != c) return rc_host_fail(RC_ERR_INVALID, "rc_render_cached: bad cache");
+    if (cfg->cache_vertex == 0 || cfg->cache_vertex + 1 >= cfg->max_path_length)
+        return rc_host_fail(RC_ERR_INVALID,
+                            "rc_render_cached: cache_vertex must satisfy 1 <= cache_vertex < max_path_length - 1");
+    RC_TRY(render_frame(c, s, f, cam, cfg, nullptr, nullptr, cache));
+    float l = 0.0f;
+    uint32_t n = 0, q = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    RC_TRY(rc_cache_train_frame(c, cache, train != 0, &l, &n, &q));
+    if (train) c->stats.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->stats.train_samples = train ? n : 0;
+    c->stats.cache_queries = q;
+    if (loss) *loss = l;
     return RC_OK;
 }
 

@@ -1,0 +1,203 @@
+"""Radiance cache on the GPU: BF16 MFMA inference and training against the
+FP64 oracle and the NumPy model of the kernels' BF16 roundings, and the
+cached path tracer.
+
+Tolerances: the GPU and the BF16 model round the same values to bfloat16 and
+differ only by FP32 vs FP64 accumulation (a few rounding flips), so they are
+compared tightly; against the FP64 oracle the BF16 error itself is allowed
+(output log-radiance within 0.03, gradient blocks within 6 % relative L2 —
+the measured BF16 model error is 0.011 and 3.2 %).
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from raycer_amd import api
+from tests import cache_ref as cr
+from tests.test_gpu_render import cfg_for, render_gpu, scene
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = np.array([-1.0, 0.0, -2.0]), np.array([3.0, 1.5, 2.0])
+
+
+def make_cache(ctx, **kw):
+    return api.RadianceCache(ctx, LO, HI, **kw)
+
+
+@pytest.mark.parametrize("n", [1, 15, 4093])
+def test_infer_matches_bf16_model_and_oracle(ctx, n):
+    p = cr.random_params(5)
+    q = cr.random_queries(n, LO, HI, 1)
+    cache = make_cache(ctx)
+    try:
+        cache.set_params(p)
+        np.testing.assert_array_equal(cache.params(), p)
+        z = np.log(cache.infer(q).astype(np.float64))
+    finally:
+        cache.close()
+    enc = cr.encode(q, LO, HI)
+    zb = cr.forward_bf16(p, enc)
+    z64 = np.log(orc.mlp_forward(p.astype(np.float64), enc, 81, 64, 4, 3))
+    d_model, d_oracle = np.abs(z - zb), np.abs(z - z64)
+    print(f"n={n}: |z - bf16 model| max {d_model.max():.2e} mean {d_model.mean():.2e}; "
+          f"|z - fp64 oracle| max {d_oracle.max():.2e} median {np.median(d_oracle):.2e}")
+    assert d_model.max() < 5e-3 and d_model.mean() < 2e-4
+    assert d_oracle.max() < 0.03 and np.median(d_oracle) < 0.005
+
+
+def test_default_initialisation_starts_near_half(ctx):
+    cache = make_cache(ctx, seed=7)
+    try:
+        y = cache.infer(cr.random_queries(1000, LO, HI, 3))
+        p = cache.params()
+    finally:
+        cache.close()
+    assert p.shape == (cr.N_PARAMS,) and np.isfinite(p).all()
+    np.testing.assert_allclose(cr.bias(p, 4), np.log(0.5), rtol=1e-6)
+    assert np.abs(cr.weights(p, 4)).max() <= 0.01
+    assert ((y > 0.35) & (y < 0.7)).all()
+
+
+def test_gradient_matches_bf16_model_and_oracle(ctx):
+    p = cr.random_params(5)
+    n = 4093
+    q = cr.random_queries(n, LO, HI, 1)
+    t = (np.exp(np.random.default_rng(9).normal(size=(n, 3))) * 0.5).astype(np.float32)
+    cache = make_cache(ctx)
+    try:
+        cache.set_params(p)
+        g, loss = cache.gradient(q, t)
+        g2, _ = cache.gradient(q, t)
+    finally:
+        cache.close()
+    np.testing.assert_array_equal(g, g2)  # fixed-order reductions
+    enc = cr.encode(q, LO, HI)
+    lb, gb = cr.loss_grad_bf16(p, enc, t)
+    l64, g64 = orc.mlp_loss_grad(p.astype(np.float64), enc, t.astype(np.float64), 81, 64, 4, 3, rel=True, eps=1e-2)
+    assert abs(loss - lb) < 2e-3 * lb and abs(loss - l64) < 1e-2 * l64
+    for l in range(5):
+        for name, s in (("W", slice(cr.OFF_W[l], cr.OFF_B[l])), ("b", slice(cr.OFF_B[l], cr.OFF_B[l] + cr.FAN_OUT[l]))):
+            rm = np.linalg.norm(g[s] - gb[s]) / np.linalg.norm(gb[s])
+            ro = np.linalg.norm(g[s] - g64[s]) / np.linalg.norm(g64[s])
+            cos = float(g[s] @ g64[s] / (np.linalg.norm(g[s]) * np.linalg.norm(g64[s])))
+            print(f"layer {l} {name}: rel L2 vs bf16 model {rm:.2e}, vs fp64 {ro:.2e}, cos {cos:.6f}")
+            assert rm < 1e-2, (l, name, rm)
+            assert ro < 0.06 and cos > 0.998, (l, name, ro, cos)
+
+
+def test_adam_steps_match_oracle(ctx):
+    p = cr.random_params(6)
+    q = cr.random_queries(2048, LO, HI, 4)
+    t = cr.smooth_target(q, LO, HI)
+    cache = make_cache(ctx, lr=1e-3)
+    p64 = p.astype(np.float64)
+    m = np.zeros_like(p64)
+    v = np.zeros_like(p64)
+    try:
+        cache.set_params(p)
+        for step in range(1, 4):
+            g, _ = cache.gradient(q, t)
+            cache.train_step(q, t)
+            p64, m, v = orc.adam_step(p64, m, v, g.astype(np.float64), lr=1e-3, b1=0.9, b2=0.99, eps=1e-8, t=step)
+            got = cache.params()
+            err = np.abs(got - p64).max()
+            print(f"adam step {step}: max |p_gpu - p_oracle| {err:.2e}")
+            assert err < 2e-6
+    finally:
+        cache.close()
+
+
+def test_training_fits_a_smooth_target(ctx):
+    cache = make_cache(ctx, seed=11, lr=1e-3)
+    qv = cr.random_queries(4096, LO, HI, 99)
+    tv = cr.smooth_target(qv, LO, HI)
+    try:
+        _, l0 = cache.gradient(qv, tv)
+        for step in range(300):
+            q = cr.random_queries(8192, LO, HI, 1000 + step)
+            cache.train_step(q, cr.smooth_target(q, LO, HI))
+        _, l1 = cache.gradient(qv, tv)
+    finally:
+        cache.close()
+    print(f"validation loss {l0:.4f} -> {l1:.4f} after 300 Adam steps")
+    assert l1 < 0.1 * l0
+
+
+def terrain_cfg(sd, w=96, h=96, spp=16, **kw):
+    return cfg_for(sd, w, h, spp, **kw)
+
+
+def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx):
+    sd, s, _ = scene(ctx, "terrain")
+    b0, b1 = np.asarray(sd.desc.bounds_min), np.asarray(sd.desc.bounds_max)
+    cache = api.RadianceCache(ctx, b0, b1, train_stride=0xFFFFFFFF)
+    p = cache.params()
+    p[cr.OFF_W[4]:cr.OFF_B[4]] = 0.0
+    p[cr.OFF_B[4]:] = -200.0  # exp underflows to exactly 0 in FP32
+    cache.set_params(p)
+    cv = 2
+    cfg = terrain_cfg(sd, cache_vertex=cv)
+    film = ctx.film(cfg.width, cfg.height)
+    try:
+        cache.render(s, film, sd.desc.camera, cfg, train=False)
+        st = ctx.stats()
+        cached = film.sums()
+    finally:
+        film.close()
+        cache.close()
+    plain_cfg = terrain_cfg(sd, max_path_length=cv + 1)
+    _, plain = render_gpu(ctx, s, sd.desc.camera, plain_cfg)
+    print(f"cache queries {st.cache_queries}, training samples {st.train_samples}")
+    assert st.cache_queries > 0 and st.train_samples == 0
+    np.testing.assert_array_equal(cached, plain)
+
+
+def test_cached_render_rejects_bad_cache_vertex(ctx):
+    sd, s, _ = scene(ctx, "terrain")
+    cache = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max)
+    film = ctx.film(32, 32)
+    try:
+        for cv in (0, 5, 6):  # max_path_length 6: needs 1 <= cv < 5
+            cfg = terrain_cfg(sd, 32, 32, 4, cache_vertex=cv)
+            with pytest.raises(api.RaycerError):
+                cache.render(s, film, sd.desc.camera, cfg)
+    finally:
+        film.close()
+        cache.close()
+
+
+def test_online_training_converges_to_the_path_traced_image(ctx):
+    """Paths end into the cache after 2 bounces; the cache trains one step per
+    frame on the suffix radiance of the training paths.  After training, the
+    cached image must agree with a converged plain render."""
+    sd, s, _ = scene(ctx, "terrain")
+    cache = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, seed=3, max_batch=16384)
+    cfg = terrain_cfg(sd, cache_vertex=2)
+    film = ctx.film(cfg.width, cfg.height)
+    losses = []
+    try:
+        for f in range(200):
+            cfg.frame = f
+            film.clear()
+            losses.append(cache.render(s, film, sd.desc.camera, cfg, train=True))
+            if f == 0:
+                st = ctx.stats()
+                assert st.cache_queries > 0 and 0 < st.train_samples <= 16384
+        film.clear()
+        ev = terrain_cfg(sd, spp=64, cache_vertex=2)
+        ev.frame = 5000
+        cache.render(s, film, sd.desc.camera, ev, train=False)
+        cached = film.linear()[:, :3].astype(np.float64)
+    finally:
+        film.close()
+        cache.close()
+    ref_cfg = terrain_cfg(sd, spp=1024)
+    ref, _ = render_gpu(ctx, s, sd.desc.camera, ref_cfg)
+    ref = ref[:, :3].astype(np.float64)
+    early, late = np.mean(losses[1:11]), np.mean(losses[-10:])
+    ratio = cached.mean() / ref.mean()
+    print(f"training loss {early:.4f} -> {late:.4f}; cached/reference image mean {ratio:.4f}")
+    assert np.isfinite(cached).all()
+    assert late < early
+    assert abs(ratio - 1.0) < 0.05
