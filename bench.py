@@ -6,7 +6,9 @@ per path with Russian roulette from vertex 3, sky radiance 0.5 and a
 directional sun with next-event estimation.  One step = one full frame
 (132,710,400 camera samples) through the wavefront path tracer; the scene is
 built and uploaded before timing, so the timed region holds only GPU work on
-device-resident inputs.
+device-resident inputs.  The BVH is the library's 32-bin SAH build (the
+reference's 6-candidate SAH build gives the same hits at ~13% lower
+throughput).
 
 Prints one JSON line.  Besides the headline Msamples/s it reports the
 roofline of the dominant kernel, k_extend_shade (closest-hit traversal fused
@@ -70,6 +72,109 @@ def cpu_baseline(sd, cam, cfg, frac: int = 2, seed: int = 7) -> dict:
     }
 
 
+def mse_pair(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+
+
+def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, cached_spp: int = 1,
+                   plain_spp: int = 64, ref_spp: int = 4096, warmup_frames: int = 16,
+                   pretrain_frames: int = 256) -> dict:
+    """Plain and radiance-cache renders given the same wall time, against a
+    ref_spp plain render.
+
+    The budget is the time of one plain frame of plain_spp samples.  Cold: a
+    new cache spends the budget on warmup_frames training frames (discarded)
+    and then on as many training frames of cached_spp samples as fit,
+    accumulated.  Warm: the same cache after pretrain_frames more untimed
+    training frames (a cache carried over from earlier frames, as in an
+    animation) renders and trains within the budget.  Paths end into the cache
+    at vertex cache_vertex (after two bounces for 2).  MSE is over linear RGB
+    and over the Reinhard-mapped RGB (key 0.18, gamma 2.2)."""
+    from raycer_amd import api
+
+    cfg = api.RenderCfg.from_buffer_copy(base_cfg)
+    cfg.width, cfg.height = base_cfg.width // scale, base_cfg.height // scale
+    cam = sd.desc.camera
+    film = ctx.film(cfg.width, cfg.height)
+
+    def image():
+        return film.linear()[:, :3].astype(np.float64), film.tonemap()[0][:, :3].astype(np.float64)
+
+    def score(lin, tm, ref_lin, ref_tm):
+        return {"mse_linear": mse_pair(lin, ref_lin), "mse_tonemapped": mse_pair(tm, ref_tm),
+                "mean_ratio": round(float(lin.mean() / ref_lin.mean()), 5)}
+
+    out = {"resolution": f"{cfg.width}x{cfg.height}", "reference_spp": ref_spp, "plain_spp": plain_spp,
+           "cache_vertex": cache_vertex}
+    try:
+        # reference: ref_spp / plain_spp frames of plain_spp samples
+        cfg.spp = plain_spp
+        film.clear()
+        for f in range(ref_spp // plain_spp):
+            cfg.frame = 100000 + f
+            ctx.render(scene, film, cam, cfg)
+        ref_lin, ref_tm = image()
+        # plain at the budget
+        film.clear()
+        cfg.frame = 7
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.render(scene, film, cam, cfg)
+        ctx.synchronize()
+        budget = time.perf_counter() - t0
+        out["budget_ms"] = round(budget * 1e3, 2)
+        out["plain"] = score(*image(), ref_lin, ref_tm)
+        cache = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, lr=1e-3, seed=1, max_batch=65536)
+        try:
+            ccfg = api.RenderCfg.from_buffer_copy(cfg)
+            ccfg.spp = cached_spp
+            ccfg.cache_vertex = cache_vertex
+            frame = [200000]
+
+            def run(warmup):
+                frames = accumulated = 0
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                film.clear()
+                while True:
+                    if frames == warmup:
+                        film.clear()
+                    ccfg.frame = frame[0]
+                    frame[0] += 1
+                    loss = cache.render(scene, film, cam, ccfg, train=True)
+                    frames += 1
+                    if frames > warmup:
+                        accumulated += 1
+                    elapsed = time.perf_counter() - t0
+                    if frames > warmup and elapsed * (frames + 1) / frames > budget:
+                        break
+                ctx.synchronize()
+                st = ctx.stats()
+                r = score(*image(), ref_lin, ref_tm)
+                r.update({"spp": accumulated * cached_spp, "ms": round((time.perf_counter() - t0) * 1e3, 2),
+                          "train_steps": frames, "train_samples_last_frame": int(st.train_samples),
+                          "cache_queries_last_frame": int(st.cache_queries), "last_loss": round(loss, 5)})
+                return r
+
+            out["cached_cold"] = run(warmup_frames)
+            out["cached_cold"]["warmup_frames"] = warmup_frames
+            for f in range(pretrain_frames):
+                ccfg.frame = frame[0]
+                frame[0] += 1
+                film.clear()
+                cache.render(scene, film, cam, ccfg, train=True)
+            out["cached_warm"] = run(0)
+            out["cached_warm"]["pretrain_frames"] = pretrain_frames
+        finally:
+            cache.close()
+    finally:
+        film.close()
+    for k in ("cached_cold", "cached_warm"):
+        out[k]["ratio_linear"] = round(out[k]["mse_linear"] / out["plain"]["mse_linear"], 4)
+        out[k]["ratio_tonemapped"] = round(out[k]["mse_tonemapped"] / out["plain"]["mse_tonemapped"], 4)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,9 +183,13 @@ def main() -> int:
     ap.add_argument("--grid", type=int, default=1024, help="terrain vertices per side (1024 = 2,093,058 triangles)")
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mse", choices=("full", "small", "off"), default="full",
+                    help="equal-time MSE of the radiance-cache renderer: at the bench resolution (full), "
+                         "at a quarter of it per side (small), or not at all")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU sharding is described in DESIGN.md",
+        print(f"bench.py measures one GPU only (asked for {a.gpus}): frames shard by pixel rows across GPUs "
+              f"with no exchange but a final film gather, so the single-GPU figure is the per-GPU figure",
               file=sys.stderr)
         return 2
 
@@ -89,8 +198,8 @@ def main() -> int:
 
     t0 = time.perf_counter()
     sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_TERRAIN, seed=1, grid=a.grid))
-    nodes, ordered, build_ms, depth = ra.build_bvh(sd.vertices)
-    log(f"terrain {sd.ntri} triangles, BVH {len(nodes)} nodes depth {depth} in {build_ms:.0f} ms")
+    nodes, ordered, build_ms, depth = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    log(f"terrain {sd.ntri} triangles, binned-SAH BVH {len(nodes)} nodes depth {depth} in {build_ms:.0f} ms")
     cfg = sd.desc.cfg
     cfg.spp = a.spp
     cam = sd.desc.camera
@@ -172,6 +281,7 @@ def main() -> int:
             "spp": cfg.spp,
             "max_path_length": cfg.max_path_length,
             "estimator": "reference (2 rho |cos|)",
+            "bvh": "32-bin SAH, collapsed to 4-wide",
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -197,6 +307,10 @@ def main() -> int:
         "bvh_build_ms": round(build_ms, 1),
     }
     film.close()
+    if a.mse != "off":
+        log(f"equal-time MSE of the radiance cache ({a.mse})")
+        out["equal_time_mse"] = equal_time_mse(ctx, scene, sd, cfg, 1 if a.mse == "full" else 4)
+        log(f"equal-time MSE: {json.dumps(out['equal_time_mse'])}")
     scene.close()
     ctx.close()
     if not a.no_cpu_baseline:

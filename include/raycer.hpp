@@ -68,11 +68,13 @@ struct Triangle {
     uint64_t materialId = 0;  // index into Scene::materials
 };
 
-// BVHBuildInfo (FlatBVH.h:19-36): only the SAH path is provided.
+// BVHBuildInfo (FlatBVH.h:19-36): only the SAH path is provided; binnedSAH
+// selects this library's 32-bin SAH instead of the reference's 6-candidate one.
 struct BVHBuildInfo {
     uint64_t maxLeafSize = 5;
     bool useSAH = true;
     uint32_t threads = 0;  // 0 = all hardware threads
+    bool binnedSAH = false;
 };
 
 using FlatBVHNode = rc_flat_node;
@@ -82,7 +84,8 @@ public:
     // FlatBVH::build (FlatBVH.cpp:89-204) over the triangles' boxes.
     void build(const std::vector<float>& vertices, const BVHBuildInfo& info) {
         if (!info.useSAH) throw RaycerError(RC_ERR_INVALID, "only the SAH build is provided");
-        rch_bvh_build_info bi{info.maxLeafSize, info.threads, 0};
+        rch_bvh_build_info bi{info.maxLeafSize, info.threads,
+                              uint32_t(info.binnedSAH ? RCH_BVH_BINNED : RCH_BVH_REFERENCE)};
         rch_bvh* h = nullptr;
         check(rch_bvh_build(vertices.data(), vertices.size() / 9, &bi, &h));
         flatNodes.resize(rch_bvh_node_count(h));

@@ -260,7 +260,8 @@ typedef struct rc_cache_cfg {
     float loss_eps;      /* relative L2 denominator epsilon (1e-2) */
     uint32_t seed;       /* weight initialisation */
     uint32_t max_batch;  /* largest training batch (65,536) */
-    uint32_t train_stride;  /* renderer: one training path per train_stride slots (0 = about max_batch a frame) */
+    uint32_t train_stride;  /* renderer: one training path per train_stride slots; 0 = adaptive, steering the
+                               training samples per frame towards 0.9 max_batch */
 } rc_cache_cfg;
 
 int rc_cache_create(rc_ctx* ctx, const rc_cache_cfg* cfg, const float bounds_min[3], const float bounds_max[3],

@@ -16,12 +16,18 @@
 extern "C" {
 #endif
 
-/* BVHBuildInfo (src/Raytracing/Primitives/FlatBVH.h:19-36); only the SAH path
- * (useSAH = true, regularSAHSplits = 0) is provided. */
+/* BVHBuildInfo (src/Raytracing/Primitives/FlatBVH.h:19-36); of the
+ * reference's options only the SAH path (useSAH = true, regularSAHSplits = 0)
+ * is provided.  method selects the split rule:
+ *   RCH_BVH_REFERENCE  the reference's 6-candidate SAH (node-for-node parity);
+ *   RCH_BVH_BINNED     a 32-bin SAH with a leaf-cost test (no counterpart in
+ *                      the reference), in the same flat format.
+ * Closest-hit and any-hit results do not depend on the method. */
+enum { RCH_BVH_REFERENCE = 0, RCH_BVH_BINNED = 1 };
 typedef struct rch_bvh_build_info {
     uint64_t max_leaf_size;  /* maxLeafSize, default 5 */
     uint32_t threads;        /* 0 = all hardware threads */
-    uint32_t pad;
+    uint32_t method;         /* RCH_BVH_REFERENCE (0) or RCH_BVH_BINNED */
 } rch_bvh_build_info;
 
 typedef struct rch_bvh rch_bvh;

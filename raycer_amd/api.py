@@ -16,7 +16,7 @@ from pathlib import Path
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = Path(os.environ.get("RAYCER_LIBRARY", str(PKG / "libraycer.so")))
+LIB_PATH = Path(os.environ.get("RAYCER_LIBRARY") or str(PKG / "libraycer.so"))
 
 RC_OK, RC_ERR_INVALID, RC_ERR_HIP, RC_ERR_ALLOC, RC_ERR_INTERRUPTED, RC_ERR_NO_DEVICE = range(6)
 RC_MAT_EMISSIVE, RC_MAT_NON_SHADOWING = 1, 2
@@ -24,6 +24,7 @@ RC_FILTER_BOX, RC_FILTER_TENT = 0, 1
 RC_SAMPLER_JITTERED, RC_SAMPLER_RANDOM = 0, 1
 RC_EST_REFERENCE, RC_EST_LAMBERT = 0, 1
 RCH_SCENE_BOX, RCH_SCENE_SPHERE_GROUND, RCH_SCENE_TERRAIN, RCH_SCENE_SOUP = 1, 2, 3, 4
+RCH_BVH_REFERENCE, RCH_BVH_BINNED = 0, 1
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("tri", "<i4")])
@@ -90,7 +91,7 @@ CACHE_PARAMS = 17923
 
 
 class BVHBuildInfo(C.Structure):
-    _fields_ = [("max_leaf_size", C.c_uint64), ("threads", C.c_uint32), ("pad", C.c_uint32)]
+    _fields_ = [("max_leaf_size", C.c_uint64), ("threads", C.c_uint32), ("method", C.c_uint32)]
 
 
 class SceneParams(C.Structure):
@@ -226,11 +227,12 @@ def generate_scene(p: SceneParams) -> SceneData:
     return SceneData(v, m, mats, d)
 
 
-def build_bvh(vertices: np.ndarray, max_leaf: int = 5, threads: int = 0):
-    """FlatBVH::build through rch_bvh_build; returns (nodes, ordered_ids, build_ms, depth)."""
+def build_bvh(vertices: np.ndarray, max_leaf: int = 5, threads: int = 0, method: int = 0):
+    """FlatBVH::build through rch_bvh_build (method RCH_BVH_REFERENCE, or
+    RCH_BVH_BINNED); returns (nodes, ordered_ids, build_ms, depth)."""
     v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 9)
     h = C.c_void_p()
-    info = BVHBuildInfo(max_leaf, threads, 0)
+    info = BVHBuildInfo(max_leaf, threads, method)
     check(lib().rch_bvh_build(_ptr(v), v.shape[0], C.byref(info), C.byref(h)))
     try:
         n = lib().rch_bvh_node_count(h)

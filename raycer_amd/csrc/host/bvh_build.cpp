@@ -142,6 +142,114 @@ struct Builder {
     }
 };
 
+// Binned SAH builder (RCH_BVH_BINNED): no counterpart in the reference.  Per
+// node, centroids are binned into 32 slabs along each axis with a non-zero
+// centroid extent; the split plane between slabs with the lowest surface area
+// cost C_t * A(node) + C_i * (A(L) N(L) + A(R) N(R)) wins, and a node of at
+// most max_leaf primitives becomes a leaf when C_i * N * A(node) is no worse.
+// Output format, preorder and threading are those of Builder.
+struct Binned {
+    static constexpr int kBins = 32;
+    std::vector<Prim> prims;
+    uint64_t max_leaf = 5;
+    unsigned spawn_depth = 0;
+    static constexpr uint64_t kParallelMin = 20000;
+
+    uint32_t build(uint64_t s, uint64_t e, std::vector<rc_flat_node>& out, unsigned depth) {
+        Box box, cb;
+        for (uint64_t i = s; i < e; ++i) {
+            box.add(prims[i]);
+            for (int k = 0; k < 3; ++k) {
+                cb.lo[k] = std::min(cb.lo[k], prims[i].c[k]);
+                cb.hi[k] = std::max(cb.hi[k], prims[i].c[k]);
+            }
+        }
+        rc_flat_node n;
+        for (int k = 0; k < 3; ++k) { n.aabb_min[k] = box.lo[k]; n.aabb_max[k] = box.hi[k]; }
+        n.start_offset = s;
+        n.primitive_count = e - s;
+        n.right_offset = 0;
+        size_t self = out.size();
+        out.push_back(n);
+        const uint64_t cnt = e - s;
+        if (cnt <= 1) return 1;
+
+        double best = std::numeric_limits<double>::infinity();
+        int axis = -1, split_bin = 0;
+        for (int a = 0; a < 3; ++a) {
+            double ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.0)) continue;
+            double scale = kBins / ext;
+            Box bb[kBins];
+            uint64_t bn[kBins] = {};
+            for (uint64_t i = s; i < e; ++i) {
+                int k = std::min(kBins - 1, int((prims[i].c[a] - cb.lo[a]) * scale));
+                bb[k].add(prims[i]);
+                ++bn[k];
+            }
+            double rarea[kBins];
+            uint64_t rcount[kBins];
+            Box acc;
+            uint64_t c = 0;
+            for (int k = kBins - 1; k > 0; --k) {
+                if (bn[k]) {
+                    for (int d = 0; d < 3; ++d) {
+                        acc.lo[d] = std::min(acc.lo[d], bb[k].lo[d]);
+                        acc.hi[d] = std::max(acc.hi[d], bb[k].hi[d]);
+                    }
+                }
+                c += bn[k];
+                rarea[k] = c ? acc.area() : 0.0;
+                rcount[k] = c;
+            }
+            Box lacc;
+            uint64_t lc = 0;
+            for (int k = 0; k < kBins - 1; ++k) {
+                if (bn[k]) {
+                    for (int d = 0; d < 3; ++d) {
+                        lacc.lo[d] = std::min(lacc.lo[d], bb[k].lo[d]);
+                        lacc.hi[d] = std::max(lacc.hi[d], bb[k].hi[d]);
+                    }
+                }
+                lc += bn[k];
+                if (lc == 0 || rcount[k + 1] == 0) continue;
+                double cost = lacc.area() * double(lc) + rarea[k + 1] * double(rcount[k + 1]);
+                if (cost < best) { best = cost; axis = a; split_bin = k; }
+            }
+        }
+        const double area = box.area();
+        const double split_cost = area + best;  // C_t = C_i = 1
+        if (cnt <= max_leaf && (axis < 0 || double(cnt) * area <= split_cost)) return 1;
+
+        uint64_t mid;
+        if (axis < 0) {
+            mid = s + cnt / 2;  // coincident centroids: halve the list
+        } else {
+            const double lo = cb.lo[axis], scale = kBins / (cb.hi[axis] - cb.lo[axis]);
+            auto it = std::partition(prims.begin() + s, prims.begin() + e, [&](const Prim& p) {
+                return std::min(kBins - 1, int((p.c[axis] - lo) * scale)) <= split_bin;
+            });
+            mid = uint64_t(it - prims.begin());
+            if (mid == s || mid == e) mid = s + cnt / 2;
+        }
+        uint32_t dl, dr;
+        if (depth < spawn_depth && cnt >= kParallelMin) {
+            std::vector<rc_flat_node> left, right;
+            std::thread th([&] { dl = build(s, mid, left, depth + 1); });
+            dr = build(mid, e, right, depth + 1);
+            th.join();
+            out[self].right_offset = int64_t(1 + left.size());
+            out.insert(out.end(), left.begin(), left.end());
+            out.insert(out.end(), right.begin(), right.end());
+        } else {
+            dl = build(s, mid, out, depth + 1);
+            out[self].right_offset = int64_t(out.size() - self);
+            dr = build(mid, e, out, depth + 1);
+        }
+        return 1 + std::max(dl, dr);
+    }
+};
+
 }  // namespace
 
 struct rch_bvh {
@@ -157,15 +265,20 @@ extern "C" int rch_bvh_build(const float* v, size_t ntri, const rch_bvh_build_in
     auto t0 = std::chrono::steady_clock::now();
     rch_bvh* b = new (std::nothrow) rch_bvh();
     if (!b) return rc_host_fail(RC_ERR_ALLOC, "rch_bvh_build: out of memory");
-    Builder B;
-    B.max_leaf = info && info->max_leaf_size ? info->max_leaf_size : 5;
+    const uint32_t method = info ? info->method : RCH_BVH_REFERENCE;
+    if (method != RCH_BVH_REFERENCE && method != RCH_BVH_BINNED) {
+        delete b;
+        return rc_host_fail(RC_ERR_INVALID, "rch_bvh_build: unknown method");
+    }
+    const uint64_t max_leaf = info && info->max_leaf_size ? info->max_leaf_size : 5;
     unsigned threads = info && info->threads ? info->threads : std::max(1u, std::thread::hardware_concurrency());
-    while ((1u << B.spawn_depth) < threads * 2 && B.spawn_depth < 12) ++B.spawn_depth;
-    if (threads == 1) B.spawn_depth = 0;
-    B.prims.resize(ntri);
+    unsigned spawn_depth = 0;
+    while ((1u << spawn_depth) < threads * 2 && spawn_depth < 12) ++spawn_depth;
+    if (threads == 1) spawn_depth = 0;
+    std::vector<Prim> prims(ntri);
     for (size_t i = 0; i < ntri; ++i) {
         const float* p = v + 9 * i;
-        Prim& q = B.prims[i];
+        Prim& q = prims[i];
         for (int k = 0; k < 3; ++k) {
             double a = p[k], bb = p[3 + k], c = p[6 + k];
             q.lo[k] = std::min(a, std::min(bb, c));
@@ -174,10 +287,25 @@ extern "C" int rch_bvh_build(const float* v, size_t ntri, const rch_bvh_build_in
         }
         q.id = i;
     }
-    b->nodes.reserve(2 * ntri / B.max_leaf + 16);
-    b->depth = B.build(0, ntri, b->nodes, 0);
+    b->nodes.reserve(2 * ntri / max_leaf + 16);
+    const std::vector<Prim>* done;
+    Builder B;
+    Binned N;
+    if (method == RCH_BVH_BINNED) {
+        N.prims.swap(prims);
+        N.max_leaf = max_leaf;
+        N.spawn_depth = spawn_depth;
+        b->depth = N.build(0, ntri, b->nodes, 0);
+        done = &N.prims;
+    } else {
+        B.prims.swap(prims);
+        B.max_leaf = max_leaf;
+        B.spawn_depth = spawn_depth;
+        b->depth = B.build(0, ntri, b->nodes, 0);
+        done = &B.prims;
+    }
     b->ordered.resize(ntri);
-    for (size_t i = 0; i < ntri; ++i) b->ordered[i] = B.prims[i].id;
+    for (size_t i = 0; i < ntri; ++i) b->ordered[i] = (*done)[i].id;
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = b;
     return RC_OK;

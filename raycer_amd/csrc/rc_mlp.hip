@@ -471,6 +471,7 @@ struct rc_cache {
     float* loss = nullptr;
     uint32_t step = 0;
     uint32_t* counters = nullptr;  // frame counters read by the renderer kernels
+    uint32_t auto_stride = 0;      // training-path stride when cfg.train_stride == 0 (0: not yet chosen)
     DevBuf act, dz, partial, queries, targets, out;
     DevBuf pass_q, pass_r;  // queries and training records of one render pass
 };
@@ -540,8 +541,13 @@ int rc_cache_frame_io(rc_ctx* c, rc_cache* k, uint32_t cache_vertex, size_t pass
     RC_TRY(k->queries.ensure(size_t(k->cfg.max_batch) * 12 * 4));
     RC_TRY(k->targets.ensure(size_t(k->cfg.max_batch) * 3 * 4));
     uint32_t stride = k->cfg.train_stride;
-    if (stride == 0)  // about max_batch training paths per frame
-        stride = uint32_t(std::max<uint64_t>(1, frame_samples / k->cfg.max_batch));
+    if (stride == 0) {
+        // adaptive: first frame as if every training path reached the cache
+        // vertex; later frames steer the record count to ~0.9 max_batch
+        if (k->auto_stride == 0)
+            k->auto_stride = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(frame_samples / k->cfg.max_batch, 1u << 20)));
+        stride = k->auto_stride;
+    }
     io->queries = k->pass_q.as<float4>();
     io->records = k->pass_r.as<float4>();
     io->counters = k->counters;
@@ -565,6 +571,11 @@ int rc_cache_train_frame(rc_ctx* c, rc_cache* k, bool train, float* loss, uint32
     uint32_t n = std::min(h[2], k->cfg.max_batch);
     *samples = n;
     *queries = h[3];
+    if (k->cfg.train_stride == 0 && k->auto_stride) {
+        double want = 0.9 * double(k->cfg.max_batch);
+        double next = h[2] ? double(k->auto_stride) * double(h[2]) / want : double(k->auto_stride) * 0.5;
+        k->auto_stride = uint32_t(std::max(1.0, std::min(next, double(1u << 20))));
+    }
     *loss = 0.0f;
     if (n == 0 || !train) return RC_OK;
     RC_TRY(rc_cache_grad_device(c, k, k->queries.as<float>(), k->targets.as<float>(), n));

@@ -167,37 +167,50 @@ def test_cached_render_rejects_bad_cache_vertex(ctx):
         cache.close()
 
 
+def block_means(img, w, h, b=8):
+    return img.reshape(h // b, b, w // b, b, -1).mean(axis=(1, 3))
+
+
 def test_online_training_converges_to_the_path_traced_image(ctx):
     """Paths end into the cache after 2 bounces; the cache trains one step per
     frame on the suffix radiance of the training paths.  After training, the
-    cached image must agree with a converged plain render."""
+    cached image must agree with a converged plain render in mean, and its
+    8x8-block means (noise averaged out, bias left) must be closer to the
+    reference's than with the untrained cache.  (The per-frame training loss is
+    not monotone: its targets are one-path estimates.)"""
     sd, s, _ = scene(ctx, "terrain")
     cache = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, seed=3, max_batch=16384)
     cfg = terrain_cfg(sd, cache_vertex=2)
+    ev = terrain_cfg(sd, spp=256, cache_vertex=2)
+    ev.frame = 5000
     film = ctx.film(cfg.width, cfg.height)
-    losses = []
+
+    def evaluate():
+        film.clear()
+        cache.render(s, film, sd.desc.camera, ev, train=False)
+        return film.linear()[:, :3].astype(np.float64)
+
     try:
-        for f in range(200):
+        untrained = evaluate()
+        for f in range(300):
             cfg.frame = f
             film.clear()
-            losses.append(cache.render(s, film, sd.desc.camera, cfg, train=True))
+            cache.render(s, film, sd.desc.camera, cfg, train=True)
             if f == 0:
                 st = ctx.stats()
                 assert st.cache_queries > 0 and 0 < st.train_samples <= 16384
-        film.clear()
-        ev = terrain_cfg(sd, spp=64, cache_vertex=2)
-        ev.frame = 5000
-        cache.render(s, film, sd.desc.camera, ev, train=False)
-        cached = film.linear()[:, :3].astype(np.float64)
+        trained = evaluate()
     finally:
         film.close()
         cache.close()
-    ref_cfg = terrain_cfg(sd, spp=1024)
-    ref, _ = render_gpu(ctx, s, sd.desc.camera, ref_cfg)
+    ref, _ = render_gpu(ctx, s, sd.desc.camera, terrain_cfg(sd, spp=4096))
     ref = ref[:, :3].astype(np.float64)
-    early, late = np.mean(losses[1:11]), np.mean(losses[-10:])
-    ratio = cached.mean() / ref.mean()
-    print(f"training loss {early:.4f} -> {late:.4f}; cached/reference image mean {ratio:.4f}")
-    assert np.isfinite(cached).all()
-    assert late < early
-    assert abs(ratio - 1.0) < 0.05
+    w, h = cfg.width, cfg.height
+    br = block_means(ref, w, h)
+    b0 = np.mean((block_means(untrained, w, h) - br) ** 2)
+    b1 = np.mean((block_means(trained, w, h) - br) ** 2)
+    r1 = trained.mean() / ref.mean()
+    print(f"trained image mean / reference {r1:.4f}; 8x8-block MSE untrained {b0:.3e} -> trained {b1:.3e}")
+    assert np.isfinite(trained).all()
+    assert abs(r1 - 1.0) < 0.03
+    assert b1 < b0

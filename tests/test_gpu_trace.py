@@ -59,6 +59,25 @@ def test_closest_hit_random_rays(ctx, name):
                   max_listed=0.05 if name == "box" else 0.01)
 
 
+@pytest.mark.parametrize("name", ["sphere5", "soup"])
+def test_results_do_not_depend_on_the_bvh(ctx, name):
+    """The binned-SAH BVH visits triangles in another order; with the canonical
+    tie rule (lower original id) the hits must be bit-identical."""
+    sd, s, _ = scene(ctx, name)
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    s2 = ctx.upload(sd, nodes, ordered)
+    try:
+        v = sd.vertices.reshape(-1, 3)
+        rays = random_rays(60000, 13, v.min(0), v.max(0))
+        rng = np.random.default_rng(4)
+        rays = np.concatenate([rays, camera_rays(sd.desc.camera, 160, 120, jitter=rng.random((160 * 120, 2)))])
+        a, b = ctx.trace_closest(s, rays), ctx.trace_closest(s2, rays)
+        assert np.array_equal(a, b)
+        assert np.array_equal(ctx.trace_any(s, rays), ctx.trace_any(s2, rays))
+    finally:
+        s2.close()
+
+
 def test_closest_hit_matches_brute_force_without_bvh(ctx):
     sd, s, _ = scene(ctx, "box")
     rays = random_rays(5000, 9, [0, 0, 0], [1, 1, 1])

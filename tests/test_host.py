@@ -54,6 +54,52 @@ def test_bvh_leaf_size_and_degenerate_input():
     assert n2[n2["right_offset"] == 0]["primitive_count"].max() <= 2
 
 
+def check_bvh_invariants(nodes, ordered, vertices, max_leaf):
+    """Preorder flat BVH: every primitive exactly once, children inside their
+    parent, leaves within max_leaf, right child after the left subtree."""
+    ntri = vertices.shape[0]
+    assert np.array_equal(np.sort(ordered), np.arange(ntri))
+    tb = vertices.reshape(-1, 3, 3)
+    lo, hi = tb.min(1).astype(np.float64), tb.max(1).astype(np.float64)
+    covered = np.zeros(ntri, np.int64)
+    stack = [(0, len(nodes))]
+    while stack:
+        i, end = stack.pop()
+        n = nodes[i]
+        s, c = int(n["start_offset"]), int(n["primitive_count"])
+        ids = ordered[s:s + c]
+        assert (lo[ids] >= n["aabb_min"] - 1e-12).all() and (hi[ids] <= n["aabb_max"] + 1e-12).all()
+        if n["right_offset"] == 0:
+            assert 1 <= c <= max_leaf
+            covered[ids] += 1
+            continue
+        left, right = i + 1, i + int(n["right_offset"])
+        assert left < right < end
+        for ch in (left, right):
+            assert (nodes[ch]["aabb_min"] >= n["aabb_min"]).all() and (nodes[ch]["aabb_max"] <= n["aabb_max"]).all()
+        stack += [(left, right), (right, end)]
+    assert (covered == 1).all()
+
+
+@pytest.mark.parametrize("name", list(SMALL_SCENES))
+def test_binned_sah_bvh_is_valid_and_cheaper(name):
+    sd = ra.generate_scene(SMALL_SCENES[name])
+    ref = ra.build_bvh(sd.vertices)
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    check_bvh_invariants(nodes, ordered, sd.vertices, 5)
+    check_bvh_invariants(ref[0], ref[1], sd.vertices, 5)
+
+    def sah(nd):
+        e = nd["aabb_max"] - nd["aabb_min"]
+        a = 2 * (e[:, 0] * e[:, 1] + e[:, 1] * e[:, 2] + e[:, 0] * e[:, 2])
+        leaf = nd["right_offset"] == 0
+        return (a[~leaf].sum() + (a[leaf] * nd["primitive_count"][leaf]).sum()) / a[0]
+
+    assert sah(nodes) <= sah(ref[0]) * 1.001
+    again = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED, threads=1)
+    assert np.array_equal(again[0], nodes) and np.array_equal(again[1], ordered)
+
+
 def test_bvh_rejects_empty_input():
     with pytest.raises(ra.RaycerError):
         ra.build_bvh(np.zeros((0, 9), np.float32))
