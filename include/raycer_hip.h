@@ -291,6 +291,16 @@ int rc_render_cached(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc
  * rendered into films[i] with cams[i] and cfgs[i]. */
 int rc_render_batch(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* const* films, const rc_camera* cams,
                     const rc_render_cfg* cfgs, size_t nscenes, const rc_tuning* tuning);
+/* The same with samples per pass chosen per scene (spp_per_pass[i] in [1, 64]). */
+int rc_render_batch_plan(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* const* films, const rc_camera* cams,
+                         const rc_render_cfg* cfgs, size_t nscenes, const uint32_t* spp_per_pass);
+/* Autotuner for rc_render_batch_plan: per size class (ceil(log2(pixels))) it
+ * renders one scene of the class with 1, 2, 4, ... 64 samples per pass (at
+ * most the frame's spp and 2^25 paths per pass) and keeps the fastest.  With a
+ * cache_path, choices are read from and added to that JSON file, keyed
+ * "<device>/<class>".  Writes one entry per scene into spp_per_pass. */
+int rc_autotune_batch(rc_ctx* ctx, const rc_scene* const* scenes, const rc_camera* cams, const rc_render_cfg* cfgs,
+                      size_t nscenes, const char* cache_path, uint32_t* spp_per_pass);
 
 #ifdef __cplusplus
 }

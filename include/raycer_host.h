@@ -74,6 +74,32 @@ int rch_scene_describe(const rch_scene_params* p, rch_scene_desc* desc);
 /* Writes ntri * 9 vertices, ntri material indices and nmat materials. */
 int rch_scene_generate(const rch_scene_params* p, float* vertices, uint32_t* tri_material, rc_material* materials);
 
+/* ------------------------------------------------------------ OBJ models
+ * ModelLoader::readObjFile (src/Utils/ObjModelLoader.cpp:332-482) for
+ * triangle scenes: v, f (i, i/t, i//n, i/t/n, negative indices, fan
+ * triangulation), mtllib / usemtl, and Ka / Kd / Ks / Ke / Ns from the MTL
+ * file; numbers read as StringUtils::parseDouble reads them.  Material 0 is
+ * the base material (faces before any usemtl, or with an unknown name); the
+ * MTL materials follow in file order.  Texture maps, groups and instances are
+ * not supported. */
+typedef struct rch_obj_info {  /* ModelLoaderInfo (src/Utils/ModelLoader.h:27-46) */
+    double scale[3];           /* default 1, 1, 1 */
+    double rotate[3];          /* EulerAngle pitch, yaw, roll (degrees), applied as rotateXYZ */
+    double translate[3];
+    rc_material base;          /* baseMaterial: every MTL material starts from it */
+    uint32_t ignore_materials;
+    uint32_t pad;
+} rch_obj_info;
+typedef struct rch_obj rch_obj;
+
+void rch_obj_default_info(rch_obj_info* info);
+int rch_obj_load(const char* path, const rch_obj_info* info /* NULL = defaults */, rch_obj** out);
+size_t rch_obj_triangle_count(const rch_obj* model);
+size_t rch_obj_material_count(const rch_obj* model);
+/* vertices: 9 floats per triangle; tri_material: one index per triangle */
+int rch_obj_get(const rch_obj* model, float* vertices, uint32_t* tri_material, rc_material* materials);
+int rch_obj_free(rch_obj* model);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,16 @@ class SceneParams(C.Structure):
                 ("tri_count", C.c_uint64)]
 
 
+class Material(C.Structure):
+    _fields_ = [("diffuse", C.c_float * 3), ("emittance", C.c_float * 3), ("ambient", C.c_float * 3),
+                ("specular", C.c_float * 3), ("shininess", C.c_float), ("flags", C.c_uint32), ("pad", C.c_uint32 * 2)]
+
+
+class ObjInfo(C.Structure):
+    _fields_ = [("scale", C.c_double * 3), ("rotate", C.c_double * 3), ("translate", C.c_double * 3),
+                ("base", Material), ("ignore_materials", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("ntri", C.c_size_t), ("nmat", C.c_size_t), ("camera", Camera), ("cfg", RenderCfg),
                 ("light", PointLight), ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3)]
@@ -147,6 +157,8 @@ def lib() -> C.CDLL:
             "rc_raycast_phong": [vp, vp, C.POINTER(Camera), u32, u32, u64, C.c_float, C.POINTER(PointLight), vp, vp,
                                  vp],
             "rc_render_batch": [vp, vp, vp, vp, vp, sz, C.POINTER(Tuning)],
+            "rc_render_batch_plan": [vp, vp, vp, vp, vp, sz, vp],
+            "rc_autotune_batch": [vp, vp, vp, vp, sz, C.c_char_p, vp],
             "rc_copy_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
             "rc_render_progress": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), vp, vp],
             "rc_cache_create": [vp, C.POINTER(CacheCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(vp)],
@@ -165,6 +177,11 @@ def lib() -> C.CDLL:
             "rch_bvh_free": [vp],
             "rch_scene_describe": [C.POINTER(SceneParams), C.POINTER(SceneDesc)],
             "rch_scene_generate": [C.POINTER(SceneParams), vp, vp, vp],
+            "rch_obj_load": [C.c_char_p, C.POINTER(ObjInfo), C.POINTER(vp)],
+            "rch_obj_triangle_count": [vp],
+            "rch_obj_material_count": [vp],
+            "rch_obj_get": [vp, vp, vp, vp],
+            "rch_obj_free": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -173,6 +190,10 @@ def lib() -> C.CDLL:
         L.rch_bvh_node_count.restype = C.c_size_t
         L.rch_bvh_build_ms.restype = C.c_double
         L.rch_bvh_depth.restype = C.c_uint32
+        L.rch_obj_triangle_count.restype = C.c_size_t
+        L.rch_obj_material_count.restype = C.c_size_t
+        L.rch_obj_default_info.argtypes = [C.POINTER(ObjInfo)]
+        L.rch_obj_default_info.restype = None
         _lib = L
     return _lib
 
@@ -224,6 +245,44 @@ def generate_scene(p: SceneParams) -> SceneData:
     m = np.empty(d.ntri, np.uint32)
     mats = np.zeros(d.nmat, MATERIAL_DTYPE)
     check(lib().rch_scene_generate(C.byref(p), _ptr(v), _ptr(m), _ptr(mats)))
+    return SceneData(v, m, mats, d)
+
+
+def load_obj(path, scale=(1.0, 1.0, 1.0), rotate=(0.0, 0.0, 0.0), translate=(0.0, 0.0, 0.0),
+             ignore_materials=False, base: Material | None = None):
+    """ModelLoader::readObjFile through rch_obj_load: (vertices (n, 9) float32,
+    tri_material (n,) uint32, materials MATERIAL_DTYPE with [0] = the base material)."""
+    info = ObjInfo()
+    lib().rch_obj_default_info(C.byref(info))
+    for k in range(3):
+        info.scale[k], info.rotate[k], info.translate[k] = scale[k], rotate[k], translate[k]
+    info.ignore_materials = int(ignore_materials)
+    if base is not None:
+        info.base = base
+    h = C.c_void_p()
+    check(lib().rch_obj_load(str(path).encode(), C.byref(info), C.byref(h)))
+    try:
+        n, nm = lib().rch_obj_triangle_count(h), lib().rch_obj_material_count(h)
+        v = np.empty((n, 9), np.float32)
+        m = np.empty(n, np.uint32)
+        mats = np.zeros(nm, MATERIAL_DTYPE)
+        check(lib().rch_obj_get(h, _ptr(v), _ptr(m), _ptr(mats)))
+        return v, m, mats
+    finally:
+        lib().rch_obj_free(h)
+
+
+def obj_scene(path, camera: Camera, cfg: RenderCfg, light: PointLight | None = None, **kw) -> SceneData:
+    """A SceneData for an OBJ model with the given camera, render settings and
+    point light (the form of raycer's test scenes built from models)."""
+    v, m, mats = load_obj(path, **kw)
+    d = SceneDesc()
+    d.ntri, d.nmat, d.camera, d.cfg = v.shape[0], mats.shape[0], camera, cfg
+    if light is not None:
+        d.light = light
+    p = v.reshape(-1, 3)
+    for k in range(3):
+        d.bounds_min[k], d.bounds_max[k] = float(p[:, k].min()), float(p[:, k].max())
     return SceneData(v, m, mats, d)
 
 
@@ -314,6 +373,30 @@ class Context:
     def render(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, interrupt=None,
                pixels_processed=None) -> None:
         check(lib().rc_render(self.h, scene.h, film.h, C.byref(cam), C.byref(cfg), interrupt, pixels_processed))
+
+    def _batch_arrays(self, scenes, cams, cfgs, films=None):
+        n = len(scenes)
+        sarr = (C.c_void_p * n)(*[s.h.value for s in scenes])
+        farr = (C.c_void_p * n)(*[f.h.value for f in films]) if films is not None else None
+        return n, sarr, farr, (Camera * n)(*cams), (RenderCfg * n)(*cfgs)
+
+    def render_batch(self, scenes, films, cams, cfgs, spp_per_pass=None, tuning: Tuning | None = None) -> None:
+        """All scenes in one captured graph: one samples-per-pass for all
+        (tuning, default the context's) or one per scene (spp_per_pass)."""
+        n, sarr, farr, carr, garr = self._batch_arrays(scenes, cams, cfgs, films)
+        if spp_per_pass is not None:
+            sp = np.ascontiguousarray(spp_per_pass, np.uint32)
+            check(lib().rc_render_batch_plan(self.h, sarr, farr, carr, garr, n, _ptr(sp)))
+        else:
+            check(lib().rc_render_batch(self.h, sarr, farr, carr, garr, n, C.byref(tuning) if tuning else None))
+
+    def autotune_batch(self, scenes, cams, cfgs, cache_path=None) -> np.ndarray:
+        """Samples per pass for each scene from rc_autotune_batch."""
+        n, sarr, _, carr, garr = self._batch_arrays(scenes, cams, cfgs)
+        out = np.zeros(n, np.uint32)
+        check(lib().rc_autotune_batch(self.h, sarr, carr, garr, n, str(cache_path).encode() if cache_path else None,
+                                      _ptr(out)))
+        return out
 
     def raycast_phong(self, scene: "Scene", cam: Camera, width: int, height: int, seed: int, offset: float,
                       light: PointLight):

@@ -232,8 +232,10 @@ __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, fl
     }
 }
 
+// 1: the if-if traversal (leaves postponed onto the stack), 2% faster on the
+// 2M-triangle terrain than testing leaf children at once (0).
 #ifndef RC_TRAV_IFIF
-#define RC_TRAV_IFIF 0
+#define RC_TRAV_IFIF 1
 #endif
 // Stack entries of the if-if traversal: an interior child's record index, or
 // a leaf tagged with kLeafTag, its triangle count (1..31) in bits 26..30 and

@@ -22,7 +22,7 @@ namespace {
 #define RC_TRACE_BLOCK 256
 #endif
 #ifndef RC_LDS_DEPTH
-#define RC_LDS_DEPTH 16
+#define RC_LDS_DEPTH 24
 #endif
 constexpr int kBlock = RC_TRACE_BLOCK;
 constexpr int kLdsDepth = RC_LDS_DEPTH;

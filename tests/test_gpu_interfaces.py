@@ -88,3 +88,99 @@ def test_batch_graph_equals_individual_renders(ctx):
         f.close()
     for s in scenes:
         s.close()
+
+
+def test_autotuned_batch_plan_equals_individual_renders(ctx, tmp_path):
+    """rc_autotune_batch picks samples per pass per size class; the batch
+    rendered with that plan equals one render per scene with the same passes,
+    and a second autotune call reads the cached JSON instead of re-timing."""
+    scenes, films, cams, cfgs = [], [], [], []
+    sizes = [(32, 32), (48, 40), (128, 96), (30, 34), (128, 128)]
+    for seed, (w, h) in enumerate(sizes):
+        sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_SOUP, seed=20 + seed, tri_count=3000 + 2000 * seed))
+        nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+        scenes.append(ctx.upload(sd, nodes, ordered))
+        cfg = sd.desc.cfg
+        cfg.width, cfg.height, cfg.spp = w, h, 8
+        films.append(ctx.film(w, h))
+        cams.append(sd.desc.camera)
+        cfgs.append(cfg)
+    cache = tmp_path / "tune.json"
+    plan = ctx.autotune_batch(scenes, cams, cfgs, cache)
+    text = cache.read_text()
+    entries = json.loads(text)
+    classes = {int(np.ceil(np.log2(c.width * c.height))) for c in cfgs}
+    assert len(entries) == len(classes)
+    assert all(1 <= v <= 8 and (v & (v - 1)) == 0 for v in entries.values())
+    assert ((plan >= 1) & (plan <= 8)).all()
+    # scenes of one class share the choice
+    for i, ci in enumerate(cfgs):
+        for j, cj in enumerate(cfgs):
+            if np.ceil(np.log2(ci.width * ci.height)) == np.ceil(np.log2(cj.width * cj.height)):
+                assert plan[i] == plan[j]
+    again = ctx.autotune_batch(scenes, cams, cfgs, cache)
+    assert np.array_equal(again, plan) and cache.read_text() == text
+    for f in films:
+        f.clear()
+    ctx.render_batch(scenes, films, cams, cfgs, spp_per_pass=plan)
+    batch = [f.linear() for f in films]
+    try:
+        for i in range(len(scenes)):
+            ctx.set_tuning(spp_per_pass=int(plan[i]))
+            f2 = ctx.film(cfgs[i].width, cfgs[i].height)
+            ctx.render(scenes[i], f2, cams[i], cfgs[i])
+            assert np.array_equal(f2.linear(), batch[i]), i
+            f2.close()
+    finally:
+        ctx.set_tuning(spp_per_pass=1)
+        for f in films:
+            f.close()
+        for s in scenes:
+            s.close()
+
+
+def cornell_scene(w, h, spp):
+    """raycer's TestScene3 (src/TestScenes/TestScene3.cpp:11-47): the Cornell
+    box OBJ, camera at (0, 1, 3.5) looking down -z with the default 75 degree
+    horizontal field of view, one point light at (0, 1.9, 0)."""
+    from tests.test_obj import MESHES
+    cam = api.Camera((0.0, 1.0, 3.5), 0.0, 0.0, 0.0, 75.0)
+    cfg = api.RenderCfg()
+    cfg.width, cfg.height, cfg.spp = w, h, spp
+    cfg.sampler, cfg.filter, cfg.filter_radius = api.RC_SAMPLER_JITTERED, api.RC_FILTER_BOX, 0.5
+    cfg.max_path_length, cfg.rr_start, cfg.estimator = 6, 3, api.RC_EST_REFERENCE
+    cfg.ray_start_offset, cfg.seed = 1e-5, 3
+    light = api.PointLight((0.0, 1.9, 0.0), (1.0, 0.71, 0.24), 1.0, 3.0, 1.0, (1.0, 1.0, 1.0), 0.01)
+    return ra.obj_scene(MESHES / "cornellbox.obj", cam, cfg, light)
+
+
+def test_reference_cornell_scene_phong_and_path_tracer(ctx):
+    import oracle as orc
+    from tests.conftest import ORACLE_THREADS
+    from tests.test_gpu_render import oracle_stats, render_gpu, replay_check
+    sd = cornell_scene(64, 64, 16)
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices)
+    s = ctx.upload(sd, nodes, ordered)
+    ob = orc.BVH(sd.vertices)
+    try:
+        # the reference's default tracer for this scene: Phong + the point light
+        w, h = 160, 120
+        light = sd.desc.light
+        ids, t, rgba = ctx.raycast_phong(s, sd.desc.camera, w, h, 1, 1e-5, light)
+        lt = [*light.position, *light.color, light.intensity, light.max_distance, light.attenuation]
+        amb = [*light.ambient_color, light.ambient_intensity]
+        hit, t_o, rgb = orc.raycast_phong(ob, sd.tri_material, sd.materials, sd.desc.camera, w, h, 1, 1e-5, lt, amb,
+                                          threads=ORACLE_THREADS)
+        same = ids == hit
+        assert same.mean() > 0.995 and (hit >= 0).mean() > 0.3  # the box is open towards the camera
+        err = np.abs(rgba[same, :3] - rgb[same])
+        assert (err <= 1e-3).mean() > 0.995
+        # the path tracer lit by the emissive ceiling quad (Ke 17 12 4), replayed by the oracle
+        cfg = sd.desc.cfg
+        lin, _ = render_gpu(ctx, s, sd.desc.camera, cfg)
+        st = oracle_stats(ob, sd, sd.desc.camera, cfg)
+        frac = replay_check(lin, st)
+        print(f"cornell: phong {same.mean():.4f} ids equal; path replay {frac:.2%} pixels equal")
+        assert lin[:, :3].mean() > 0.01
+    finally:
+        s.close()
