@@ -175,6 +175,54 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
     return out
 
 
+def batch_c4(ctx, n_scenes: int = 64) -> dict:
+    """Many-scene workload: soup scenes with seeds 0..n-1 (1k-500k triangles,
+    log-uniform; 128^2 to 1024^2; 32 jittered spp; depth 5), autotuned samples
+    per pass per size class, all frames in one captured HIP graph."""
+    import tempfile
+
+    import raycer_amd as ra
+    from raycer_amd import api
+
+    scenes, films, cams, cfgs = [], [], [], []
+    tris = 0
+    t0 = time.perf_counter()
+    try:
+        for seed in range(n_scenes):
+            sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_SOUP, seed=seed))
+            nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+            scenes.append(ctx.upload(sd, nodes, ordered))
+            films.append(ctx.film(sd.desc.cfg.width, sd.desc.cfg.height))
+            cams.append(sd.desc.camera)
+            cfgs.append(sd.desc.cfg)
+            tris += sd.ntri
+        setup = time.perf_counter() - t0
+        with tempfile.TemporaryDirectory() as d:
+            t0 = time.perf_counter()
+            plan = ctx.autotune_batch(scenes, cams, cfgs, os.path.join(d, "tune.json"))
+            tune = time.perf_counter() - t0
+        for f in films:
+            f.clear()
+        ctx.render_batch(scenes, films, cams, cfgs, spp_per_pass=plan)  # warm
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.render_batch(scenes, films, cams, cfgs, spp_per_pass=plan)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        for f in films:
+            f.close()
+        for s in scenes:
+            s.close()
+    samples = sum(c.width * c.height * c.spp for c in cfgs)
+    by_res = {}
+    for c, p in zip(cfgs, plan):
+        by_res[f"{c.width}x{c.height}"] = int(p)
+    return {"scenes": n_scenes, "triangles": tris, "samples": samples, "setup_s": round(setup, 2),
+            "autotune_ms": round(tune * 1e3, 1), "render_ms": round(dt * 1e3, 1),
+            "msamples_per_s": round(samples / dt / 1e6, 1), "spp_per_pass": by_res}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +231,8 @@ def main() -> int:
     ap.add_argument("--grid", type=int, default=1024, help="terrain vertices per side (1024 = 2,093,058 triangles)")
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c4", choices=("on", "off"), default="on",
+                    help="also time the 64-scene batch (autotuned, one captured graph)")
     ap.add_argument("--mse", choices=("full", "small", "off"), default="full",
                     help="equal-time MSE of the radiance-cache renderer: at the bench resolution (full), "
                          "at a quarter of it per side (small), or not at all")
@@ -307,10 +357,25 @@ def main() -> int:
         "bvh_build_ms": round(build_ms, 1),
     }
     film.close()
+    if a.c4 == "on":
+        log("64-scene batch (autotuned, one captured graph)")
+        out["batch_64_scenes"] = batch_c4(ctx)
+        log(f"batch: {json.dumps(out['batch_64_scenes'])}")
     if a.mse != "off":
         log(f"equal-time MSE of the radiance cache ({a.mse})")
         out["equal_time_mse"] = equal_time_mse(ctx, scene, sd, cfg, 1 if a.mse == "full" else 4)
         log(f"equal-time MSE: {json.dumps(out['equal_time_mse'])}")
+        # the same comparison on the closed box (emitter-lit, up to 8 rays per
+        # path), where every path reaches the cache vertex
+        bd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_BOX))
+        bn, bo, _, _ = ra.build_bvh(bd.vertices, method=api.RCH_BVH_BINNED)
+        box = ctx.upload(bd, bn, bo)
+        try:
+            out["equal_time_mse_box"] = equal_time_mse(ctx, box, bd, bd.desc.cfg, 1 if a.mse == "full" else 2,
+                                                       cached_spp=16, warmup_frames=4, pretrain_frames=64)
+        finally:
+            box.close()
+        log(f"equal-time MSE (box): {json.dumps(out['equal_time_mse_box'])}")
     scene.close()
     ctx.close()
     if not a.no_cpu_baseline:

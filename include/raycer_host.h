@@ -94,6 +94,10 @@ typedef struct rch_obj rch_obj;
 
 void rch_obj_default_info(rch_obj_info* info);
 int rch_obj_load(const char* path, const rch_obj_info* info /* NULL = defaults */, rch_obj** out);
+/* ModelLoader::readPlyFile (src/Utils/PlyModelLoader.cpp): ascii or binary
+ * PLY with a vertex element (x, y, z) and a face element (vertex_indices
+ * lists, fanned); one material, the base material. */
+int rch_ply_load(const char* path, const rch_obj_info* info /* NULL = defaults */, rch_obj** out);
 size_t rch_obj_triangle_count(const rch_obj* model);
 size_t rch_obj_material_count(const rch_obj* model);
 /* vertices: 9 floats per triangle; tri_material: one index per triangle */

@@ -178,6 +178,7 @@ def lib() -> C.CDLL:
             "rch_scene_describe": [C.POINTER(SceneParams), C.POINTER(SceneDesc)],
             "rch_scene_generate": [C.POINTER(SceneParams), vp, vp, vp],
             "rch_obj_load": [C.c_char_p, C.POINTER(ObjInfo), C.POINTER(vp)],
+            "rch_ply_load": [C.c_char_p, C.POINTER(ObjInfo), C.POINTER(vp)],
             "rch_obj_triangle_count": [vp],
             "rch_obj_material_count": [vp],
             "rch_obj_get": [vp, vp, vp, vp],
@@ -250,7 +251,8 @@ def generate_scene(p: SceneParams) -> SceneData:
 
 def load_obj(path, scale=(1.0, 1.0, 1.0), rotate=(0.0, 0.0, 0.0), translate=(0.0, 0.0, 0.0),
              ignore_materials=False, base: Material | None = None):
-    """ModelLoader::readObjFile through rch_obj_load: (vertices (n, 9) float32,
+    """ModelLoader::readObjFile (readPlyFile for .ply) through rch_obj_load /
+    rch_ply_load: (vertices (n, 9) float32,
     tri_material (n,) uint32, materials MATERIAL_DTYPE with [0] = the base material)."""
     info = ObjInfo()
     lib().rch_obj_default_info(C.byref(info))
@@ -260,7 +262,8 @@ def load_obj(path, scale=(1.0, 1.0, 1.0), rotate=(0.0, 0.0, 0.0), translate=(0.0
     if base is not None:
         info.base = base
     h = C.c_void_p()
-    check(lib().rch_obj_load(str(path).encode(), C.byref(info), C.byref(h)))
+    loader = lib().rch_ply_load if str(path).lower().endswith(".ply") else lib().rch_obj_load
+    check(loader(str(path).encode(), C.byref(info), C.byref(h)))
     try:
         n, nm = lib().rch_obj_triangle_count(h), lib().rch_obj_material_count(h)
         v = np.empty((n, 9), np.float32)

@@ -1,4 +1,4 @@
-// Wavefront OBJ + MTL loader for triangle scenes: the behaviour of raycer's
+// Wavefront OBJ + MTL (and PLY, below) loader for triangle scenes: the behaviour of raycer's
 // ModelLoader::readObjFile (src/Utils/ObjModelLoader.cpp:332-482) with
 // processMaterialFile (:35-230) and processFace (:232-330), reduced to what the
 // triangle path tracer uses.
@@ -103,6 +103,16 @@ M4 rotation(int axis, double deg) {
     return r;
 }
 
+M4 model_transform(const rch_obj_info& info) {
+    M4 s = M4::identity(), t = M4::identity();
+    for (int k = 0; k < 3; ++k) {
+        s.m[k][k] = info.scale[k];
+        t.m[k][3] = info.translate[k];
+    }
+    // translation * rotateXYZ(pitch, yaw, roll) * scaling
+    return t * (rotation(0, info.rotate[0]) * rotation(1, info.rotate[1]) * rotation(2, info.rotate[2])) * s;
+}
+
 struct Loader {
     rch_obj_info info;
     std::vector<rc_material> materials;  // [0] = the base material
@@ -198,16 +208,7 @@ struct Loader {
         std::string dir(path);
         size_t slash = dir.find_last_of('/');
         dir = slash == std::string::npos ? std::string() : dir.substr(0, slash + 1);
-        xf = M4::identity();
-        {
-            M4 s = M4::identity(), t = M4::identity();
-            for (int k = 0; k < 3; ++k) {
-                s.m[k][k] = info.scale[k];
-                t.m[k][3] = info.translate[k];
-            }
-            // translation * rotateXYZ(pitch, yaw, roll) * scaling
-            xf = t * (rotation(0, info.rotate[0]) * rotation(1, info.rotate[1]) * rotation(2, info.rotate[2])) * s;
-        }
+        xf = model_transform(info);
         materials.assign(1, info.base);
         uint32_t cur_mat = 0;
         char buf[1024];
@@ -245,6 +246,183 @@ struct Loader {
     }
 };
 
+// ------------------------------------------------------------------ PLY
+// ModelLoader::readPlyFile (src/Utils/PlyModelLoader.cpp): header with
+// element / property lines (scalar types char..double and lists), ascii or
+// binary bodies (binary read in the host's little-endian order, as the
+// reference reads it), vertex x y z, face vertex_indices fanned like OBJ
+// faces; every triangle gets material 0.
+enum class PlyT { I8, U8, I16, U16, I32, U32, F32, F64, LIST, NONE };
+
+PlyT ply_type(const std::string& t) {
+    if (t == "char" || t == "int8") return PlyT::I8;
+    if (t == "uchar" || t == "uint8") return PlyT::U8;
+    if (t == "short" || t == "int16") return PlyT::I16;
+    if (t == "ushort" || t == "uint16") return PlyT::U16;
+    if (t == "int" || t == "int32") return PlyT::I32;
+    if (t == "uint" || t == "uint32") return PlyT::U32;
+    if (t == "float" || t == "float32") return PlyT::F32;
+    if (t == "double" || t == "float64") return PlyT::F64;
+    if (t == "list") return PlyT::LIST;
+    return PlyT::NONE;
+}
+
+struct PlyProp {
+    PlyT type = PlyT::NONE, len = PlyT::NONE, item = PlyT::NONE;
+    std::string name;
+};
+struct PlyElem {
+    std::string name;
+    uint64_t count = 0;
+    std::vector<PlyProp> props;
+};
+
+bool read_binary(std::ifstream& f, PlyT t, double& out) {
+    auto rd = [&](auto v) {
+        f.read(reinterpret_cast<char*>(&v), sizeof(v));
+        out = double(v);
+    };
+    switch (t) {
+        case PlyT::I8: rd(int8_t()); break;
+        case PlyT::U8: rd(uint8_t()); break;
+        case PlyT::I16: rd(int16_t()); break;
+        case PlyT::U16: rd(uint16_t()); break;
+        case PlyT::I32: rd(int32_t()); break;
+        case PlyT::U32: rd(uint32_t()); break;
+        case PlyT::F32: rd(float()); break;
+        case PlyT::F64: rd(double()); break;
+        default: return false;
+    }
+    return bool(f);
+}
+
+int load_ply(const char* path, const M4& xf, std::vector<float>& tris, std::vector<uint32_t>& tri_mat) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.good()) return rc_host_fail(RC_ERR_INVALID, std::string("rch_ply_load: cannot open ") + path);
+    auto bad = [&](const char* why) { return rc_host_fail(RC_ERR_INVALID, std::string("rch_ply_load: ") + why); };
+    std::string line, t;
+    if (!std::getline(f, line) || line.find("ply") != 0) return bad("not a PLY file");
+    bool ascii = true, ended = false;
+    std::vector<PlyElem> elems;
+    while (std::getline(f, line)) {
+        size_t i = 0;
+        if (!next_token(line, i, t)) continue;
+        if (t == "format") {
+            next_token(line, i, t);
+            if (t == "ascii") ascii = true;
+            else if (t == "binary_little_endian" || t == "binary_big_endian") ascii = false;
+            else return bad("unknown PLY format");
+            next_token(line, i, t);
+            if (t != "1.0") return bad("unknown PLY version");
+        } else if (t == "element") {
+            PlyElem e;
+            next_token(line, i, e.name);
+            next_token(line, i, t);
+            e.count = std::strtoull(t.c_str(), nullptr, 10);
+            elems.push_back(e);
+        } else if (t == "property") {
+            if (elems.empty()) return bad("property before element");
+            PlyProp p;
+            next_token(line, i, t);
+            p.type = ply_type(t);
+            if (p.type == PlyT::NONE) return bad("unknown PLY type");
+            if (p.type == PlyT::LIST) {
+                next_token(line, i, t);
+                p.len = ply_type(t);
+                next_token(line, i, t);
+                p.item = ply_type(t);
+                if (p.len == PlyT::NONE || p.len == PlyT::LIST || p.item == PlyT::NONE || p.item == PlyT::LIST)
+                    return bad("unknown PLY type");
+            }
+            next_token(line, i, p.name);
+            elems.back().props.push_back(p);
+        } else if (t == "end_header") {
+            ended = true;
+            break;
+        }
+    }
+    if (!ended) return bad("invalid PLY file");
+    bool has_v = false, has_f = false;
+    for (auto& e : elems) {
+        auto has = [&](const char* n) {
+            for (auto& p : e.props)
+                if (p.name == n) return true;
+            return false;
+        };
+        if (e.name == "vertex") {
+            has_v = true;
+            if (!has("x") || !has("y") || !has("z")) return bad("vertex element without x, y, z");
+        }
+        if (e.name == "face") {
+            has_f = true;
+            if (!has("vertex_indices")) return bad("face element without vertex_indices");
+        }
+    }
+    if (!has_v || !has_f) return bad("no vertex or face element");
+    std::vector<double> verts;
+    std::vector<std::vector<uint64_t>> faces;
+    for (auto& e : elems) {
+        for (uint64_t r = 0; r < e.count; ++r) {
+            size_t i = 0;
+            if (ascii && !std::getline(f, line)) return bad("truncated PLY body");
+            double xyz[3] = {0.0, 0.0, 0.0};
+            std::vector<uint64_t> idx;
+            for (auto& p : e.props) {
+                if (p.type == PlyT::LIST) {
+                    double n = 0.0;
+                    if (ascii) {
+                        next_token(line, i, t);
+                        n = double(std::strtoull(t.c_str(), nullptr, 10));
+                    } else if (!read_binary(f, p.len, n)) {
+                        return bad("truncated PLY body");
+                    }
+                    const bool keep = e.name == "face" && p.name == "vertex_indices";
+                    for (uint64_t k = 0; k < uint64_t(n); ++k) {
+                        double v = 0.0;
+                        if (ascii) {
+                            next_token(line, i, t);
+                            v = double(std::strtoull(t.c_str(), nullptr, 10));
+                        } else if (!read_binary(f, p.item, v)) {
+                            return bad("truncated PLY body");
+                        }
+                        if (keep) idx.push_back(uint64_t(v));
+                    }
+                    continue;
+                }
+                double v = 0.0;
+                if (ascii) {
+                    next_token(line, i, t);
+                    v = parse_number(t);
+                } else if (!read_binary(f, p.type, v)) {
+                    return bad("truncated PLY body");
+                }
+                if (e.name == "vertex") {
+                    if (p.name == "x") xyz[0] = v;
+                    else if (p.name == "y") xyz[1] = v;
+                    else if (p.name == "z") xyz[2] = v;
+                }
+            }
+            if (e.name == "vertex")
+                for (int row = 0; row < 3; ++row)
+                    verts.push_back(xf.m[row][0] * xyz[0] + xf.m[row][1] * xyz[1] + xf.m[row][2] * xyz[2] + xf.m[row][3]);
+            if (e.name == "face") faces.push_back(idx);
+        }
+    }
+    const uint64_t nv = verts.size() / 3;
+    for (auto& fc : faces) {
+        if (fc.size() < 3) continue;
+        for (uint64_t a : fc)
+            if (a >= nv) return bad("face index out of range");
+        for (size_t k = 2; k < fc.size(); ++k) {
+            const uint64_t c[3] = {fc[0], fc[k - 1], fc[k]};
+            for (uint64_t a : c)
+                for (int d = 0; d < 3; ++d) tris.push_back(float(verts[3 * a + d]));
+            tri_mat.push_back(0);
+        }
+    }
+    return RC_OK;
+}
+
 }  // namespace
 
 struct rch_obj {
@@ -273,6 +451,25 @@ extern "C" int rch_obj_load(const char* path, const rch_obj_info* info, rch_obj*
     o->tris.swap(L.tris);
     o->tri_mat.swap(L.tri_mat);
     o->materials.swap(L.materials);
+    *out = o;
+    return RC_OK;
+}
+
+extern "C" int rch_ply_load(const char* path, const rch_obj_info* info, rch_obj** out) {
+    if (!path || !out) return rc_host_fail(RC_ERR_INVALID, "rch_ply_load: null argument");
+    rch_obj_info in;
+    if (info) in = *info;
+    else rch_obj_default_info(&in);
+    std::vector<float> tris;
+    std::vector<uint32_t> tri_mat;
+    int rc = load_ply(path, model_transform(in), tris, tri_mat);
+    if (rc != RC_OK) return rc;
+    if (tri_mat.empty()) return rc_host_fail(RC_ERR_INVALID, std::string("rch_ply_load: no triangles in ") + path);
+    rch_obj* o = new (std::nothrow) rch_obj();
+    if (!o) return rc_host_fail(RC_ERR_ALLOC, "rch_ply_load: out of memory");
+    o->tris.swap(tris);
+    o->tri_mat.swap(tri_mat);
+    o->materials.assign(1, in.base);
     *out = o;
     return RC_OK;
 }

@@ -140,3 +140,37 @@ def test_missing_files_raise(tmp_path):
     empty.write_text("v 0 0 0\n")
     with pytest.raises(ra.RaycerError):
         ra.load_obj(empty)
+
+
+def area(v):
+    t = v.reshape(-1, 3, 3).astype(np.float64)
+    return 0.5 * np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1)
+
+
+@pytest.mark.parametrize("name", ["cube1", "cube2", "cube3"])
+def test_reference_ply_cubes(name):
+    """ModelLoaderTest's PLY half (src/Tests/ModelLoaderTest.cpp:39-49): 12
+    triangles each; cube3.ply is the binary little-endian one."""
+    v, m, mats = ra.load_obj(MESHES / f"{name}.ply")
+    assert v.shape == (12, 9) and (m == 0).all() and mats.shape == (1,)
+    # the same 2x2x2 cube as cube1.obj: surface 24, same corner set
+    assert abs(area(v).sum() - 24.0) < 1e-4
+    corners = np.unique(np.round(v.reshape(-1, 3), 4), axis=0)
+    ref = np.unique(np.round(ra.load_obj(MESHES / "cube1.obj")[0].reshape(-1, 3), 4), axis=0)
+    np.testing.assert_array_equal(corners, ref)
+
+
+def test_ply_errors(tmp_path):
+    bad = tmp_path / "b.ply"
+    bad.write_text("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+                   "element face 1\nproperty list uchar int vertex_indices\nend_header\n0 0 0\n1 0 0\n0 1 0\n3 0 1 7\n")
+    with pytest.raises(ra.RaycerError):
+        ra.load_obj(bad)  # index 7 out of range
+    good = tmp_path / "g.ply"
+    good.write_text(bad.read_text().replace("3 0 1 7", "4 0 1 2 0"))
+    v, _, _ = ra.load_obj(good, translate=(0, 0, 1))
+    assert v.shape == (2, 9)
+    np.testing.assert_array_equal(v[0], np.float32([0, 0, 1, 1, 0, 1, 0, 1, 1]))
+    (tmp_path / "n.ply").write_text("not a ply\n")
+    with pytest.raises(ra.RaycerError):
+        ra.load_obj(tmp_path / "n.ply")
