@@ -85,7 +85,7 @@ public:
     void build(const std::vector<float>& vertices, const BVHBuildInfo& info) {
         if (!info.useSAH) throw RaycerError(RC_ERR_INVALID, "only the SAH build is provided");
         rch_bvh_build_info bi{info.maxLeafSize, info.threads,
-                              uint32_t(info.binnedSAH ? RCH_BVH_BINNED : RCH_BVH_REFERENCE)};
+                              uint32_t(info.binnedSAH ? RCH_BVH_BINNED : RCH_BVH_REFERENCE), 0.0f, 0u};
         rch_bvh* h = nullptr;
         check(rch_bvh_build(vertices.data(), vertices.size() / 9, &bi, &h));
         flatNodes.resize(rch_bvh_node_count(h));
@@ -155,9 +155,22 @@ public:
     std::vector<Material> materials;
     rc_point_light pointLight{};  // used by the RAY tracer
 
-    // Scene::initialize (Scene.cpp:203-500): flattens the triangles and builds
-    // the root BVH unless one was restored.
+    // ModelLoaderInfo (src/Utils/ModelLoader.h:27-46): models listed here are
+    // read by initialize(), OBJ or PLY by file extension.
+    struct ModelLoaderInfo {
+        std::string modelFilePath;
+        Vector3 scale{1.0, 1.0, 1.0};
+        EulerAngle rotate{0.0, 0.0, 0.0};
+        Vector3 translate{0.0, 0.0, 0.0};
+        bool ignoreMaterials = false;
+    };
+    std::vector<ModelLoaderInfo> models;
+
+    // Scene::initialize (Scene.cpp:203-500): loads the models, flattens the
+    // triangles and builds the root BVH unless one was restored.
     void initialize() {
+        for (const ModelLoaderInfo& mi : models) loadModel(mi);
+        models.clear();
         vertices_.resize(primitives.triangles.size() * 9);
         materialIds_.resize(primitives.triangles.size());
         for (size_t i = 0; i < primitives.triangles.size(); ++i) {
@@ -229,11 +242,64 @@ public:
         return s;
     }
 
+    // TestScene3 (src/TestScenes/TestScene3.cpp:11-47): the Cornell box model,
+    // camera at (0, 1, 3.5) looking down -z, one point light, Phong tracer.
+    static Scene createTestScene3(const std::string& cornellBoxObjPath) {
+        Scene s;
+        s.general.tracerType = TracerType::RAY;
+        s.camera.position = Vector3{0.0, 1.0, 3.5};
+        s.camera.orientation = EulerAngle{0.0, 0.0, 0.0};
+        ModelLoaderInfo mi;
+        mi.modelFilePath = cornellBoxObjPath;
+        s.models.push_back(mi);
+        s.pointLight = rc_point_light{{0.0f, 1.9f, 0.0f}, {1.0f, 0.71f, 0.24f}, 1.0f, 3.0f, 1.0f, {1.0f, 1.0f, 1.0f}, 0.01f};
+        s.defaultWidth = 1280;
+        s.defaultHeight = 800;
+        return s;
+    }
+
     const std::vector<float>& flatVertices() const { return vertices_; }
     const std::vector<uint32_t>& materialIds() const { return materialIds_; }
     uint32_t defaultWidth = 0, defaultHeight = 0;
 
 private:
+    void loadModel(const ModelLoaderInfo& mi) {
+        rch_obj_info info;
+        rch_obj_default_info(&info);
+        info.scale[0] = mi.scale.x; info.scale[1] = mi.scale.y; info.scale[2] = mi.scale.z;
+        info.rotate[0] = mi.rotate.pitch; info.rotate[1] = mi.rotate.yaw; info.rotate[2] = mi.rotate.roll;
+        info.translate[0] = mi.translate.x; info.translate[1] = mi.translate.y; info.translate[2] = mi.translate.z;
+        info.ignore_materials = mi.ignoreMaterials ? 1u : 0u;
+        const std::string& f = mi.modelFilePath;
+        const bool ply = f.size() >= 4 && (f.compare(f.size() - 4, 4, ".ply") == 0 || f.compare(f.size() - 4, 4, ".PLY") == 0);
+        rch_obj* h = nullptr;
+        check(ply ? rch_ply_load(f.c_str(), &info, &h) : rch_obj_load(f.c_str(), &info, &h));
+        const size_t nt = rch_obj_triangle_count(h), nm = rch_obj_material_count(h);
+        std::vector<float> v(nt * 9);
+        std::vector<uint32_t> m(nt);
+        std::vector<rc_material> mats(nm);
+        int rc = rch_obj_get(h, v.data(), m.data(), mats.data());
+        rch_obj_free(h);
+        check(rc);
+        const uint64_t base = materials.size();
+        for (const rc_material& rm : mats) {
+            Material mt;
+            mt.diffuseReflectance = Color{rm.diffuse[0], rm.diffuse[1], rm.diffuse[2]};
+            mt.emittance = Color{rm.emittance[0], rm.emittance[1], rm.emittance[2]};
+            mt.ambientReflectance = Color{rm.ambient[0], rm.ambient[1], rm.ambient[2]};
+            mt.specularReflectance = Color{rm.specular[0], rm.specular[1], rm.specular[2]};
+            mt.specularShininess = rm.shininess;
+            mt.isEmissive = rm.flags & RC_MAT_EMISSIVE;
+            materials.push_back(mt);
+        }
+        for (size_t i = 0; i < nt; ++i) {
+            Triangle t;
+            for (int k = 0; k < 3; ++k) t.vertices[k] = Vector3{v[9 * i + 3 * k], v[9 * i + 3 * k + 1], v[9 * i + 3 * k + 2]};
+            t.materialId = base + m[i];
+            primitives.triangles.push_back(t);
+        }
+    }
+
     std::vector<float> vertices_;
     std::vector<uint32_t> materialIds_;
 };
@@ -418,6 +484,21 @@ public:
         if (rc == RC_ERR_INTERRUPTED) return;  // the film keeps the passes done, as Tracer::run
         check(rc);
         state.pixelsProcessed = state.pixelCount ? state.pixelCount : width_ * height_;
+    }
+
+    // Raytracer with one point light (the RAY tracer type): primary ray,
+    // ambient + Phong + hard shadow per pixel (Raytracer.cpp:312-407,
+    // 524-551).  Returns the linear RGBA image; the film is not touched.
+    Image raycastPhong(const Scene& sc, uint64_t seed = 1) {
+        if (!scene_) throw RaycerError(RC_ERR_INVALID, "HipTracer buffers are not initialized");
+        Image img = blank();
+        rc_camera cam{{float(sc.camera.position.x), float(sc.camera.position.y), float(sc.camera.position.z)},
+                      float(sc.camera.orientation.pitch), float(sc.camera.orientation.yaw),
+                      float(sc.camera.orientation.roll), float(sc.camera.fov)};
+        // pixel (x, y) at y * width + x with y = 0 at the bottom, as Image
+        check(rc_raycast_phong(ctx_, scene_, &cam, uint32_t(width_), uint32_t(height_), seed,
+                               float(sc.general.rayStartOffset), &sc.pointLight, nullptr, nullptr, img.pixels.data()));
+        return img;
     }
 
     // ReinhardToneMapper on the device, then download (CLTracer::downloadImage).

@@ -28,6 +28,8 @@ typedef struct rch_bvh_build_info {
     uint64_t max_leaf_size;  /* maxLeafSize, default 5 */
     uint32_t threads;        /* 0 = all hardware threads */
     uint32_t method;         /* RCH_BVH_REFERENCE (0) or RCH_BVH_BINNED */
+    float tri_cost;          /* RCH_BVH_BINNED: cost of a triangle test relative to a node visit (0 = 1) */
+    uint32_t pad;
 } rch_bvh_build_info;
 
 typedef struct rch_bvh rch_bvh;

@@ -91,7 +91,8 @@ CACHE_PARAMS = 17923
 
 
 class BVHBuildInfo(C.Structure):
-    _fields_ = [("max_leaf_size", C.c_uint64), ("threads", C.c_uint32), ("method", C.c_uint32)]
+    _fields_ = [("max_leaf_size", C.c_uint64), ("threads", C.c_uint32), ("method", C.c_uint32),
+                ("tri_cost", C.c_float), ("pad", C.c_uint32)]
 
 
 class SceneParams(C.Structure):
@@ -289,12 +290,12 @@ def obj_scene(path, camera: Camera, cfg: RenderCfg, light: PointLight | None = N
     return SceneData(v, m, mats, d)
 
 
-def build_bvh(vertices: np.ndarray, max_leaf: int = 5, threads: int = 0, method: int = 0):
+def build_bvh(vertices: np.ndarray, max_leaf: int = 5, threads: int = 0, method: int = 0, tri_cost: float = 0.0):
     """FlatBVH::build through rch_bvh_build (method RCH_BVH_REFERENCE, or
     RCH_BVH_BINNED); returns (nodes, ordered_ids, build_ms, depth)."""
     v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 9)
     h = C.c_void_p()
-    info = BVHBuildInfo(max_leaf, threads, method)
+    info = BVHBuildInfo(max_leaf, threads, method, tri_cost, 0)
     check(lib().rch_bvh_build(_ptr(v), v.shape[0], C.byref(info), C.byref(h)))
     try:
         n = lib().rch_bvh_node_count(h)

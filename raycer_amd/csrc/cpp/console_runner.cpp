@@ -5,9 +5,14 @@
 // SIGINT interrupts the render; the passes done are still saved, as the
 // reference saves partial_result.png (ConsoleRunner.cpp:63-65).
 //
-//   raycer_render --scene box|sphere|terrain|soup [--seed N] [--level N] [--grid N]
-//                 [--tris N] [--width W] [--height H] [--spp N] [--frames F]
-//                 [--linear out.pfm] [--tonemapped out.ppm]
+//   raycer_render --scene box|sphere|terrain|soup|cornell [--seed N] [--level N] [--grid N]
+//                 [--tris N] [--mesh cornellbox.obj] [--tracer ray|path] [--width W] [--height H]
+//                 [--spp N] [--frames F] [--linear out.pfm] [--tonemapped out.ppm]
+//
+// cornell is raycer's TestScene3 (the Cornell box OBJ, by default the copy in
+// tests/golden/meshes next to this program's package); its default tracer is
+// the RAY tracer (Phong + the point light), --tracer path renders it with the
+// path tracer, lit by the box's emissive ceiling quad.
 #include <atomic>
 #include <chrono>
 #include <csignal>
@@ -33,7 +38,7 @@ uint32_t scene_kind(const std::string& s) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::string scene_name = "box", linear_out, tm_out;
+    std::string scene_name = "box", linear_out, tm_out, mesh, tracer_name;
     uint32_t seed = 0, level = 0, grid = 0, width = 0, height = 0, spp = 0, frames = 1;
     uint64_t tris = 0;
     for (int i = 1; i < argc; ++i) {
@@ -54,6 +59,8 @@ int main(int argc, char** argv) {
             else if (a == "--frames") frames = std::stoul(next());
             else if (a == "--linear") linear_out = next();
             else if (a == "--tonemapped") tm_out = next();
+            else if (a == "--mesh") mesh = next();
+            else if (a == "--tracer") tracer_name = next();
             else {
                 std::fprintf(stderr, "unknown option %s\n", a.c_str());
                 return 2;
@@ -66,7 +73,23 @@ int main(int argc, char** argv) {
     std::signal(SIGINT, on_sigint);
     try {
         auto t0 = std::chrono::steady_clock::now();
-        raycer::Scene scene = raycer::Scene::createTestScene(scene_kind(scene_name), seed, level, grid, tris);
+        raycer::Scene scene;
+        if (scene_name == "cornell") {
+            if (mesh.empty()) {
+                std::string exe(argv[0]);
+                size_t slash = exe.find_last_of('/');
+                mesh = (slash == std::string::npos ? std::string(".") : exe.substr(0, slash)) +
+                       "/../tests/golden/meshes/cornellbox.obj";
+            }
+            scene = raycer::Scene::createTestScene3(mesh);
+            scene.general.maxPathLength = 8;
+            scene.general.russianRouletteStart = 3;
+        } else {
+            scene = raycer::Scene::createTestScene(scene_kind(scene_name), seed, level, grid, tris);
+        }
+        if (tracer_name == "ray") scene.general.tracerType = raycer::TracerType::RAY;
+        else if (tracer_name == "path") scene.general.tracerType = raycer::TracerType::PATH;
+        else if (!tracer_name.empty()) throw raycer::RaycerError(RC_ERR_INVALID, "unknown tracer " + tracer_name);
         if (spp) {
             if (scene.general.multiSamplerType == raycer::SamplerType::RANDOM) {
                 scene.general.randomSampleCount = spp;
@@ -92,6 +115,19 @@ int main(int argc, char** argv) {
         state.pixelCount = uint64_t(w) * h;
         double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
+        if (scene.general.tracerType == raycer::TracerType::RAY) {
+            auto r0 = std::chrono::steady_clock::now();
+            raycer::Image img = tracer.raycastPhong(scene);
+            double render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
+            bool ok = true;
+            if (!linear_out.empty()) ok &= img.savePFM(linear_out);
+            if (!tm_out.empty()) ok &= img.savePPM(tm_out);
+            std::printf("{\"scene\": \"%s\", \"tracer\": \"ray\", \"triangles\": %zu, \"bvh_ms\": %.1f, "
+                        "\"setup_ms\": %.1f, \"render_ms\": %.1f}\n",
+                        scene_name.c_str(), scene.primitives.triangles.size(), scene.rootBVH.bvh.buildMilliseconds,
+                        setup_ms, render_ms);
+            return ok ? 0 : 1;
+        }
         std::exception_ptr render_error = nullptr;
         auto r0 = std::chrono::steady_clock::now();
         std::thread render([&]() {

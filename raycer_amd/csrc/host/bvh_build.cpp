@@ -146,12 +146,14 @@ struct Builder {
 // node, centroids are binned into 32 slabs along each axis with a non-zero
 // centroid extent; the split plane between slabs with the lowest surface area
 // cost C_t * A(node) + C_i * (A(L) N(L) + A(R) N(R)) wins, and a node of at
-// most max_leaf primitives becomes a leaf when C_i * N * A(node) is no worse.
+// most max_leaf primitives becomes a leaf when C_i * N * A(node) is no worse
+// (C_t = 1, C_i = tri_cost).
 // Output format, preorder and threading are those of Builder.
 struct Binned {
     static constexpr int kBins = 32;
     std::vector<Prim> prims;
     uint64_t max_leaf = 5;
+    double tri_cost = 1.0;  // C_i, with C_t = 1
     unsigned spawn_depth = 0;
     static constexpr uint64_t kParallelMin = 20000;
 
@@ -218,8 +220,8 @@ struct Binned {
             }
         }
         const double area = box.area();
-        const double split_cost = area + best;  // C_t = C_i = 1
-        if (cnt <= max_leaf && (axis < 0 || double(cnt) * area <= split_cost)) return 1;
+        const double split_cost = area + tri_cost * best;
+        if (cnt <= max_leaf && (axis < 0 || tri_cost * double(cnt) * area <= split_cost)) return 1;
 
         uint64_t mid;
         if (axis < 0) {
@@ -294,6 +296,7 @@ extern "C" int rch_bvh_build(const float* v, size_t ntri, const rch_bvh_build_in
     if (method == RCH_BVH_BINNED) {
         N.prims.swap(prims);
         N.max_leaf = max_leaf;
+        N.tri_cost = info && info->tri_cost > 0.0f ? double(info->tri_cost) : 1.0;
         N.spawn_depth = spawn_depth;
         b->depth = N.build(0, ntri, b->nodes, 0);
         done = &N.prims;

@@ -184,3 +184,31 @@ def test_reference_cornell_scene_phong_and_path_tracer(ctx):
         assert lin[:, :3].mean() > 0.01
     finally:
         s.close()
+
+
+def test_console_runner_renders_the_reference_cornell_scene(ctx, tmp_path):
+    """raycer_render --scene cornell: TestScene3 through the C++ interface
+    (Scene::models -> OBJ loader -> FlatBVH::build -> HipTracer).  Its RAY
+    tracer image equals rc_raycast_phong on the same scene; the path tracer
+    renders the emitter-lit box."""
+    out = tmp_path / "cornell_ray.pfm"
+    r = subprocess.run([str(RUNNER), "--scene", "cornell", "--width", "80", "--height", "60", "--linear", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["tracer"] == "ray" and info["triangles"] == 36
+    img = read_pfm(out)
+    sd = cornell_scene(80, 60, 1)
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices)
+    s = ctx.upload(sd, nodes, ordered)
+    try:
+        _, _, rgba = ctx.raycast_phong(s, sd.desc.camera, 80, 60, 1, 1e-5, sd.desc.light)
+    finally:
+        s.close()
+    np.testing.assert_array_equal(img, rgba[:, :3])
+    pt = tmp_path / "cornell_path.pfm"
+    r = subprocess.run([str(RUNNER), "--scene", "cornell", "--tracer", "path", "--width", "64", "--height", "48",
+                        "--spp", "16", "--linear", str(pt)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lin = read_pfm(pt)
+    assert np.isfinite(lin).all() and lin.mean() > 0.0
