@@ -323,90 +323,120 @@ __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict
 }
 
 // The same traversal with leaves postponed ("if-if"): a node visit pushes its
-// hit children, leaves included, far-to-near and keeps the nearest; each loop
-// iteration then does at most one node visit and one leaf, so the lanes of a
-// wave stay in step instead of waiting while one lane tests up to four leaves.
+// hit children, leaves included, far-to-near and keeps the nearest; each step
+// then does at most one node visit and one leaf, so the lanes of a wave stay
+// in step instead of waiting while one lane tests up to four leaves.  The
+// state is explicit so a persistent kernel can interleave rays per lane.
+struct Trav {
+    float3 o, d, inv;
+    float tmin, cull;
+    double tmin64, tmax64, best;
+    TraceResult R;
+    uint32_t cur;
+    int sp;
+};
+
+__device__ __forceinline__ void trav_begin(Trav& S, float3 o, float3 d, float tmin, float tmax) {
+    S.o = o;
+    S.d = d;
+    S.inv = safe_inv(d);
+    S.tmin = fmaxf(tmin, 0.0f);
+    S.tmin64 = S.tmin;
+    S.tmax64 = tmax;
+    S.best = S.tmax64;
+    S.cull = tmax * (1.0f + kRefineWindow);
+    S.R = TraceResult{tmax, 0.0f, 0.0f, kNoHit, false};
+    S.cur = 0;  // root: interior record 0
+    S.sp = 0;
+}
+
+// One step; true when the traversal has ended (S.R holds the result,
+// S.R.overflow when the stack would have outgrown max_depth).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_step(Trav& S, const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+                                          uint32_t* stack, int stride, int max_depth, uint32_t& n_rec,
+                                          uint32_t& n_tri) {
+    const float INF = __builtin_inff();
+    if (!(S.cur & kLeafTag)) {
+        const float4* p = reinterpret_cast<const float4*>(nodes + S.cur);
+        float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6], q7 = p[7];
+        if (COUNT) n_rec += kWidth;
+        uint32_t l0 = __float_as_uint(q1.z), c0 = __float_as_uint(q1.w);
+        uint32_t l1 = __float_as_uint(q3.z), c1 = __float_as_uint(q3.w);
+        uint32_t l2 = __float_as_uint(q5.z), c2 = __float_as_uint(q5.w);
+        uint32_t l3 = __float_as_uint(q7.z), c3 = __float_as_uint(q7.w);
+        float t0 = c0 == kEmptyChild ? INF : slab(q0, q1, S.o, S.inv, S.tmin, S.cull);
+        float t1 = c1 == kEmptyChild ? INF : slab(q2, q3, S.o, S.inv, S.tmin, S.cull);
+        float t2 = c2 == kEmptyChild ? INF : slab(q4, q5, S.o, S.inv, S.tmin, S.cull);
+        float t3 = c3 == kEmptyChild ? INF : slab(q6, q7, S.o, S.inv, S.tmin, S.cull);
+        order2(t0, l0, c0, t1, l1, c1);
+        order2(t2, l2, c2, t3, l3, c3);
+        order2(t0, l0, c0, t2, l2, c2);
+        order2(t1, l1, c1, t3, l3, c3);
+        order2(t1, l1, c1, t2, l2, c2);
+        bool have = false;
+        uint32_t next = 0;
+#define RC_CHILD(t, l, c)                                                                                      \
+        if (t != INF && t <= S.cull) {                                                                         \
+            uint32_t e = l;                                                                                    \
+            bool take = true;                                                                                  \
+            if (c != 0) {                                                                                      \
+                if (leaf_fits(l, c)) {                                                                         \
+                    e = kLeafTag | (c << 26) | l;                                                              \
+                } else {                                                                                       \
+                    take = false;                                                                              \
+                    if (test_leaf<ANY, COUNT>(tris, l, c, S.o, S.d, S.tmin, S.tmin64, S.tmax64, S.best, S.cull, \
+                                              S.R, n_tri))                                                     \
+                        return true;                                                                           \
+                }                                                                                              \
+            }                                                                                                  \
+            if (take) {                                                                                        \
+                if (have) {                                                                                    \
+                    if (S.sp >= max_depth) {                                                                   \
+                        S.R.overflow = true;                                                                   \
+                        return true;                                                                           \
+                    }                                                                                          \
+                    stack[S.sp * stride] = next;                                                               \
+                    ++S.sp;                                                                                    \
+                }                                                                                              \
+                next = e;                                                                                      \
+                have = true;                                                                                   \
+            }                                                                                                  \
+        }
+        RC_CHILD(t3, l3, c3)
+        RC_CHILD(t2, l2, c2)
+        RC_CHILD(t1, l1, c1)
+        RC_CHILD(t0, l0, c0)
+#undef RC_CHILD
+        if (have) {
+            S.cur = next;
+        } else {
+            if (S.sp == 0) return true;
+            --S.sp;
+            S.cur = stack[S.sp * stride];
+        }
+    }
+    if (S.cur & kLeafTag) {
+        const uint32_t first = S.cur & ((1u << 26) - 1u), count = (S.cur >> 26) & 31u;
+        if (test_leaf<ANY, COUNT>(tris, first, count, S.o, S.d, S.tmin, S.tmin64, S.tmax64, S.best, S.cull, S.R,
+                                  n_tri))
+            return true;
+        if (S.sp == 0) return true;
+        --S.sp;
+        S.cur = stack[S.sp * stride];
+    }
+    return false;
+}
+
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ TraceResult traverse_ifif(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                      float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                      int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
-    const float INF = __builtin_inff();
-    TraceResult R{tmax, 0.0f, 0.0f, kNoHit, false};
-    tmin = fmaxf(tmin, 0.0f);
-    const double tmin64 = tmin, tmax64 = tmax;
-    double best = tmax64;
-    float cull = tmax * (1.0f + kRefineWindow);
-    float3 inv = safe_inv(d);
-    uint32_t cur = 0;  // root: interior record 0
-    int sp = 0;
-    while (true) {
-        if (!(cur & kLeafTag)) {
-            const float4* p = reinterpret_cast<const float4*>(nodes + cur);
-            float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6], q7 = p[7];
-            if (COUNT) n_rec += kWidth;
-            uint32_t l0 = __float_as_uint(q1.z), c0 = __float_as_uint(q1.w);
-            uint32_t l1 = __float_as_uint(q3.z), c1 = __float_as_uint(q3.w);
-            uint32_t l2 = __float_as_uint(q5.z), c2 = __float_as_uint(q5.w);
-            uint32_t l3 = __float_as_uint(q7.z), c3 = __float_as_uint(q7.w);
-            float t0 = c0 == kEmptyChild ? INF : slab(q0, q1, o, inv, tmin, cull);
-            float t1 = c1 == kEmptyChild ? INF : slab(q2, q3, o, inv, tmin, cull);
-            float t2 = c2 == kEmptyChild ? INF : slab(q4, q5, o, inv, tmin, cull);
-            float t3 = c3 == kEmptyChild ? INF : slab(q6, q7, o, inv, tmin, cull);
-            order2(t0, l0, c0, t1, l1, c1);
-            order2(t2, l2, c2, t3, l3, c3);
-            order2(t0, l0, c0, t2, l2, c2);
-            order2(t1, l1, c1, t3, l3, c3);
-            order2(t1, l1, c1, t2, l2, c2);
-            bool have = false;
-            uint32_t next = 0;
-#define RC_CHILD(t, l, c)                                                                                 \
-            if (t != INF && t <= cull) {                                                                  \
-                uint32_t e = l;                                                                           \
-                bool take = true;                                                                         \
-                if (c != 0) {                                                                             \
-                    if (leaf_fits(l, c)) {                                                                \
-                        e = kLeafTag | (c << 26) | l;                                                     \
-                    } else {                                                                              \
-                        take = false;                                                                     \
-                        if (test_leaf<ANY, COUNT>(tris, l, c, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) \
-                            return R;                                                                     \
-                    }                                                                                     \
-                }                                                                                         \
-                if (take) {                                                                               \
-                    if (have) {                                                                           \
-                        if (sp >= max_depth) {                                                            \
-                            R.overflow = true;                                                            \
-                            return R;                                                                     \
-                        }                                                                                 \
-                        stack[sp * stride] = next;                                                        \
-                        ++sp;                                                                             \
-                    }                                                                                     \
-                    next = e;                                                                             \
-                    have = true;                                                                          \
-                }                                                                                         \
-            }
-            RC_CHILD(t3, l3, c3)
-            RC_CHILD(t2, l2, c2)
-            RC_CHILD(t1, l1, c1)
-            RC_CHILD(t0, l0, c0)
-#undef RC_CHILD
-            if (have) {
-                cur = next;
-            } else {
-                if (sp == 0) break;
-                --sp;
-                cur = stack[sp * stride];
-            }
-        }
-        if (cur & kLeafTag) {
-            const uint32_t first = cur & ((1u << 26) - 1u), count = (cur >> 26) & 31u;
-            if (test_leaf<ANY, COUNT>(tris, first, count, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
-            if (sp == 0) break;
-            --sp;
-            cur = stack[sp * stride];
-        }
+    Trav S;
+    trav_begin(S, o, d, tmin, tmax);
+    while (!trav_step<ANY, COUNT>(S, nodes, tris, stack, stride, max_depth, n_rec, n_tri)) {
     }
-    return R;
+    return S.R;
 }
 
 template <bool ANY, bool COUNT>
