@@ -133,8 +133,8 @@ typedef struct rc_point_light {
     float ambient_intensity;
 } rc_point_light;
 
-/* Launch tuning (autotuned): threads per block of the traversal kernels, pass
- * size in samples per pixel, persistent grid size in blocks per CU. */
+/* Launch tuning: threads per block of the ray-query kernels, pass size in
+ * samples per pixel, grid limit of the ray-query kernels in blocks per CU. */
 typedef struct rc_tuning {
     uint32_t block;
     uint32_t spp_per_pass;
@@ -287,7 +287,7 @@ int rc_cache_gradient(rc_ctx* ctx, rc_cache* cache, const float* queries, const 
 int rc_render_cached(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam,
                      const rc_render_cfg* cfg, rc_cache* cache, int train, float* loss);
 
-/* Many scenes in one captured HIP graph (persistent launch): scenes[i] is
+/* Many scenes in one captured HIP graph, launched once: scenes[i] is
  * rendered into films[i] with cams[i] and cfgs[i]. */
 int rc_render_batch(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* const* films, const rc_camera* cams,
                     const rc_render_cfg* cfgs, size_t nscenes, const rc_tuning* tuning);

@@ -326,7 +326,7 @@ __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict
 // hit children, leaves included, far-to-near and keeps the nearest; each step
 // then does at most one node visit and one leaf, so the lanes of a wave stay
 // in step instead of waiting while one lane tests up to four leaves.  The
-// state is explicit so a persistent kernel can interleave rays per lane.
+// state is explicit (trav_begin / trav_step).
 struct Trav {
     float3 o, d, inv;
     float tmin, cull;

@@ -42,7 +42,6 @@ struct DevCounters {
     uint32_t ovf_count;
     uint32_t queue_count[4];   // wavefront queue sizes (ping, pong, shadow, scratch)
     uint32_t error;
-    uint32_t fetch[2];         // persistent kernels: next queue entry (extend, shadow)
 };
 
 enum ProfKind { PK_EXTEND = 0, PK_SHADOW = 1, PK_SHADE = 2, PK_OTHER = 3, PK_CACHE = 4 };
@@ -68,7 +67,6 @@ struct rc_ctx {
     std::vector<hipEvent_t> ev_free;
     std::vector<Pending> pending;
     hipGraphExec_t batch_graph = nullptr;
-    uint32_t persistent_blocks[2] = {0, 0};  // resident blocks per CU of the persistent extend / shadow kernels
 
     int prof_begin(hipEvent_t* a);
     int prof_end(hipEvent_t a, int kind);

@@ -33,15 +33,6 @@ constexpr float kPi = 3.14159265358979323846f;
 #endif
 constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the traversal kernels
 constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
-// 1: persistent extend / shadow kernels whose lanes take new queue entries
-// as others finish (refilling a wave when at most RC_REFILL of its 64 lanes
-// are still traversing); 0: one lane per queue entry, grid sized to the queue.
-#ifndef RC_PERSISTENT
-#define RC_PERSISTENT 1
-#endif
-#ifndef RC_REFILL
-#define RC_REFILL 32
-#endif
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
 #endif
@@ -355,137 +346,6 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
     }
 }
 
-// Persistent form of k_extend_shade.  Each lane holds one queue entry at a
-// time and advances its traversal one step per loop iteration.  When at most
-// RC_REFILL lanes of the wave are still traversing, the lanes whose traversal
-// ended shade together and the idle lanes take the next queue entries (one
-// atomic per wave); the wave exits when the queue is drained and no lane has
-// work.  Results per queue entry are the same as k_extend_shade's.
-#ifndef RC_MIN_WAVES_P
-#define RC_MIN_WAVES_P 4  // the persistent extend kernel holds traversal and shading state at once
-#endif
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES_P) void k_extend_shade_p(
-    PassParams P, uint32_t k, const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
-    const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
-    const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
-    rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
-    DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI) {
-    __shared__ uint32_t stack[kTraceLds * kTraceBlock];
-    const uint32_t n = ctr->queue_count[cin];
-    const uint32_t lane = __lane_id();
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
-    uint32_t n_rec = 0, n_tri = 0;
-    uint32_t* my_stack = stack + threadIdx.x;
-    Trav S;
-    float4 r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), r1 = r0, s = r0;
-    uint32_t idx = 0;
-    bool trav = false, fin = false, drained = false;
-    while (true) {
-        if (__popcll(__ballot(trav)) <= RC_REFILL) {
-            // the finished lanes shade together
-            ShadeOut O;
-            O.alive = O.shadow = false;
-            uint32_t slot = 0;
-            if (fin) {
-                slot = __float_as_uint(s.w);
-                if (S.R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = idx;
-                else O = shade_vertex(P, k, r0, r1, s, S.R.t, S.R.id == kNoHit ? -1 : int(S.R.id), shade, mats, L, CI);
-                fin = false;
-            }
-            emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout);
-            // the idle lanes take the next queue entries
-            if (!drained) {
-                const uint64_t idle = __ballot(!trav);
-                const uint32_t want = __popcll(idle);
-                const int leader = __ffsll((unsigned long long)idle) - 1;
-                uint32_t base = 0;
-                if (int(lane) == leader) base = atomicAdd(&ctr->fetch[0], want);
-                base = __shfl(base, leader, 64);
-                if (base + want >= n) drained = true;
-                if (!trav) {
-                    const uint32_t q = base + __popcll(idle & ((1ull << lane) - 1ull));
-                    if (q < n) {
-                        idx = q;
-                        const float4* rp = reinterpret_cast<const float4*>(rays + q);
-                        r0 = rp[0];
-                        r1 = rp[1];
-                        s = st[q];
-                        trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
-                        trav = true;
-                    }
-                }
-            }
-            if (__ballot(trav) == 0) break;
-        }
-        if (trav && trav_step<false, COUNT>(S, nodes, tris, my_stack, kTraceBlock, kTraceLds, n_rec, n_tri)) {
-            trav = false;
-            fin = true;
-        }
-    }
-    if (COUNT) {
-        wave_add(&ctr->node_records, n_rec);
-        wave_add(&ctr->tri_tests, n_tri);
-    }
-}
-
-// Persistent form of k_shadow (the same refill scheme; an ended any-hit query
-// is applied at once).
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow_p(const NodeRec* __restrict__ nodes,
-                                                         const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
-                                                         const float4* __restrict__ scontrib, float4* __restrict__ L,
-                                                         uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr) {
-    __shared__ uint32_t stack[kTraceLds * kTraceBlock];
-    const uint32_t n = ctr->queue_count[2];
-    const uint32_t lane = __lane_id();
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
-    uint32_t n_rec = 0, n_tri = 0;
-    uint32_t* my_stack = stack + threadIdx.x;
-    Trav S;
-    uint32_t idx = 0;
-    bool trav = false, drained = false;
-    while (true) {
-        if (__popcll(__ballot(trav)) <= RC_REFILL) {
-            if (!drained) {
-                const uint64_t idle = __ballot(!trav);
-                const uint32_t want = __popcll(idle);
-                const int leader = __ffsll((unsigned long long)idle) - 1;
-                uint32_t base = 0;
-                if (int(lane) == leader) base = atomicAdd(&ctr->fetch[1], want);
-                base = __shfl(base, leader, 64);
-                if (base + want >= n) drained = true;
-                if (!trav) {
-                    const uint32_t q = base + __popcll(idle & ((1ull << lane) - 1ull));
-                    if (q < n) {
-                        idx = q;
-                        const float4* rp = reinterpret_cast<const float4*>(srays + q);
-                        float4 a = rp[0], b = rp[1];
-                        trav_begin(S, f3(a), f3(b), a.w, b.w);
-                        trav = true;
-                    }
-                }
-            }
-            if (__ballot(trav) == 0) break;
-        }
-        if (trav && trav_step<true, COUNT>(S, nodes, tris, my_stack, kTraceBlock, kTraceLds, n_rec, n_tri)) {
-            trav = false;
-            if (S.R.overflow) {
-                ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = idx;
-            } else if (S.R.id == kNoHit) {
-                float4 c = scontrib[idx];
-                uint32_t slot = __float_as_uint(c.w);
-                float4 l = L[slot];
-                L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
-            }
-        }
-    }
-    if (COUNT) {
-        wave_add(&ctr->node_records, n_rec);
-        wave_add(&ctr->tri_tests, n_tri);
-    }
-}
-
 // Shading of the listed queue entries (rays the deep-stack kernel traced).
 __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t k, const rc_ray* __restrict__ rays,
                                                          const float4* __restrict__ st, const rc_hit* __restrict__ hits,
@@ -598,13 +458,9 @@ __global__ void k_prep(DevCounters* ctr, uint32_t cout) {
     ctr->queue_count[cout] = 0;
     ctr->queue_count[2] = 0;
     ctr->ovf_count = 0;
-    ctr->fetch[0] = 0;
 }
 
-__global__ void k_reset_ovf(DevCounters* ctr) {
-    ctr->ovf_count = 0;
-    ctr->fetch[1] = 0;
-}
+__global__ void k_reset_ovf(DevCounters* ctr) { ctr->ovf_count = 0; }
 
 __device__ __forceinline__ void kahan(float& s, float& c, float v) {
     float y = v - c;
@@ -632,20 +488,6 @@ __global__ __launch_bounds__(kBlock) void k_film_add(uint32_t npix, uint32_t spp
     }
     sum[p] = s;
     comp[p] = c;
-}
-
-// Resident blocks per CU of the persistent kernels (the smaller of the timing
-// and counting instantiations), queried once per context.
-int persistent_occupancy(rc_ctx* c) {
-    if (!RC_PERSISTENT || c->persistent_blocks[0]) return RC_OK;
-    int a = 0, b = 0, a2 = 0, b2 = 0;
-    RC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_shade_p<false>, kTraceBlock, 0));
-    RC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a2, k_extend_shade_p<true>, kTraceBlock, 0));
-    RC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow_p<false>, kTraceBlock, 0));
-    RC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_shadow_p<true>, kTraceBlock, 0));
-    c->persistent_blocks[0] = uint32_t(std::max(1, std::min(a, a2)));
-    c->persistent_blocks[1] = uint32_t(std::max(1, std::min(b, b2)));
-    return RC_OK;
 }
 
 int alloc_pass(rc_ctx* c, size_t npaths) {
@@ -710,7 +552,7 @@ int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass) {
     RC_TRY(alloc_pass(c, npaths));
     RC_TRY(c->ovf_list.ensure(npaths * sizeof(uint32_t)));
     RC_TRY(c->deep_stack.ensure(size_t(2048) * kDeepStack * sizeof(uint32_t)));
-    return persistent_occupancy(c);
+    return RC_OK;
 }
 
 // One frame of a scene into a film: the passes of rc_render, enqueued on the
@@ -746,7 +588,6 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     P.has_sun = (cfg->sun_irradiance[0] > 0.0f || cfg->sun_irradiance[1] > 0.0f || cfg->sun_irradiance[2] > 0.0f);
     P.seed = cfg->seed;
     const uint32_t grid_small = uint32_t(c->cu_count) * 2u;  // list-driven cold-path kernels
-    RC_TRY(persistent_occupancy(c));
     rc_ray* ra = c->ray_a.as<rc_ray>();
     rc_ray* rb = c->ray_b.as<rc_ray>();
     float4* sa = c->st_a.as<float4>();
@@ -777,17 +618,7 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
             uint32_t cin = k & 1u, cout = cin ^ 1u;
             k_prep<<<1, 1, 0, c->stream>>>(c->counters, cout);
             RC_TRY(c->prof_begin(&ev));
-            if (RC_PERSISTENT) {
-                const uint32_t grid_p = std::min(grid_t, uint32_t(c->cu_count) * c->persistent_blocks[0]);
-                if (c->counting)
-                    k_extend_shade_p<true><<<grid_p, kTraceBlock, 0, c->stream>>>(
-                        P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
-                        c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
-                else
-                    k_extend_shade_p<false><<<grid_p, kTraceBlock, 0, c->stream>>>(
-                        P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
-                        c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
-            } else if (c->counting) {
+            if (c->counting) {
                 k_extend_shade<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
                     P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
                     c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
@@ -806,17 +637,7 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
             if (P.has_sun && k + 1 < cfg->max_path_length) {
                 k_reset_ovf<<<1, 1, 0, c->stream>>>(c->counters);
                 RC_TRY(c->prof_begin(&ev));
-                if (RC_PERSISTENT) {
-                    const uint32_t grid_p = std::min(grid_t, uint32_t(c->cu_count) * c->persistent_blocks[1]);
-                    if (c->counting)
-                        k_shadow_p<true><<<grid_p, kTraceBlock, 0, c->stream>>>(
-                            s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(),
-                            ovf, c->counters);
-                    else
-                        k_shadow_p<false><<<grid_p, kTraceBlock, 0, c->stream>>>(
-                            s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(),
-                            ovf, c->counters);
-                } else if (c->counting) {
+                if (c->counting) {
                     k_shadow<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
                         s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
                         c->counters);
