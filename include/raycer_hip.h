@@ -282,8 +282,9 @@ int rc_cache_gradient(rc_ctx* ctx, rc_cache* cache, const float* queries, const 
  * emission adds throughput x cache(query) and stops.  Paths of one slot in
  * train_stride run to the end instead and give up to max_batch training
  * samples (target = radiance gathered after the vertex / throughput at it);
- * with train != 0 one Adam step runs on them after the frame and loss (may be
- * NULL) gets that batch's loss. */
+ * with train = k > 0, k (at most 16) Adam steps run after the frame, each on
+ * an interleaved 1/k of those samples, and loss (may be NULL) gets their mean
+ * loss. */
 int rc_render_cached(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam,
                      const rc_render_cfg* cfg, rc_cache* cache, int train, float* loss);
 

@@ -456,9 +456,10 @@ class RadianceCache:
         check(lib().rc_cache_train_step(self.ctx.h, self.h, _ptr(q), _ptr(t), q.shape[0], C.byref(loss)))
         return loss.value
 
-    def render(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, train: bool = True) -> float:
-        """One frame with paths ending into the cache at cfg.cache_vertex; with
-        train, one Adam step on the frame's training samples (returns its loss)."""
+    def render(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, train: int = 1) -> float:
+        """One frame with paths ending into the cache at cfg.cache_vertex; then
+        `train` Adam steps (0 = none) on interleaved slices of the frame's
+        training samples (returns their mean loss)."""
         loss = C.c_float()
         check(lib().rc_render_cached(self.ctx.h, scene.h, film.h, C.byref(cam), C.byref(cfg), self.h, int(train),
                                      C.byref(loss)))

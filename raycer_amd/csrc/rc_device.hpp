@@ -115,13 +115,24 @@ __device__ __forceinline__ float slab(float4 r0, float4 r1, float3 o, float3 inv
 // widened by kFilterBand so FP32 rounding never rejects a triangle the FP64
 // decision (mt_hit64, inclusive bounds) would accept.
 constexpr float kFilterBand = 1e-3f;
+#ifndef RC_FAST_RCP
+#define RC_FAST_RCP 0
+#endif
+#ifndef RC_SLAB_SELECT
+#define RC_SLAB_SELECT 0
+#endif
 __device__ __forceinline__ bool mt_hit(const TriRec& T, float3 o, float3 d, float& t) {
     float3 v0 = f3(T.v0);
     float3 e1 = f3(T.v1) - v0, e2 = f3(T.v2) - v0;
     float3 pv = cross3(d, e2);
     float det = dot3(e1, pv);
     if (det == 0.0f) return false;
+#if RC_FAST_RCP
+    // hardware reciprocal (1 ulp): the bands above absorb it, FP64 decides
+    float inv = __builtin_amdgcn_rcpf(det);
+#else
     float inv = 1.0f / det;
+#endif
     float3 tv = o - v0;
     float u = dot3(tv, pv) * inv;
     if (u < -kFilterBand || u > 1.0f + kFilterBand) return false;
@@ -224,12 +235,15 @@ __device__ __forceinline__ bool test_leaf(const TriRec* __restrict__ tris, uint3
     return false;
 }
 
+// Compare-exchange of two children by entry distance, as selects (no branch).
 __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, float& tb, uint32_t& lb, uint32_t& cb) {
-    if (tb < ta) {
-        float t = ta; ta = tb; tb = t;
-        uint32_t x = la; la = lb; lb = x;
-        x = ca; ca = cb; cb = x;
-    }
+    const bool sw = tb < ta;
+    const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
+    const uint32_t l0 = sw ? lb : la, l1 = sw ? la : lb;
+    const uint32_t c0 = sw ? cb : ca, c1 = sw ? ca : cb;
+    ta = t0; tb = t1;
+    la = l0; lb = l1;
+    ca = c0; cb = c1;
 }
 
 // 1: the if-if traversal (leaves postponed onto the stack), 2% faster on the
@@ -365,10 +379,19 @@ __device__ __forceinline__ bool trav_step(Trav& S, const NodeRec* __restrict__ n
         uint32_t l1 = __float_as_uint(q3.z), c1 = __float_as_uint(q3.w);
         uint32_t l2 = __float_as_uint(q5.z), c2 = __float_as_uint(q5.w);
         uint32_t l3 = __float_as_uint(q7.z), c3 = __float_as_uint(q7.w);
+#if RC_SLAB_SELECT
+        float t0 = slab(q0, q1, S.o, S.inv, S.tmin, S.cull), t1 = slab(q2, q3, S.o, S.inv, S.tmin, S.cull);
+        float t2 = slab(q4, q5, S.o, S.inv, S.tmin, S.cull), t3 = slab(q6, q7, S.o, S.inv, S.tmin, S.cull);
+        t0 = c0 == kEmptyChild ? INF : t0;
+        t1 = c1 == kEmptyChild ? INF : t1;
+        t2 = c2 == kEmptyChild ? INF : t2;
+        t3 = c3 == kEmptyChild ? INF : t3;
+#else
         float t0 = c0 == kEmptyChild ? INF : slab(q0, q1, S.o, S.inv, S.tmin, S.cull);
         float t1 = c1 == kEmptyChild ? INF : slab(q2, q3, S.o, S.inv, S.tmin, S.cull);
         float t2 = c2 == kEmptyChild ? INF : slab(q4, q5, S.o, S.inv, S.tmin, S.cull);
         float t3 = c3 == kEmptyChild ? INF : slab(q6, q7, S.o, S.inv, S.tmin, S.cull);
+#endif
         order2(t0, l0, c0, t1, l1, c1);
         order2(t2, l2, c2, t3, l3, c3);
         order2(t0, l0, c0, t2, l2, c2);

@@ -131,9 +131,9 @@ int rc_cache_apply_device(rc_ctx* c, rc_cache* k, const float4* queries, const u
 // of frame_samples samples, fills io and enqueues the clearing of its counters.
 int rc_cache_frame_io(rc_ctx* c, rc_cache* k, uint32_t cache_vertex, size_t pass_paths, uint64_t frame_samples,
                       CacheIO* io);
-// After a frame: the frame's query and training-sample counts and, if train,
-// one optimizer step over the training samples it gathered.
-int rc_cache_train_frame(rc_ctx* c, rc_cache* k, bool train, float* loss, uint32_t* samples, uint32_t* queries);
+// After a frame: the frame's query and training-sample counts and `steps`
+// optimizer steps over interleaved slices of the training samples it gathered.
+int rc_cache_train_frame(rc_ctx* c, rc_cache* k, uint32_t steps, float* loss, uint32_t* samples, uint32_t* queries);
 rc_ctx* rc_cache_owner(const rc_cache* k);
 int rc_check_ctx(const rc_ctx* ctx, const char* fn);
 int rc_check_scene(const rc_ctx* ctx, const rc_scene* s, const char* fn);

@@ -234,7 +234,7 @@ __device__ __forceinline__ size_t quad_index(uint32_t s, int f, int F) {
 __global__ __launch_bounds__(kThreads) void k_cache_fwd_bwd(const v4s* __restrict__ wt_g, const v4s* __restrict__ wtb_g,
                                                             const float* __restrict__ params, Bounds B,
                                                             const float* __restrict__ queries,
-                                                            const float* __restrict__ targets, uint32_t n,
+                                                            const float* __restrict__ targets, uint32_t n, uint32_t stride, uint32_t offset,
                                                             uint32_t n_pad, float loss_eps, short* __restrict__ act,
                                                             short* __restrict__ dz, float* __restrict__ loss_out) {
     __shared__ v4s wt[kFwdTiles * 64];
@@ -252,7 +252,9 @@ __global__ __launch_bounds__(kThreads) void k_cache_fwd_bwd(const v4s* __restric
         const bool valid = s < n;
         float q[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) q[k] = valid ? queries[size_t(s) * 12 + k] : 0.0f;
+        const size_t row = size_t(s) * stride + offset;  // sample s of the slice
+#pragma unroll
+        for (int k = 0; k < 12; ++k) q[k] = valid ? queries[row * 12 + k] : 0.0f;
         float p01[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) p01[a] = (q[a] - B.lo[a]) * B.inv[a];
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void k_cache_fwd_bwd(const v4s* __restric
 #pragma unroll
             for (int e = 0; e < kOut; ++e) {
                 float y = expf(z[e]);
-                float r = y - targets[size_t(s) * 3 + e];
+                float r = y - targets[row * 3 + e];
                 float den = y * y + loss_eps;
                 loss += r * r / den * norm;
                 d4[e] = 2.0f * r / den * norm * y;
@@ -505,7 +507,8 @@ int rc_cache_apply_device(rc_ctx* c, rc_cache* k, const float4* queries, const u
 }
 
 // Gradient of the loss over n samples already on the device (n known on the host).
-int rc_cache_grad_device(rc_ctx* c, rc_cache* k, const float* d_queries, const float* d_targets, uint32_t n) {
+int rc_cache_grad_device(rc_ctx* c, rc_cache* k, const float* d_queries, const float* d_targets, uint32_t n,
+                         uint32_t stride = 1, uint32_t offset = 0) {
     uint32_t n_pad = (n + 15) / 16 * 16;
     RC_TRY(k->act.ensure(size_t(n_pad) * kActF * sizeof(short)));
     RC_TRY(k->dz.ensure(size_t(n_pad) * kDzF * sizeof(short)));
@@ -513,7 +516,8 @@ int rc_cache_grad_device(rc_ctx* c, rc_cache* k, const float* d_queries, const f
     float* partial_b = k->partial.as<float>() + size_t(kSplits) * kWgTiles * 256;
     k_zero_f<<<1, kThreads, 0, c->stream>>>(k->loss, 1);
     k_cache_fwd_bwd<<<waves_grid(n_pad), kThreads, 0, c->stream>>>(
-        k->fwd, k->bwd, k->params, k->bounds, d_queries, d_targets, n, n_pad, k->cfg.loss_eps, k->act.as<short>(),
+        k->fwd, k->bwd, k->params, k->bounds, d_queries, d_targets, n, stride, offset, n_pad, k->cfg.loss_eps,
+        k->act.as<short>(),
         k->dz.as<short>(), k->loss);
     const uint32_t wg_waves = kWgTiles * kSplits;
     k_cache_wgrad<<<(wg_waves + 3) / 4, kThreads, 0, c->stream>>>(k->act.as<short>(), k->dz.as<short>(), n_pad,
@@ -564,7 +568,7 @@ int rc_cache_frame_io(rc_ctx* c, rc_cache* k, uint32_t cache_vertex, size_t pass
 
 rc_ctx* rc_cache_owner(const rc_cache* k) { return k->ctx; }
 
-int rc_cache_train_frame(rc_ctx* c, rc_cache* k, bool train, float* loss, uint32_t* samples, uint32_t* queries) {
+int rc_cache_train_frame(rc_ctx* c, rc_cache* k, uint32_t steps, float* loss, uint32_t* samples, uint32_t* queries) {
     uint32_t h[4] = {0, 0, 0, 0};
     RC_TRY_HIP(hipMemcpyAsync(h, k->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
@@ -577,11 +581,20 @@ int rc_cache_train_frame(rc_ctx* c, rc_cache* k, bool train, float* loss, uint32
         k->auto_stride = uint32_t(std::max(1.0, std::min(next, double(1u << 20))));
     }
     *loss = 0.0f;
-    if (n == 0 || !train) return RC_OK;
-    RC_TRY(rc_cache_grad_device(c, k, k->queries.as<float>(), k->targets.as<float>(), n));
-    RC_TRY(rc_cache_adam_device(c, k));
-    RC_TRY_HIP(hipMemcpyAsync(loss, k->loss, 4, hipMemcpyDeviceToHost, c->stream));
-    RC_TRY_HIP(hipStreamSynchronize(c->stream));
+    if (n == 0 || steps == 0) return RC_OK;
+    steps = std::min(steps, n);
+    // steps Adam steps on interleaved slices j, j + steps, j + 2 steps, ...
+    float sum = 0.0f;
+    for (uint32_t j = 0; j < steps; ++j) {
+        const uint32_t nj = (n - j + steps - 1) / steps;
+        RC_TRY(rc_cache_grad_device(c, k, k->queries.as<float>(), k->targets.as<float>(), nj, steps, j));
+        RC_TRY(rc_cache_adam_device(c, k));
+        float l = 0.0f;
+        RC_TRY_HIP(hipMemcpyAsync(&l, k->loss, 4, hipMemcpyDeviceToHost, c->stream));
+        RC_TRY_HIP(hipStreamSynchronize(c->stream));
+        sum += l;
+    }
+    *loss = sum / float(steps);
     return RC_OK;
 }
 

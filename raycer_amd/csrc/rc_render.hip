@@ -758,7 +758,7 @@ extern "C" int rc_render_cached(rc_ctx* c, const rc_scene* s, rc_film* f, const 
     float l = 0.0f;
     uint32_t n = 0, q = 0;
     auto t0 = std::chrono::steady_clock::now();
-    RC_TRY(rc_cache_train_frame(c, cache, train != 0, &l, &n, &q));
+    RC_TRY(rc_cache_train_frame(c, cache, train > 0 ? uint32_t(std::min(train, 16)) : 0u, &l, &n, &q));
     if (train) c->stats.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->stats.train_samples = train ? n : 0;
     c->stats.cache_queries = q;

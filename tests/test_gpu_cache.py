@@ -214,3 +214,28 @@ def test_online_training_converges_to_the_path_traced_image(ctx):
     assert np.isfinite(trained).all()
     assert abs(r1 - 1.0) < 0.03
     assert b1 < b0
+
+
+def test_several_adam_steps_per_frame(ctx):
+    """train = k runs k Adam steps on interleaved slices of the frame's
+    training samples: parameters move k times, the loss is finite."""
+    sd, s, _ = scene(ctx, "terrain")
+    cfg = terrain_cfg(sd, cache_vertex=2)
+    film = ctx.film(cfg.width, cfg.height)
+    caches = [api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, seed=5, max_batch=8192)
+              for _ in range(2)]
+    try:
+        p0 = caches[0].params()
+        l1 = caches[0].render(s, film, sd.desc.camera, cfg, train=1)
+        l4 = caches[1].render(s, film, sd.desc.camera, cfg, train=4)
+        p1, p4 = caches[0].params(), caches[1].params()
+    finally:
+        film.close()
+        for c in caches:
+            c.close()
+    assert np.isfinite(l1) and np.isfinite(l4) and l1 > 0 and l4 > 0
+    # Adam moves every parameter by about lr per step (first steps: |m/sqrt(v)| ~ 1)
+    d1, d4 = np.abs(p1 - p0), np.abs(p4 - p0)
+    print(f"median |dp|: 1 step {np.median(d1):.2e}, 4 steps {np.median(d4):.2e}")
+    assert 0.5e-3 < np.median(d1) < 1.5e-3
+    assert np.median(d4) > 2.0 * np.median(d1)
