@@ -251,7 +251,6 @@ __global__ __launch_bounds__(kThreads) void k_cache_fwd_bwd(const v4s* __restric
         const uint32_t s = g * 16 + (lane & 15);
         const bool valid = s < n;
         float q[12];
-#pragma unroll
         const size_t row = size_t(s) * stride + offset;  // sample s of the slice
 #pragma unroll
         for (int k = 0; k < 12; ++k) q[k] = valid ? queries[row * 12 + k] : 0.0f;
