@@ -97,6 +97,23 @@ public:
         hasBeenBuilt = true;
     }
 
+    // Serialization of the built tree (FlatBVH.h:99-105) and its restore on
+    // load (FlatBVH.cpp:206-214); restore() refuses a file saved for other
+    // triangles.
+    void save(const std::string& path, const std::vector<float>& vertices) const {
+        check(rch_bvh_save(path.c_str(), vertices.data(), vertices.size() / 9, flatNodes.data(), flatNodes.size(),
+                           orderedPrimitiveIds.data()));
+    }
+    void restore(const std::string& path, const std::vector<float>& vertices) {
+        size_t nn = 0, nt = 0;
+        check(rch_bvh_file_counts(path.c_str(), &nn, &nt));
+        if (nt != vertices.size() / 9) throw RaycerError(RC_ERR_INVALID, "saved BVH was built over other triangles");
+        flatNodes.resize(nn);
+        orderedPrimitiveIds.resize(nt);
+        check(rch_bvh_load(path.c_str(), vertices.data(), nt, flatNodes.data(), orderedPrimitiveIds.data()));
+        hasBeenBuilt = true;
+    }
+
     bool hasBeenBuilt = false;
     std::vector<FlatBVHNode> flatNodes;
     std::vector<uint64_t> orderedPrimitiveIds;

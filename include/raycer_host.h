@@ -45,6 +45,15 @@ double rch_bvh_build_ms(const rch_bvh* bvh);
 uint32_t rch_bvh_depth(const rch_bvh* bvh);
 int rch_bvh_free(rch_bvh* bvh);
 
+/* A built BVH on disk (the reference serializes FlatBVH's flatNodes and
+ * orderedPrimitiveIds, FlatBVH.h:99-105, and restores them on load instead of
+ * rebuilding, FlatBVH.cpp:206-214, Scene.cpp:487-488).  Loading fails unless
+ * the triangles hash to what was saved.  The upload validates the tree again. */
+int rch_bvh_save(const char* path, const float* vertices, size_t ntri, const rc_flat_node* nodes, size_t nnodes,
+                 const uint64_t* ordered_ids);
+int rch_bvh_file_counts(const char* path, size_t* nnodes, size_t* ntri);
+int rch_bvh_load(const char* path, const float* vertices, size_t ntri, rc_flat_node* nodes, uint64_t* ordered_ids);
+
 /* ------------------------------------------------------------ scenes */
 enum {
     RCH_SCENE_BOX = 1,          /* closed box + two rotated inner boxes + ceiling light, 36 triangles */

@@ -7,5 +7,5 @@ the benchmark.
 """
 from .api import (  # noqa: F401
     Camera, Context, Film, RaycerError, RenderCfg, Scene, SceneData, build_bvh, describe_scene, generate_scene,
-    lib, load_obj, make_rays, obj_scene, scene_params,
+    lib, load_bvh, load_obj, make_rays, obj_scene, save_bvh, scene_params,
 )

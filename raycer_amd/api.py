@@ -178,6 +178,9 @@ def lib() -> C.CDLL:
             "rch_bvh_free": [vp],
             "rch_scene_describe": [C.POINTER(SceneParams), C.POINTER(SceneDesc)],
             "rch_scene_generate": [C.POINTER(SceneParams), vp, vp, vp],
+            "rch_bvh_save": [C.c_char_p, vp, sz, vp, sz, vp],
+            "rch_bvh_file_counts": [C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+            "rch_bvh_load": [C.c_char_p, vp, sz, vp, vp],
             "rch_obj_load": [C.c_char_p, C.POINTER(ObjInfo), C.POINTER(vp)],
             "rch_ply_load": [C.c_char_p, C.POINTER(ObjInfo), C.POINTER(vp)],
             "rch_obj_triangle_count": [vp],
@@ -248,6 +251,25 @@ def generate_scene(p: SceneParams) -> SceneData:
     mats = np.zeros(d.nmat, MATERIAL_DTYPE)
     check(lib().rch_scene_generate(C.byref(p), _ptr(v), _ptr(m), _ptr(mats)))
     return SceneData(v, m, mats, d)
+
+
+def save_bvh(path, vertices, nodes, ordered) -> None:
+    """rch_bvh_save: the flat BVH with a hash of the triangles it was built over."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 9)
+    nodes, ordered = np.ascontiguousarray(nodes), np.ascontiguousarray(ordered, np.uint64)
+    check(lib().rch_bvh_save(str(path).encode(), _ptr(v), v.shape[0], _ptr(nodes), nodes.shape[0], _ptr(ordered)))
+
+
+def load_bvh(path, vertices):
+    """rch_bvh_load: (nodes, ordered_ids) saved for these triangles."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 9)
+    nn, nt = C.c_size_t(), C.c_size_t()
+    check(lib().rch_bvh_file_counts(str(path).encode(), C.byref(nn), C.byref(nt)))
+    if nt.value != v.shape[0]:
+        raise RaycerError(RC_ERR_INVALID, "rch_bvh_load: the saved BVH was built over other triangles")
+    nodes, ordered = np.empty(nn.value, NODE_DTYPE), np.empty(nt.value, np.uint64)
+    check(lib().rch_bvh_load(str(path).encode(), _ptr(v), v.shape[0], _ptr(nodes), _ptr(ordered)))
+    return nodes, ordered
 
 
 def load_obj(path, scale=(1.0, 1.0, 1.0), rotate=(0.0, 0.0, 0.0), translate=(0.0, 0.0, 0.0),

@@ -327,3 +327,83 @@ extern "C" int rch_bvh_free(rch_bvh* b) {
     delete b;
     return RC_OK;
 }
+
+// ------------------------------------------------------------------ save / load
+// A built FlatBVH on disk: the counterpart of the reference's serialized
+// FlatBVH (flatNodes + orderedPrimitiveIds, src/Raytracing/Primitives/FlatBVH.h:99-105)
+// and its restore on load, which skips the build (FlatBVH.cpp:206-214,
+// Scene.cpp:487-488).  Binary, little-endian: magic "RCBVH001", node count,
+// triangle count, FNV-1a hash of the triangles' vertex bytes, the nodes, the
+// ordered ids.  Loading checks the hash against the triangles it is given.
+namespace {
+constexpr char kMagic[8] = {'R', 'C', 'B', 'V', 'H', '0', '0', '1'};
+
+uint64_t fnv1a(const float* v, size_t ntri) {
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(v);
+    for (size_t i = 0; i < ntri * 9 * sizeof(float); ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+struct FileHeader {
+    char magic[8];
+    uint64_t nnodes, ntri, hash;
+};
+}  // namespace
+
+extern "C" int rch_bvh_save(const char* path, const float* vertices, size_t ntri, const rc_flat_node* nodes,
+                            size_t nnodes, const uint64_t* ordered) {
+    if (!path || !vertices || !nodes || !ordered || ntri == 0 || nnodes == 0)
+        return rc_host_fail(RC_ERR_INVALID, "rch_bvh_save: null or empty argument");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_save: cannot write ") + path);
+    FileHeader h;
+    std::memcpy(h.magic, kMagic, 8);
+    h.nnodes = nnodes;
+    h.ntri = ntri;
+    h.hash = fnv1a(vertices, ntri);
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(nodes, sizeof(rc_flat_node), nnodes, f) == nnodes &&
+              std::fwrite(ordered, sizeof(uint64_t), ntri, f) == ntri;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? RC_OK : rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_save: write failed for ") + path);
+}
+
+namespace {
+int read_header(FILE* f, const char* path, FileHeader& h) {
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0)
+        return rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_load: not a saved BVH: ") + path);
+    return RC_OK;
+}
+}  // namespace
+
+extern "C" int rch_bvh_file_counts(const char* path, size_t* nnodes, size_t* ntri) {
+    if (!path || !nnodes || !ntri) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_file_counts: null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_file_counts: cannot open ") + path);
+    FileHeader h;
+    int rc = read_header(f, path, h);
+    std::fclose(f);
+    if (rc != RC_OK) return rc;
+    *nnodes = size_t(h.nnodes);
+    *ntri = size_t(h.ntri);
+    return RC_OK;
+}
+
+extern "C" int rch_bvh_load(const char* path, const float* vertices, size_t ntri, rc_flat_node* nodes,
+                            uint64_t* ordered) {
+    if (!path || !vertices || !nodes || !ordered) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_load: null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_load: cannot open ") + path);
+    FileHeader h;
+    int rc = read_header(f, path, h);
+    if (rc == RC_OK && (h.ntri != ntri || h.hash != fnv1a(vertices, ntri)))
+        rc = rc_host_fail(RC_ERR_INVALID, "rch_bvh_load: the saved BVH was built over other triangles");
+    if (rc == RC_OK && (std::fread(nodes, sizeof(rc_flat_node), h.nnodes, f) != h.nnodes ||
+                        std::fread(ordered, sizeof(uint64_t), ntri, f) != ntri))
+        rc = rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_load: truncated file ") + path);
+    std::fclose(f);
+    return rc;
+}

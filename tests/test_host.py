@@ -169,3 +169,27 @@ def test_create_without_gpu_reports_an_error():
         ra.Context(0)
     assert e.value.code in (api.RC_ERR_NO_DEVICE, api.RC_ERR_HIP)
     assert api.last_error()
+
+
+def test_bvh_save_and_restore(tmp_path):
+    """A saved BVH restores node-for-node for the same triangles and is refused
+    for others (FlatBVH serialization and restore, FlatBVH.h:99-105)."""
+    sd = ra.generate_scene(SMALL_SCENES["sphere4"])
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    path = tmp_path / "scene.bvh"
+    ra.save_bvh(path, sd.vertices, nodes, ordered)
+    n2, o2 = ra.load_bvh(path, sd.vertices)
+    assert np.array_equal(n2, nodes) and np.array_equal(o2, ordered)
+    moved = sd.vertices.copy()
+    moved[0, 0] += 1e-3
+    with pytest.raises(ra.RaycerError):
+        ra.load_bvh(path, moved)
+    with pytest.raises(ra.RaycerError):
+        ra.load_bvh(path, sd.vertices[:-1])
+    trunc = tmp_path / "cut.bvh"
+    trunc.write_bytes(path.read_bytes()[:-8])
+    with pytest.raises(ra.RaycerError):
+        ra.load_bvh(trunc, sd.vertices)
+    (tmp_path / "junk.bvh").write_bytes(b"not a bvh at all, no magic here")
+    with pytest.raises(ra.RaycerError):
+        ra.load_bvh(tmp_path / "junk.bvh", sd.vertices)
