@@ -78,7 +78,7 @@ def mse_pair(a: np.ndarray, b: np.ndarray) -> float:
 
 def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, cached_spp: int = 1,
                    plain_spp: int = 64, ref_spp: int = 4096, warmup_frames: int = 16,
-                   pretrain_frames: int = 256) -> dict:
+                   pretrain_frames: int = 256, train_steps: int = 1) -> dict:
     """Plain and radiance-cache renders given the same wall time, against a
     ref_spp plain render.
 
@@ -88,7 +88,8 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
     accumulated.  Warm: the same cache after pretrain_frames more untimed
     training frames (a cache carried over from earlier frames, as in an
     animation) renders and trains within the budget.  Paths end into the cache
-    at vertex cache_vertex (after two bounces for 2).  MSE is over linear RGB
+    at vertex cache_vertex (after two bounces for 2); every frame ends with
+    train_steps Adam steps on its training samples.  MSE is over linear RGB
     and over the Reinhard-mapped RGB (key 0.18, gamma 2.2)."""
     from raycer_amd import api
 
@@ -105,7 +106,7 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
                 "mean_ratio": round(float(lin.mean() / ref_lin.mean()), 5)}
 
     out = {"resolution": f"{cfg.width}x{cfg.height}", "reference_spp": ref_spp, "plain_spp": plain_spp,
-           "cache_vertex": cache_vertex}
+           "cache_vertex": cache_vertex, "train_steps_per_frame": train_steps}
     try:
         # reference: ref_spp / plain_spp frames of plain_spp samples
         cfg.spp = plain_spp
@@ -141,7 +142,7 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
                         film.clear()
                     ccfg.frame = frame[0]
                     frame[0] += 1
-                    loss = cache.render(scene, film, cam, ccfg, train=True)
+                    loss = cache.render(scene, film, cam, ccfg, train=train_steps)
                     frames += 1
                     if frames > warmup:
                         accumulated += 1
@@ -162,7 +163,7 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
                 ccfg.frame = frame[0]
                 frame[0] += 1
                 film.clear()
-                cache.render(scene, film, cam, ccfg, train=True)
+                cache.render(scene, film, cam, ccfg, train=train_steps)
             out["cached_warm"] = run(0)
             out["cached_warm"]["pretrain_frames"] = pretrain_frames
         finally:

@@ -234,8 +234,10 @@ def test_several_adam_steps_per_frame(ctx):
         for c in caches:
             c.close()
     assert np.isfinite(l1) and np.isfinite(l4) and l1 > 0 and l4 > 0
-    # Adam moves every parameter by about lr per step (first steps: |m/sqrt(v)| ~ 1)
+    # Adam's first step moves every parameter with a non-zero gradient by
+    # lr |g| / (|g| + eps) ~ lr; four steps move most of them further
     d1, d4 = np.abs(p1 - p0), np.abs(p4 - p0)
-    print(f"median |dp|: 1 step {np.median(d1):.2e}, 4 steps {np.median(d4):.2e}")
-    assert 0.5e-3 < np.median(d1) < 1.5e-3
-    assert np.median(d4) > 2.0 * np.median(d1)
+    m1, m4 = np.median(d1[d1 > 0]), np.median(d4[d4 > 0])
+    print(f"median non-zero |dp|: 1 step {m1:.2e}, 4 steps {m4:.2e}")
+    assert 0.9e-3 < m1 <= 1.0001e-3
+    assert m4 > 1.5e-3
