@@ -115,11 +115,15 @@ __device__ __forceinline__ float slab(float4 r0, float4 r1, float3 o, float3 inv
 // widened by kFilterBand so FP32 rounding never rejects a triangle the FP64
 // decision (mt_hit64, inclusive bounds) would accept.
 constexpr float kFilterBand = 1e-3f;
+// RC_FAST_RCP 1: the FP32 filter's 1/det is the hardware reciprocal (1 ulp;
+// the filter bands absorb it and FP64 decides).  RC_SLAB_SELECT 1: the four
+// child slab tests run unconditionally and empty slots are masked by a select
+// (no branch).  Together 10% faster on the 2M-triangle terrain.
 #ifndef RC_FAST_RCP
-#define RC_FAST_RCP 0
+#define RC_FAST_RCP 1
 #endif
 #ifndef RC_SLAB_SELECT
-#define RC_SLAB_SELECT 0
+#define RC_SLAB_SELECT 1
 #endif
 __device__ __forceinline__ bool mt_hit(const TriRec& T, float3 o, float3 d, float& t) {
     float3 v0 = f3(T.v0);
