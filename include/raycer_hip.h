@@ -279,9 +279,11 @@ int rc_cache_gradient(rc_ctx* ctx, rc_cache* cache, const float* queries, const 
                       float* grad, float* loss);
 /* rc_render with paths ending into the cache at cfg->cache_vertex (1 <=
  * cache_vertex < max_path_length - 1): a path that reaches that vertex without
- * emission adds throughput x cache(query) and stops.  Paths of one slot in
+ * emission traces its sun next-event ray as usual, adds throughput x
+ * cache(query) for the indirect light and stops.  Paths of one slot in
  * train_stride run to the end instead and give up to max_batch training
- * samples (target = radiance gathered after the vertex / throughput at it);
+ * samples (target = radiance gathered after the vertex's next-event ray /
+ * throughput at the vertex);
  * with train = k > 0, k (at most 16) Adam steps run after the frame, each on
  * an interleaved 1/k of those samples, and loss (may be NULL) gets their mean
  * loss. */

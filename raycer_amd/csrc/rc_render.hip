@@ -190,7 +190,9 @@ __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
 struct ShadeOut {
     bool alive, shadow;
     float3 thr, newo, newd, contrib, so;
+    uint32_t rec;  // radiance-cache training record written at this vertex, or kNoRecord
 };
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 
 // Training paths: a fixed-rate, frame-varying subset of the slots that run
 // to the end instead of stopping into the cache.
@@ -213,6 +215,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
                                                  const CacheIO& CI) {
     ShadeOut O;
     O.alive = O.shadow = false;
+    O.rec = kNoRecord;
     const bool more = (k + 1) < P.max_len;
     uint32_t slot = __float_as_uint(s.w);
     float3 thr = f3(s);
@@ -233,11 +236,23 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
     float3 nrm = f3(sh);
     float3 p = f3(r0) + (t * f3(r1));
     float3 rho = f3(dif);
+    if (!more) return O;
+    float4 u = draw4(P.seed, slot % P.npix, P.sample0 + slot / P.npix, P.frame, k);
+    if (P.has_sun) {
+        float cs = dot3(nrm, P.sun);
+        if (cs > 0.0f) {
+            float f = P.estimator == RC_EST_REFERENCE ? (2.0f / kPi) * cs * cs : (1.0f / kPi) * cs;
+            O.contrib = mul3(thr, mul3(rho, P.sun_e)) * f;
+            O.so = p + P.sun * P.offset;
+            O.shadow = true;
+        }
+    }
     if (CI.cache_vertex > 0 && k == CI.cache_vertex) {
+        // The cache holds the indirect part of the radiance leaving this
+        // vertex (the direct sun is the next-event ray above, for every path).
         const float3 wo = make_float3(-r1.x, -r1.y, -r1.z);
-        bool train = training_path(P, slot, CI.train_stride);
-        if (!train) {
-            // the path ends here: its remaining radiance comes from the cache
+        if (!training_path(P, slot, CI.train_stride)) {
+            // the path ends here: its indirect radiance comes from the cache
             uint32_t q = atomicAdd(&CI.counters[0], 1u);
             if (q < CI.cap) {
                 float4* Q = CI.queries + 4 * size_t(q);
@@ -248,30 +263,21 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
             }
             return O;
         }
-        {
-            // training vertex: the radiance the full path gathers after it,
-            // over the throughput arriving at it, is the target for this query
-            uint32_t r = atomicAdd(&CI.counters[1], 1u);
-            if (r < CI.cap) {
-                float4 lb = L[slot];
-                float4* R = CI.records + 5 * size_t(r);
-                R[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
-                R[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
-                R[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.y);
-                R[3] = make_float4(rho.x, rho.y, rho.z, thr.z);
-                R[4] = make_float4(lb.x, lb.y, lb.z, 0.0f);
-            }
-        }
-    }
-    if (!more) return O;
-    float4 u = draw4(P.seed, slot % P.npix, P.sample0 + slot / P.npix, P.frame, k);
-    if (P.has_sun) {
-        float cs = dot3(nrm, P.sun);
-        if (cs > 0.0f) {
-            float f = P.estimator == RC_EST_REFERENCE ? (2.0f / kPi) * cs * cs : (1.0f / kPi) * cs;
-            O.contrib = mul3(thr, mul3(rho, P.sun_e)) * f;
-            O.so = p + P.sun * P.offset;
-            O.shadow = true;
+        // training vertex: the radiance the path gathers after this vertex's
+        // next-event ray, over the throughput arriving here, is the target;
+        // the record's "before" value gets that ray's contribution added once
+        // its occlusion is known (k_cache_nee_fix), and its queue position
+        // once it is appended (emit_vertex)
+        uint32_t r = atomicAdd(&CI.counters[1], 1u);
+        if (r < CI.cap) {
+            float4 lb = L[slot];
+            float4* R = CI.records + 5 * size_t(r);
+            R[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+            R[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
+            R[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.y);
+            R[3] = make_float4(rho.x, rho.y, rho.z, thr.z);
+            R[4] = make_float4(lb.x, lb.y, lb.z, __uint_as_float(kNoRecord));
+            O.rec = r;
         }
     }
     float3 w = hemisphere(nrm, u.x, u.y);
@@ -294,7 +300,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
 __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut& O, uint32_t slot,
                                             rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
-                                            DevCounters* __restrict__ ctr, uint32_t cout) {
+                                            DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI) {
     uint32_t pos = wave_append(O.alive, &ctr->queue_count[cout]);
     if (O.alive) {
         float4* ro = reinterpret_cast<float4*>(rays_out + pos);
@@ -308,6 +314,7 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
         ro[0] = make_float4(O.so.x, O.so.y, O.so.z, 0.0f);
         ro[1] = make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff());
         scontrib[spos] = make_float4(O.contrib.x, O.contrib.y, O.contrib.z, __uint_as_float(slot));
+        if (O.rec != kNoRecord) CI.records[5 * size_t(O.rec) + 4].w = __uint_as_float(spos);
     }
 }
 
@@ -328,6 +335,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
     O.alive = O.shadow = false;
+    O.rec = kNoRecord;
     uint32_t slot = 0;
     if (i < n) {
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
         if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
     }
-    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout);
+    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
@@ -360,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
         uint32_t q = base + threadIdx.x;
         ShadeOut O;
         O.alive = O.shadow = false;
+        O.rec = kNoRecord;
         uint32_t slot = 0;
         if (q < n) {
             uint32_t i = list[q];
@@ -369,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
             slot = __float_as_uint(s.w);
             O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L, CI);
         }
-        emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout);
+        emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI);
     }
 }
 
@@ -380,7 +389,8 @@ template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const NodeRec* __restrict__ nodes,
                                                        const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
                                                        const float4* __restrict__ scontrib, float4* __restrict__ L,
-                                                       uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr) {
+                                                       uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr,
+                                                       uint8_t* __restrict__ occ_out) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[2];
     const uint32_t i = trace_block() * kTraceBlock + threadIdx.x;
@@ -393,11 +403,14 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const Node
                                               kTraceBlock, kTraceLds, n_rec, n_tri);
         if (R.overflow) {
             ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
-        } else if (R.id == kNoHit) {
-            float4 c = scontrib[i];
-            uint32_t slot = __float_as_uint(c.w);
-            float4 l = L[slot];
-            L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+        } else {
+            if (occ_out) occ_out[i] = R.id == kNoHit ? 0 : 1;
+            if (R.id == kNoHit) {
+                float4 c = scontrib[i];
+                uint32_t slot = __float_as_uint(c.w);
+                float4 l = L[slot];
+                L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+            }
         }
     }
     if (COUNT) {
@@ -444,6 +457,23 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(const float4* __restri
         q[6] = r2.x; q[7] = r2.y; q[8] = r2.z;
         q[9] = r3.x; q[10] = r3.y; q[11] = r3.z;
         for (int c = 0; c < 3; ++c) batch_t[size_t(pos) * 3 + c] = T[c] > 1e-8f ? fmaxf(d[c], 0.0f) / T[c] : 0.0f;
+    }
+}
+
+// Adds to each training record of the pass its vertex's next-event
+// contribution when that shadow ray was unoccluded, so "before" covers the
+// direct light at the cache vertex.
+__global__ __launch_bounds__(kBlock) void k_cache_nee_fix(float4* __restrict__ records,
+                                                          const uint32_t* __restrict__ counters, uint32_t pass_cap,
+                                                          const float4* __restrict__ scontrib,
+                                                          const uint8_t* __restrict__ occ) {
+    const uint32_t n = min(counters[1], pass_cap);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        float4 r4 = records[5 * size_t(i) + 4];
+        const uint32_t spos = __float_as_uint(r4.w);
+        if (spos == kNoRecord || occ[spos]) continue;
+        const float4 c = scontrib[spos];
+        records[5 * size_t(i) + 4] = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, r4.w);
     }
 }
 
@@ -635,21 +665,28 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
                                                                  c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
                                                                  c->counters, cout, CI);
             if (P.has_sun && k + 1 < cfg->max_path_length) {
+                // at the cache vertex the shadow results are kept for the training records
+                const bool nee_fix = CI.cache_vertex && k == CI.cache_vertex;
+                uint8_t* occ_out = nee_fix ? c->socc.as<uint8_t>() : nullptr;
                 k_reset_ovf<<<1, 1, 0, c->stream>>>(c->counters);
                 RC_TRY(c->prof_begin(&ev));
                 if (c->counting) {
                     k_shadow<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
                         s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
-                        c->counters);
+                        c->counters, occ_out);
                 } else {
                     k_shadow<false><<<grid_t, kTraceBlock, 0, c->stream>>>(
                         s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
-                        c->counters);
+                        c->counters, occ_out);
                 }
                 RC_TRY(c->prof_end(ev, PK_SHADOW));
                 RC_TRY(rc_launch_deep(c, s, true, c->srays.as<rc_ray>(), nullptr, c->socc.as<uint8_t>()));
                 k_shadow_apply_listed<<<grid_small, kBlock, 0, c->stream>>>(
                     c->scontrib.as<float4>(), c->socc.as<uint8_t>(), ovf, c->slotL.as<float4>(), c->counters);
+                if (nee_fix)
+                    k_cache_nee_fix<<<grid_small, kBlock, 0, c->stream>>>(CI.records, CI.counters, CI.cap,
+                                                                         c->scontrib.as<float4>(),
+                                                                         c->socc.as<uint8_t>());
             }
             std::swap(rin, rout);
             std::swap(sin_, sout);

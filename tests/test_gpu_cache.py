@@ -129,6 +129,9 @@ def terrain_cfg(sd, w=96, h=96, spp=16, **kw):
 
 
 def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx):
+    """A cache that returns exactly 0: every path ends after its vertex-2
+    next-event ray, which (without sky) is what a plain render with at most 4
+    rays per path gathers — bit for bit."""
     sd, s, _ = scene(ctx, "terrain")
     b0, b1 = np.asarray(sd.desc.bounds_min), np.asarray(sd.desc.bounds_max)
     cache = api.RadianceCache(ctx, b0, b1, train_stride=0xFFFFFFFF)
@@ -138,6 +141,7 @@ def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx):
     cache.set_params(p)
     cv = 2
     cfg = terrain_cfg(sd, cache_vertex=cv)
+    cfg.sky[0] = cfg.sky[1] = cfg.sky[2] = 0.0
     film = ctx.film(cfg.width, cfg.height)
     try:
         cache.render(s, film, sd.desc.camera, cfg, train=False)
@@ -146,7 +150,8 @@ def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx):
     finally:
         film.close()
         cache.close()
-    plain_cfg = terrain_cfg(sd, max_path_length=cv + 1)
+    plain_cfg = terrain_cfg(sd, max_path_length=cv + 2)
+    plain_cfg.sky[0] = plain_cfg.sky[1] = plain_cfg.sky[2] = 0.0
     _, plain = render_gpu(ctx, s, sd.desc.camera, plain_cfg)
     print(f"cache queries {st.cache_queries}, training samples {st.train_samples}")
     assert st.cache_queries > 0 and st.train_samples == 0
