@@ -237,7 +237,8 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     RC_TRY(rc_check_ctx(c, "rc_scene_upload"));
     if (!out || !v || !tri_mat || !mats || !nodes || !ordered || ntri == 0 || nnodes == 0 || nmat == 0)
         return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: null or empty input");
-    if (ntri >= 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: too many triangles");
+    if (ntri > kMaxTriangles)
+        return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: more than 2^26 triangles");
     *out = nullptr;
     // Validate the FlatBVH: a permutation of the triangles, leaves inside the
     // ordered range, right offsets pointing forward inside the array.
@@ -274,6 +275,11 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     auto is_leaf = [&](size_t i) { return nodes[i].right_offset == 0; };
     std::vector<NodeRec> recs;
     recs.reserve(2 * nnodes / 3 + kWidth);
+    // explicit stack of (binary node, record slot that links to its wide node);
+    // a leaf of kMaxLeafRecord or more triangles becomes an interior record
+    // whose wide node splits the range (split = true, range start / count)
+    struct Pending { size_t node; size_t slot; bool split; uint64_t start, count; };
+    std::vector<Pending> todo;
     auto fill = [&](size_t at, size_t child, uint32_t link) {
         NodeRec& r = recs[at];
         const rc_flat_node& n = nodes[child];
@@ -281,17 +287,46 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
             r.lo[k] = round_down(n.aabb_min[k]);
             r.hi[k] = round_up(n.aabb_max[k]);
         }
-        if (is_leaf(child)) {
+        if (is_leaf(child) && n.primitive_count < kMaxLeafRecord) {
             r.link = uint32_t(n.start_offset);
             r.count = uint32_t(n.primitive_count);
         } else {
             r.link = link;
             r.count = 0;
+            if (is_leaf(child)) todo.push_back({child, at, true, n.start_offset, n.primitive_count});
         }
     };
-    // explicit stack of (binary node, record slot that links to its wide node)
-    struct Pending { size_t node; size_t slot; };
-    std::vector<Pending> todo;
+    // the wide node of a split leaf: up to four parts of the range, each a
+    // leaf record or (still too long) another split; every part keeps the
+    // leaf's box
+    auto emit_split = [&](size_t leaf, uint64_t start, uint64_t count) -> uint32_t {
+        uint32_t base = uint32_t(recs.size());
+        recs.resize(recs.size() + kWidth);
+        const uint64_t part = (count + kWidth - 1) / kWidth;
+        for (int k = 0; k < kWidth; ++k) {
+            NodeRec& r = recs[base + k];
+            const uint64_t s0 = start + k * part, c0 = s0 < start + count ? std::min(part, start + count - s0) : 0;
+            if (c0 == 0) {
+                for (int q = 0; q < 3; ++q) { r.lo[q] = 0.0f; r.hi[q] = 0.0f; }
+                r.link = 0;
+                r.count = kEmptyChild;
+                continue;
+            }
+            for (int q = 0; q < 3; ++q) {
+                r.lo[q] = round_down(nodes[leaf].aabb_min[q]);
+                r.hi[q] = round_up(nodes[leaf].aabb_max[q]);
+            }
+            if (c0 < kMaxLeafRecord) {
+                r.link = uint32_t(s0);
+                r.count = uint32_t(c0);
+            } else {
+                r.link = 0;
+                r.count = 0;
+                todo.push_back({leaf, base + size_t(k), true, s0, c0});
+            }
+        }
+        return base;
+    };
     auto emit = [&](size_t n) -> uint32_t {  // returns the first record of n's wide node
         size_t kids[kWidth];
         int nk = 0;
@@ -322,7 +357,7 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
                 continue;
             }
             fill(base + k, kids[k], 0);
-            if (!is_leaf(kids[k])) todo.push_back({kids[k], base + size_t(k)});
+            if (!is_leaf(kids[k])) todo.push_back({kids[k], base + size_t(k), false, 0, 0});
         }
         return base;
     };
@@ -330,7 +365,7 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     while (!todo.empty()) {
         Pending p = todo.back();
         todo.pop_back();
-        uint32_t b = emit(p.node);
+        uint32_t b = p.split ? emit_split(p.node, p.start, p.count) : emit(p.node);
         recs[p.slot].link = b;
     }
     size_t nrec = recs.size();

@@ -257,10 +257,11 @@ __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, fl
 #endif
 // Stack entries of the if-if traversal: an interior child's record index, or
 // a leaf tagged with kLeafTag, its triangle count (1..31) in bits 26..30 and
-// its first triangle (< 2^26) below.  Leaves that do not fit are tested at
-// once instead.
+// its first triangle (< 2^26) below.  The upload guarantees both: scenes hold
+// at most 2^26 triangles and longer leaves are split into wide nodes.
 constexpr uint32_t kLeafTag = 0x80000000u;
-__device__ __forceinline__ bool leaf_fits(uint32_t link, uint32_t count) { return count < 32u && link < (1u << 26); }
+constexpr uint32_t kMaxLeafRecord = 32;           // leaf records hold fewer triangles
+constexpr size_t kMaxTriangles = size_t(1) << 26;
 
 // Ordered, t-culled traversal over 4-wide nodes.  The four child boxes are
 // tested and sorted by entry distance (a 5-comparator network); leaf children
@@ -405,30 +406,17 @@ __device__ __forceinline__ bool trav_step(Trav& S, const NodeRec* __restrict__ n
         uint32_t next = 0;
 #define RC_CHILD(t, l, c)                                                                                      \
         if (t != INF && t <= S.cull) {                                                                         \
-            uint32_t e = l;                                                                                    \
-            bool take = true;                                                                                  \
-            if (c != 0) {                                                                                      \
-                if (leaf_fits(l, c)) {                                                                         \
-                    e = kLeafTag | (c << 26) | l;                                                              \
-                } else {                                                                                       \
-                    take = false;                                                                              \
-                    if (test_leaf<ANY, COUNT>(tris, l, c, S.o, S.d, S.tmin, S.tmin64, S.tmax64, S.best, S.cull, \
-                                              S.R, n_tri))                                                     \
-                        return true;                                                                           \
+            const uint32_t e = c != 0 ? (kLeafTag | (c << 26) | l) : l;                                       \
+            if (have) {                                                                                        \
+                if (S.sp >= max_depth) {                                                                       \
+                    S.R.overflow = true;                                                                       \
+                    return true;                                                                               \
                 }                                                                                              \
+                stack[S.sp * stride] = next;                                                                   \
+                ++S.sp;                                                                                        \
             }                                                                                                  \
-            if (take) {                                                                                        \
-                if (have) {                                                                                    \
-                    if (S.sp >= max_depth) {                                                                   \
-                        S.R.overflow = true;                                                                   \
-                        return true;                                                                           \
-                    }                                                                                          \
-                    stack[S.sp * stride] = next;                                                               \
-                    ++S.sp;                                                                                    \
-                }                                                                                              \
-                next = e;                                                                                      \
-                have = true;                                                                                   \
-            }                                                                                                  \
+            next = e;                                                                                          \
+            have = true;                                                                                       \
         }
         RC_CHILD(t3, l3, c3)
         RC_CHILD(t2, l2, c2)

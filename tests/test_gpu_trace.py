@@ -59,12 +59,18 @@ def test_closest_hit_random_rays(ctx, name):
                   max_listed=0.05 if name == "box" else 0.01)
 
 
-@pytest.mark.parametrize("name", ["sphere5", "soup"])
-def test_results_do_not_depend_on_the_bvh(ctx, name):
-    """The binned-SAH BVH visits triangles in another order; with the canonical
-    tie rule (lower original id) the hits must be bit-identical."""
+@pytest.mark.parametrize("name,method,max_leaf", [("sphere5", api.RCH_BVH_BINNED, 5), ("soup", api.RCH_BVH_BINNED, 5),
+                                                  ("soup", api.RCH_BVH_REFERENCE, 100)])
+def test_results_do_not_depend_on_the_bvh(ctx, name, method, max_leaf):
+    """Another BVH visits triangles in another order; with the canonical tie
+    rule (lower original id) the hits must be bit-identical.  Leaves of up to
+    100 triangles exercise the upload's splitting of long leaves (a leaf
+    record holds at most 31)."""
     sd, s, _ = scene(ctx, name)
-    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, max_leaf=max_leaf, method=method)
+    if max_leaf > 31:
+        leaves = nodes[nodes["right_offset"] == 0]
+        assert leaves["primitive_count"].max() >= 32
     s2 = ctx.upload(sd, nodes, ordered)
     try:
         v = sd.vertices.reshape(-1, 3)
