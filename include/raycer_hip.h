@@ -181,9 +181,11 @@ int rc_copy_bandwidth(rc_ctx* ctx, size_t bytes, int iters, double* gbps);
 
 /* ------------------------------------------------------------ scene */
 /* CLTracer::initializeBuffers / CLScene::readSceneFull (CLScene.cpp:72-263):
- * the host FlatBVH (nodes + orderedPrimitiveIds) is repacked into 32-byte
- * child-pair records and 48-byte triangle records and uploaded.
- * vertices: ntri * 9 floats in original triangle order; tri_material: ntri. */
+ * the host FlatBVH (nodes + orderedPrimitiveIds) is validated, collapsed into
+ * 4-wide nodes of four 32-byte child records (leaves of 32 or more triangles
+ * are split over extra wide nodes) and 48-byte triangle records, and uploaded.
+ * vertices: ntri * 9 floats in original triangle order (ntri <= 2^26);
+ * tri_material: ntri. */
 int rc_scene_upload(rc_ctx* ctx, const float* vertices, size_t ntri, const uint32_t* tri_material,
                     const rc_material* materials, size_t nmat, const rc_flat_node* nodes, size_t nnodes,
                     const uint64_t* ordered_ids, rc_scene** out);
