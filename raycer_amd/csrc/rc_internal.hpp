@@ -42,7 +42,6 @@ struct DevCounters {
     uint32_t ovf_count;
     uint32_t queue_count[4];   // wavefront queue sizes (ping, pong, shadow, scratch)
     uint32_t error;
-    uint32_t oct_count[2][8];  // ping / pong queues binned by ray-direction octant
 };
 
 enum ProfKind { PK_EXTEND = 0, PK_SHADOW = 1, PK_SHADE = 2, PK_OTHER = 3, PK_CACHE = 4 };

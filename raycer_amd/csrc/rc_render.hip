@@ -33,13 +33,6 @@ constexpr float kPi = 3.14159265358979323846f;
 #endif
 constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the traversal kernels
 constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
-// 1: continuation rays are appended to eight regions of the next queue by
-// the octant of their direction, so a wave of the next bounce traces rays of
-// one octant; 0: one compact queue in shading order.
-#ifndef RC_OCTANT_QUEUES
-#define RC_OCTANT_QUEUES 0
-#endif
-constexpr uint32_t kQueueRegions = RC_OCTANT_QUEUES ? 8u : 1u;
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
 #endif
@@ -111,7 +104,7 @@ struct PassParams {
     uint32_t max_len, rr_start, estimator, frame;
     float3 sky, sun, sun_e;
     uint64_t seed;
-    uint32_t has_sun, cap;  // cap: paths of a pass (the capacity of each queue region)
+    uint32_t has_sun, pad;
 };
 
 __device__ __forceinline__ float filter_1d(uint32_t f, float r, float x) {
@@ -131,8 +124,6 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
         ctr->queue_count[0] = n;
         ctr->queue_count[1] = 0;
         ctr->queue_count[2] = 0;
-        for (int o = 0; o < 8; ++o) ctr->oct_count[0][o] = ctr->oct_count[1][o] = 0;
-        ctr->oct_count[0][0] = n;  // camera rays: all in region 0
     }
     if (slot >= n) return;
     uint32_t j = slot / P.npix, pixel = slot - j * P.npix;
@@ -181,28 +172,6 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
     if (int(lane) == leader && total) base = atomicAdd(counter, total);
     base = __shfl(base, leader, 64);
     return base + prefix;
-}
-
-// wave_append over eight counters: lane appends to counters[bin] when pred.
-__device__ __forceinline__ uint32_t wave_append8(bool pred, uint32_t bin, uint32_t* counters) {
-    const uint32_t lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
-    uint32_t pos = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) {
-        const bool pk = pred && bin == k;
-        const uint64_t mask = __ballot(pk);
-        if (mask == 0) continue;
-        uint32_t base = 0;
-        if (int(lane) == leader) base = atomicAdd(&counters[k], uint32_t(__popcll(mask)));
-        base = __shfl(base, leader, 64);
-        if (pk) pos = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-    }
-    return pos;
-}
-
-__device__ __forceinline__ uint32_t octant(float3 d) {
-    return (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
 }
 
 // mapToHemisphere with distribution 1 (Sampler.cpp:50-64) in the ONB of a
@@ -332,12 +301,7 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
                                             rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
                                             DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI) {
-#if RC_OCTANT_QUEUES
-    const uint32_t bin = octant(O.newd);
-    uint32_t pos = wave_append8(O.alive, bin, ctr->oct_count[cout]) + bin * P.cap;
-#else
     uint32_t pos = wave_append(O.alive, &ctr->queue_count[cout]);
-#endif
     if (O.alive) {
         float4* ro = reinterpret_cast<float4*>(rays_out + pos);
         ro[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f);
@@ -365,28 +329,15 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
     DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
-    const uint32_t v = trace_block() * kTraceBlock + threadIdx.x;  // entry of the (virtual) queue
-#if RC_OCTANT_QUEUES
-    // the queue is eight regions of P.cap entries; entry v is in the region
-    // where the running count passes it
-    uint32_t n = 0, i = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t ck = ctr->oct_count[cin][k];
-        if (v >= n && v < n + ck) i = k * P.cap + (v - n);
-        n += ck;
-    }
-#else
     const uint32_t n = ctr->queue_count[cin];
-    const uint32_t i = v;
-#endif
-    if (v == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
+    const uint32_t i = trace_block() * kTraceBlock + threadIdx.x;
+    if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
     O.alive = O.shadow = false;
     O.rec = kNoRecord;
     uint32_t slot = 0;
-    if (v < n) {
+    if (i < n) {
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
         float4 r0 = rp[0], r1 = rp[1];
         float4 s = st[i];
@@ -536,7 +487,6 @@ __global__ void k_cache_reset(uint32_t* counters, uint32_t which) {
 __global__ void k_prep(DevCounters* ctr, uint32_t cout) {
     ctr->queue_count[cout] = 0;
     ctr->queue_count[2] = 0;
-    for (int o = 0; o < 8; ++o) ctr->oct_count[cout][o] = 0;
     ctr->ovf_count = 0;
 }
 
@@ -571,11 +521,11 @@ __global__ __launch_bounds__(kBlock) void k_film_add(uint32_t npix, uint32_t spp
 }
 
 int alloc_pass(rc_ctx* c, size_t npaths) {
-    RC_TRY(c->ray_a.ensure(npaths * kQueueRegions * sizeof(rc_ray)));
-    RC_TRY(c->ray_b.ensure(npaths * kQueueRegions * sizeof(rc_ray)));
-    RC_TRY(c->st_a.ensure(npaths * kQueueRegions * sizeof(float4)));
-    RC_TRY(c->st_b.ensure(npaths * kQueueRegions * sizeof(float4)));
-    RC_TRY(c->hits.ensure(npaths * kQueueRegions * sizeof(rc_hit)));
+    RC_TRY(c->ray_a.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(c->ray_b.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(c->st_a.ensure(npaths * sizeof(float4)));
+    RC_TRY(c->st_b.ensure(npaths * sizeof(float4)));
+    RC_TRY(c->hits.ensure(npaths * sizeof(rc_hit)));
     RC_TRY(c->srays.ensure(npaths * sizeof(rc_ray)));
     RC_TRY(c->scontrib.ensure(npaths * sizeof(float4)));
     RC_TRY(c->socc.ensure(npaths));
@@ -667,7 +617,6 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     P.sun_e = make_float3(cfg->sun_irradiance[0], cfg->sun_irradiance[1], cfg->sun_irradiance[2]);
     P.has_sun = (cfg->sun_irradiance[0] > 0.0f || cfg->sun_irradiance[1] > 0.0f || cfg->sun_irradiance[2] > 0.0f);
     P.seed = cfg->seed;
-    P.cap = uint32_t(npaths);
     const uint32_t grid_small = uint32_t(c->cu_count) * 2u;  // list-driven cold-path kernels
     rc_ray* ra = c->ray_a.as<rc_ray>();
     rc_ray* rb = c->ray_b.as<rc_ray>();
