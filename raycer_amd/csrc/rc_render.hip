@@ -4,7 +4,8 @@
 // shadow part of Raytracer (Raytracer.cpp:312-407, 524-551).
 //
 // A render is a sequence of passes; a pass traces spp_per_pass samples of
-// every pixel.  Slot s = j * npix + pixel holds sample j of the pass.  Per
+// every pixel.  Slot s = j * npix + q holds sample j of the pass for the
+// pixel at position q of the tiled pixel order (PixelOrder below).  Per
 // bounce: extend (closest hit over the live queue) -> shade (emission, sky,
 // sun next-event ray, cosine bounce, Russian roulette; survivors compacted
 // into the next queue with one atomic per wave) -> shadow (any hit) -> apply.
@@ -96,8 +97,50 @@ __device__ __forceinline__ float3 cam_dir(const CamBasis& b, float px, float py)
     return normalize3(p - b.pos);
 }
 
+// Pixel order of the slots of a pass: the 8x8 tiles of the largest tiled
+// region first (tiles row by row, pixels row by row inside a tile), then the
+// right strip and the bottom strip row by row.  A wave of 64 consecutive slots
+// is one 8x8 tile, and since the queues keep slot order from bounce to bounce
+// its rays start from nearby points at every bounce.
+struct PixelOrder {
+    uint32_t width, tiles_x, x0, y0, a;  // x0 = 8 tiles_x, y0 = 8 tiles_y, a = x0 y0
+};
+
+PixelOrder make_order(uint32_t w, uint32_t h) {
+    PixelOrder o;
+    o.width = w;
+    o.tiles_x = w / 8;
+    o.x0 = 8 * (w / 8);
+    o.y0 = 8 * (h / 8);
+    o.a = o.x0 * o.y0;
+    return o;
+}
+
+__device__ __forceinline__ uint32_t order_to_pixel(const PixelOrder& o, uint32_t q) {
+    if (q < o.a) {
+        const uint32_t t = q >> 6, w = q & 63u, ty = t / o.tiles_x, tx = t - ty * o.tiles_x;
+        return (ty * 8 + (w >> 3)) * o.width + tx * 8 + (w & 7u);
+    }
+    uint32_t r = q - o.a;
+    const uint32_t rw = o.width - o.x0;
+    if (r < rw * o.y0) {
+        const uint32_t y = r / rw;
+        return y * o.width + o.x0 + (r - y * rw);
+    }
+    r -= rw * o.y0;
+    return o.y0 * o.width + r;
+}
+
+__device__ __forceinline__ uint32_t pixel_to_order(const PixelOrder& o, uint32_t p) {
+    const uint32_t y = p / o.width, x = p - y * o.width;
+    if (x < o.x0 && y < o.y0) return ((((y >> 3) * o.tiles_x + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u));
+    if (y < o.y0) return o.a + y * (o.width - o.x0) + (x - o.x0);
+    return o.a + (o.width - o.x0) * o.y0 + (p - o.y0 * o.width);
+}
+
 struct PassParams {
     CamBasis cam;
+    PixelOrder order;
     uint32_t width, npix, spp_pass, sample0;
     uint32_t spp, sampler, jitter_n, filter;
     float radius, offset;
@@ -126,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
         ctr->queue_count[2] = 0;
     }
     if (slot >= n) return;
-    uint32_t j = slot / P.npix, pixel = slot - j * P.npix;
+    uint32_t j = slot / P.npix, pixel = order_to_pixel(P.order, slot - j * P.npix);
     uint32_t s = P.sample0 + j;
     float px = float(pixel % P.width), py = float(pixel / P.width);
     float w = 1.0f;
@@ -197,7 +240,7 @@ constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 // Training paths: a fixed-rate, frame-varying subset of the slots that run
 // to the end instead of stopping into the cache.
 __device__ __forceinline__ bool training_path(const PassParams& P, uint32_t slot, uint32_t stride) {
-    uint32_t pixel = slot % P.npix, sample = P.sample0 + slot / P.npix;
+    uint32_t pixel = order_to_pixel(P.order, slot % P.npix), sample = P.sample0 + slot / P.npix;
     uint32_t h = pixel * 2654435761u ^ (sample * 40503u + P.frame * 2246822519u);
     h ^= h >> 15;
     h *= 0x2C1B3C6Du;
@@ -237,7 +280,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
     float3 p = f3(r0) + (t * f3(r1));
     float3 rho = f3(dif);
     if (!more) return O;
-    float4 u = draw4(P.seed, slot % P.npix, P.sample0 + slot / P.npix, P.frame, k);
+    float4 u = draw4(P.seed, order_to_pixel(P.order, slot % P.npix), P.sample0 + slot / P.npix, P.frame, k);
     if (P.has_sun) {
         float cs = dot3(nrm, P.sun);
         if (cs > 0.0f) {
@@ -501,14 +544,15 @@ __device__ __forceinline__ void kahan(float& s, float& c, float v) {
 
 // Film::addSample (Film.cpp:54-59) for the pass's samples of each pixel, in
 // slot order, with FP32 Kahan compensation of the running sums.
-__global__ __launch_bounds__(kBlock) void k_film_add(uint32_t npix, uint32_t spp_pass, const float4* __restrict__ L,
-                                                     const float* __restrict__ W, float4* __restrict__ sum,
-                                                     float4* __restrict__ comp) {
+__global__ __launch_bounds__(kBlock) void k_film_add(PixelOrder order, uint32_t npix, uint32_t spp_pass,
+                                                     const float4* __restrict__ L, const float* __restrict__ W,
+                                                     float4* __restrict__ sum, float4* __restrict__ comp) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npix) return;
+    const uint32_t q = pixel_to_order(order, p);
     float4 s = sum[p], c = comp[p];
     for (uint32_t j = 0; j < spp_pass; ++j) {
-        uint32_t slot = j * npix + p;
+        uint32_t slot = j * npix + q;
         float w = W[slot];
         float4 l = L[slot];
         kahan(s.x, c.x, w * l.x);
@@ -600,6 +644,7 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     PassParams P;
     std::memset(&P, 0, sizeof(P));
     P.cam = make_basis(*cam, cfg->width, cfg->height);
+    P.order = make_order(cfg->width, cfg->height);
     P.width = cfg->width;
     P.npix = npix;
     P.spp = cfg->spp;
@@ -701,8 +746,9 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
                                                                   CI.batch_cap);
         }
         RC_TRY(c->prof_begin(&ev));
-        k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(npix, this_pass, c->slotL.as<float4>(),
-                                                                          c->slotW.as<float>(), f->sum, f->comp);
+        k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(P.order, npix, this_pass,
+                                                                          c->slotL.as<float4>(), c->slotW.as<float>(),
+                                                                          f->sum, f->comp);
         RC_TRY(c->prof_end(ev, PK_OTHER));
         RC_TRY_HIP(hipGetLastError());
     }
