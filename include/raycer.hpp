@@ -37,9 +37,40 @@ inline void check(int rc) {
 }
 
 enum class TracerType { RAY, PATH };
-enum class FilterType { BOX, TENT };
-enum class SamplerType { JITTERED, RANDOM };
-enum class ToneMapperType { REINHARD };
+// The reference's enums, in its order (Filter.h:12, Sampler.h:23, ToneMapper.h:14).
+enum class FilterType { BOX, TENT, BELL, GAUSSIAN, MITCHELL, LANCZOS_SINC };
+enum class SamplerType { CENTER, RANDOM, REGULAR, JITTERED, CMJ, POISSON_DISC };
+enum class ToneMapperType { PASSTHROUGH, LINEAR, SIMPLE, REINHARD };
+
+inline uint32_t to_rc(FilterType f) { return uint32_t(f); }  // RC_FILTER_* share the order
+inline uint32_t to_rc(SamplerType s) {
+    switch (s) {
+    case SamplerType::CENTER: return RC_SAMPLER_CENTER;
+    case SamplerType::RANDOM: return RC_SAMPLER_RANDOM;
+    case SamplerType::REGULAR: return RC_SAMPLER_REGULAR;
+    case SamplerType::JITTERED: return RC_SAMPLER_JITTERED;
+    case SamplerType::CMJ: return RC_SAMPLER_CMJ;
+    default: return RC_SAMPLER_POISSON_DISC;
+    }
+}
+inline SamplerType sampler_of(uint32_t s) {
+    switch (s) {
+    case RC_SAMPLER_CENTER: return SamplerType::CENTER;
+    case RC_SAMPLER_RANDOM: return SamplerType::RANDOM;
+    case RC_SAMPLER_REGULAR: return SamplerType::REGULAR;
+    case RC_SAMPLER_CMJ: return SamplerType::CMJ;
+    case RC_SAMPLER_POISSON_DISC: return SamplerType::POISSON_DISC;
+    default: return SamplerType::JITTERED;
+    }
+}
+inline uint32_t to_rc(ToneMapperType t) {
+    switch (t) {
+    case ToneMapperType::PASSTHROUGH: return RC_TONEMAP_PASSTHROUGH;
+    case ToneMapperType::LINEAR: return RC_TONEMAP_LINEAR;
+    case ToneMapperType::SIMPLE: return RC_TONEMAP_SIMPLE;
+    default: return RC_TONEMAP_REINHARD;
+    }
+}
 
 struct Color {
     double r = 0.0, g = 0.0, b = 0.0;
@@ -137,7 +168,9 @@ public:
         SamplerType multiSamplerType = SamplerType::JITTERED;
         uint64_t randomSampleCount = 0;  // RANDOM sampler: samples per pixel
         FilterType multiSamplerFilterType = FilterType::BOX;
-        double filterRadius = 0.5;
+        double filterRadius = 0.5;              // box, tent; lanczos_sinc (integer)
+        double gaussianStandardDeviation = 1.0; // GaussianFilter's default
+        double mitchellB = 1.0 / 3.0, mitchellC = 1.0 / 3.0;  // MitchellFilter's defaults
         double rayStartOffset = 0.00001;
         uint64_t russianRouletteStart = 0;  // path vertex where roulette starts; 0 = off
         bool lambertEstimator = false;      // false: the reference's 2 rho |cos| weight
@@ -149,13 +182,16 @@ public:
 
     Camera camera;
 
-    // Scene::ToneMapper (Scene.h:137-160), Reinhard.
+    // Scene::ToneMapper (Scene.h:137-160), with the reference's defaults.
     struct ToneMapper {
-        ToneMapperType type = ToneMapperType::REINHARD;
+        ToneMapperType type = ToneMapperType::LINEAR;
         bool applyGamma = true;
         bool shouldClamp = true;
         double gamma = 2.2;
+        double exposure = 0.0;
         double key = 0.18;
+        bool enableAveraging = false;
+        double averagingAlpha = 0.1;
         bool fastPow = true;  // MathUtils::fastPow, as the reference
     } toneMapper;
 
@@ -236,16 +272,22 @@ public:
         s.camera.orientation = EulerAngle{d.camera.pitch, d.camera.yaw, d.camera.roll};
         s.camera.fov = d.camera.fov;
         s.general.maxPathLength = c.max_path_length;
-        if (c.sampler == RC_SAMPLER_JITTERED) {
+        s.general.multiSamplerType = sampler_of(c.sampler);
+        if (c.sampler != RC_SAMPLER_RANDOM) {
             uint64_t n = 1;
             while ((n + 1) * (n + 1) <= c.spp) ++n;
             s.general.multiSampleCountSqrt = n;
         } else {
-            s.general.multiSamplerType = SamplerType::RANDOM;
             s.general.randomSampleCount = c.spp;
         }
-        s.general.multiSamplerFilterType = c.filter == RC_FILTER_TENT ? FilterType::TENT : FilterType::BOX;
+        s.general.multiSamplerFilterType = FilterType(c.filter);
         s.general.filterRadius = c.filter_radius;
+        if (c.filter == RC_FILTER_GAUSSIAN) s.general.gaussianStandardDeviation = c.filter_param[0];
+        if (c.filter == RC_FILTER_MITCHELL) {
+            s.general.mitchellB = c.filter_param[0];
+            s.general.mitchellC = c.filter_param[1];
+        }
+        s.toneMapper.type = ToneMapperType::REINHARD;  // the benchmark scenes are shown with Reinhard
         s.general.rayStartOffset = c.ray_start_offset;
         s.general.russianRouletteStart = c.rr_start;
         s.general.lambertEstimator = c.estimator == RC_EST_LAMBERT;
@@ -461,17 +503,21 @@ public:
         rc_render_cfg c{};
         c.width = uint32_t(width_);
         c.height = uint32_t(height_);
-        if (sc.general.multiSamplerType == SamplerType::JITTERED) {
-            c.sampler = RC_SAMPLER_JITTERED;
+        c.sampler = to_rc(sc.general.multiSamplerType);
+        if (sc.general.multiSamplerType != SamplerType::RANDOM)
             c.spp = uint32_t(sc.general.multiSampleCountSqrt * sc.general.multiSampleCountSqrt);
-        } else {
-            c.sampler = RC_SAMPLER_RANDOM;
+        else
             c.spp = uint32_t(sc.general.randomSampleCount);
-        }
         c.max_path_length = uint32_t(sc.general.maxPathLength);
         c.rr_start = uint32_t(sc.general.russianRouletteStart);
-        c.filter = sc.general.multiSamplerFilterType == FilterType::TENT ? RC_FILTER_TENT : RC_FILTER_BOX;
+        c.filter = to_rc(sc.general.multiSamplerFilterType);
         c.filter_radius = float(sc.general.filterRadius);
+        if (sc.general.multiSamplerFilterType == FilterType::GAUSSIAN) {
+            c.filter_param[0] = float(sc.general.gaussianStandardDeviation);
+        } else if (sc.general.multiSamplerFilterType == FilterType::MITCHELL) {
+            c.filter_param[0] = float(sc.general.mitchellB);
+            c.filter_param[1] = float(sc.general.mitchellC);
+        }
         c.estimator = sc.general.lambertEstimator ? RC_EST_LAMBERT : RC_EST_REFERENCE;
         c.ray_start_offset = float(sc.general.rayStartOffset);
         c.sky[0] = float(sc.general.skyRadiance.r);
@@ -518,11 +564,21 @@ public:
         return img;
     }
 
-    // ReinhardToneMapper on the device, then download (CLTracer::downloadImage).
+    // Film::generateToneMappedImage on the device with the scene's tone mapper,
+    // then download (CLTracer::downloadImage).  Reinhard's moving average of the
+    // maximum luminance lives with the film, as it lives with the reference's.
     Image downloadImage(const Scene::ToneMapper& tm = Scene::ToneMapper()) {
         Image img = blank();
-        rc_tonemap t{float(tm.key), float(tm.gamma), tm.shouldClamp ? 1u : 0u, tm.applyGamma ? 1u : 0u,
-                     tm.fastPow ? 1u : 0u, 0u};
+        rc_tonemap t{};
+        t.key = float(tm.key);
+        t.gamma = float(tm.gamma);
+        t.clamp = tm.shouldClamp ? 1u : 0u;
+        t.apply_gamma = tm.applyGamma ? 1u : 0u;
+        t.fast_pow = tm.fastPow ? 1u : 0u;
+        t.type = to_rc(tm.type);
+        t.exposure = float(tm.exposure);
+        t.enable_averaging = tm.enableAveraging ? 1u : 0u;
+        t.averaging_alpha = float(tm.averagingAlpha);
         check(rc_film_tonemap(ctx_, film_, &t, img.pixels.data(), nullptr, nullptr));
         return img;
     }

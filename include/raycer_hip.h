@@ -88,22 +88,46 @@ typedef struct rc_camera {
     float fov;
 } rc_camera;
 
-enum { RC_FILTER_BOX = 0, RC_FILTER_TENT = 1 };
-enum { RC_SAMPLER_JITTERED = 0, RC_SAMPLER_RANDOM = 1 };
+/* Filter::getFilter's FilterType (src/Rendering/Filters/Filter.h:12) with the
+ * reference's shapes: box and tent take filter_radius (BoxFilter.cpp:15-37,
+ * TentFilter.cpp:12-44); bell has radius 1.5 (BellFilter.cpp:12-28); gaussian
+ * takes the standard deviation filter_param[0] and has radius 7.43384 sigma / 2
+ * (GaussianFilter.cpp:15-23); mitchell takes B = filter_param[0], C =
+ * filter_param[1] and has radius 2 (MitchellFilter.cpp:12-37; the reference's
+ * default is B = C = 1/3); lanczos_sinc takes the integer radius filter_radius
+ * (LanczosSincFilter.cpp:12-40; the reference's default is 2). */
+enum {
+    RC_FILTER_BOX = 0, RC_FILTER_TENT = 1, RC_FILTER_BELL = 2, RC_FILTER_GAUSSIAN = 3,
+    RC_FILTER_MITCHELL = 4, RC_FILTER_LANCZOS_SINC = 5
+};
+/* Sampler::getSampler's SamplerType (src/Rendering/Samplers/Sampler.h:23) as
+ * Tracer::generateMultiSamples uses it (Tracer.cpp:115-144): every sampler but
+ * random needs spp = n*n and draws sample s = y*n + x of the n x n grid; one
+ * sample is the pixel centre with weight 1.  center: (0.5, 0.5)
+ * (CenterSampler.cpp:19-33); regular: ((x + 0.5)/n, (y + 0.5)/n)
+ * (RegularSampler.cpp:19-34); cmj: correlated multi-jittered with one 64-bit
+ * permutation per pixel and frame (CMJSampler.cpp:12-77); poisson_disc: the
+ * point (0, 0), which is what PoissonDiscSampler::getSample2D returns
+ * (PoissonDiscSampler.cpp:22-32): every sample then sits at offset (-r, -r)
+ * from the pixel, r the filter radius, as in the reference. */
+enum {
+    RC_SAMPLER_JITTERED = 0, RC_SAMPLER_RANDOM = 1, RC_SAMPLER_CENTER = 2, RC_SAMPLER_REGULAR = 3,
+    RC_SAMPLER_CMJ = 4, RC_SAMPLER_POISSON_DISC = 5
+};
 enum { RC_EST_REFERENCE = 0, RC_EST_LAMBERT = 1 };
 
 /* Render settings: Scene::General names where they exist (Scene.h:89-131). */
 typedef struct rc_render_cfg {
     uint32_t width, height;
-    uint32_t spp;               /* samples per pixel; jittered needs spp = n*n (multiSampleCountSqrt = n) */
-    uint32_t sampler;           /* RC_SAMPLER_JITTERED (1 sample = pixel centre, weight 1) or RC_SAMPLER_RANDOM */
+    uint32_t spp;               /* samples per pixel; all samplers but random need spp = n*n (multiSampleCountSqrt = n) */
+    uint32_t sampler;           /* RC_SAMPLER_* (multiSamplerType) */
     uint32_t max_path_length;   /* maxPathLength: rays per path including the camera ray */
     uint32_t rr_start;          /* Russian roulette at path vertex k >= rr_start; 0 = off */
     uint32_t filter;            /* RC_FILTER_* (multiSamplerFilterType) */
     uint32_t estimator;         /* RC_EST_REFERENCE: throughput *= 2 rho |n.w|; RC_EST_LAMBERT: *= rho */
     uint32_t cache_vertex;      /* with a radiance cache: path vertex that ends in the cache (0 = off) */
-    uint32_t pad0;
-    float filter_radius;
+    float filter_radius;        /* box, tent: radius; lanczos_sinc: integer radius */
+    float filter_param[2];      /* gaussian: standard deviation; mitchell: B, C */
     float ray_start_offset;     /* rayStartOffset */
     float sky[3];               /* radiance on a miss (the reference returns black) */
     float sun_dir[3];           /* unit vector towards a directional sun */
@@ -112,13 +136,25 @@ typedef struct rc_render_cfg {
     uint64_t seed;
 } rc_render_cfg;
 
-/* Tone-mapper settings (Scene::ToneMapper, Scene.h:137-160), Reinhard only. */
+/* Tone mappers (ToneMapper::getToneMapper, src/Rendering/ToneMappers/ToneMapper.cpp:13-23).
+ * reinhard: ReinhardToneMapper.cpp:18-86; linear: c * 2^exposure
+ * (LinearToneMapper.cpp:12-35); simple: c' / (1 + c') with c' = c * 2^exposure
+ * (SimpleToneMapper.cpp:12-37); passthrough: the linear image as it is
+ * (PassthroughToneMapper.cpp:12-22).  Reinhard is 0 so that a zeroed struct
+ * selects it. */
+enum { RC_TONEMAP_REINHARD = 0, RC_TONEMAP_LINEAR = 1, RC_TONEMAP_SIMPLE = 2, RC_TONEMAP_PASSTHROUGH = 3 };
+
+/* Tone-mapper settings (Scene::ToneMapper, Scene.h:137-160). */
 typedef struct rc_tonemap {
-    float key;
+    float key;                  /* reinhard */
     float gamma;
-    uint32_t clamp;
-    uint32_t apply_gamma;
-    uint32_t fast_pow;          /* 1 = MathUtils::fastPow (reference), 0 = pow */
+    uint32_t clamp;             /* shouldClamp */
+    uint32_t apply_gamma;       /* applyGamma */
+    uint32_t fast_pow;          /* 1 = MathUtils::fastPow for gamma and 2^exposure (reference), 0 = pow */
+    uint32_t type;              /* RC_TONEMAP_* */
+    float exposure;             /* linear, simple */
+    uint32_t enable_averaging;  /* reinhard: moving average of the maximum luminance, kept by the film */
+    float averaging_alpha;      /* in (0, 1] */
     uint32_t pad;
 } rc_tonemap;
 
@@ -219,8 +255,12 @@ int rc_film_clear(rc_ctx* ctx, rc_film* film);
 int rc_film_download_linear(rc_ctx* ctx, const rc_film* film, float* rgba);
 /* raw accumulators: per pixel sum.rgb, weight (4 floats) */
 int rc_film_download_sums(rc_ctx* ctx, const rc_film* film, float* sums);
-/* ReinhardToneMapper::apply on the device (ReinhardToneMapper.cpp:18-86);
- * log_avg / max_lum may be NULL. */
+/* Film::generateToneMappedImage's tone map on the device (Film.cpp:61-69) with
+ * tm->type's mapper.  For reinhard, log_avg / max_lum (either may be NULL) get
+ * the log-average luminance and the maximum luminance used (after the moving
+ * average when tm->enable_averaging; the film keeps the average, starting at
+ * 1, as ReinhardToneMapper's maxLuminanceAverage does, and rc_film_clear does
+ * not reset it); other mappers set them to 0. */
 int rc_film_tonemap(rc_ctx* ctx, const rc_film* film, const rc_tonemap* tm, float* rgba, double* log_avg,
                     double* max_lum);
 

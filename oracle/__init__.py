@@ -24,7 +24,7 @@ class RenderCfg(C.Structure):
                 ("estimator", C.c_uint32), ("traversal", C.c_uint32), ("filter_radius", C.c_double),
                 ("ray_start_offset", C.c_double), ("sky", C.c_double * 3), ("sun_dir", C.c_double * 3),
                 ("sun_irradiance", C.c_double * 3), ("seed", C.c_uint64), ("frame", C.c_uint32),
-                ("cache_vertex", C.c_uint32)]
+                ("cache_vertex", C.c_uint32), ("filter_param", C.c_double * 2)]
 
 
 _libs: dict[str, C.CDLL] = {}
@@ -43,7 +43,13 @@ def lib(fast: bool = False) -> C.CDLL:
         L.or_uniform.restype = d
         L.or_uniform.argtypes = [C.c_uint32]
         L.or_filter_weight.restype = d
-        L.or_filter_weight.argtypes = [C.c_uint32, d, d, d]
+        L.or_filter_weight.argtypes = [C.c_uint32, d, d, d, d, d]
+        L.or_filter_radius.restype = d
+        L.or_filter_radius.argtypes = [C.c_uint32, d, d, d]
+        L.or_cmj_permute.restype = C.c_uint64
+        L.or_cmj_permute.argtypes = [C.c_uint64] * 3
+        L.or_cmj_sample.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, d, d, vp]
+        L.or_tonemap.argtypes = [vp, sz, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
         L.or_bvh_build.argtypes = [vp, sz, C.c_uint64, C.POINTER(vp)]
         L.or_bvh_node_count.restype = sz
         L.or_bvh_node_count.argtypes = [vp]
@@ -205,6 +211,7 @@ def render_cfg(cfg, traversal: int = 1) -> RenderCfg:
               "frame", "cache_vertex"):
         setattr(o, f, getattr(cfg, f))
     o.filter_radius = cfg.filter_radius
+    o.filter_param[0], o.filter_param[1] = cfg.filter_param[0], cfg.filter_param[1]
     o.ray_start_offset = cfg.ray_start_offset
     for k in range(3):
         o.sky[k] = cfg.sky[k]
@@ -263,8 +270,43 @@ def reinhard(linear_rgba: np.ndarray, key=0.18, gamma=2.2, clamp=True, apply_gam
     return out, la.value, ml.value
 
 
-def filter_weight(f: int, r: float, x: float, y: float) -> float:
-    return lib().or_filter_weight(f, r, x, y)
+def filter_weight(f: int, r: float, x: float, y: float, p0: float = 0.0, p1: float = 0.0) -> float:
+    """2-D filter weight; p0, p1 = Gaussian standard deviation, or Mitchell's B and C."""
+    return lib().or_filter_weight(f, r, p0, p1, x, y)
+
+
+def filter_radius(f: int, r: float, p0: float = 0.0, p1: float = 0.0) -> float:
+    return lib().or_filter_radius(f, r, p0, p1)
+
+
+def cmj_permute(i: int, n: int, p: int) -> int:
+    return lib().or_cmj_permute(i, n, p)
+
+
+def cmj_sample(x: int, y: int, n: int, p: int, r0: float, r1: float):
+    o = np.empty(2)
+    lib().or_cmj_sample(x, y, n, p, r0, r1, _p(o))
+    return o
+
+
+class ToneMapState:
+    """ReinhardToneMapper's moving average of the maximum luminance (starts at 1)."""
+
+    def __init__(self):
+        self.max_lum_average = C.c_double(1.0)
+
+
+def tonemap(linear_rgba: np.ndarray, type=0, key=0.18, gamma=2.2, clamp=True, apply_gamma=True, fast_pow=True,
+            exposure=0.0, enable_averaging=False, averaging_alpha=0.1, state: ToneMapState | None = None):
+    """type: 0 Reinhard, 1 linear, 2 simple, 3 passthrough.  Returns (rgba, log_avg, max_lum)."""
+    lin = np.ascontiguousarray(linear_rgba, np.float32).reshape(-1, 4)
+    out = np.empty_like(lin)
+    prm = np.array([type, key, gamma, float(clamp), float(apply_gamma), float(fast_pow), exposure,
+                    float(enable_averaging), averaging_alpha])
+    st = state or ToneMapState()
+    la, ml = C.c_double(), C.c_double()
+    lib().or_tonemap(_p(lin), lin.shape[0], _p(prm), _p(out), C.byref(la), C.byref(ml), C.byref(st.max_lum_average))
+    return out, la.value, ml.value
 
 
 def hemisphere(n, u1, u2):

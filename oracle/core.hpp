@@ -72,6 +72,17 @@ inline void draw4(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t frame
     for (int i = 0; i < 4; ++i) u[i] = u01(o[i]);
 }
 
+// The CMJ sampler's per-pixel 64-bit permutation: counter (pixel, 0, frame,
+// PERM_DIM), low word first.
+static const uint32_t PERM_DIM = 0xFFFFFFFEu;
+inline uint64_t pixel_permutation(uint64_t seed, uint32_t pixel, uint32_t frame) {
+    uint32_t ctr[4] = {pixel, 0u, frame, PERM_DIM};
+    uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t o[4];
+    philox4x32_10(ctr, key, o);
+    return uint64_t(o[0]) | (uint64_t(o[1]) << 32);
+}
+
 // ---------------------------------------------------------------- geometry
 struct Tri {
     V3 v0, v1, v2;

@@ -19,9 +19,12 @@
  *                               JitteredSampler.cpp:21-34)
  *   - mapToHemisphere           (src/Rendering/Samplers/Sampler.cpp:50-64)
  *   - PathTracer::traceRecursive (src/Raytracing/Tracers/PathTracer.cpp:25-75)
- *   - Box/Tent filters          (BoxFilter.cpp:15-37, TentFilter.cpp:12-44)
+ *   - the six filters           (src/Rendering/Filters/)
+ *   - the six samplers as generateMultiSamples uses them, CMJ's permute
+ *                               (src/Rendering/Samplers/, CMJSampler.cpp:12-77)
  *   - Film::addSample / generateToneMappedImage (src/Rendering/Film.cpp:54-69)
- *   - ReinhardToneMapper::apply (ReinhardToneMapper.cpp:18-86)
+ *   - ReinhardToneMapper::apply (ReinhardToneMapper.cpp:18-86), Linear,
+ *     Simple and Passthrough tone mappers (src/Rendering/ToneMappers/)
  *   - MathUtils::fastPow        (src/Math/MathUtils.cpp:56-68)
  *   - Raytracer Phong + point-light shadow (Raytracer.cpp:312-407, 524-551)
  * plus the estimator terms the reference does not have (Russian roulette, sky
@@ -101,10 +104,10 @@ void or_camera_ray(const double cam[7], uint32_t width, uint32_t height,
 typedef struct or_render_cfg {
     uint32_t width, height;
     uint32_t spp;               /* samples per pixel (jittered: n*n) */
-    uint32_t sampler;           /* 0 jittered (JitteredSampler), 1 random (RandomSampler) */
+    uint32_t sampler;           /* 0 jittered, 1 random, 2 center, 3 regular, 4 CMJ, 5 Poisson disc */
     uint32_t max_path_length;   /* maxPathLength */
     uint32_t rr_start;          /* RR at path vertex k >= rr_start (0 = off) */
-    uint32_t filter;            /* 0 box, 1 tent */
+    uint32_t filter;            /* 0 box, 1 tent, 2 bell, 3 gaussian, 4 mitchell, 5 lanczos sinc */
     uint32_t estimator;         /* 0 reference 2*rho*|cos|, 1 lambert rho */
     uint32_t traversal;         /* 0 reference unordered BVH, 1 ordered culled BVH */
     double filter_radius;
@@ -115,6 +118,7 @@ typedef struct or_render_cfg {
     uint64_t seed;
     uint32_t frame;
     uint32_t cache_vertex;      /* unused by the oracle renderer (no cache) */
+    double filter_param[2];     /* gaussian: standard deviation; mitchell: B, C */
 } or_render_cfg;
 
 /* materials: nmat * 16 doubles:
@@ -154,8 +158,19 @@ void or_raycast_phong(const or_bvh* b, const float* vertices, const uint32_t* tr
 void or_reinhard(const float* linear, size_t npix, const double params[5], float* out,
                  double* log_avg_out, double* max_lum_out);
 
+/* general tone map: params = type (0 reinhard, 1 linear, 2 simple, 3 passthrough),
+ * key, gamma, clamp, apply_gamma, use_fastpow, exposure, enable_averaging,
+ * averaging_alpha.  *max_lum_average is the moving average's state (read and,
+ * with averaging, updated; start it at 1). */
+void or_tonemap(const float* linear, size_t npix, const double params[9], float* out, double* log_avg_out,
+                double* max_lum_out, double* max_lum_average);
+
 /* ---------------------------------------------------------------- filters */
-double or_filter_weight(uint32_t filter, double radius, double x, double y);
+/* p0, p1: gaussian standard deviation; mitchell B, C (ignored otherwise) */
+double or_filter_weight(uint32_t filter, double radius, double p0, double p1, double x, double y);
+double or_filter_radius(uint32_t filter, double radius, double p0, double p1);
+uint64_t or_cmj_permute(uint64_t i, uint64_t l, uint64_t p);
+void or_cmj_sample(uint64_t x, uint64_t y, uint64_t n, uint64_t p, double r0, double r1, double out[2]);
 /* cosine hemisphere (Sampler.cpp:50-64) with ONB::fromNormal-style tangents of a
  * triangle without texcoords (Triangle.cpp:40-44). */
 void or_hemisphere(const double n[3], double u1, double u2, double out[3]);

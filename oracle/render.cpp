@@ -37,16 +37,114 @@ extern "C" void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], u
 }
 extern "C" double or_uniform(uint32_t x) { return u01(x); }
 
-// BoxFilter (BoxFilter.cpp:15-37) and TentFilter (TentFilter.cpp:12-44); the
-// 2-D weight is the product wx * wy (Filter.cpp:31-39).
-static double filter_1d(uint32_t f, double r, double x) {
-    if (f == 0) return (x >= -r && x < r) ? 1.0 / (2.0 * r) : 0.0;
-    double rinv = 1.0 / r;
-    double s = std::abs(x * rinv);
-    return (s < 1.0 ? 1.0 - s : 0.0) * rinv;
+// The filters of Filter::getFilter (src/Rendering/Filters/Filter.cpp:17-29);
+// the 2-D weight is the product wx * wy (Filter.cpp:31-39).  p0, p1: the
+// Gaussian's standard deviation, or Mitchell's B and C.
+namespace {
+struct FilterShape {
+    uint32_t f;
+    double radius, a, b;
+};
+FilterShape filter_shape(uint32_t f, double r, double p0, double p1) {
+    FilterShape s{f, r, 0.0, 0.0};
+    switch (f) {
+    case 2: s.radius = 1.5; break;                     // BellFilter.cpp:24-28
+    case 3:                                            // GaussianFilter.cpp:15-23
+        s.a = 1.0 / (std::sqrt(2.0 * M_PI) * p0);
+        s.b = -1.0 / (2.0 * p0 * p0);
+        s.radius = (7.43384 * p0) / 2.0;
+        break;
+    case 4: s.radius = 2.0; s.a = p0; s.b = p1; break;  // MitchellFilter.cpp:30-37
+    default: break;                                    // box, tent, lanczos: r
+    }
+    return s;
 }
-extern "C" double or_filter_weight(uint32_t f, double r, double x, double y) {
-    return filter_1d(f, r, x) * filter_1d(f, r, y);
+double filter_1d(const FilterShape& F, double x) {
+    const double r = F.radius;
+    switch (F.f) {
+    case 0: return (x >= -r && x < r) ? 1.0 / (2.0 * r) : 0.0;  // BoxFilter.cpp:22-37
+    case 1: {                                                  // TentFilter.cpp:12-44
+        double rinv = 1.0 / r;
+        double s = std::abs(x * rinv);
+        return (s < 1.0 ? 1.0 - s : 0.0) * rinv;
+    }
+    case 2: {  // BellFilter.cpp:12-22
+        double s = std::abs(x);
+        if (s < 0.5) return 0.75 - s * s;
+        if (s <= 1.5) return 0.5 * std::pow(s - 1.5, 2.0);
+        return 0.0;
+    }
+    case 3: return F.a * std::exp(x * x * F.b);  // GaussianFilter.cpp:25-33
+    case 4: {                                    // MitchellFilter.cpp:12-23
+        const double B = F.a, C = F.b, s = std::abs(x);
+        if (s <= 1.0)
+            return ((12.0 - 9.0 * B - 6.0 * C) * (s * s * s) + (-18.0 + 12.0 * B + 6.0 * C) * (s * s) + (6.0 - 2.0 * B)) *
+                   (1.0 / 6.0);
+        if (s <= 2.0)
+            return ((-B - 6.0 * C) * (s * s * s) + (6.0 * B + 30.0 * C) * (s * s) + (-12.0 * B - 48.0 * C) * s +
+                    (8.0 * B + 24.0 * C)) *
+                   (1.0 / 6.0);
+        return 0.0;
+    }
+    default: {  // LanczosSincFilter.cpp:12-30
+        double s = std::abs(x);
+        if (s == 0.0) return 1.0;
+        if (s > r) return 0.0;
+        auto sinc = [](double v) { return std::sin(M_PI * v) / (M_PI * v); };
+        return sinc(s) * sinc(s / r);
+    }
+    }
+}
+
+// CMJSampler's permute (CMJSampler.cpp:12-44), Kensler's hash permutation
+// ("Correlated Multi-Jittered Sampling", Pixar Technical Memo 13-01) in the
+// reference's 64-bit arithmetic.
+uint64_t cmj_permute(uint64_t i, uint64_t l, uint64_t p) {
+    uint64_t w = l - 1;
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    w |= w >> 8;
+    w |= w >> 16;
+    do {
+        i ^= p;
+        i *= 0xe170893dull;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8;
+        i *= 0x0929eb3full;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1;
+        i *= 1 | p >> 27;
+        i *= 0x6935fa69ull;
+        i ^= (i & w) >> 11;
+        i *= 0x74dcb303ull;
+        i ^= (i & w) >> 2;
+        i *= 0x9e501cc3ull;
+        i ^= (i & w) >> 2;
+        i *= 0xc860a3dfull;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return (i + p) % l;
+}
+}  // namespace
+
+extern "C" double or_filter_weight(uint32_t f, double r, double p0, double p1, double x, double y) {
+    FilterShape F = filter_shape(f, r, p0, p1);
+    return filter_1d(F, x) * filter_1d(F, y);
+}
+extern "C" double or_filter_radius(uint32_t f, double r, double p0, double p1) {
+    return filter_shape(f, r, p0, p1).radius;
+}
+extern "C" uint64_t or_cmj_permute(uint64_t i, uint64_t l, uint64_t p) { return cmj_permute(i, l, p); }
+/* The CMJ sample (x, y) of an n x n pattern (CMJSampler.cpp:57-77) for
+ * permutation p and jitter (r0, r1). */
+extern "C" void or_cmj_sample(uint64_t x, uint64_t y, uint64_t n, uint64_t p, double r0, double r1, double out[2]) {
+    const uint64_t sx = cmj_permute(x, n, p * 0x68bc21ebull);
+    const uint64_t sy = cmj_permute(y, n, p * 0x02e5be93ull);
+    out[0] = (double(x) + (double(sy) + r0) / double(n)) / double(n);
+    out[1] = (double(y) + (double(sx) + r1) / double(n)) / double(n);
 }
 extern "C" void or_hemisphere(const double n[3], double u1, double u2, double out[3]) {
     V3 r = hemisphere(V3(n[0], n[1], n[2]), u1, u2);
@@ -221,24 +319,44 @@ void sample_camera(const Cam& k, const or_render_cfg& c, uint32_t pixel, uint32_
     // stratum (x, y) of an n x n grid (JitteredSampler.cpp:21-34) or a uniform
     // point (RandomSampler.cpp:23-35), offset (s - 0.5) * 2 * radius, weighted by
     // the filter.
-    if (c.sampler == 0 && c.spp == 1) {
+    // Samplers (src/Rendering/Samplers/*.cpp): 0 jittered, 1 random, 2 center,
+    // 3 regular, 4 CMJ with one 64-bit permutation per pixel and frame, 5
+    // Poisson disc, whose getSample2D returns (0, 0) (PoissonDiscSampler.cpp:22-32).
+    if (c.sampler != 1 && c.spp == 1) {
         w = 1.0;
     } else {
         double u[4];
         draw4(c.seed, pixel, s, c.frame, CAM_DIM, u);
+        const uint32_t n = isqrt_exact(c.spp);
+        const uint32_t sx = s % n, sy = s / n;
         double ox, oy;
-        if (c.sampler == 0) {
-            uint32_t n = isqrt_exact(c.spp);
-            uint32_t sx = s % n, sy = s / n;
+        switch (c.sampler) {
+        case 0:
             ox = (double(sx) + u[0]) / double(n);
             oy = (double(sy) + u[1]) / double(n);
-        } else {
+            break;
+        case 1:
             ox = u[0];
             oy = u[1];
+            break;
+        case 2: ox = oy = 0.5; break;
+        case 3:
+            ox = (double(sx) + 0.5) / double(n);
+            oy = (double(sy) + 0.5) / double(n);
+            break;
+        case 4: {
+            double o[2];
+            or_cmj_sample(sx, sy, n, pixel_permutation(c.seed, pixel, c.frame), u[0], u[1], o);
+            ox = o[0];
+            oy = o[1];
+            break;
         }
-        ox = (ox - 0.5) * 2.0 * c.filter_radius;
-        oy = (oy - 0.5) * 2.0 * c.filter_radius;
-        w = filter_1d(c.filter, c.filter_radius, ox) * filter_1d(c.filter, c.filter_radius, oy);
+        default: ox = oy = 0.0; break;
+        }
+        const FilterShape F = filter_shape(c.filter, c.filter_radius, c.filter_param[0], c.filter_param[1]);
+        ox = (ox - 0.5) * 2.0 * F.radius;
+        oy = (oy - 0.5) * 2.0 * F.radius;
+        w = filter_1d(F, ox) * filter_1d(F, oy);
         px += ox;
         py += oy;
     }
@@ -368,6 +486,65 @@ extern "C" void or_raycast_phong(const or_bvh* b, const float* vertices, const u
             col = col + ph * (da * (shadow ? 0.0 : 1.0));
         }
         rgb_out[3 * i] = col.x; rgb_out[3 * i + 1] = col.y; rgb_out[3 * i + 2] = col.z;
+    }
+}
+
+// ---------------------------------------------------------------- tone mappers
+// ToneMapper::getToneMapper's four mappers (ToneMapper.cpp:13-23) on the linear
+// image: Reinhard below (with MovingAverage::addMeasurement on the maximum,
+// MovingAverage.cpp:28-31, when averaging), Linear c * fastPow(2, exposure)
+// (LinearToneMapper.cpp:12-35), Simple c' / (1 + c') of that
+// (SimpleToneMapper.cpp:12-37), Passthrough a copy (PassthroughToneMapper.cpp:12-22).
+extern "C" void or_tonemap(const float* lin, size_t npix, const double prm[9], float* out, double* log_avg_out,
+                           double* max_lum_out, double* max_lum_average) {
+    const uint32_t type = uint32_t(prm[0]);
+    const double key = prm[1], gamma = prm[2], exposure = prm[6], alpha = prm[8];
+    const bool do_clamp = prm[3] != 0.0, do_gamma = prm[4] != 0.0, fast = prm[5] != 0.0, avg = prm[7] != 0.0;
+    if (log_avg_out) *log_avg_out = 0.0;
+    if (max_lum_out) *max_lum_out = 0.0;
+    if (type == 3) {
+        for (size_t i = 0; i < 4 * npix; ++i) out[i] = lin[i];
+        return;
+    }
+    double scale = 0.0, inv2 = 0.0;
+    if (type == 0) {
+        double logsum = 0.0, maxl = 1.0;
+        for (size_t i = 0; i < npix; ++i) {
+            double l = 0.2126 * double(lin[4 * i]) + 0.7152 * double(lin[4 * i + 1]) + 0.0722 * double(lin[4 * i + 2]);
+            logsum += std::log(0.01 + l);
+            if (l > maxl) maxl = l;
+        }
+        if (avg) {
+            *max_lum_average = alpha * maxl + (1.0 - alpha) * *max_lum_average;
+            maxl = *max_lum_average;
+        }
+        double logavg = std::exp(logsum / double(npix));
+        scale = key / logavg;
+        inv2 = 1.0 / (maxl * maxl);
+        if (log_avg_out) *log_avg_out = logavg;
+        if (max_lum_out) *max_lum_out = maxl;
+    }
+    const double expo = fast ? or_fastpow(2.0, exposure) : std::pow(2.0, exposure);
+    const double ig = 1.0 / gamma;
+    for (size_t i = 0; i < npix; ++i) {
+        double c[3] = {lin[4 * i], lin[4 * i + 1], lin[4 * i + 2]};
+        if (type == 0) {
+            double l = 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2];
+            double sl = scale * l;
+            double cs = ((sl * (1.0 + (sl * inv2))) / (1.0 + sl)) / l;
+            for (double& v : c) v *= cs;
+        } else {
+            for (double& v : c) v *= expo;
+            if (type == 2)
+                for (double& v : c) v = v / (1.0 + v);
+        }
+        for (int k = 0; k < 3; ++k) {
+            double v = c[k];
+            if (do_clamp) v = std::max(0.0, std::min(v, 1.0));
+            if (do_gamma) v = fast ? or_fastpow(v, ig) : std::pow(v, ig);
+            out[4 * i + k] = float(v);
+        }
+        out[4 * i + 3] = 1.0f;
     }
 }
 

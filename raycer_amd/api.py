@@ -20,8 +20,10 @@ LIB_PATH = Path(os.environ.get("RAYCER_LIBRARY") or str(PKG / "libraycer.so"))
 
 RC_OK, RC_ERR_INVALID, RC_ERR_HIP, RC_ERR_ALLOC, RC_ERR_INTERRUPTED, RC_ERR_NO_DEVICE = range(6)
 RC_MAT_EMISSIVE, RC_MAT_NON_SHADOWING = 1, 2
-RC_FILTER_BOX, RC_FILTER_TENT = 0, 1
-RC_SAMPLER_JITTERED, RC_SAMPLER_RANDOM = 0, 1
+RC_FILTER_BOX, RC_FILTER_TENT, RC_FILTER_BELL, RC_FILTER_GAUSSIAN, RC_FILTER_MITCHELL, RC_FILTER_LANCZOS_SINC = range(6)
+(RC_SAMPLER_JITTERED, RC_SAMPLER_RANDOM, RC_SAMPLER_CENTER, RC_SAMPLER_REGULAR, RC_SAMPLER_CMJ,
+ RC_SAMPLER_POISSON_DISC) = range(6)
+RC_TONEMAP_REINHARD, RC_TONEMAP_LINEAR, RC_TONEMAP_SIMPLE, RC_TONEMAP_PASSTHROUGH = range(4)
 RC_EST_REFERENCE, RC_EST_LAMBERT = 0, 1
 RCH_SCENE_BOX, RCH_SCENE_SPHERE_GROUND, RCH_SCENE_TERRAIN, RCH_SCENE_SOUP = 1, 2, 3, 4
 RCH_BVH_REFERENCE, RCH_BVH_BINNED = 0, 1
@@ -50,15 +52,16 @@ class Camera(C.Structure):
 class RenderCfg(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("sampler", C.c_uint32),
                 ("max_path_length", C.c_uint32), ("rr_start", C.c_uint32), ("filter", C.c_uint32),
-                ("estimator", C.c_uint32), ("cache_vertex", C.c_uint32), ("pad0", C.c_uint32),
-                ("filter_radius", C.c_float), ("ray_start_offset", C.c_float), ("sky", C.c_float * 3),
+                ("estimator", C.c_uint32), ("cache_vertex", C.c_uint32), ("filter_radius", C.c_float),
+                ("filter_param", C.c_float * 2), ("ray_start_offset", C.c_float), ("sky", C.c_float * 3),
                 ("sun_dir", C.c_float * 3), ("sun_irradiance", C.c_float * 3), ("frame", C.c_uint32),
                 ("seed", C.c_uint64)]
 
 
 class ToneMap(C.Structure):
     _fields_ = [("key", C.c_float), ("gamma", C.c_float), ("clamp", C.c_uint32), ("apply_gamma", C.c_uint32),
-                ("fast_pow", C.c_uint32), ("pad", C.c_uint32)]
+                ("fast_pow", C.c_uint32), ("type", C.c_uint32), ("exposure", C.c_float),
+                ("enable_averaging", C.c_uint32), ("averaging_alpha", C.c_float), ("pad", C.c_uint32)]
 
 
 class PointLight(C.Structure):
@@ -537,9 +540,12 @@ class Film:
         check(lib().rc_film_download_sums(self.ctx.h, self.h, _ptr(out)))
         return out
 
-    def tonemap(self, key=0.18, gamma=2.2, clamp=True, apply_gamma=True, fast_pow=True):
+    def tonemap(self, key=0.18, gamma=2.2, clamp=True, apply_gamma=True, fast_pow=True, type=RC_TONEMAP_REINHARD,
+                exposure=0.0, enable_averaging=False, averaging_alpha=0.1):
+        """(rgba, log_avg, max_lum); log_avg and max_lum are Reinhard's (0 for the other mappers)."""
         out = np.empty((self.width * self.height, 4), np.float32)
-        tm = ToneMap(key, gamma, int(clamp), int(apply_gamma), int(fast_pow), 0)
+        tm = ToneMap(key, gamma, int(clamp), int(apply_gamma), int(fast_pow), type, exposure, int(enable_averaging),
+                     averaging_alpha, 0)
         la, ml = C.c_double(), C.c_double()
         check(lib().rc_film_tonemap(self.ctx.h, self.h, C.byref(tm), _ptr(out), C.byref(la), C.byref(ml)))
         return out, la.value, ml.value

@@ -8,6 +8,9 @@
 //   raycer_render --scene box|sphere|terrain|soup|cornell [--seed N] [--level N] [--grid N]
 //                 [--tris N] [--mesh cornellbox.obj] [--tracer ray|path] [--width W] [--height H]
 //                 [--spp N] [--frames F] [--linear out.pfm] [--tonemapped out.ppm]
+//                 [--sampler center|random|regular|jittered|cmj|poisson_disc]
+//                 [--filter box|tent|bell|gaussian|mitchell|lanczos_sinc]
+//                 [--tonemapper passthrough|linear|simple|reinhard]
 //
 // cornell is raycer's TestScene3 (the Cornell box OBJ, by default the copy in
 // tests/golden/meshes next to this program's package); its default tracer is
@@ -35,10 +38,21 @@ uint32_t scene_kind(const std::string& s) {
     if (s == "soup") return RCH_SCENE_SOUP;
     throw raycer::RaycerError(RC_ERR_INVALID, "unknown scene " + s);
 }
+
+// the reference's SamplerType / FilterType / ToneMapperType names, lower case
+template <typename E, size_t N>
+E parse_enum(const std::string& s, const char* const (&names)[N], const char* what) {
+    for (size_t i = 0; i < N; ++i)
+        if (s == names[i]) return E(i);
+    throw raycer::RaycerError(RC_ERR_INVALID, std::string("unknown ") + what + " " + s);
+}
+const char* const kSamplers[] = {"center", "random", "regular", "jittered", "cmj", "poisson_disc"};
+const char* const kFilters[] = {"box", "tent", "bell", "gaussian", "mitchell", "lanczos_sinc"};
+const char* const kToneMappers[] = {"passthrough", "linear", "simple", "reinhard"};
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::string scene_name = "box", linear_out, tm_out, mesh, tracer_name;
+    std::string scene_name = "box", linear_out, tm_out, mesh, tracer_name, sampler_name, filter_name, tm_name;
     uint32_t seed = 0, level = 0, grid = 0, width = 0, height = 0, spp = 0, frames = 1;
     uint64_t tris = 0;
     for (int i = 1; i < argc; ++i) {
@@ -61,6 +75,9 @@ int main(int argc, char** argv) {
             else if (a == "--tonemapped") tm_out = next();
             else if (a == "--mesh") mesh = next();
             else if (a == "--tracer") tracer_name = next();
+            else if (a == "--sampler") sampler_name = next();
+            else if (a == "--filter") filter_name = next();
+            else if (a == "--tonemapper") tm_name = next();
             else {
                 std::fprintf(stderr, "unknown option %s\n", a.c_str());
                 return 2;
@@ -87,6 +104,19 @@ int main(int argc, char** argv) {
         } else {
             scene = raycer::Scene::createTestScene(scene_kind(scene_name), seed, level, grid, tris);
         }
+        if (!sampler_name.empty()) {
+            scene.general.multiSamplerType = parse_enum<raycer::SamplerType>(sampler_name, kSamplers, "sampler");
+            if (scene.general.multiSamplerType == raycer::SamplerType::RANDOM && scene.general.randomSampleCount == 0)
+                scene.general.randomSampleCount = scene.general.multiSampleCountSqrt * scene.general.multiSampleCountSqrt;
+        }
+        if (!filter_name.empty()) {
+            scene.general.multiSamplerFilterType = parse_enum<raycer::FilterType>(filter_name, kFilters, "filter");
+            // the reference filters' default sizes (BoxFilter.h, TentFilter.h, LanczosSincFilter.h)
+            if (scene.general.multiSamplerFilterType == raycer::FilterType::BOX) scene.general.filterRadius = 0.5;
+            if (scene.general.multiSamplerFilterType == raycer::FilterType::TENT) scene.general.filterRadius = 1.0;
+            if (scene.general.multiSamplerFilterType == raycer::FilterType::LANCZOS_SINC) scene.general.filterRadius = 2.0;
+        }
+        if (!tm_name.empty()) scene.toneMapper.type = parse_enum<raycer::ToneMapperType>(tm_name, kToneMappers, "tone mapper");
         if (tracer_name == "ray") scene.general.tracerType = raycer::TracerType::RAY;
         else if (tracer_name == "path") scene.general.tracerType = raycer::TracerType::PATH;
         else if (!tracer_name.empty()) throw raycer::RaycerError(RC_ERR_INVALID, "unknown tracer " + tracer_name);

@@ -93,6 +93,7 @@ struct rc_film {
     float4* sum = nullptr;   // w*rgb, w
     float4* comp = nullptr;  // Kahan compensation terms
     DevBuf linear, partial;
+    double max_lum_average = 1.0;  // ReinhardToneMapper's maxLuminanceAverage (ReinhardToneMapper.cpp:12-15)
 };
 
 // Launches the closest-hit (ANY = false) or any-hit traversal over a device

@@ -144,17 +144,94 @@ struct PassParams {
     uint32_t width, npix, spp_pass, sample0;
     uint32_t spp, sampler, jitter_n, filter;
     float radius, offset;
+    float fa, fb;  // filter shape: gaussian alpha, beta; mitchell B, C
     uint32_t max_len, rr_start, estimator, frame;
     float3 sky, sun, sun_e;
     uint64_t seed;
     uint32_t has_sun, pad;
 };
 
-__device__ __forceinline__ float filter_1d(uint32_t f, float r, float x) {
-    if (f == RC_FILTER_BOX) return (x >= -r && x < r) ? 1.0f / (2.0f * r) : 0.0f;
-    float rinv = 1.0f / r;
-    float s = fabsf(x * rinv);
-    return (s < 1.0f ? 1.0f - s : 0.0f) * rinv;
+// Filter::getWeightX of each FilterType (src/Rendering/Filters/*.cpp), in
+// FP32; r is the filter's radius.
+__device__ __forceinline__ float filter_1d(const PassParams& P, float x) {
+    const float r = P.radius;
+    switch (P.filter) {
+    case RC_FILTER_BOX:  // BoxFilter.cpp:22-37
+        return (x >= -r && x < r) ? 1.0f / (2.0f * r) : 0.0f;
+    case RC_FILTER_TENT: {  // TentFilter.cpp:12-44
+        float rinv = 1.0f / r;
+        float s = fabsf(x * rinv);
+        return (s < 1.0f ? 1.0f - s : 0.0f) * rinv;
+    }
+    case RC_FILTER_BELL: {  // BellFilter.cpp:12-22: the quadratic B-spline
+        float s = fabsf(x);
+        if (s < 0.5f) return 0.75f - s * s;
+        if (s <= 1.5f) return 0.5f * (s - 1.5f) * (s - 1.5f);
+        return 0.0f;
+    }
+    case RC_FILTER_GAUSSIAN:  // GaussianFilter.cpp:25-33: alpha exp(beta x^2)
+        return P.fa * expf(x * x * P.fb);
+    case RC_FILTER_MITCHELL: {  // MitchellFilter.cpp:12-23, Mitchell & Netravali's cubic
+        const float B = P.fa, C = P.fb, s = fabsf(x), s2 = s * s, s3 = s2 * s;
+        if (s <= 1.0f)
+            return ((12.0f - 9.0f * B - 6.0f * C) * s3 + (-18.0f + 12.0f * B + 6.0f * C) * s2 + (6.0f - 2.0f * B)) *
+                   (1.0f / 6.0f);
+        if (s <= 2.0f)
+            return ((-B - 6.0f * C) * s3 + (6.0f * B + 30.0f * C) * s2 + (-12.0f * B - 48.0f * C) * s +
+                    (8.0f * B + 24.0f * C)) *
+                   (1.0f / 6.0f);
+        return 0.0f;
+    }
+    default: {  // LanczosSincFilter.cpp:12-30: sinc(s) sinc(s / r), sinc(x) = sin(pi x) / (pi x)
+        float s = fabsf(x);
+        if (s == 0.0f) return 1.0f;
+        if (s > r) return 0.0f;
+        const float q = s / r;
+        return (sinpif(s) / (3.14159265358979f * s)) * (sinpif(q) / (3.14159265358979f * q));
+    }
+    }
+}
+
+// CMJSampler's permute (CMJSampler.cpp:12-44): Kensler's hash-based
+// permutation of [0, l) ("Correlated Multi-Jittered Sampling", Pixar Technical
+// Memo 13-01, 2013) in the reference's 64-bit arithmetic.  The hash is a
+// bijection of [0, 2^k) with 2^k >= l, so cycle walking ends within 2^k steps.
+__device__ __forceinline__ uint64_t cmj_permute(uint64_t i, uint64_t l, uint64_t p) {
+    uint64_t w = l - 1;
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    w |= w >> 8;
+    w |= w >> 16;
+    do {
+        i ^= p;
+        i *= 0xe170893dull;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8;
+        i *= 0x0929eb3full;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1;
+        i *= 1 | p >> 27;
+        i *= 0x6935fa69ull;
+        i ^= (i & w) >> 11;
+        i *= 0x74dcb303ull;
+        i ^= (i & w) >> 2;
+        i *= 0x9e501cc3ull;
+        i ^= (i & w) >> 2;
+        i *= 0xc860a3dfull;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return (i + p) % l;
+}
+
+// The per-pixel permutation of generateMultiSamples (Tracer.cpp:129-130), one
+// 64-bit draw per pixel and frame, shared by all the pixel's samples.
+constexpr uint32_t kPermDim = 0xFFFFFFFEu;
+__device__ __forceinline__ uint64_t pixel_permutation(uint64_t seed, uint32_t pixel, uint32_t frame) {
+    uint4 o = philox(make_uint4(pixel, 0u, frame, kPermDim), make_uint2(uint32_t(seed), uint32_t(seed >> 32)));
+    return uint64_t(o.x) | (uint64_t(o.y) << 32);
 }
 
 // Tracer::generateMultiSamples (Tracer.cpp:115-144) for one slot.
@@ -173,20 +250,41 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
     uint32_t s = P.sample0 + j;
     float px = float(pixel % P.width), py = float(pixel / P.width);
     float w = 1.0f;
-    if (!(P.sampler == RC_SAMPLER_JITTERED && P.spp == 1)) {
+    if (!(P.sampler != RC_SAMPLER_RANDOM && P.spp == 1)) {
         float4 u = draw4(P.seed, pixel, s, P.frame, kCamDim);
+        const uint32_t n = P.jitter_n, sy = P.sampler == RC_SAMPLER_RANDOM ? 0u : s / n, sx = s - sy * n;
         float ox, oy;
-        if (P.sampler == RC_SAMPLER_JITTERED) {
-            uint32_t sy = s / P.jitter_n, sx = s - sy * P.jitter_n;
-            ox = (float(sx) + u.x) / float(P.jitter_n);
-            oy = (float(sy) + u.y) / float(P.jitter_n);
-        } else {
+        switch (P.sampler) {
+        case RC_SAMPLER_JITTERED:  // JitteredSampler.cpp:21-34
+            ox = (float(sx) + u.x) / float(n);
+            oy = (float(sy) + u.y) / float(n);
+            break;
+        case RC_SAMPLER_RANDOM:  // RandomSampler.cpp:23-35
             ox = u.x;
             oy = u.y;
+            break;
+        case RC_SAMPLER_CENTER:  // CenterSampler.cpp:19-33
+            ox = oy = 0.5f;
+            break;
+        case RC_SAMPLER_REGULAR:  // RegularSampler.cpp:19-34
+            ox = (float(sx) + 0.5f) / float(n);
+            oy = (float(sy) + 0.5f) / float(n);
+            break;
+        case RC_SAMPLER_CMJ: {  // CMJSampler.cpp:57-77
+            const uint64_t perm = pixel_permutation(P.seed, pixel, P.frame);
+            const float px_ = float(cmj_permute(sx, n, perm * 0x68bc21ebull));
+            const float py_ = float(cmj_permute(sy, n, perm * 0x02e5be93ull));
+            ox = (float(sx) + (py_ + u.x) / float(n)) / float(n);
+            oy = (float(sy) + (px_ + u.y) / float(n)) / float(n);
+            break;
+        }
+        default:  // PoissonDiscSampler::getSample2D returns (0, 0) (PoissonDiscSampler.cpp:22-32)
+            ox = oy = 0.0f;
+            break;
         }
         ox = (ox - 0.5f) * 2.0f * P.radius;
         oy = (oy - 0.5f) * 2.0f * P.radius;
-        w = filter_1d(P.filter, P.radius, ox) * filter_1d(P.filter, P.radius, oy);
+        w = filter_1d(P, ox) * filter_1d(P, oy);
         px += ox;
         py += oy;
     }
@@ -598,22 +696,51 @@ __global__ void k_zero_counters(DevCounters* ctr) {
 
 }  // namespace
 
+// The filter's radius and shape constants, in double as the reference's
+// constructors compute them (GaussianFilter.cpp:15-23), rounded to float.
+static void filter_shape(const rc_render_cfg& c, float& radius, float& a, float& b) {
+    a = b = 0.0f;
+    switch (c.filter) {
+    case RC_FILTER_BELL: radius = 1.5f; break;
+    case RC_FILTER_GAUSSIAN: {
+        const double sd = c.filter_param[0];
+        a = float(1.0 / (std::sqrt(2.0 * M_PI) * sd));
+        b = float(-1.0 / (2.0 * sd * sd));
+        radius = float((7.43384 * sd) / 2.0);
+        break;
+    }
+    case RC_FILTER_MITCHELL:
+        radius = 2.0f;
+        a = c.filter_param[0];
+        b = c.filter_param[1];
+        break;
+    default: radius = c.filter_radius; break;
+    }
+}
+
 int rc_validate_cfg(const rc_render_cfg* cfg, const char* fn) {
     if (!cfg) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": null config");
     if (cfg->width < 2 || cfg->height < 2 || cfg->width > 16384 || cfg->height > 16384)
         return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": film size out of range");
     if (cfg->spp == 0 || cfg->spp > 65536) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": spp out of range");
-    if (cfg->sampler == RC_SAMPLER_JITTERED) {
+    if (cfg->sampler > RC_SAMPLER_POISSON_DISC) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": unknown sampler");
+    if (cfg->sampler != RC_SAMPLER_RANDOM) {
         uint32_t n = isqrt_exact(cfg->spp);
         if (n * n != cfg->spp)
-            return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": jittered sampling needs spp = n*n");
-    } else if (cfg->sampler != RC_SAMPLER_RANDOM) {
-        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": unknown sampler");
+            return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": this sampler needs spp = n*n");
     }
     if (cfg->max_path_length == 0 || cfg->max_path_length > 64)
         return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": max_path_length out of range");
-    if (cfg->filter > RC_FILTER_TENT || !(cfg->filter_radius > 0.0f))
-        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": bad filter");
+    if (cfg->filter > RC_FILTER_LANCZOS_SINC) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": unknown filter");
+    if ((cfg->filter == RC_FILTER_BOX || cfg->filter == RC_FILTER_TENT) && !(cfg->filter_radius > 0.0f && cfg->filter_radius < 1e6f))
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": filter_radius must be positive");
+    if (cfg->filter == RC_FILTER_LANCZOS_SINC &&
+        !(cfg->filter_radius >= 1.0f && cfg->filter_radius <= 64.0f && cfg->filter_radius == std::floor(cfg->filter_radius)))
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": lanczos_sinc needs an integer filter_radius in [1, 64]");
+    if (cfg->filter == RC_FILTER_GAUSSIAN && !(cfg->filter_param[0] > 0.0f && cfg->filter_param[0] < 1e6f))
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": gaussian needs a positive standard deviation");
+    if (cfg->filter == RC_FILTER_MITCHELL && !(std::isfinite(cfg->filter_param[0]) && std::isfinite(cfg->filter_param[1])))
+        return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": mitchell needs finite B and C");
     if (cfg->estimator > RC_EST_LAMBERT) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": bad estimator");
     return RC_OK;
 }
@@ -651,7 +778,7 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     P.sampler = cfg->sampler;
     P.jitter_n = isqrt_exact(cfg->spp);
     P.filter = cfg->filter;
-    P.radius = cfg->filter_radius;
+    filter_shape(*cfg, P.radius, P.fa, P.fb);
     P.offset = cfg->ray_start_offset;
     P.max_len = cfg->max_path_length;
     P.rr_start = cfg->rr_start;

@@ -108,6 +108,80 @@ def test_random_sampler_replay(ctx):
     replay_check(lin, oracle_stats(ob, sd, sd.desc.camera, cfg))
 
 
+# (sampler, filter, filter_radius, filter_param) over every sampler and filter of the reference
+SAMPLING = [
+    (api.RC_SAMPLER_CMJ, api.RC_FILTER_TENT, 1.0, (0.0, 0.0)),
+    (api.RC_SAMPLER_REGULAR, api.RC_FILTER_BELL, 0.5, (0.0, 0.0)),
+    (api.RC_SAMPLER_JITTERED, api.RC_FILTER_GAUSSIAN, 0.5, (0.5, 0.0)),
+    (api.RC_SAMPLER_CMJ, api.RC_FILTER_MITCHELL, 0.5, (1 / 3, 1 / 3)),
+    (api.RC_SAMPLER_JITTERED, api.RC_FILTER_LANCZOS_SINC, 2.0, (0.0, 0.0)),
+    (api.RC_SAMPLER_CENTER, api.RC_FILTER_TENT, 1.0, (0.0, 0.0)),
+    (api.RC_SAMPLER_POISSON_DISC, api.RC_FILTER_BOX, 0.5, (0.0, 0.0)),
+]
+
+
+@pytest.mark.parametrize("sampler,filt,radius,param", SAMPLING)
+def test_sampler_and_filter_replay(ctx, sampler, filt, radius, param):
+    """Camera samples come from the shared RNG, so the per-pixel filter weight
+    sums equal the oracle's to FP32 rounding, and the images replay it."""
+    sd, s, ob = scene(ctx, "box")
+    spp = 16
+    cfg = cfg_for(sd, 48, 48, spp, sampler=sampler, filter=filt, filter_radius=radius)
+    cfg.filter_param[0], cfg.filter_param[1] = param
+    lin, sums = render_gpu(ctx, s, sd.desc.camera, cfg)
+    st = oracle_stats(ob, sd, sd.desc.camera, cfg)
+    peak = orc.filter_weight(filt, radius, 0.0, 0.0, *param)
+    np.testing.assert_allclose(sums[:, 3], st[:, 0], rtol=1e-5, atol=2e-6 * spp * peak)
+    # negative lobes (Mitchell, Lanczos) can bring a pixel's weight sum near 0;
+    # replay the pixels whose sum is not small
+    ok = np.abs(st[:, 0]) > 0.2 * np.median(np.abs(st[:, 0]))
+    assert ok.mean() > 0.8
+    replay_check(lin[ok], st[ok])
+
+
+@pytest.mark.parametrize("tm_type", [api.RC_TONEMAP_LINEAR, api.RC_TONEMAP_SIMPLE, api.RC_TONEMAP_PASSTHROUGH])
+def test_tone_mappers_match_oracle(ctx, tm_type):
+    sd, s, _ = scene(ctx, "box")
+    cfg = cfg_for(sd, 64, 64, 4)
+    film = ctx.film(64, 64)
+    ctx.render(s, film, sd.desc.camera, cfg)
+    lin = film.linear()
+    for exposure, fast in [(0.0, True), (0.7, True), (-1.5, False)]:
+        out, la, ml = film.tonemap(type=tm_type, exposure=exposure, fast_pow=fast)
+        ref, _, _ = orc.tonemap(lin, type=tm_type, exposure=exposure, fast_pow=fast)
+        assert la == 0.0 and ml == 0.0
+        if tm_type == api.RC_TONEMAP_PASSTHROUGH:
+            np.testing.assert_array_equal(out, lin)
+        else:
+            assert np.max(np.abs(out - ref)) <= 4e-6
+            assert np.all(out[:, 3] == 1.0)
+    film.close()
+
+
+def test_reinhard_moving_average_lives_with_the_film(ctx):
+    sd, s, _ = scene(ctx, "box")
+    cfg = cfg_for(sd, 64, 64, 4)
+    film = ctx.film(64, 64)
+    ctx.render(s, film, sd.desc.camera, cfg)
+    lin = film.linear()
+    state = orc.ToneMapState()
+    for _ in range(3):
+        out, la, ml = film.tonemap(enable_averaging=True, averaging_alpha=0.25)
+        ref, rla, rml = orc.tonemap(lin, enable_averaging=True, averaging_alpha=0.25, state=state)
+        assert ml == rml and la == pytest.approx(rla, rel=1e-12)
+        assert np.max(np.abs(out - ref)) <= 4e-6
+    film.clear()  # Film::clear leaves the tone mapper's average alone
+    ctx.render(s, film, sd.desc.camera, cfg)
+    _, _, ml = film.tonemap(enable_averaging=True, averaging_alpha=0.25)
+    _, _, rml = orc.tonemap(lin, enable_averaging=True, averaging_alpha=0.25, state=state)
+    assert ml == rml
+    with pytest.raises(ra.RaycerError):
+        film.tonemap(enable_averaging=True, averaging_alpha=0.0)
+    with pytest.raises(ra.RaycerError):
+        film.tonemap(type=7)
+    film.close()
+
+
 def binomial_bound(n, p=0.0027, z=4.0):
     return n * p + z * math.sqrt(n * p * (1 - p)) + 1
 
@@ -251,6 +325,14 @@ def test_render_rejects_bad_config(ctx):
         ctx.render(s, film, sd.desc.camera, cfg_for(sd, 16, 16, 12))   # jittered needs n*n
     with pytest.raises(ra.RaycerError):
         ctx.render(s, film, sd.desc.camera, cfg_for(sd, 32, 16, 4))    # film size mismatch
+    bad = [dict(sampler=api.RC_SAMPLER_CMJ, spp=8),                     # n*n samplers
+           dict(sampler=6), dict(filter=6),
+           dict(filter=api.RC_FILTER_LANCZOS_SINC, filter_radius=1.5),  # integer radius
+           dict(filter=api.RC_FILTER_GAUSSIAN)]                         # standard deviation 0
+    for kw in bad:
+        spp = kw.pop("spp", 16)
+        with pytest.raises(ra.RaycerError):
+            ctx.render(s, film, sd.desc.camera, cfg_for(sd, 16, 16, spp, **kw))
     film.close()
 
 
