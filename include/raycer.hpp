@@ -84,6 +84,7 @@ struct EulerAngle {
     double pitch = 0.0, yaw = 0.0, roll = 0.0;
 };
 
+// Material (Material.h:20-66) without the texture maps.
 struct Material {
     Color diffuseReflectance;
     Color emittance;
@@ -92,6 +93,51 @@ struct Material {
     double specularShininess = 2.0;
     bool isEmissive = false;
     bool nonShadowing = false;
+    bool skipLighting = false;
+    bool fresnelReflection = false;
+    double refractiveIndex = 1.0;
+    double rayReflectance = 0.0;
+    SamplerType rayReflectanceGlossinessSamplerType = SamplerType::CMJ;
+    uint64_t rayReflectanceGlossinessSampleCountSqrt = 0;
+    double rayReflectanceGlossiness = 1000.0;
+    double rayTransmittance = 0.0;
+    SamplerType rayTransmittanceGlossinessSamplerType = SamplerType::CMJ;
+    uint64_t rayTransmittanceGlossinessSampleCountSqrt = 0;
+    double rayTransmittanceGlossiness = 1000.0;
+    bool enableRayTransmissionAttenuation = false;
+    double rayTransmissionAttenuationFactor = 1.0;
+    Color rayTransmissionAttenuationColor;
+};
+
+// Lights.h: the ambient light (with ambient occlusion) and the directional,
+// point (optionally an area light) and spot lights of the ray tracer.
+struct Light {
+    Color color{1.0, 1.0, 1.0};
+    double intensity = 1.0;
+};
+struct AmbientLight : Light {
+    bool enableAmbientOcclusion = false;
+    SamplerType ambientOcclusionSamplerType = SamplerType::CMJ;
+    uint64_t ambientOcclusionSampleCountSqrt = 0;
+    double ambientOcclusionMaxSampleDistance = 1.0;
+    double ambientOcclusionSampleDistribution = 1.0;
+};
+struct DirectionalLight : Light {
+    Vector3 direction;
+};
+struct PointLight : Light {
+    Vector3 position;
+    double maxDistance = 1000.0;
+    double attenuation = 1.0;
+    bool enableAreaLight = false;
+    SamplerType areaLightSamplerType = SamplerType::CMJ;
+    uint64_t areaLightSampleCountSqrt = 0;
+    double areaLightRadius = 1.0;
+};
+struct SpotLight : PointLight {
+    Vector3 direction;
+    double angle = 45.0;
+    double sideAttenuation = 1.0;
 };
 
 struct Triangle {
@@ -178,7 +224,30 @@ public:
         Vector3 sunDirection;               // towards the sun
         Color sunIrradiance;
         uint64_t seed = 0;
+        // ray tracer (Scene.h:92-105)
+        uint64_t maxRayIterations = 3;
+        Color backgroundColor;
+        bool visualizeDepth = false;
+        double visualizeDepthMaxDistance = 25.0;
     } general;
+
+    struct Lights {
+        AmbientLight ambientLight;
+        std::vector<DirectionalLight> directionalLights;
+        std::vector<PointLight> pointLights;
+        std::vector<SpotLight> spotLights;
+    } lights;
+
+    // Scene::SimpleFog (Scene.h:162-183)
+    struct SimpleFog {
+        bool enabled = false;
+        Color color{1.0, 1.0, 1.0};
+        double distance = 100.0;
+        double steepness = 1.0;
+        bool heightDispersion = false;
+        double height = 100.0;
+        double heightSteepness = 1.0;
+    } simpleFog;
 
     Camera camera;
 
@@ -206,7 +275,6 @@ public:
     } primitives;
 
     std::vector<Material> materials;
-    rc_point_light pointLight{};  // used by the RAY tracer
 
     // ModelLoaderInfo (src/Utils/ModelLoader.h:27-46): models listed here are
     // read by initialize(), OBJ or PLY by file extension.
@@ -295,14 +363,24 @@ public:
         s.general.sunDirection = Vector3{c.sun_dir[0], c.sun_dir[1], c.sun_dir[2]};
         s.general.sunIrradiance = Color{c.sun_irradiance[0], c.sun_irradiance[1], c.sun_irradiance[2]};
         s.general.seed = c.seed;
-        s.pointLight = d.light;
+        const rc_point_light& l = d.light;
+        PointLight pl;
+        pl.position = Vector3{l.position[0], l.position[1], l.position[2]};
+        pl.color = Color{l.color[0], l.color[1], l.color[2]};
+        pl.intensity = l.intensity;
+        pl.maxDistance = l.max_distance;
+        pl.attenuation = l.attenuation;
+        s.lights.pointLights.push_back(pl);
+        s.lights.ambientLight.color = Color{l.ambient_color[0], l.ambient_color[1], l.ambient_color[2]};
+        s.lights.ambientLight.intensity = l.ambient_intensity;
         s.defaultWidth = c.width;
         s.defaultHeight = c.height;
         return s;
     }
 
     // TestScene3 (src/TestScenes/TestScene3.cpp:11-47): the Cornell box model,
-    // camera at (0, 1, 3.5) looking down -z, one point light, Phong tracer.
+    // camera at (0, 1, 3.5) looking down -z, ambient 0.01 and one point light,
+    // ray tracer.
     static Scene createTestScene3(const std::string& cornellBoxObjPath) {
         Scene s;
         s.general.tracerType = TracerType::RAY;
@@ -311,7 +389,15 @@ public:
         ModelLoaderInfo mi;
         mi.modelFilePath = cornellBoxObjPath;
         s.models.push_back(mi);
-        s.pointLight = rc_point_light{{0.0f, 1.9f, 0.0f}, {1.0f, 0.71f, 0.24f}, 1.0f, 3.0f, 1.0f, {1.0f, 1.0f, 1.0f}, 0.01f};
+        s.lights.ambientLight.color = Color{1.0, 1.0, 1.0};
+        s.lights.ambientLight.intensity = 0.01;
+        PointLight pl;
+        pl.color = Color{1.0, 0.71, 0.24};
+        pl.intensity = 1.0;
+        pl.position = Vector3{0.0, 1.9, 0.0};
+        pl.maxDistance = 3.0;
+        pl.attenuation = 1.0;
+        s.lights.pointLights.push_back(pl);
         s.defaultWidth = 1280;
         s.defaultHeight = 800;
         return s;
@@ -534,6 +620,15 @@ public:
         rc_camera cam{{float(sc.camera.position.x), float(sc.camera.position.y), float(sc.camera.position.z)},
                       float(sc.camera.orientation.pitch), float(sc.camera.orientation.yaw),
                       float(sc.camera.orientation.roll), float(sc.camera.fov)};
+        if (sc.general.tracerType == TracerType::RAY) {
+            // Raytracer (Raytracer.cpp:21-597) over the same camera samples
+            if (interrupted.load()) return;
+            RayInputs in = rayInputs(sc);
+            check(rc_raytrace(ctx_, scene_, film_, &cam, &c, &in.settings, in.materials.data(),
+                              uint32_t(in.materials.size()), in.lights.data(), uint32_t(in.lights.size())));
+            state.pixelsProcessed = state.pixelCount ? state.pixelCount : width_ * height_;
+            return;
+        }
         struct Poll {
             TracerState* state;
             std::atomic<bool>* interrupted;
@@ -549,18 +644,112 @@ public:
         state.pixelsProcessed = state.pixelCount ? state.pixelCount : width_ * height_;
     }
 
-    // Raytracer with one point light (the RAY tracer type): primary ray,
-    // ambient + Phong + hard shadow per pixel (Raytracer.cpp:312-407,
-    // 524-551).  Returns the linear RGBA image; the film is not touched.
+    // The ray tracer's inputs from the scene: Scene::General's ray-tracer
+    // fields, the lights, SimpleFog and the materials' ray-tracing fields.
+    struct RayInputs {
+        rc_ray_settings settings{};
+        std::vector<rc_ray_material> materials;
+        std::vector<rc_light> lights;
+    };
+    static RayInputs rayInputs(const Scene& sc) {
+        RayInputs in;
+        rc_ray_settings& r = in.settings;
+        auto put3 = [](float* d, const Color& c) { d[0] = float(c.r); d[1] = float(c.g); d[2] = float(c.b); };
+        auto putv = [](float* d, const Vector3& v) { d[0] = float(v.x); d[1] = float(v.y); d[2] = float(v.z); };
+        put3(r.background, sc.general.backgroundColor);
+        r.max_ray_iterations = uint32_t(sc.general.maxRayIterations);
+        const AmbientLight& a = sc.lights.ambientLight;
+        put3(r.ambient_color, a.color);
+        r.ambient_intensity = float(a.intensity);
+        r.ao_enabled = a.enableAmbientOcclusion ? 1u : 0u;
+        r.ao_sampler = to_rc(a.ambientOcclusionSamplerType);
+        r.ao_n = uint32_t(a.ambientOcclusionSampleCountSqrt);
+        r.ao_max_distance = float(a.ambientOcclusionMaxSampleDistance);
+        r.ao_distribution = float(a.ambientOcclusionSampleDistribution);
+        r.visualize_depth = sc.general.visualizeDepth ? 1u : 0u;
+        r.visualize_depth_max = float(sc.general.visualizeDepthMaxDistance);
+        const Scene::SimpleFog& f = sc.simpleFog;
+        r.fog_enabled = f.enabled ? 1u : 0u;
+        put3(r.fog_color, f.color);
+        r.fog_distance = float(f.distance);
+        r.fog_steepness = float(f.steepness);
+        r.fog_height_dispersion = f.heightDispersion ? 1u : 0u;
+        r.fog_height = float(f.height);
+        r.fog_height_steepness = float(f.heightSteepness);
+        for (const Material& m : sc.materials) {
+            rc_ray_material x{};
+            x.ray_reflectance = float(m.rayReflectance);
+            x.ray_transmittance = float(m.rayTransmittance);
+            x.refractive_index = float(m.refractiveIndex);
+            x.fresnel_reflection = m.fresnelReflection ? 1u : 0u;
+            x.skip_lighting = m.skipLighting ? 1u : 0u;
+            x.refl_sampler = to_rc(m.rayReflectanceGlossinessSamplerType);
+            x.refl_n = uint32_t(m.rayReflectanceGlossinessSampleCountSqrt);
+            x.refl_glossiness = float(m.rayReflectanceGlossiness);
+            x.trans_sampler = to_rc(m.rayTransmittanceGlossinessSamplerType);
+            x.trans_n = uint32_t(m.rayTransmittanceGlossinessSampleCountSqrt);
+            x.trans_glossiness = float(m.rayTransmittanceGlossiness);
+            x.enable_attenuation = m.enableRayTransmissionAttenuation ? 1u : 0u;
+            x.attenuation_factor = float(m.rayTransmissionAttenuationFactor);
+            put3(x.attenuation_color, m.rayTransmissionAttenuationColor);
+            in.materials.push_back(x);
+        }
+        auto point = [&](const PointLight& p, uint32_t type) {
+            rc_light l{};
+            l.type = type;
+            put3(l.color, p.color);
+            l.intensity = float(p.intensity);
+            putv(l.position, p.position);
+            l.max_distance = float(p.maxDistance);
+            l.attenuation = float(p.attenuation);
+            l.area_light = p.enableAreaLight ? 1u : 0u;
+            l.area_sampler = to_rc(p.areaLightSamplerType);
+            l.area_n = uint32_t(p.areaLightSampleCountSqrt);
+            l.area_radius = float(p.areaLightRadius);
+            return l;
+        };
+        for (const DirectionalLight& d : sc.lights.directionalLights) {
+            rc_light l{};
+            l.type = RC_LIGHT_DIRECTIONAL;
+            put3(l.color, d.color);
+            l.intensity = float(d.intensity);
+            putv(l.direction, d.direction);
+            in.lights.push_back(l);
+        }
+        for (const PointLight& p : sc.lights.pointLights) in.lights.push_back(point(p, RC_LIGHT_POINT));
+        for (const SpotLight& sp : sc.lights.spotLights) {
+            rc_light l = point(sp, RC_LIGHT_SPOT);
+            putv(l.direction, sp.direction);
+            l.angle = float(sp.angle);
+            l.side_attenuation = float(sp.sideAttenuation);
+            in.lights.push_back(l);
+        }
+        return in;
+    }
+
+    // The one-light raycast of the 1080p Phong workload: primary ray through a
+    // jittered point of each pixel, ambient + Phong + hard shadow for the
+    // scene's first point light (Raytracer.cpp:312-407, 524-551).  Returns the
+    // linear RGBA image; the film is not touched.
     Image raycastPhong(const Scene& sc, uint64_t seed = 1) {
         if (!scene_) throw RaycerError(RC_ERR_INVALID, "HipTracer buffers are not initialized");
+        if (sc.lights.pointLights.empty()) throw RaycerError(RC_ERR_INVALID, "raycastPhong needs a point light");
+        const PointLight& p = sc.lights.pointLights.front();
+        const AmbientLight& a = sc.lights.ambientLight;
+        rc_point_light pl{{float(p.position.x), float(p.position.y), float(p.position.z)},
+                          {float(p.color.r), float(p.color.g), float(p.color.b)},
+                          float(p.intensity),
+                          float(p.maxDistance),
+                          float(p.attenuation),
+                          {float(a.color.r), float(a.color.g), float(a.color.b)},
+                          float(a.intensity)};
         Image img = blank();
         rc_camera cam{{float(sc.camera.position.x), float(sc.camera.position.y), float(sc.camera.position.z)},
                       float(sc.camera.orientation.pitch), float(sc.camera.orientation.yaw),
                       float(sc.camera.orientation.roll), float(sc.camera.fov)};
         // pixel (x, y) at y * width + x with y = 0 at the bottom, as Image
         check(rc_raycast_phong(ctx_, scene_, &cam, uint32_t(width_), uint32_t(height_), seed,
-                               float(sc.general.rayStartOffset), &sc.pointLight, nullptr, nullptr, img.pixels.data()));
+                               float(sc.general.rayStartOffset), &pl, nullptr, nullptr, img.pixels.data()));
         return img;
     }
 

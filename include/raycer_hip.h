@@ -285,6 +285,83 @@ int rc_raycast_phong(rc_ctx* ctx, const rc_scene* scene, const rc_camera* cam, u
                      uint64_t seed, float ray_start_offset, const rc_point_light* light, int32_t* hit_ids,
                      float* t, float* rgba);
 
+/* ------------------------------------------------------------ Whitted ray tracer
+ * Raytracer::traceRecursive (src/Raytracing/Tracers/Raytracer.cpp:27-310) with
+ * calculateLightColor, Phong shading, ambient occlusion and hard or area-light
+ * shadows (:312-597), run as Tracer::run runs it: the camera samples of
+ * rc_render_cfg (sampler, filter, spp) accumulate into the film.  The settings
+ * are Scene::General's ray-tracer fields, Scene::Lights, Scene::SimpleFog
+ * (Scene.h:89-200, Lights.h) and Material's ray-tracing fields (Material.h:20-66);
+ * the scene's rc_material gives the ambient, diffuse and specular reflectances
+ * and the shininess.  Textures, normal maps and volumetric fog are not
+ * supported. */
+enum { RC_LIGHT_DIRECTIONAL = 0, RC_LIGHT_POINT = 1, RC_LIGHT_SPOT = 2 };
+
+typedef struct rc_light {
+    uint32_t type;              /* RC_LIGHT_* */
+    float color[3];
+    float intensity;
+    float position[3];          /* point, spot */
+    float direction[3];         /* directional: direction of travel; spot: axis */
+    float max_distance;         /* point, spot: 1 - min(1, d / max_distance)^attenuation */
+    float attenuation;
+    uint32_t area_light;        /* point, spot: soft shadows from a disc (enableAreaLight) */
+    uint32_t area_sampler;      /* RC_SAMPLER_* (areaLightSamplerType) */
+    uint32_t area_n;            /* areaLightSampleCountSqrt, 1..16 */
+    float area_radius;
+    float angle;                /* spot: cone angle in degrees */
+    float side_attenuation;     /* spot */
+    uint32_t pad;
+} rc_light;
+
+/* Material's ray-tracing fields, one per scene material, in the scene's order. */
+typedef struct rc_ray_material {
+    float ray_reflectance;
+    float ray_transmittance;
+    float refractive_index;
+    uint32_t fresnel_reflection;
+    uint32_t skip_lighting;     /* the colour is the diffuse reflectance */
+    uint32_t refl_sampler;      /* rayReflectanceGlossinessSamplerType */
+    uint32_t refl_n;            /* rayReflectanceGlossinessSampleCountSqrt, 0 = mirror, at most 16 */
+    float refl_glossiness;      /* rayReflectanceGlossiness (hemisphere exponent) */
+    uint32_t trans_sampler;
+    uint32_t trans_n;
+    float trans_glossiness;
+    uint32_t enable_attenuation;    /* enableRayTransmissionAttenuation */
+    float attenuation_factor;
+    float attenuation_color[3];
+} rc_ray_material;
+
+typedef struct rc_ray_settings {
+    float background[3];        /* backgroundColor */
+    uint32_t max_ray_iterations;    /* at most 16 */
+    float ambient_color[3];
+    float ambient_intensity;
+    uint32_t ao_enabled;        /* enableAmbientOcclusion */
+    uint32_t ao_sampler;
+    uint32_t ao_n;              /* ambientOcclusionSampleCountSqrt, 1..16 when enabled */
+    float ao_max_distance;
+    float ao_distribution;
+    uint32_t visualize_depth;
+    float visualize_depth_max;
+    uint32_t fog_enabled;       /* SimpleFog */
+    float fog_color[3];
+    float fog_distance;
+    float fog_steepness;
+    uint32_t fog_height_dispersion;
+    float fog_height;
+    float fog_height_steepness;
+} rc_ray_settings;
+
+/* Renders one frame with the Whitted tracer into the film (accumulating, as
+ * rc_render does).  mats has one entry per scene material (nmat must equal the
+ * scene's material count); at most 64 lights.  Each pixel's rays form a tree
+ * (reflection and transmission children, glossy ones n*n per hit) traced level
+ * by level; a level that would outgrow the buffers is processed in parts. */
+int rc_raytrace(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam, const rc_render_cfg* cfg,
+                const rc_ray_settings* settings, const rc_ray_material* mats, uint32_t nmat, const rc_light* lights,
+                uint32_t nlights);
+
 /* ------------------------------------------------------------ radiance cache
  * No counterpart in the reference.  An MLP (81 -> 64 x 4 ReLU -> 3 exp,
  * 17,923 parameters) with BF16 weights on MFMA and FP32 accumulation, trained

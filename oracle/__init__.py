@@ -50,6 +50,8 @@ def lib(fast: bool = False) -> C.CDLL:
         L.or_cmj_permute.argtypes = [C.c_uint64] * 3
         L.or_cmj_sample.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, d, d, vp]
         L.or_tonemap.argtypes = [vp, sz, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
+        L.or_raytrace.argtypes = [vp, vp, sz, vp, vp, vp, C.POINTER(RenderCfg), vp, vp, vp, C.c_uint32, vp, sz, vp,
+                                  C.c_int]
         L.or_bvh_build.argtypes = [vp, sz, C.c_uint64, C.POINTER(vp)]
         L.or_bvh_node_count.restype = sz
         L.or_bvh_node_count.argtypes = [vp]
@@ -234,6 +236,24 @@ def render(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, pixel_ids=None, threads
     ca = camera_array(cam)
     L.or_render(bvh.h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), ma.shape[0],
                 _p(ca), C.byref(cfg), _p(ids), npix, _p(out), threads, C.byref(stop))
+    return out
+
+
+def raytrace(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, settings, ray_mats, lights=(), pixel_ids=None,
+             threads: int = 0):
+    """The Whitted tracer's per-pixel stats (npix, 12) like render().  settings,
+    ray_mats and lights are the product's ctypes structures (same layouts)."""
+    L = lib(bvh.fast)
+    tm = np.ascontiguousarray(tri_mat, np.uint32)
+    ma = np.ascontiguousarray(materials_array(mats) if mats.dtype.names else mats, np.float64)
+    ids = None if pixel_ids is None else np.ascontiguousarray(pixel_ids, np.uint32)
+    npix = cfg.width * cfg.height if ids is None else ids.shape[0]
+    out = np.empty((npix, 12))
+    ca = camera_array(cam)
+    lts = lights if isinstance(lights, C.Array) or len(lights) == 0 else (type(lights[0]) * len(lights))(*lights)
+    L.or_raytrace(bvh.h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), _p(ca), C.byref(cfg),
+                  C.addressof(settings), C.addressof(ray_mats), C.addressof(lts) if len(lights) else None, len(lights),
+                  _p(ids), npix, _p(out), threads)
     return out
 
 

@@ -27,6 +27,10 @@
  *     Simple and Passthrough tone mappers (src/Rendering/ToneMappers/)
  *   - MathUtils::fastPow        (src/Math/MathUtils.cpp:56-68)
  *   - Raytracer Phong + point-light shadow (Raytracer.cpp:312-407, 524-551)
+ *   - the Whitted ray tracer: Raytracer::traceRecursive with reflection,
+ *     refraction, Fresnel, glossy sampling, attenuation, simple fog, ambient
+ *     occlusion and directional / point / spot lights with area-light shadows
+ *     (Raytracer.cpp:27-597)
  * plus the estimator terms the reference does not have (Russian roulette, sky
  * radiance on miss, directional-sun next-event estimation), the counter-based
  * RNG (Philox4x32-10, Salmon et al., SC'11) and an FP64 MLP (forward, backward,
@@ -142,6 +146,59 @@ int or_trace_path(const or_bvh* b, const float* vertices, size_t ntri, const uin
                   const double* materials, const double cam[7], const or_render_cfg* cfg,
                   uint32_t pixel, uint32_t sample, double L_out[3], double* weight_out,
                   double* vertices_out);
+
+/* ---------------------------------------------------------------- Whitted ray tracer */
+/* Same layouts as the product's rc_light / rc_ray_material / rc_ray_settings. */
+typedef struct or_light {
+    uint32_t type;              /* 0 directional, 1 point, 2 spot */
+    float color[3];
+    float intensity;
+    float position[3];
+    float direction[3];
+    float max_distance;
+    float attenuation;
+    uint32_t area_light;
+    uint32_t area_sampler;
+    uint32_t area_n;
+    float area_radius;
+    float angle;
+    float side_attenuation;
+    uint32_t pad;
+} or_light;
+typedef struct or_ray_material {
+    float ray_reflectance, ray_transmittance, refractive_index;
+    uint32_t fresnel_reflection, skip_lighting;
+    uint32_t refl_sampler, refl_n;
+    float refl_glossiness;
+    uint32_t trans_sampler, trans_n;
+    float trans_glossiness;
+    uint32_t enable_attenuation;
+    float attenuation_factor;
+    float attenuation_color[3];
+} or_ray_material;
+typedef struct or_ray_settings {
+    float background[3];
+    uint32_t max_ray_iterations;
+    float ambient_color[3];
+    float ambient_intensity;
+    uint32_t ao_enabled, ao_sampler, ao_n;
+    float ao_max_distance, ao_distribution;
+    uint32_t visualize_depth;
+    float visualize_depth_max;
+    uint32_t fog_enabled;
+    float fog_color[3];
+    float fog_distance, fog_steepness;
+    uint32_t fog_height_dispersion;
+    float fog_height, fog_height_steepness;
+} or_ray_settings;
+
+/* Raytracer::traceRecursive (Raytracer.cpp:27-597) under Tracer::run's camera
+ * sampling, recursively, with the ray tree's keyed draws.  stats_out: per
+ * listed pixel the 12 sums of or_render (the last is the number of rays). */
+void or_raytrace(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,
+                 const double* materials, const double cam[7], const or_render_cfg* cfg,
+                 const or_ray_settings* settings, const or_ray_material* rmats, const or_light* lights,
+                 uint32_t nlights, const uint32_t* pixel_ids, size_t npix, double* stats_out, int nthreads);
 
 /* ---------------------------------------------------------------- raycast (Phong) */
 /* light: position.xyz, color.rgb, intensity, maxDistance, attenuation

@@ -587,3 +587,348 @@ extern "C" void or_reinhard(const float* lin, size_t npix, const double prm[5], 
     if (log_avg_out) *log_avg_out = logavg;
     if (max_lum_out) *max_lum_out = maxl;
 }
+
+// ---------------------------------------------------------------- Whitted ray tracer
+// Raytracer::traceRecursive (Raytracer.cpp:27-310) and its helpers
+// (:312-597), recursive as the reference is, in FP64.  The per-hit draws are
+// keyed by the node's place in the ray tree (root key 0, child key =
+// hash(key + golden * (kind << 16 | s) + golden)), so the GPU's level-by-level
+// wavefront reproduces the same rays.
+namespace {
+uint32_t wt_hash(uint32_t x) {  // lowbias32 (C. Wellons)
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+uint32_t wt_child(uint32_t key, uint32_t kind, uint32_t s) { return wt_hash(key + 0x9E3779B9u * (1u + ((kind << 16) | s))); }
+const uint32_t P_REFL = 1, P_TRANS = 2, P_AO = 3, P_LIGHT = 16, PERM_S = 0xFFFFu;
+
+struct WS {
+    Scene sc;
+    const or_ray_settings* rs;
+    const or_ray_material* rm;
+    const or_light* lights;
+    uint32_t nlights;
+    double offset;
+    uint64_t seed;
+    uint32_t frame;
+};
+struct WNode {
+    uint32_t pixel, sample, key;
+};
+
+void node_bits(const WS& S, const WNode& N, uint32_t p, uint32_t s, uint32_t o[4]) {
+    uint32_t ctr[4] = {N.pixel, N.sample, S.frame, wt_hash(N.key ^ wt_hash((p << 16) | s))};
+    uint32_t key[2] = {uint32_t(S.seed), uint32_t(S.seed >> 32)};
+    philox4x32_10(ctr, key, o);
+}
+void node_uv(const WS& S, const WNode& N, uint32_t p, uint32_t s, double& u, double& v) {
+    uint32_t o[4];
+    node_bits(S, N, p, s, o);
+    u = u01(o[0]);
+    v = u01(o[1]);
+}
+uint64_t node_perm(const WS& S, const WNode& N, uint32_t p) {
+    uint32_t o[4];
+    node_bits(S, N, p, PERM_S, o);
+    return uint64_t(o[0]) | (uint64_t(o[1]) << 32);
+}
+// Sampler::getSample2D of each SamplerType
+void sample_point(uint32_t type, uint32_t x, uint32_t y, uint32_t n, uint64_t perm, double r0, double r1, double& px,
+                  double& py) {
+    switch (type) {
+    case 0: px = (x + r0) / n; py = (y + r1) / n; break;
+    case 1: px = r0; py = r1; break;
+    case 2: px = py = 0.5; break;
+    case 3: px = (x + 0.5) / n; py = (y + 0.5) / n; break;
+    case 4: {
+        double o[2];
+        or_cmj_sample(x, y, n, perm, r0, r1, o);
+        px = o[0];
+        py = o[1];
+        break;
+    }
+    default: px = py = 0.0; break;
+    }
+}
+// ONB::fromNormal (ONB.cpp:32-39)
+void onb(V3 w, V3& u, V3& v) {
+    u = normalized(cross(w, ALMOST_UP));
+    v = normalized(cross(u, w));
+}
+// mapToHemisphere (Sampler.cpp:50-64)
+V3 map_hemisphere(V3 u, V3 v, V3 w, double e, double px, double py) {
+    double phi = 2.0 * M_PI * px;
+    double ct = std::pow(1.0 - py, 1.0 / (e + 1.0));
+    double st = std::sqrt(1.0 - ct * ct);
+    return (st * std::cos(phi)) * u + (st * std::sin(phi)) * v + ct * w;
+}
+V3 reflect(V3 a, V3 n) { return a - (2.0 * dot(a, n)) * n; }
+V3 lerp(V3 a, V3 b, double t) { return a + (b - a) * t; }
+V3 f3(const float* a) { return V3(a[0], a[1], a[2]); }
+
+// calculatePhongShadingColor (Raytracer.cpp:386-407)
+V3 phong(V3 n, V3 l, V3 v, V3 lc, V3 kd, V3 ks, double shin) {
+    V3 c(0, 0, 0);
+    double diff = dot(l, n);
+    if (diff > 0.0) {
+        c = mul(lc * diff, kd);
+        if (!(ks.x == 0.0 && ks.y == 0.0 && ks.z == 0.0)) {
+            V3 r = normalized((2.0 * diff * n) - l);
+            double sa = dot(r, v);
+            if (sa > 0.0) c = c + mul(lc * std::pow(sa, shin), ks);
+        }
+    }
+    return c;
+}
+
+bool wt_closest(const WS& S, V3 o, V3 d, double tmax, double& t, int64_t& id) {
+    return closest(S.sc, make_ray(o, d, 0.0, tmax), 1, t, id);
+}
+bool wt_shadowed(const WS& S, V3 o, V3 d, double tmax) { return occluded(S.sc, make_ray(o, d, 0.0, tmax)); }
+
+// calculateSimpleFogColor (Raytracer.cpp:409-425)
+V3 fog(const WS& S, double dist, V3 p, V3 c) {
+    const or_ray_settings& R = *S.rs;
+    double t1 = std::max(0.0, std::min(dist / R.fog_distance, 1.0));
+    t1 = std::pow(t1, R.fog_steepness);
+    if (R.fog_height_dispersion && p.y > 0.0) {
+        double t2 = std::max(0.0, std::min(p.y / R.fog_height, 1.0));
+        t2 = std::pow(t2, R.fog_height_steepness);
+        t1 *= 1.0 - t2;
+    }
+    return lerp(c, f3(R.fog_color), t1);
+}
+
+// calculateShadowAmount for point and spot lights (Raytracer.cpp:524-597)
+double shadow_amount(const WS& S, const WNode& N, uint32_t li, V3 p) {
+    const or_light& L = S.lights[li];
+    V3 lp = f3(L.position);
+    V3 l = normalized(lp - p);
+    if (!L.area_light) return wt_shadowed(S, p + l * S.offset, l, length(lp - p)) ? 1.0 : 0.0;
+    V3 right = normalized(cross(l, ALMOST_UP));
+    V3 up = normalized(cross(right, l));
+    uint64_t perm = node_perm(S, N, P_LIGHT + li);
+    uint32_t n = L.area_n;
+    double amount = 0.0;
+    for (uint32_t y = 0; y < n; ++y)
+        for (uint32_t x = 0; x < n; ++x) {
+            double r0, r1, px, py;
+            node_uv(S, N, P_LIGHT + li, y * n + x, r0, r1);
+            sample_point(L.area_sampler, x, y, n, perm, r0, r1, px, py);
+            // getDiscSample: mapToDisc (Sampler.cpp:36-48) times the radius
+            double th = 2.0 * M_PI * px, rr = std::sqrt(py);
+            double jx = rr * std::cos(th) * L.area_radius, jy = rr * std::sin(th) * L.area_radius;
+            V3 np = lp + (jx * right + jy * up);
+            V3 dl = normalized(np - p);
+            if (wt_shadowed(S, p + dl * S.offset, dl, length(np - p))) amount += 1.0;
+        }
+    return amount / (double(n) * double(n));
+}
+
+// calculateAmbientOcclusionAmount (Raytracer.cpp:453-494); AO rays are not
+// shadow rays, so nonShadowing triangles occlude them
+double ao_amount(const WS& S, const WNode& N, V3 p, V3 nrm) {
+    const or_ray_settings& R = *S.rs;
+    V3 u, v;
+    onb(nrm, u, v);
+    uint64_t perm = node_perm(S, N, P_AO);
+    uint32_t n = R.ao_n;
+    double occ = 0.0;
+    for (uint32_t y = 0; y < n; ++y)
+        for (uint32_t x = 0; x < n; ++x) {
+            double r0, r1, px, py;
+            node_uv(S, N, P_AO, y * n + x, r0, r1);
+            sample_point(R.ao_sampler, x, y, n, perm, r0, r1, px, py);
+            V3 sd = map_hemisphere(u, v, nrm, R.ao_distribution, px, py);
+            double t;
+            int64_t id;
+            if (wt_closest(S, p + sd * S.offset, sd, R.ao_max_distance, t, id)) occ += 1.0;
+        }
+    return 1.0 - occ / (double(n) * double(n));
+}
+
+// calculateLightColor (Raytracer.cpp:312-384): ambient (with occlusion), then
+// the directional, point and spot lights, each group in list order.
+V3 light_color(const WS& S, const WNode& N, V3 d, V3 p, V3 nrm, const double* m) {
+    const or_ray_settings& R = *S.rs;
+    V3 kd(m[0], m[1], m[2]), ka(m[6], m[7], m[8]), ks(m[9], m[10], m[11]);
+    double shin = m[12];
+    V3 v(-d.x, -d.y, -d.z);
+    double ao = R.ao_enabled ? ao_amount(S, N, p, nrm) : 1.0;
+    V3 c = mul(f3(R.ambient_color) * double(R.ambient_intensity) * ao, ka);
+    for (uint32_t type = 0; type < 3; ++type)
+        for (uint32_t li = 0; li < S.nlights; ++li) {
+            const or_light& L = S.lights[li];
+            if (L.type != type) continue;
+            V3 lc = f3(L.color) * double(L.intensity);
+            if (type == 0) {
+                V3 l = f3(L.direction) * -1.0;
+                V3 pc = phong(nrm, l, v, lc, kd, ks, shin);
+                double sh = wt_shadowed(S, p + l * S.offset, l, std::numeric_limits<double>::max()) ? 1.0 : 0.0;
+                c = c + pc * (1.0 - sh);
+                continue;
+            }
+            V3 tl = f3(L.position) - p;
+            double dist = length(tl);
+            V3 l = normalized(tl);
+            V3 pc = phong(nrm, l, v, lc, kd, ks, shin);
+            double sh = shadow_amount(S, N, li, p);
+            double da = std::min(1.0, dist / L.max_distance);
+            da = 1.0 - std::pow(da, double(L.attenuation));
+            if (type == 1) {
+                c = c + pc * (da * (1.0 - sh));
+            } else {
+                double sa = dot(f3(L.direction), l * -1.0);
+                if (sa > 0.0) {
+                    sa = std::min(1.0, (1.0 - sa) / (L.angle / 180.0));
+                    sa = 1.0 - std::pow(sa, double(L.side_attenuation));
+                    c = c + pc * (da * sa * (1.0 - sh));
+                }
+            }
+        }
+    return c;
+}
+
+V3 wt_trace(const WS& S, V3 o, V3 d, uint32_t iter, const WNode& N, bool& found, double& dist, uint64_t& rays);
+
+// calculateReflectedColor / calculateTransmittedColor (Raytracer.cpp:160-310)
+V3 child_color(const WS& S, const or_ray_material& M, uint32_t kind, V3 p, V3 nrm, bool outside, V3 dir, double k,
+               uint32_t n, uint32_t sampler, double glossiness, uint32_t iter, const WNode& N, uint64_t& rays) {
+    V3 attc = f3(M.attenuation_color);
+    if (n == 0) {
+        bool cf = false;
+        double cd = 0.0;
+        WNode C{N.pixel, N.sample, wt_child(N.key, kind, 0)};
+        V3 c = wt_trace(S, p + dir * S.offset, dir, iter + 1, C, cf, cd, rays) * k;
+        // only attenuate if the ray has travelled inside a primitive
+        bool inside_path = kind == P_REFL ? !outside : outside;
+        if (inside_path && cf && M.enable_attenuation) c = lerp(attc, c, std::exp(-M.attenuation_factor * cd));
+        return c;
+    }
+    V3 u, v;
+    onb(dir, u, v);
+    uint64_t perm = node_perm(S, N, kind);
+    V3 sum(0, 0, 0);
+    for (uint32_t y = 0; y < n; ++y)
+        for (uint32_t x = 0; x < n; ++x) {
+            uint32_t s = y * n + x;
+            double r0, r1, px, py;
+            node_uv(S, N, kind, s, r0, r1);
+            sample_point(sampler, x, y, n, perm, r0, r1, px, py);
+            V3 sd = map_hemisphere(u, v, dir, glossiness, px, py);
+            double sn = dot(sd, nrm);
+            bool flip = kind == P_REFL ? ((outside && sn < 0.0) || (!outside && sn > 0.0))
+                                       : ((outside && sn > 0.0) || (!outside && sn < 0.0));
+            if (flip) sd = reflect(sd, dir);
+            bool cf = false;
+            double cd = 0.0;
+            WNode C{N.pixel, N.sample, wt_child(N.key, kind, s)};
+            V3 c = wt_trace(S, p + sd * S.offset, sd, iter + 1, C, cf, cd, rays) * k;
+            if (!outside && cf && M.enable_attenuation) c = lerp(attc, c, std::exp(-M.attenuation_factor * cd));
+            sum = sum + c;
+        }
+    return sum * (1.0 / (double(n) * double(n)));
+}
+
+V3 wt_trace(const WS& S, V3 o, V3 d, uint32_t iter, const WNode& N, bool& found, double& dist, uint64_t& rays) {
+    const or_ray_settings& R = *S.rs;
+    double t;
+    int64_t id;
+    ++rays;
+    found = wt_closest(S, o, d, std::numeric_limits<double>::max(), t, id);
+    if (!found) return f3(R.background);
+    dist = t;
+    if (R.visualize_depth) {
+        double z = 1.0 - std::min(t, double(R.visualize_depth_max)) / R.visualize_depth_max;
+        z = std::pow(z, 2.0);
+        return V3(z, z, z);
+    }
+    const uint32_t mid = S.sc.tri_mat[id];
+    const double* m = S.sc.mats + OR_MAT_STRIDE * mid;
+    const or_ray_material& M = S.rm[mid];
+    V3 p = o + t * d;
+    V3 nrm = tri_normal(load_tri(S.sc.verts, uint64_t(id)));
+    bool outside = dot(d, nrm) < 0.0;
+    if (M.skip_lighting) {
+        V3 c(m[0], m[1], m[2]);
+        if (R.fog_enabled && outside) c = fog(S, t, p, c);
+        return c;
+    }
+    // calculateRayReflectanceAndTransmittance (Raytracer.cpp:126-158)
+    double fr = 1.0, ft = 1.0;
+    double eta = M.refractive_index;
+    if (M.fresnel_reflection) {
+        double cs = dot(d, nrm);
+        bool out = cs < 0.0;
+        double n1 = out ? 1.0 : eta, n2 = out ? eta : 1.0;
+        double rf0 = (n2 - n1) / (n2 + n1);
+        rf0 = rf0 * rf0;
+        fr = rf0 + (1.0 - rf0) * std::pow(1.0 - std::abs(cs), 5.0);
+        ft = 1.0 - fr;
+    }
+    double Rr = M.ray_reflectance * fr, Tt = M.ray_transmittance * ft;
+    V3 refl(0, 0, 0), trans(0, 0, 0);
+    if (Rr > 0.0 && iter < R.max_ray_iterations) {
+        V3 rd = normalized(d + (2.0 * -dot(d, nrm)) * nrm);
+        refl = child_color(S, M, P_REFL, p, nrm, outside, rd, Rr, M.refl_n, M.refl_sampler, M.refl_glossiness, iter, N,
+                           rays);
+    }
+    if (Tt > 0.0 && iter < R.max_ray_iterations) {
+        double c1 = dot(d, nrm);
+        bool out = c1 < 0.0;
+        double n1 = out ? 1.0 : eta, n2 = out ? eta : 1.0;
+        double n3 = n1 / n2;
+        double c2 = 1.0 - (n3 * n3) * (1.0 - c1 * c1);
+        if (c2 > 0.0) {
+            V3 td = normalized(d * n3 + (std::abs(c1) * n3 - std::sqrt(c2)) * nrm);
+            trans = child_color(S, M, P_TRANS, p, nrm, outside, td, Tt, M.trans_n, M.trans_sampler,
+                                M.trans_glossiness, iter, N, rays);
+        }
+    }
+    V3 c = light_color(S, N, d, p, nrm, m) + refl + trans;
+    if (outside && R.fog_enabled) c = fog(S, t, p, c);
+    return c;
+}
+}  // namespace
+
+extern "C" void or_raytrace(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,
+                            const double* mats, const double cam[7], const or_render_cfg* cfg,
+                            const or_ray_settings* settings, const or_ray_material* rmats, const or_light* lights,
+                            uint32_t nlights, const uint32_t* pixel_ids, size_t npix, double* stats, int nthreads) {
+    WS S{Scene{b, vertices, tri_mat, mats, {}}, settings, rmats, lights, nlights, cfg->ray_start_offset, cfg->seed,
+         cfg->frame};
+    S.sc.tri_flags.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) S.sc.tri_flags[i] = uint32_t(mats[OR_MAT_STRIDE * tri_mat[i] + 13]);
+    const or_render_cfg c = *cfg;
+    Cam k = make_cam(cam, c.width, c.height);
+    size_t count = pixel_ids ? npix : size_t(c.width) * c.height;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < int64_t(count); ++i) {
+        uint32_t pixel = pixel_ids ? pixel_ids[i] : uint32_t(i);
+        double* st = stats + 12 * i;
+        for (int q = 0; q < 12; ++q) st[q] = 0.0;
+        for (uint32_t smp = 0; smp < c.spp; ++smp) {
+            V3 o, d;
+            double w;
+            sample_camera(k, c, pixel, smp, o, d, w);
+            bool found = false;
+            double dist = 0.0;
+            uint64_t rays = 0;
+            V3 L = wt_trace(S, o, d, 0, WNode{pixel, smp, 0u}, found, dist, rays);
+            double Lc[3] = {L.x, L.y, L.z};
+            st[0] += w;
+            st[1] += w * w;
+            for (int ch = 0; ch < 3; ++ch) {
+                st[2 + ch] += w * Lc[ch];
+                st[5 + ch] += w * w * Lc[ch];
+                st[8 + ch] += w * w * Lc[ch] * Lc[ch];
+            }
+            st[11] += double(rays);
+        }
+    }
+}

@@ -64,6 +64,94 @@ class ToneMap(C.Structure):
                 ("enable_averaging", C.c_uint32), ("averaging_alpha", C.c_float), ("pad", C.c_uint32)]
 
 
+RC_LIGHT_DIRECTIONAL, RC_LIGHT_POINT, RC_LIGHT_SPOT = range(3)
+
+
+class Light(C.Structure):
+    """rc_light: a directional, point or spot light of the Whitted tracer (Lights.h)."""
+    _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("intensity", C.c_float), ("position", C.c_float * 3),
+                ("direction", C.c_float * 3), ("max_distance", C.c_float), ("attenuation", C.c_float),
+                ("area_light", C.c_uint32), ("area_sampler", C.c_uint32), ("area_n", C.c_uint32),
+                ("area_radius", C.c_float), ("angle", C.c_float), ("side_attenuation", C.c_float),
+                ("pad", C.c_uint32)]
+
+
+class RayMaterial(C.Structure):
+    """rc_ray_material: Material's ray-tracing fields (Material.h:20-66)."""
+    _fields_ = [("ray_reflectance", C.c_float), ("ray_transmittance", C.c_float), ("refractive_index", C.c_float),
+                ("fresnel_reflection", C.c_uint32), ("skip_lighting", C.c_uint32), ("refl_sampler", C.c_uint32),
+                ("refl_n", C.c_uint32), ("refl_glossiness", C.c_float), ("trans_sampler", C.c_uint32),
+                ("trans_n", C.c_uint32), ("trans_glossiness", C.c_float), ("enable_attenuation", C.c_uint32),
+                ("attenuation_factor", C.c_float), ("attenuation_color", C.c_float * 3)]
+
+
+class RaySettings(C.Structure):
+    """rc_ray_settings: Scene::General's ray-tracer fields, the ambient light and SimpleFog."""
+    _fields_ = [("background", C.c_float * 3), ("max_ray_iterations", C.c_uint32), ("ambient_color", C.c_float * 3),
+                ("ambient_intensity", C.c_float), ("ao_enabled", C.c_uint32), ("ao_sampler", C.c_uint32),
+                ("ao_n", C.c_uint32), ("ao_max_distance", C.c_float), ("ao_distribution", C.c_float),
+                ("visualize_depth", C.c_uint32), ("visualize_depth_max", C.c_float), ("fog_enabled", C.c_uint32),
+                ("fog_color", C.c_float * 3), ("fog_distance", C.c_float), ("fog_steepness", C.c_float),
+                ("fog_height_dispersion", C.c_uint32), ("fog_height", C.c_float),
+                ("fog_height_steepness", C.c_float)]
+
+
+def ray_materials(n: int, **defaults):
+    """n RayMaterial with the reference's defaults (Material.h: glossiness 1000,
+    refractive index 1, CMJ samplers), then `defaults` applied to each."""
+    arr = (RayMaterial * n)()
+    for m in arr:
+        m.refractive_index = 1.0
+        m.refl_glossiness = m.trans_glossiness = 1000.0
+        m.refl_sampler = m.trans_sampler = RC_SAMPLER_CMJ
+        m.attenuation_factor = 1.0
+        for k, v in defaults.items():
+            setattr(m, k, v)
+    return arr
+
+
+def ray_settings(**kw) -> RaySettings:
+    """RaySettings with the reference's defaults (Scene.h, Lights.h), then kw."""
+    r = RaySettings()
+    r.max_ray_iterations = 3
+    r.ambient_color[:] = (1.0, 1.0, 1.0)
+    r.ambient_intensity = 1.0
+    r.ao_sampler = RC_SAMPLER_CMJ
+    r.ao_max_distance = 1.0
+    r.ao_distribution = 1.0
+    r.visualize_depth_max = 25.0
+    r.fog_color[:] = (1.0, 1.0, 1.0)
+    r.fog_distance = r.fog_height = 100.0
+    r.fog_steepness = r.fog_height_steepness = 1.0
+    for k, v in kw.items():
+        if isinstance(v, (tuple, list)):
+            getattr(r, k)[:] = v
+        else:
+            setattr(r, k, v)
+    return r
+
+
+def light(type: int, **kw) -> Light:
+    """A Light with the reference's defaults (Lights.h: white, intensity 1,
+    maxDistance 1000, attenuation 1, CMJ area sampler, radius 1, angle 45)."""
+    L = Light()
+    L.type = type
+    L.color[:] = (1.0, 1.0, 1.0)
+    L.intensity = 1.0
+    L.max_distance = 1000.0
+    L.attenuation = 1.0
+    L.area_sampler = RC_SAMPLER_CMJ
+    L.area_radius = 1.0
+    L.angle = 45.0
+    L.side_attenuation = 1.0
+    for k, v in kw.items():
+        if isinstance(v, (tuple, list)):
+            getattr(L, k)[:] = v
+        else:
+            setattr(L, k, v)
+    return L
+
+
 class PointLight(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("color", C.c_float * 3), ("intensity", C.c_float),
                 ("max_distance", C.c_float), ("attenuation", C.c_float), ("ambient_color", C.c_float * 3),
@@ -150,6 +238,8 @@ def lib() -> C.CDLL:
             "rc_device_free": [vp, vp],
             "rc_memcpy_h2d": [vp, vp, vp, sz],
             "rc_memcpy_d2h": [vp, vp, vp, sz],
+            "rc_raytrace": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), C.POINTER(RaySettings), vp, u32,
+                            vp, u32],
             "rc_film_create": [vp, u32, u32, C.POINTER(vp)],
             "rc_film_free": [vp],
             "rc_film_clear": [vp, vp],
@@ -426,6 +516,14 @@ class Context:
         check(lib().rc_autotune_batch(self.h, sarr, carr, garr, n, str(cache_path).encode() if cache_path else None,
                                       _ptr(out)))
         return out
+
+    def raytrace(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, settings: RaySettings,
+                 materials, lights=()) -> None:
+        """rc_raytrace: the Whitted tracer's frame, accumulated into the film."""
+        mats = materials if isinstance(materials, C.Array) else (RayMaterial * len(materials))(*materials)
+        lts = lights if isinstance(lights, C.Array) else (Light * max(1, len(lights)))(*lights)
+        check(lib().rc_raytrace(self.h, scene.h, film.h, C.byref(cam), C.byref(cfg), C.byref(settings),
+                                C.cast(mats, C.c_void_p), len(mats), C.cast(lts, C.c_void_p), len(lights)))
 
     def raycast_phong(self, scene: "Scene", cam: Camera, width: int, height: int, seed: int, offset: float,
                       light: PointLight):

@@ -150,6 +150,7 @@ extern "C" int rc_destroy(rc_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->batch_graph) (void)hipGraphExecDestroy(c->batch_graph);
+    rc_whitted_release(c);
     DevBuf* bufs[] = {&c->trace_rays, &c->trace_hits, &c->trace_occ, &c->ovf_list, &c->deep_stack, &c->ray_a,
                       &c->ray_b, &c->st_a, &c->st_b, &c->hits, &c->srays, &c->scontrib, &c->socc, &c->slotL,
                       &c->slotW};

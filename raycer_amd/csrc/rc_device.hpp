@@ -74,6 +74,113 @@ __device__ __forceinline__ float4 draw4(uint64_t seed, uint32_t pixel, uint32_t 
     return make_float4(u01(o.x), u01(o.y), u01(o.z), u01(o.w));
 }
 
+// Pixel order of the slots of a pass: the 8x8 tiles of the largest tiled
+// region first (tiles row by row, pixels row by row inside a tile), then the
+// right strip and the bottom strip row by row.  A wave of 64 consecutive slots
+// is one 8x8 tile, and since the queues keep slot order from bounce to bounce
+// its rays start from nearby points at every bounce.
+struct PixelOrder {
+    uint32_t width, tiles_x, x0, y0, a;  // x0 = 8 tiles_x, y0 = 8 tiles_y, a = x0 y0
+};
+
+inline PixelOrder make_order(uint32_t w, uint32_t h) {
+    PixelOrder o;
+    o.width = w;
+    o.tiles_x = w / 8;
+    o.x0 = 8 * (w / 8);
+    o.y0 = 8 * (h / 8);
+    o.a = o.x0 * o.y0;
+    return o;
+}
+
+__device__ __forceinline__ uint32_t order_to_pixel(const PixelOrder& o, uint32_t q) {
+    if (q < o.a) {
+        const uint32_t t = q >> 6, w = q & 63u, ty = t / o.tiles_x, tx = t - ty * o.tiles_x;
+        return (ty * 8 + (w >> 3)) * o.width + tx * 8 + (w & 7u);
+    }
+    uint32_t r = q - o.a;
+    const uint32_t rw = o.width - o.x0;
+    if (r < rw * o.y0) {
+        const uint32_t y = r / rw;
+        return y * o.width + o.x0 + (r - y * rw);
+    }
+    r -= rw * o.y0;
+    return o.y0 * o.width + r;
+}
+
+__device__ __forceinline__ uint32_t pixel_to_order(const PixelOrder& o, uint32_t p) {
+    const uint32_t y = p / o.width, x = p - y * o.width;
+    if (x < o.x0 && y < o.y0) return ((((y >> 3) * o.tiles_x + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u));
+    if (y < o.y0) return o.a + y * (o.width - o.x0) + (x - o.x0);
+    return o.a + (o.width - o.x0) * o.y0 + (p - o.y0 * o.width);
+}
+
+// CMJSampler's permute (CMJSampler.cpp:12-44): Kensler's hash-based
+// permutation of [0, l) ("Correlated Multi-Jittered Sampling", Pixar Technical
+// Memo 13-01, 2013) in the reference's 64-bit arithmetic.  The hash is a
+// bijection of [0, 2^k) with 2^k >= l, so cycle walking ends within 2^k steps.
+__device__ __forceinline__ uint64_t cmj_permute(uint64_t i, uint64_t l, uint64_t p) {
+    uint64_t w = l - 1;
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    w |= w >> 8;
+    w |= w >> 16;
+    do {
+        i ^= p;
+        i *= 0xe170893dull;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8;
+        i *= 0x0929eb3full;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1;
+        i *= 1 | p >> 27;
+        i *= 0x6935fa69ull;
+        i ^= (i & w) >> 11;
+        i *= 0x74dcb303ull;
+        i ^= (i & w) >> 2;
+        i *= 0x9e501cc3ull;
+        i ^= (i & w) >> 2;
+        i *= 0xc860a3dfull;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return (i + p) % l;
+}
+
+// The per-pixel permutation of generateMultiSamples (Tracer.cpp:129-130), one
+// 64-bit draw per pixel and frame, shared by all the pixel's samples.
+constexpr uint32_t kPermDim = 0xFFFFFFFEu;
+__device__ __forceinline__ uint64_t pixel_permutation(uint64_t seed, uint32_t pixel, uint32_t frame) {
+    uint4 o = philox(make_uint4(pixel, 0u, frame, kPermDim), make_uint2(uint32_t(seed), uint32_t(seed >> 32)));
+    return uint64_t(o.x) | (uint64_t(o.y) << 32);
+}
+
+// Sampler::getSample2D of each SamplerType (src/Rendering/Samplers/) for
+// stratum (x, y) of an n x n pattern, with the jitter (r0, r1) and the
+// permutation perm (CMJ only).
+__device__ __forceinline__ float2 sample_point(uint32_t type, uint32_t x, uint32_t y, uint32_t n, uint64_t perm,
+                                               float r0, float r1) {
+    switch (type) {
+    case RC_SAMPLER_JITTERED:  // JitteredSampler.cpp:21-34
+        return make_float2((float(x) + r0) / float(n), (float(y) + r1) / float(n));
+    case RC_SAMPLER_RANDOM:  // RandomSampler.cpp:23-35
+        return make_float2(r0, r1);
+    case RC_SAMPLER_CENTER:  // CenterSampler.cpp:19-33
+        return make_float2(0.5f, 0.5f);
+    case RC_SAMPLER_REGULAR:  // RegularSampler.cpp:19-34
+        return make_float2((float(x) + 0.5f) / float(n), (float(y) + 0.5f) / float(n));
+    case RC_SAMPLER_CMJ: {  // CMJSampler.cpp:57-77
+        const float sx = float(cmj_permute(x, n, perm * 0x68bc21ebull));
+        const float sy = float(cmj_permute(y, n, perm * 0x02e5be93ull));
+        return make_float2((float(x) + (sy + r0) / float(n)) / float(n), (float(y) + (sx + r1) / float(n)) / float(n));
+    }
+    default:  // PoissonDiscSampler::getSample2D returns (0, 0) (PoissonDiscSampler.cpp:22-32)
+        return make_float2(0.0f, 0.0f);
+    }
+}
+
 // ---------------------------------------------------------------- vec math
 __device__ __forceinline__ float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
 __device__ __forceinline__ float3 operator+(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }

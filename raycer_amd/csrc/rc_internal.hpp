@@ -67,6 +67,7 @@ struct rc_ctx {
     std::vector<hipEvent_t> ev_free;
     std::vector<Pending> pending;
     hipGraphExec_t batch_graph = nullptr;
+    struct WhittedBufs* whitted = nullptr;  // the Whitted tracer's ray-tree levels
 
     int prof_begin(hipEvent_t* a);
     int prof_end(hipEvent_t a, int kind);
@@ -102,6 +103,16 @@ struct rc_film {
 int rc_launch_trace(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, const uint32_t* n_dev,
                     uint32_t n_bound, rc_hit* hits, uint8_t* occ);
 int rc_launch_deep(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, rc_hit* hits, uint8_t* occ);
+// Camera rays of one pass of Tracer::generateMultiSamples (slot = j * npix + q
+// for sample sample0 + j of the pixel at position q of the tiled order): ray,
+// state (1, 1, 1, slot), L = 0 and the filter weight W.
+int rc_launch_camera(rc_ctx* c, const rc_camera* cam, const rc_render_cfg* cfg, uint32_t sample0, uint32_t spp_pass,
+                     rc_ray* rays, float4* st, float4* L, float* W);
+// Film::addSample of a pass's slot results, per pixel in sample order.
+int rc_launch_film_add(rc_ctx* c, rc_film* f, const rc_render_cfg* cfg, uint32_t spp_pass, const float4* L,
+                       const float* W);
+// Frees the Whitted tracer's buffers of a context.
+void rc_whitted_release(rc_ctx* c);
 
 // Radiance-cache plumbing of a frame (all null / 0 when the cache is off).
 struct CacheIO {

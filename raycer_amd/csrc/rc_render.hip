@@ -97,47 +97,6 @@ __device__ __forceinline__ float3 cam_dir(const CamBasis& b, float px, float py)
     return normalize3(p - b.pos);
 }
 
-// Pixel order of the slots of a pass: the 8x8 tiles of the largest tiled
-// region first (tiles row by row, pixels row by row inside a tile), then the
-// right strip and the bottom strip row by row.  A wave of 64 consecutive slots
-// is one 8x8 tile, and since the queues keep slot order from bounce to bounce
-// its rays start from nearby points at every bounce.
-struct PixelOrder {
-    uint32_t width, tiles_x, x0, y0, a;  // x0 = 8 tiles_x, y0 = 8 tiles_y, a = x0 y0
-};
-
-PixelOrder make_order(uint32_t w, uint32_t h) {
-    PixelOrder o;
-    o.width = w;
-    o.tiles_x = w / 8;
-    o.x0 = 8 * (w / 8);
-    o.y0 = 8 * (h / 8);
-    o.a = o.x0 * o.y0;
-    return o;
-}
-
-__device__ __forceinline__ uint32_t order_to_pixel(const PixelOrder& o, uint32_t q) {
-    if (q < o.a) {
-        const uint32_t t = q >> 6, w = q & 63u, ty = t / o.tiles_x, tx = t - ty * o.tiles_x;
-        return (ty * 8 + (w >> 3)) * o.width + tx * 8 + (w & 7u);
-    }
-    uint32_t r = q - o.a;
-    const uint32_t rw = o.width - o.x0;
-    if (r < rw * o.y0) {
-        const uint32_t y = r / rw;
-        return y * o.width + o.x0 + (r - y * rw);
-    }
-    r -= rw * o.y0;
-    return o.y0 * o.width + r;
-}
-
-__device__ __forceinline__ uint32_t pixel_to_order(const PixelOrder& o, uint32_t p) {
-    const uint32_t y = p / o.width, x = p - y * o.width;
-    if (x < o.x0 && y < o.y0) return ((((y >> 3) * o.tiles_x + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u));
-    if (y < o.y0) return o.a + y * (o.width - o.x0) + (x - o.x0);
-    return o.a + (o.width - o.x0) * o.y0 + (p - o.y0 * o.width);
-}
-
 struct PassParams {
     CamBasis cam;
     PixelOrder order;
@@ -192,48 +151,6 @@ __device__ __forceinline__ float filter_1d(const PassParams& P, float x) {
     }
 }
 
-// CMJSampler's permute (CMJSampler.cpp:12-44): Kensler's hash-based
-// permutation of [0, l) ("Correlated Multi-Jittered Sampling", Pixar Technical
-// Memo 13-01, 2013) in the reference's 64-bit arithmetic.  The hash is a
-// bijection of [0, 2^k) with 2^k >= l, so cycle walking ends within 2^k steps.
-__device__ __forceinline__ uint64_t cmj_permute(uint64_t i, uint64_t l, uint64_t p) {
-    uint64_t w = l - 1;
-    w |= w >> 1;
-    w |= w >> 2;
-    w |= w >> 4;
-    w |= w >> 8;
-    w |= w >> 16;
-    do {
-        i ^= p;
-        i *= 0xe170893dull;
-        i ^= p >> 16;
-        i ^= (i & w) >> 4;
-        i ^= p >> 8;
-        i *= 0x0929eb3full;
-        i ^= p >> 23;
-        i ^= (i & w) >> 1;
-        i *= 1 | p >> 27;
-        i *= 0x6935fa69ull;
-        i ^= (i & w) >> 11;
-        i *= 0x74dcb303ull;
-        i ^= (i & w) >> 2;
-        i *= 0x9e501cc3ull;
-        i ^= (i & w) >> 2;
-        i *= 0xc860a3dfull;
-        i &= w;
-        i ^= i >> 5;
-    } while (i >= l);
-    return (i + p) % l;
-}
-
-// The per-pixel permutation of generateMultiSamples (Tracer.cpp:129-130), one
-// 64-bit draw per pixel and frame, shared by all the pixel's samples.
-constexpr uint32_t kPermDim = 0xFFFFFFFEu;
-__device__ __forceinline__ uint64_t pixel_permutation(uint64_t seed, uint32_t pixel, uint32_t frame) {
-    uint4 o = philox(make_uint4(pixel, 0u, frame, kPermDim), make_uint2(uint32_t(seed), uint32_t(seed >> 32)));
-    return uint64_t(o.x) | (uint64_t(o.y) << 32);
-}
-
 // Tracer::generateMultiSamples (Tracer.cpp:115-144) for one slot.
 __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restrict__ rays, float4* __restrict__ st,
                                                    float4* __restrict__ L, float* __restrict__ W,
@@ -253,37 +170,10 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
     if (!(P.sampler != RC_SAMPLER_RANDOM && P.spp == 1)) {
         float4 u = draw4(P.seed, pixel, s, P.frame, kCamDim);
         const uint32_t n = P.jitter_n, sy = P.sampler == RC_SAMPLER_RANDOM ? 0u : s / n, sx = s - sy * n;
-        float ox, oy;
-        switch (P.sampler) {
-        case RC_SAMPLER_JITTERED:  // JitteredSampler.cpp:21-34
-            ox = (float(sx) + u.x) / float(n);
-            oy = (float(sy) + u.y) / float(n);
-            break;
-        case RC_SAMPLER_RANDOM:  // RandomSampler.cpp:23-35
-            ox = u.x;
-            oy = u.y;
-            break;
-        case RC_SAMPLER_CENTER:  // CenterSampler.cpp:19-33
-            ox = oy = 0.5f;
-            break;
-        case RC_SAMPLER_REGULAR:  // RegularSampler.cpp:19-34
-            ox = (float(sx) + 0.5f) / float(n);
-            oy = (float(sy) + 0.5f) / float(n);
-            break;
-        case RC_SAMPLER_CMJ: {  // CMJSampler.cpp:57-77
-            const uint64_t perm = pixel_permutation(P.seed, pixel, P.frame);
-            const float px_ = float(cmj_permute(sx, n, perm * 0x68bc21ebull));
-            const float py_ = float(cmj_permute(sy, n, perm * 0x02e5be93ull));
-            ox = (float(sx) + (py_ + u.x) / float(n)) / float(n);
-            oy = (float(sy) + (px_ + u.y) / float(n)) / float(n);
-            break;
-        }
-        default:  // PoissonDiscSampler::getSample2D returns (0, 0) (PoissonDiscSampler.cpp:22-32)
-            ox = oy = 0.0f;
-            break;
-        }
-        ox = (ox - 0.5f) * 2.0f * P.radius;
-        oy = (oy - 0.5f) * 2.0f * P.radius;
+        const uint64_t perm = P.sampler == RC_SAMPLER_CMJ ? pixel_permutation(P.seed, pixel, P.frame) : 0ull;
+        const float2 pt = sample_point(P.sampler, sx, sy, n, perm, u.x, u.y);
+        float ox = (pt.x - 0.5f) * 2.0f * P.radius;
+        float oy = (pt.y - 0.5f) * 2.0f * P.radius;
         w = filter_1d(P, ox) * filter_1d(P, oy);
         px += ox;
         py += oy;
@@ -756,24 +646,14 @@ int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass) {
     return RC_OK;
 }
 
-// One frame of a scene into a film: the passes of rc_render, enqueued on the
-// context stream.  Used by rc_render and the batched graph.
-int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                     uint32_t spp_pass, rc_progress_fn progress, void* user, const CacheIO* cache_io,
-                     rc_cache* cache) {
-    CacheIO CI{};
-    if (cache_io) CI = *cache_io;
-    const uint32_t npix = cfg->width * cfg->height;
-    spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
-    const size_t npaths = size_t(npix) * spp_pass;
-    if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
-    RC_TRY(alloc_pass(c, npaths));
+// The parameters of a frame's passes (the pass fields are set per pass).
+static PassParams base_params(const rc_camera* cam, const rc_render_cfg* cfg) {
     PassParams P;
     std::memset(&P, 0, sizeof(P));
     P.cam = make_basis(*cam, cfg->width, cfg->height);
     P.order = make_order(cfg->width, cfg->height);
     P.width = cfg->width;
-    P.npix = npix;
+    P.npix = cfg->width * cfg->height;
     P.spp = cfg->spp;
     P.sampler = cfg->sampler;
     P.jitter_n = isqrt_exact(cfg->spp);
@@ -789,6 +669,42 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     P.sun_e = make_float3(cfg->sun_irradiance[0], cfg->sun_irradiance[1], cfg->sun_irradiance[2]);
     P.has_sun = (cfg->sun_irradiance[0] > 0.0f || cfg->sun_irradiance[1] > 0.0f || cfg->sun_irradiance[2] > 0.0f);
     P.seed = cfg->seed;
+    return P;
+}
+
+int rc_launch_camera(rc_ctx* c, const rc_camera* cam, const rc_render_cfg* cfg, uint32_t sample0, uint32_t spp_pass,
+                     rc_ray* rays, float4* st, float4* L, float* W) {
+    PassParams P = base_params(cam, cfg);
+    P.spp_pass = spp_pass;
+    P.sample0 = sample0;
+    const uint32_t n = P.npix * spp_pass;
+    k_camera<<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(P, rays, st, L, W, c->counters);
+    RC_TRY_HIP(hipGetLastError());
+    return RC_OK;
+}
+
+int rc_launch_film_add(rc_ctx* c, rc_film* f, const rc_render_cfg* cfg, uint32_t spp_pass, const float4* L,
+                       const float* W) {
+    const uint32_t npix = cfg->width * cfg->height;
+    k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(make_order(cfg->width, cfg->height), npix,
+                                                                      spp_pass, L, W, f->sum, f->comp);
+    RC_TRY_HIP(hipGetLastError());
+    return RC_OK;
+}
+
+// One frame of a scene into a film: the passes of rc_render, enqueued on the
+// context stream.  Used by rc_render and the batched graph.
+int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                     uint32_t spp_pass, rc_progress_fn progress, void* user, const CacheIO* cache_io,
+                     rc_cache* cache) {
+    CacheIO CI{};
+    if (cache_io) CI = *cache_io;
+    const uint32_t npix = cfg->width * cfg->height;
+    spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
+    const size_t npaths = size_t(npix) * spp_pass;
+    if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
+    RC_TRY(alloc_pass(c, npaths));
+    PassParams P = base_params(cam, cfg);
     const uint32_t grid_small = uint32_t(c->cu_count) * 2u;  // list-driven cold-path kernels
     rc_ray* ra = c->ray_a.as<rc_ray>();
     rc_ray* rb = c->ray_b.as<rc_ray>();

@@ -189,8 +189,9 @@ def test_reference_cornell_scene_phong_and_path_tracer(ctx):
 def test_console_runner_renders_the_reference_cornell_scene(ctx, tmp_path):
     """raycer_render --scene cornell: TestScene3 through the C++ interface
     (Scene::models -> OBJ loader -> FlatBVH::build -> HipTracer).  Its RAY
-    tracer image equals rc_raycast_phong on the same scene; the path tracer
-    renders the emitter-lit box."""
+    tracer image equals rc_raytrace with the scene's ambient light and point
+    light; --tracer phong equals rc_raycast_phong; the path tracer renders the
+    emitter-lit box."""
     out = tmp_path / "cornell_ray.pfm"
     r = subprocess.run([str(RUNNER), "--scene", "cornell", "--width", "80", "--height", "60", "--linear", str(out)],
                        capture_output=True, text=True, timeout=120)
@@ -198,14 +199,31 @@ def test_console_runner_renders_the_reference_cornell_scene(ctx, tmp_path):
     info = json.loads(r.stdout.strip().splitlines()[-1])
     assert info["tracer"] == "ray" and info["triangles"] == 36
     img = read_pfm(out)
+    ph = tmp_path / "cornell_phong.pfm"
+    r = subprocess.run([str(RUNNER), "--scene", "cornell", "--tracer", "phong", "--width", "80", "--height", "60",
+                        "--linear", str(ph)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert json.loads(r.stdout.strip().splitlines()[-1])["tracer"] == "phong"
     sd = cornell_scene(80, 60, 1)
     nodes, ordered, _, _ = ra.build_bvh(sd.vertices)
     s = ctx.upload(sd, nodes, ordered)
     try:
         _, _, rgba = ctx.raycast_phong(s, sd.desc.camera, 80, 60, 1, 1e-5, sd.desc.light)
+        np.testing.assert_array_equal(read_pfm(ph), rgba[:, :3])
+        # Tracer::run with the RAY tracer: TestScene3's lights, default materials,
+        # one sample (the pixel centre), seed 0, frame 0
+        cfg = sd.desc.cfg
+        cfg.spp, cfg.seed, cfg.frame = 1, 0, 0
+        rs = api.ray_settings(ambient_intensity=0.01)
+        lt = api.light(api.RC_LIGHT_POINT, position=(0.0, 1.9, 0.0), color=(1.0, 0.71, 0.24), max_distance=3.0)
+        film = ctx.film(80, 60)
+        ctx.raytrace(s, film, sd.desc.camera, cfg, rs, api.ray_materials(len(sd.materials)), [lt])
+        ref = film.linear()[:, :3]
+        film.close()
     finally:
         s.close()
-    np.testing.assert_array_equal(img, rgba[:, :3])
+    np.testing.assert_array_equal(img, ref)
+    assert ref.max() > 0.05
     pt = tmp_path / "cornell_path.pfm"
     r = subprocess.run([str(RUNNER), "--scene", "cornell", "--tracer", "path", "--width", "64", "--height", "48",
                         "--spp", "16", "--linear", str(pt)], capture_output=True, text=True, timeout=120)
