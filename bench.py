@@ -87,7 +87,9 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
     and then on as many training frames of cached_spp samples as fit,
     accumulated.  Warm: the same cache after pretrain_frames more untimed
     training frames (a cache carried over from earlier frames, as in an
-    animation) renders and trains within the budget.  Paths end into the cache
+    animation) renders and trains within the budget.  Frames of one sample use
+    random sub-pixel positions under the same filter (one jittered sample would
+    be the unfiltered pixel centre).  Paths end into the cache
     at vertex cache_vertex (after two bounces for 2); every frame ends with
     train_steps Adam steps on its training samples.  MSE is over linear RGB
     and over the Reinhard-mapped RGB (key 0.18, gamma 2.2)."""
@@ -130,6 +132,10 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
             ccfg = api.RenderCfg.from_buffer_copy(cfg)
             ccfg.spp = cached_spp
             ccfg.cache_vertex = cache_vertex
+            if cached_spp == 1:
+                # one jittered sample is the pixel centre with weight 1 (no
+                # filtering); random positions keep the filter's footprint
+                ccfg.sampler = api.RC_SAMPLER_RANDOM
             frame = [200000]
 
             def run(warmup):
