@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -139,6 +140,66 @@ struct SpotLight : PointLight {
     double angle = 45.0;
     double sideAttenuation = 1.0;
 };
+
+// Material <-> the C ABI's rc_material (shading) and rc_ray_material (ray tracing).
+inline rc_material to_rc_material(const Material& m) {
+    rc_material r{};
+    const Color* src[4] = {&m.diffuseReflectance, &m.emittance, &m.ambientReflectance, &m.specularReflectance};
+    float* dst[4] = {r.diffuse, r.emittance, r.ambient, r.specular};
+    for (int k = 0; k < 4; ++k) {
+        dst[k][0] = float(src[k]->r);
+        dst[k][1] = float(src[k]->g);
+        dst[k][2] = float(src[k]->b);
+    }
+    r.shininess = float(m.specularShininess);
+    r.flags = (m.isEmissive ? RC_MAT_EMISSIVE : 0u) | (m.nonShadowing ? RC_MAT_NON_SHADOWING : 0u);
+    return r;
+}
+inline rc_ray_material to_rc_ray_material(const Material& m) {
+    rc_ray_material x{};
+    x.ray_reflectance = float(m.rayReflectance);
+    x.ray_transmittance = float(m.rayTransmittance);
+    x.refractive_index = float(m.refractiveIndex);
+    x.fresnel_reflection = m.fresnelReflection ? 1u : 0u;
+    x.skip_lighting = m.skipLighting ? 1u : 0u;
+    x.refl_sampler = to_rc(m.rayReflectanceGlossinessSamplerType);
+    x.refl_n = uint32_t(m.rayReflectanceGlossinessSampleCountSqrt);
+    x.refl_glossiness = float(m.rayReflectanceGlossiness);
+    x.trans_sampler = to_rc(m.rayTransmittanceGlossinessSamplerType);
+    x.trans_n = uint32_t(m.rayTransmittanceGlossinessSampleCountSqrt);
+    x.trans_glossiness = float(m.rayTransmittanceGlossiness);
+    x.enable_attenuation = m.enableRayTransmissionAttenuation ? 1u : 0u;
+    x.attenuation_factor = float(m.rayTransmissionAttenuationFactor);
+    x.attenuation_color[0] = float(m.rayTransmissionAttenuationColor.r);
+    x.attenuation_color[1] = float(m.rayTransmissionAttenuationColor.g);
+    x.attenuation_color[2] = float(m.rayTransmissionAttenuationColor.b);
+    return x;
+}
+inline Material from_rc_material(const rc_material& a, const rc_ray_material& r) {
+    Material m;
+    m.diffuseReflectance = Color{a.diffuse[0], a.diffuse[1], a.diffuse[2]};
+    m.emittance = Color{a.emittance[0], a.emittance[1], a.emittance[2]};
+    m.ambientReflectance = Color{a.ambient[0], a.ambient[1], a.ambient[2]};
+    m.specularReflectance = Color{a.specular[0], a.specular[1], a.specular[2]};
+    m.specularShininess = a.shininess;
+    m.isEmissive = a.flags & RC_MAT_EMISSIVE;
+    m.nonShadowing = a.flags & RC_MAT_NON_SHADOWING;
+    m.skipLighting = r.skip_lighting != 0;
+    m.fresnelReflection = r.fresnel_reflection != 0;
+    m.refractiveIndex = r.refractive_index;
+    m.rayReflectance = r.ray_reflectance;
+    m.rayReflectanceGlossinessSamplerType = sampler_of(r.refl_sampler);
+    m.rayReflectanceGlossinessSampleCountSqrt = r.refl_n;
+    m.rayReflectanceGlossiness = r.refl_glossiness;
+    m.rayTransmittance = r.ray_transmittance;
+    m.rayTransmittanceGlossinessSamplerType = sampler_of(r.trans_sampler);
+    m.rayTransmittanceGlossinessSampleCountSqrt = r.trans_n;
+    m.rayTransmittanceGlossiness = r.trans_glossiness;
+    m.enableRayTransmissionAttenuation = r.enable_attenuation != 0;
+    m.rayTransmissionAttenuationFactor = r.attenuation_factor;
+    m.rayTransmissionAttenuationColor = Color{r.attenuation_color[0], r.attenuation_color[1], r.attenuation_color[2]};
+    return m;
+}
 
 struct Triangle {
     Vector3 vertices[3];
@@ -364,13 +425,15 @@ public:
         s.general.sunIrradiance = Color{c.sun_irradiance[0], c.sun_irradiance[1], c.sun_irradiance[2]};
         s.general.seed = c.seed;
         const rc_point_light& l = d.light;
-        PointLight pl;
-        pl.position = Vector3{l.position[0], l.position[1], l.position[2]};
-        pl.color = Color{l.color[0], l.color[1], l.color[2]};
-        pl.intensity = l.intensity;
-        pl.maxDistance = l.max_distance;
-        pl.attenuation = l.attenuation;
-        s.lights.pointLights.push_back(pl);
+        if (l.max_distance > 0.0f) {  // the scenes lit for the ray tracer carry a point light
+            PointLight pl;
+            pl.position = Vector3{l.position[0], l.position[1], l.position[2]};
+            pl.color = Color{l.color[0], l.color[1], l.color[2]};
+            pl.intensity = l.intensity;
+            pl.maxDistance = l.max_distance;
+            pl.attenuation = l.attenuation;
+            s.lights.pointLights.push_back(pl);
+        }
         s.lights.ambientLight.color = Color{l.ambient_color[0], l.ambient_color[1], l.ambient_color[2]};
         s.lights.ambientLight.intensity = l.ambient_intensity;
         s.defaultWidth = c.width;
@@ -403,11 +466,132 @@ public:
         return s;
     }
 
+    // Scene::loadFromFile / saveToFile (Scene.cpp:67-183): cereal XML or JSON
+    // by the file's extension.  Triangles, materials, lights, settings, the
+    // model list and a built root BVH; other primitive types are skipped.
+    static Scene loadFromFile(const std::string& path) {
+        rch_scene_file_data* d = nullptr;
+        check(rch_scene_file_load(path.c_str(), &d));
+        Scene s;
+        try {
+            s.readFileData(*d);
+        } catch (...) {
+            rch_scene_file_free(d);
+            throw;
+        }
+        rch_scene_file_free(d);
+        return s;
+    }
+    inline void saveToFile(const std::string& path) const;  // defined after HipTracer
+
     const std::vector<float>& flatVertices() const { return vertices_; }
     const std::vector<uint32_t>& materialIds() const { return materialIds_; }
     uint32_t defaultWidth = 0, defaultHeight = 0;
 
 private:
+    void readFileData(const rch_scene_file_data& d) {
+        auto col = [](const float* c) { return Color{c[0], c[1], c[2]}; };
+        auto vec = [](const float* v) { return Vector3{v[0], v[1], v[2]}; };
+        const rc_render_cfg& c = d.cfg;
+        const rc_ray_settings& r = d.ray;
+        general.tracerType = d.tracer_type == 0 ? TracerType::RAY : TracerType::PATH;
+        general.maxRayIterations = r.max_ray_iterations;
+        general.maxPathLength = c.max_path_length;
+        general.rayStartOffset = c.ray_start_offset;
+        general.backgroundColor = col(r.background);
+        general.multiSamplerType = sampler_of(c.sampler);
+        uint64_t n = 1;
+        while ((n + 1) * (n + 1) <= c.spp) ++n;
+        general.multiSampleCountSqrt = n;
+        general.multiSamplerFilterType = FilterType(c.filter);
+        general.filterRadius = c.filter_radius;
+        if (c.filter == RC_FILTER_GAUSSIAN) general.gaussianStandardDeviation = c.filter_param[0];
+        if (c.filter == RC_FILTER_MITCHELL) {
+            general.mitchellB = c.filter_param[0];
+            general.mitchellC = c.filter_param[1];
+        }
+        general.visualizeDepth = r.visualize_depth != 0;
+        general.visualizeDepthMaxDistance = r.visualize_depth_max;
+        camera.position = vec(d.camera.position);
+        camera.orientation = EulerAngle{d.camera.pitch, d.camera.yaw, d.camera.roll};
+        camera.fov = d.camera.fov;
+        const ToneMapperType tt[4] = {ToneMapperType::REINHARD, ToneMapperType::LINEAR, ToneMapperType::SIMPLE,
+                                      ToneMapperType::PASSTHROUGH};
+        toneMapper.type = tt[d.tonemap.type & 3u];
+        toneMapper.applyGamma = d.tonemap.apply_gamma != 0;
+        toneMapper.shouldClamp = d.tonemap.clamp != 0;
+        toneMapper.gamma = d.tonemap.gamma;
+        toneMapper.exposure = d.tonemap.exposure;
+        toneMapper.key = d.tonemap.key;
+        toneMapper.enableAveraging = d.tonemap.enable_averaging != 0;
+        toneMapper.averagingAlpha = d.tonemap.averaging_alpha;
+        simpleFog.enabled = r.fog_enabled != 0;
+        simpleFog.color = col(r.fog_color);
+        simpleFog.distance = r.fog_distance;
+        simpleFog.steepness = r.fog_steepness;
+        simpleFog.heightDispersion = r.fog_height_dispersion != 0;
+        simpleFog.height = r.fog_height;
+        simpleFog.heightSteepness = r.fog_height_steepness;
+        AmbientLight& a = lights.ambientLight;
+        a.color = col(r.ambient_color);
+        a.intensity = r.ambient_intensity;
+        a.enableAmbientOcclusion = r.ao_enabled != 0;
+        a.ambientOcclusionSamplerType = sampler_of(r.ao_sampler);
+        a.ambientOcclusionSampleCountSqrt = r.ao_n;
+        a.ambientOcclusionMaxSampleDistance = r.ao_max_distance;
+        a.ambientOcclusionSampleDistribution = r.ao_distribution;
+        for (size_t i = 0; i < d.nlights; ++i) {
+            const rc_light& L = d.lights[i];
+            if (L.type == RC_LIGHT_DIRECTIONAL) {
+                DirectionalLight x;
+                x.color = col(L.color);
+                x.intensity = L.intensity;
+                x.direction = vec(L.direction);
+                lights.directionalLights.push_back(x);
+                continue;
+            }
+            SpotLight x;
+            x.color = col(L.color);
+            x.intensity = L.intensity;
+            x.position = vec(L.position);
+            x.maxDistance = L.max_distance;
+            x.attenuation = L.attenuation;
+            x.enableAreaLight = L.area_light != 0;
+            x.areaLightSamplerType = sampler_of(L.area_sampler);
+            x.areaLightSampleCountSqrt = L.area_n;
+            x.areaLightRadius = L.area_radius;
+            if (L.type == RC_LIGHT_POINT) {
+                lights.pointLights.push_back(x);
+            } else {
+                x.direction = vec(L.direction);
+                x.angle = L.angle;
+                x.sideAttenuation = L.side_attenuation;
+                lights.spotLights.push_back(x);
+            }
+        }
+        for (size_t i = 0; i < d.nmat; ++i) materials.push_back(from_rc_material(d.materials[i], d.ray_materials[i]));
+        primitives.triangles.resize(d.ntri);
+        for (size_t i = 0; i < d.ntri; ++i) {
+            for (int k = 0; k < 3; ++k) primitives.triangles[i].vertices[k] = vec(d.vertices + 9 * i + 3 * k);
+            primitives.triangles[i].materialId = d.tri_material[i];
+        }
+        for (size_t i = 0; i < d.nmodels; ++i) {
+            const rch_scene_model& m = d.models[i];
+            ModelLoaderInfo mi;
+            mi.modelFilePath = m.path;
+            mi.scale = Vector3{m.info.scale[0], m.info.scale[1], m.info.scale[2]};
+            mi.rotate = EulerAngle{m.info.rotate[0], m.info.rotate[1], m.info.rotate[2]};
+            mi.translate = Vector3{m.info.translate[0], m.info.translate[1], m.info.translate[2]};
+            mi.ignoreMaterials = m.info.ignore_materials != 0;
+            models.push_back(mi);
+        }
+        if (d.nnodes) {
+            rootBVH.bvh.flatNodes.assign(d.nodes, d.nodes + d.nnodes);
+            rootBVH.bvh.orderedPrimitiveIds.assign(d.ordered, d.ordered + d.ntri);
+            rootBVH.bvh.hasBeenBuilt = true;
+        }
+    }
+
     void loadModel(const ModelLoaderInfo& mi) {
         rch_obj_info info;
         rch_obj_default_info(&info);
@@ -554,20 +738,7 @@ public:
         if (!scene.rootBVH.bvh.hasBeenBuilt) throw RaycerError(RC_ERR_INVALID, "scene is not initialized");
         releaseBuffers();
         std::vector<rc_material> mats(scene.materials.size());
-        for (size_t i = 0; i < mats.size(); ++i) {
-            const Material& m = scene.materials[i];
-            rc_material& r = mats[i];
-            r = rc_material{};
-            const Color* src[4] = {&m.diffuseReflectance, &m.emittance, &m.ambientReflectance, &m.specularReflectance};
-            float* dst[4] = {r.diffuse, r.emittance, r.ambient, r.specular};
-            for (int k = 0; k < 4; ++k) {
-                dst[k][0] = float(src[k]->r);
-                dst[k][1] = float(src[k]->g);
-                dst[k][2] = float(src[k]->b);
-            }
-            r.shininess = float(m.specularShininess);
-            r.flags = (m.isEmissive ? RC_MAT_EMISSIVE : 0u) | (m.nonShadowing ? RC_MAT_NON_SHADOWING : 0u);
-        }
+        for (size_t i = 0; i < mats.size(); ++i) mats[i] = to_rc_material(scene.materials[i]);
         const auto& bvh = scene.rootBVH.bvh;
         check(rc_scene_upload(ctx_, scene.flatVertices().data(), scene.flatVertices().size() / 9,
                               scene.materialIds().data(), mats.data(), mats.size(), bvh.flatNodes.data(),
@@ -676,24 +847,7 @@ public:
         r.fog_height_dispersion = f.heightDispersion ? 1u : 0u;
         r.fog_height = float(f.height);
         r.fog_height_steepness = float(f.heightSteepness);
-        for (const Material& m : sc.materials) {
-            rc_ray_material x{};
-            x.ray_reflectance = float(m.rayReflectance);
-            x.ray_transmittance = float(m.rayTransmittance);
-            x.refractive_index = float(m.refractiveIndex);
-            x.fresnel_reflection = m.fresnelReflection ? 1u : 0u;
-            x.skip_lighting = m.skipLighting ? 1u : 0u;
-            x.refl_sampler = to_rc(m.rayReflectanceGlossinessSamplerType);
-            x.refl_n = uint32_t(m.rayReflectanceGlossinessSampleCountSqrt);
-            x.refl_glossiness = float(m.rayReflectanceGlossiness);
-            x.trans_sampler = to_rc(m.rayTransmittanceGlossinessSamplerType);
-            x.trans_n = uint32_t(m.rayTransmittanceGlossinessSampleCountSqrt);
-            x.trans_glossiness = float(m.rayTransmittanceGlossiness);
-            x.enable_attenuation = m.enableRayTransmissionAttenuation ? 1u : 0u;
-            x.attenuation_factor = float(m.rayTransmissionAttenuationFactor);
-            put3(x.attenuation_color, m.rayTransmissionAttenuationColor);
-            in.materials.push_back(x);
-        }
+        for (const Material& m : sc.materials) in.materials.push_back(to_rc_ray_material(m));
         auto point = [&](const PointLight& p, uint32_t type) {
             rc_light l{};
             l.type = type;
@@ -803,5 +957,85 @@ private:
     uint64_t width_ = 0, height_ = 0;
     uint32_t frame_ = 0;
 };
+
+inline void Scene::saveToFile(const std::string& path) const {
+    rch_scene_file_data d{};
+    d.tracer_type = general.tracerType == TracerType::RAY ? 0u : 1u;
+    d.camera = rc_camera{{float(camera.position.x), float(camera.position.y), float(camera.position.z)},
+                         float(camera.orientation.pitch), float(camera.orientation.yaw),
+                         float(camera.orientation.roll), float(camera.fov)};
+    rc_render_cfg& c = d.cfg;
+    c.sampler = to_rc(general.multiSamplerType);
+    c.spp = uint32_t(general.multiSampleCountSqrt * general.multiSampleCountSqrt);
+    c.filter = to_rc(general.multiSamplerFilterType);
+    c.filter_radius = float(general.filterRadius);
+    c.max_path_length = uint32_t(general.maxPathLength);
+    c.ray_start_offset = float(general.rayStartOffset);
+    d.tonemap.type = to_rc(toneMapper.type);
+    d.tonemap.apply_gamma = toneMapper.applyGamma ? 1u : 0u;
+    d.tonemap.clamp = toneMapper.shouldClamp ? 1u : 0u;
+    d.tonemap.gamma = float(toneMapper.gamma);
+    d.tonemap.exposure = float(toneMapper.exposure);
+    d.tonemap.key = float(toneMapper.key);
+    d.tonemap.enable_averaging = toneMapper.enableAveraging ? 1u : 0u;
+    d.tonemap.averaging_alpha = float(toneMapper.averagingAlpha);
+    HipTracer::RayInputs in = HipTracer::rayInputs(*this);
+    d.ray = in.settings;
+    std::vector<rc_material> mats;
+    std::vector<uint64_t> ids;
+    for (size_t i = 0; i < materials.size(); ++i) {
+        mats.push_back(to_rc_material(materials[i]));
+        ids.push_back(i + 1);
+    }
+    std::vector<float> v(primitives.triangles.size() * 9);
+    std::vector<uint32_t> tm(primitives.triangles.size());
+    std::vector<uint64_t> tid(primitives.triangles.size());
+    for (size_t i = 0; i < primitives.triangles.size(); ++i) {
+        const Triangle& t = primitives.triangles[i];
+        for (int k = 0; k < 3; ++k) {
+            v[9 * i + 3 * k] = float(t.vertices[k].x);
+            v[9 * i + 3 * k + 1] = float(t.vertices[k].y);
+            v[9 * i + 3 * k + 2] = float(t.vertices[k].z);
+        }
+        tm[i] = uint32_t(t.materialId);
+        tid[i] = i + 1;
+    }
+    std::vector<rch_scene_model> ms(models.size());
+    for (size_t i = 0; i < models.size(); ++i) {
+        const ModelLoaderInfo& mi = models[i];
+        rch_scene_model& m = ms[i];
+        m = rch_scene_model{};
+        if (mi.modelFilePath.size() >= sizeof(m.path)) throw RaycerError(RC_ERR_INVALID, "model path too long");
+        std::memcpy(m.path, mi.modelFilePath.c_str(), mi.modelFilePath.size() + 1);
+        rch_obj_default_info(&m.info);
+        m.info.scale[0] = mi.scale.x; m.info.scale[1] = mi.scale.y; m.info.scale[2] = mi.scale.z;
+        m.info.rotate[0] = mi.rotate.pitch; m.info.rotate[1] = mi.rotate.yaw; m.info.rotate[2] = mi.rotate.roll;
+        m.info.translate[0] = mi.translate.x; m.info.translate[1] = mi.translate.y; m.info.translate[2] = mi.translate.z;
+        m.info.ignore_materials = mi.ignoreMaterials ? 1u : 0u;
+    }
+    d.nmat = mats.size();
+    d.materials = mats.data();
+    d.ray_materials = in.materials.data();
+    d.material_ids = ids.data();
+    d.ntri = tm.size();
+    d.vertices = v.data();
+    d.tri_material = tm.data();
+    d.tri_ids = tid.data();
+    d.nlights = in.lights.size();
+    d.lights = in.lights.data();
+    d.nmodels = ms.size();
+    d.models = ms.data();
+    // a built tree is written only when it covers exactly these triangles
+    std::vector<rc_flat_node> nodes;
+    std::vector<uint64_t> ordered;
+    if (rootBVH.bvh.hasBeenBuilt && models.empty() && rootBVH.bvh.orderedPrimitiveIds.size() == tm.size()) {
+        nodes = rootBVH.bvh.flatNodes;
+        ordered = rootBVH.bvh.orderedPrimitiveIds;
+        d.nnodes = nodes.size();
+        d.nodes = nodes.data();
+        d.ordered = ordered.data();
+    }
+    check(rch_scene_file_save(path.c_str(), &d));
+}
 
 }  // namespace raycer

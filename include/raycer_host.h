@@ -115,6 +115,54 @@ size_t rch_obj_material_count(const rch_obj* model);
 int rch_obj_get(const rch_obj* model, float* vertices, uint32_t* tri_material, rc_material* materials);
 int rch_obj_free(rch_obj* model);
 
+/* ------------------------------------------------------------ scene files
+ * Scene::loadFromFile / saveToFile (src/Raytracing/Scene.cpp:67-183) for the
+ * cereal XML (.xml) and JSON (.json) forms of Scene::serialize (Scene.h:330-345):
+ * the settings, camera, tone mapper, simple fog, lights, materials, triangle
+ * primitives, the model list and a built root BVH.  Other primitive types
+ * (planes, spheres, boxes, ...) are counted and skipped, as are textures and
+ * the volumetric fog; cereal's binary form is not read. */
+typedef struct rch_scene_model {   /* ModelLoaderInfo (src/Utils/ModelLoader.h:27-85) */
+    char path[1024];               /* modelFilePath */
+    rch_obj_info info;             /* scale, rotate, translate, baseMaterial, ignoreMaterials */
+    uint32_t invisible_triangles;
+    uint32_t pad;
+} rch_scene_model;
+
+typedef struct rch_scene_file_data {
+    uint32_t tracer_type;          /* 0 ray tracer, 1 path tracer (TracerType) */
+    uint32_t pad;
+    rc_camera camera;
+    rc_render_cfg cfg;             /* sampler, filter, spp = multiSampleCountSqrt^2, max_path_length,
+                                      ray_start_offset; width and height 0 (kept outside the scene) */
+    rc_tonemap tonemap;
+    rc_ray_settings ray;
+    size_t nmat;
+    rc_material* materials;
+    rc_ray_material* ray_materials;
+    uint64_t* material_ids;        /* Material::id of each entry */
+    size_t ntri;                   /* visible triangles */
+    float* vertices;               /* 9 floats per triangle */
+    uint32_t* tri_material;        /* index into materials (resolved from Primitive::materialId) */
+    uint64_t* tri_ids;             /* Primitive::id of each triangle */
+    size_t nlights;
+    rc_light* lights;              /* directional, then point, then spot lights */
+    size_t nmodels;
+    rch_scene_model* models;
+    size_t nnodes;                 /* 0 unless the file holds a built BVH over exactly these triangles */
+    rc_flat_node* nodes;
+    uint64_t* ordered;             /* orderedPrimitiveIds as indices into the triangle arrays */
+    size_t skipped_primitives;     /* non-triangle primitives and invisible triangles in the file */
+} rch_scene_file_data;
+
+/* Reads a scene file; the result is freed with rch_scene_file_free. */
+int rch_scene_file_load(const char* path, rch_scene_file_data** out);
+/* Parses a scene from text (json != 0: JSON, else XML). */
+int rch_scene_file_parse(const char* text, size_t len, int json, rch_scene_file_data** out);
+/* Writes a scene (.xml or .json by the path's extension) in cereal's layout. */
+int rch_scene_file_save(const char* path, const rch_scene_file_data* scene);
+void rch_scene_file_free(rch_scene_file_data* scene);
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,6 +201,24 @@ class ObjInfo(C.Structure):
                 ("base", Material), ("ignore_materials", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class SceneModel(C.Structure):
+    _fields_ = [("path", C.c_char * 1024), ("info", ObjInfo), ("invisible_triangles", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class SceneFileData(C.Structure):
+    """rch_scene_file_data: a scene read from or written to a cereal XML / JSON scene file."""
+    _fields_ = [("tracer_type", C.c_uint32), ("pad", C.c_uint32), ("camera", Camera), ("cfg", RenderCfg),
+                ("tonemap", ToneMap), ("ray", RaySettings),
+                ("nmat", C.c_size_t), ("materials", C.POINTER(Material)), ("ray_materials", C.POINTER(RayMaterial)),
+                ("material_ids", C.POINTER(C.c_uint64)),
+                ("ntri", C.c_size_t), ("vertices", C.POINTER(C.c_float)), ("tri_material", C.POINTER(C.c_uint32)),
+                ("tri_ids", C.POINTER(C.c_uint64)),
+                ("nlights", C.c_size_t), ("lights", C.POINTER(Light)),
+                ("nmodels", C.c_size_t), ("models", C.POINTER(SceneModel)),
+                ("nnodes", C.c_size_t), ("nodes", C.c_void_p), ("ordered", C.POINTER(C.c_uint64)),
+                ("skipped_primitives", C.c_size_t)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("ntri", C.c_size_t), ("nmat", C.c_size_t), ("camera", Camera), ("cfg", RenderCfg),
                 ("light", PointLight), ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3)]
@@ -275,6 +293,9 @@ def lib() -> C.CDLL:
             "rch_bvh_file_counts": [C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
             "rch_bvh_load": [C.c_char_p, vp, sz, vp, vp],
             "rch_obj_load": [C.c_char_p, C.POINTER(ObjInfo), C.POINTER(vp)],
+            "rch_scene_file_load": [C.c_char_p, C.POINTER(C.POINTER(SceneFileData))],
+            "rch_scene_file_parse": [C.c_char_p, sz, i32, C.POINTER(C.POINTER(SceneFileData))],
+            "rch_scene_file_save": [C.c_char_p, C.POINTER(SceneFileData)],
             "rch_ply_load": [C.c_char_p, C.POINTER(ObjInfo), C.POINTER(vp)],
             "rch_obj_triangle_count": [vp],
             "rch_obj_material_count": [vp],
@@ -292,6 +313,8 @@ def lib() -> C.CDLL:
         L.rch_obj_material_count.restype = C.c_size_t
         L.rch_obj_default_info.argtypes = [C.POINTER(ObjInfo)]
         L.rch_obj_default_info.restype = None
+        L.rch_scene_file_free.argtypes = [C.POINTER(SceneFileData)]
+        L.rch_scene_file_free.restype = None
         _lib = L
     return _lib
 
@@ -403,6 +426,118 @@ def obj_scene(path, camera: Camera, cfg: RenderCfg, light: PointLight | None = N
     for k in range(3):
         d.bounds_min[k], d.bounds_max[k] = float(p[:, k].min()), float(p[:, k].max())
     return SceneData(v, m, mats, d)
+
+
+@dataclass
+class SceneFile:
+    """A scene file's contents (Scene::loadFromFile): settings and arrays."""
+    tracer_type: int
+    camera: Camera
+    cfg: RenderCfg
+    tonemap: ToneMap
+    ray: RaySettings
+    materials: np.ndarray       # MATERIAL_DTYPE
+    ray_materials: C.Array      # RayMaterial * nmat
+    material_ids: np.ndarray    # uint64
+    vertices: np.ndarray        # (ntri, 9) float32
+    tri_material: np.ndarray    # uint32 index into materials
+    tri_ids: np.ndarray         # uint64 Primitive::id
+    lights: list
+    models: list                # (path, ObjInfo, invisible_triangles)
+    nodes: np.ndarray | None    # NODE_DTYPE, None without a built BVH
+    ordered: np.ndarray | None
+    skipped_primitives: int
+
+    def scene_data(self) -> "SceneData":
+        d = SceneDesc()
+        d.ntri, d.nmat, d.camera, d.cfg = len(self.tri_material), len(self.materials), self.camera, self.cfg
+        if len(self.vertices):
+            p = self.vertices.reshape(-1, 3)
+            for k in range(3):
+                d.bounds_min[k], d.bounds_max[k] = float(p[:, k].min()), float(p[:, k].max())
+        return SceneData(self.vertices, self.tri_material, self.materials, d)
+
+
+def _scene_file(ptr) -> SceneFile:
+    d = ptr.contents
+    try:
+        def arr(p, n, dt, shape=None):
+            if n == 0:
+                return np.zeros(shape or (0,), dt)
+            a = np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,))
+            return a.copy().reshape(shape) if shape else a.copy()
+        mats = np.zeros(d.nmat, MATERIAL_DTYPE)
+        if d.nmat:
+            C.memmove(mats.ctypes.data, d.materials, d.nmat * MATERIAL_DTYPE.itemsize)
+        rm = (RayMaterial * d.nmat)()
+        if d.nmat:
+            C.memmove(rm, d.ray_materials, d.nmat * C.sizeof(RayMaterial))
+        nodes = ordered = None
+        if d.nnodes:
+            nodes = np.zeros(d.nnodes, NODE_DTYPE)
+            C.memmove(nodes.ctypes.data, d.nodes, d.nnodes * NODE_DTYPE.itemsize)
+            ordered = arr(d.ordered, d.ntri, np.uint64)
+        return SceneFile(
+            d.tracer_type, Camera.from_buffer_copy(d.camera), RenderCfg.from_buffer_copy(d.cfg),
+            ToneMap.from_buffer_copy(d.tonemap), RaySettings.from_buffer_copy(d.ray), mats, rm,
+            arr(d.material_ids, d.nmat, np.uint64), arr(d.vertices, d.ntri * 9, np.float32, (d.ntri, 9)),
+            arr(d.tri_material, d.ntri, np.uint32), arr(d.tri_ids, d.ntri, np.uint64),
+            [Light.from_buffer_copy(d.lights[i]) for i in range(d.nlights)],
+            [(d.models[i].path.decode(), ObjInfo.from_buffer_copy(d.models[i].info), d.models[i].invisible_triangles)
+             for i in range(d.nmodels)],
+            nodes, ordered, int(d.skipped_primitives))
+    finally:
+        lib().rch_scene_file_free(ptr)
+
+
+def load_scene_file(path) -> SceneFile:
+    """rch_scene_file_load: a cereal XML or JSON scene file (Scene::loadFromFile)."""
+    p = C.POINTER(SceneFileData)()
+    check(lib().rch_scene_file_load(str(path).encode(), C.byref(p)))
+    return _scene_file(p)
+
+
+def parse_scene_file(text: str, json: bool) -> SceneFile:
+    b = text.encode()
+    p = C.POINTER(SceneFileData)()
+    check(lib().rch_scene_file_parse(b, len(b), int(json), C.byref(p)))
+    return _scene_file(p)
+
+
+def save_scene_file(path, sf: SceneFile) -> None:
+    """rch_scene_file_save: writes the scene as cereal XML or JSON (by extension)."""
+    d = SceneFileData()
+    d.tracer_type, d.camera, d.cfg, d.tonemap, d.ray = sf.tracer_type, sf.camera, sf.cfg, sf.tonemap, sf.ray
+    mats = np.ascontiguousarray(sf.materials)
+    rm = sf.ray_materials
+    ids = np.ascontiguousarray(sf.material_ids, np.uint64)
+    v = np.ascontiguousarray(sf.vertices, np.float32)
+    tm = np.ascontiguousarray(sf.tri_material, np.uint32)
+    tid = np.ascontiguousarray(sf.tri_ids, np.uint64)
+    lts = (Light * max(1, len(sf.lights)))(*sf.lights)
+    models = (SceneModel * max(1, len(sf.models)))()
+    for i, (path_, info, inv) in enumerate(sf.models):
+        models[i].path, models[i].info, models[i].invisible_triangles = path_.encode(), info, inv
+    d.nmat = len(mats)
+    d.materials = C.cast(mats.ctypes.data, C.POINTER(Material))
+    d.ray_materials = C.cast(rm, C.POINTER(RayMaterial))
+    d.material_ids = C.cast(ids.ctypes.data, C.POINTER(C.c_uint64))
+    d.ntri = len(tm)
+    d.vertices = C.cast(v.ctypes.data, C.POINTER(C.c_float))
+    d.tri_material = C.cast(tm.ctypes.data, C.POINTER(C.c_uint32))
+    d.tri_ids = C.cast(tid.ctypes.data, C.POINTER(C.c_uint64))
+    d.nlights = len(sf.lights)
+    d.lights = C.cast(lts, C.POINTER(Light))
+    d.nmodels = len(sf.models)
+    d.models = C.cast(models, C.POINTER(SceneModel))
+    nodes = ordered = None
+    if sf.nodes is not None and len(sf.nodes):
+        nodes = np.ascontiguousarray(sf.nodes)
+        ordered = np.ascontiguousarray(sf.ordered, np.uint64)
+        d.nnodes = len(nodes)
+        d.nodes = nodes.ctypes.data
+        d.ordered = C.cast(ordered.ctypes.data, C.POINTER(C.c_uint64))
+    check(lib().rch_scene_file_save(str(path).encode(), C.byref(d)))
 
 
 def build_bvh(vertices: np.ndarray, max_leaf: int = 5, threads: int = 0, method: int = 0, tri_cost: float = 0.0):

@@ -11,7 +11,10 @@
 //                 [--sampler center|random|regular|jittered|cmj|poisson_disc]
 //                 [--filter box|tent|bell|gaussian|mitchell|lanczos_sinc]
 //                 [--tonemapper passthrough|linear|simple|reinhard]
+//                 [--scene-file scene.xml|scene.json] [--save-scene out.xml|out.json]
 //
+// --scene-file reads a cereal XML or JSON scene (Scene::loadFromFile) instead
+// of a generated one; --save-scene writes the scene that was rendered.
 // cornell is raycer's TestScene3 (the Cornell box OBJ, by default the copy in
 // tests/golden/meshes next to this program's package); its default tracer is
 // the RAY (Whitted) tracer with the scene's ambient and point light, --tracer
@@ -54,6 +57,7 @@ const char* const kToneMappers[] = {"passthrough", "linear", "simple", "reinhard
 
 int main(int argc, char** argv) {
     std::string scene_name = "box", linear_out, tm_out, mesh, tracer_name, sampler_name, filter_name, tm_name;
+    std::string scene_file, save_scene;
     uint32_t seed = 0, level = 0, grid = 0, width = 0, height = 0, spp = 0, frames = 1;
     uint64_t tris = 0;
     for (int i = 1; i < argc; ++i) {
@@ -79,6 +83,8 @@ int main(int argc, char** argv) {
             else if (a == "--sampler") sampler_name = next();
             else if (a == "--filter") filter_name = next();
             else if (a == "--tonemapper") tm_name = next();
+            else if (a == "--scene-file") scene_file = next();
+            else if (a == "--save-scene") save_scene = next();
             else {
                 std::fprintf(stderr, "unknown option %s\n", a.c_str());
                 return 2;
@@ -92,7 +98,10 @@ int main(int argc, char** argv) {
     try {
         auto t0 = std::chrono::steady_clock::now();
         raycer::Scene scene;
-        if (scene_name == "cornell") {
+        if (!scene_file.empty()) {
+            scene = raycer::Scene::loadFromFile(scene_file);
+            scene_name = scene_file;
+        } else if (scene_name == "cornell") {
             if (mesh.empty()) {
                 std::string exe(argv[0]);
                 size_t slash = exe.find_last_of('/');
@@ -132,7 +141,10 @@ int main(int argc, char** argv) {
             }
         }
         scene.initialize();
-        uint32_t w = width ? width : scene.defaultWidth, h = height ? height : scene.defaultHeight;
+        if (!save_scene.empty()) scene.saveToFile(save_scene);
+        // a scene file carries no film size: 1280 x 800 unless given
+        uint32_t w = width ? width : (scene.defaultWidth ? scene.defaultWidth : 1280),
+                 h = height ? height : (scene.defaultHeight ? scene.defaultHeight : 800);
         raycer::Film film;
         film.resize(w, h);
         raycer::HipTracer tracer(0);

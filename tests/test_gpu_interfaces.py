@@ -230,3 +230,25 @@ def test_console_runner_renders_the_reference_cornell_scene(ctx, tmp_path):
     assert r.returncode == 0, r.stderr
     lin = read_pfm(pt)
     assert np.isfinite(lin).all() and lin.mean() > 0.0
+
+
+@pytest.mark.parametrize("ext", ["xml", "json"])
+def test_console_runner_scene_file_round_trip(ctx, tmp_path, ext):
+    """raycer_render --save-scene then --scene-file (Scene::saveToFile /
+    loadFromFile): the reloaded scene, its built BVH included, renders the
+    same RAY-tracer image bit for bit."""
+    scene_file = tmp_path / f"sphere.{ext}"
+    a, b = tmp_path / "a.pfm", tmp_path / "b.pfm"
+    common = ["--width", "48", "--height", "40", "--spp", "4"]
+    r = subprocess.run([str(RUNNER), "--scene", "sphere", "--level", "2", "--grid", "8", "--tracer", "ray", *common,
+                        "--save-scene", str(scene_file), "--linear", str(a)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(RUNNER), "--scene-file", str(scene_file), *common, "--linear", str(b)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert json.loads(r.stdout.strip().splitlines()[-1])["tracer"] == "ray"
+    ia, ib = read_pfm(a), read_pfm(b)
+    assert ia.max() > 0 and np.array_equal(ia, ib)
+    sf = api.load_scene_file(scene_file)
+    assert sf.nodes is not None and len(sf.vertices) == len(sf.tri_material) > 0 and len(sf.lights) == 1
