@@ -230,6 +230,57 @@ float round_up(double x) {
     return f;
 }
 
+// One wide node (four child records) as a QNode.  Per axis the scale is the
+// smallest power of two 2^e with 253 * 2^e >= the node's extent and 2^e >= 8
+// ulps of its largest coordinate; children are quantized one unit outward on
+// each side, so the decoded box origin + q * 2^e contains the exact one with a
+// unit to spare, which covers the traversal's rounding of the decode.
+void quantize_node(const NodeRec* r, QNode& q) {
+    std::memset(&q, 0, sizeof(q));
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < kWidth; ++k) {
+        if (r[k].count == kEmptyChild) continue;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], double(r[k].lo[a]));
+            hi[a] = std::max(hi[a], double(r[k].hi[a]));
+        }
+    }
+    int ebias[3];
+    double scale[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!(lo[a] <= hi[a])) lo[a] = hi[a] = 0.0;  // no child: any box
+        const double ext = hi[a] - lo[a], mag = std::max(std::fabs(lo[a]), std::fabs(hi[a]));
+        int e = -126;
+        if (ext > 0.0) e = std::max(e, int(std::ceil(std::log2(ext / 253.0))));
+        if (mag > 0.0) e = std::max(e, int(std::floor(std::log2(mag))) - 23 + 3);
+        while (std::ldexp(253.0, e) < ext) ++e;  // guard log2 rounding
+        e = std::min(e, 127);
+        ebias[a] = e + 127;
+        scale[a] = std::ldexp(1.0, e);
+        q.origin[a] = float(lo[a]);  // lo is a float (a child's bound)
+    }
+    q.exps = uint32_t(ebias[0]) | (uint32_t(ebias[1]) << 8) | (uint32_t(ebias[2]) << 16);
+    for (int k = 0; k < kWidth; ++k) {
+        const NodeRec& c = r[k];
+        uint32_t meta;
+        if (c.count == kEmptyChild) {
+            meta = 0xFFu;
+        } else {
+            meta = c.count;  // 0 interior, else 1..31 triangles
+            for (int a = 0; a < 3; ++a) {
+                double ql = std::floor((double(c.lo[a]) - double(q.origin[a])) / scale[a]) - 1.0;
+                double qh = std::ceil((double(c.hi[a]) - double(q.origin[a])) / scale[a]) + 1.0;
+                ql = std::max(0.0, std::min(255.0, ql));
+                qh = std::max(0.0, std::min(255.0, qh));
+                q.qlo[a] |= uint32_t(ql) << (8 * k);
+                q.qhi[a] |= uint32_t(qh) << (8 * k);
+            }
+        }
+        q.meta |= meta << (8 * k);
+        q.link[k] = c.count == 0 ? c.link / kWidth : c.link;  // interior links become node indices
+    }
+}
+
 }  // namespace
 
 extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uint32_t* tri_mat,
@@ -392,6 +443,15 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     // a traversal stack holds at most (kWidth - 1) entries per level
     if (depth * (kWidth - 1) >= uint32_t(kDeepStack))
         return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: BVH too deep for the traversal stack");
+#if RC_QNODE
+    std::vector<QNode> qn(nrec / kWidth);
+    for (size_t n = 0; n < qn.size(); ++n) quantize_node(&recs[n * kWidth], qn[n]);
+    const void* node_src = qn.data();
+    const size_t b_nodes = qn.size() * sizeof(QNode);
+#else
+    const void* node_src = recs.data();
+    const size_t b_nodes = nrec * sizeof(NodeRec);
+#endif
     std::vector<TriRec> tris(ntri);
     std::vector<float4> shade(ntri);
     std::vector<DevMaterial> dm(nmat);
@@ -445,8 +505,7 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     s->depth = depth;
     std::memcpy(s->bmin, bmin, sizeof(bmin));
     std::memcpy(s->bmax, bmax, sizeof(bmax));
-    size_t b_nodes = nrec * sizeof(NodeRec), b_tris = ntri * sizeof(TriRec), b_shade = ntri * sizeof(float4),
-           b_mats = nmat * sizeof(DevMaterial);
+    size_t b_tris = ntri * sizeof(TriRec), b_shade = ntri * sizeof(float4), b_mats = nmat * sizeof(DevMaterial);
     s->bytes = b_nodes + b_tris + b_shade + b_mats;
     auto cleanup = [&](int code, const std::string& msg) {
         if (s->nodes) (void)hipFree(s->nodes);
@@ -459,7 +518,7 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     if (hipMalloc(&s->nodes, b_nodes) != hipSuccess || hipMalloc(&s->tris, b_tris) != hipSuccess ||
         hipMalloc(&s->shade, b_shade) != hipSuccess || hipMalloc(&s->mats, b_mats) != hipSuccess)
         return cleanup(RC_ERR_ALLOC, "rc_scene_upload: device allocation failed");
-    if (hipMemcpy(s->nodes, recs.data(), b_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(s->nodes, node_src, b_nodes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->tris, tris.data(), b_tris, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->shade, shade.data(), b_shade, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->mats, dm.data(), b_mats, hipMemcpyHostToDevice) != hipSuccess)

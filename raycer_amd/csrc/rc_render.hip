@@ -354,7 +354,7 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
 // listed for the deep-stack kernel and shaded afterwards by k_shade_listed.
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
-    PassParams P, uint32_t k, const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+    PassParams P, uint32_t k, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
     const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
     const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
 // slot.  Overflowing rays are listed for the deep-stack kernel and applied by
 // k_shadow_apply_listed.
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const NodeRec* __restrict__ nodes,
+__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const NodeT* __restrict__ nodes,
                                                        const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
                                                        const float4* __restrict__ scontrib, float4* __restrict__ L,
                                                        uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr,

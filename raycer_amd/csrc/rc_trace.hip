@@ -43,7 +43,7 @@ __device__ __forceinline__ void store_result(bool any, const TraceResult& R, uin
 }
 
 template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_trace(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+__global__ __launch_bounds__(kBlock) void k_trace(const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                   const rc_ray* __restrict__ rays, const uint32_t* n_dev,
                                                   uint32_t n_host, rc_hit* __restrict__ hits, uint8_t* __restrict__ occ,
                                                   uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr) {
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const NodeRec* __restrict__ no
 }
 
 template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(64) void k_trace_deep(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
+__global__ __launch_bounds__(64) void k_trace_deep(const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                    const rc_ray* __restrict__ rays, rc_hit* __restrict__ hits,
                                                    uint8_t* __restrict__ occ, const uint32_t* __restrict__ ovf_list,
                                                    uint32_t* __restrict__ gstack, DevCounters* __restrict__ ctr) {
