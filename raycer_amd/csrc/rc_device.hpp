@@ -482,7 +482,7 @@ __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict
 struct Trav {
     float3 o, d, inv;
     float tmin, cull;
-    double tmin64, tmax64, best;
+    double best;
     TraceResult R;
     uint32_t cur;
     int sp;
@@ -493,9 +493,7 @@ __device__ __forceinline__ void trav_begin(Trav& S, float3 o, float3 d, float tm
     S.d = d;
     S.inv = safe_inv(d);
     S.tmin = fmaxf(tmin, 0.0f);
-    S.tmin64 = S.tmin;
-    S.tmax64 = tmax;
-    S.best = S.tmax64;
+    S.best = tmax;
     S.cull = tmax * (1.0f + kRefineWindow);
     S.R = TraceResult{tmax, 0.0f, 0.0f, kNoHit, false};
     S.cur = 0;  // root: interior record 0
@@ -602,7 +600,9 @@ __device__ __forceinline__ bool trav_step(Trav& S, const NodeT* __restrict__ nod
     }
     if (S.cur & kLeafTag) {
         const uint32_t first = S.cur & ((1u << 26) - 1u), count = (S.cur >> 26) & 31u;
-        if (test_leaf<ANY, COUNT>(tris, first, count, S.o, S.d, S.tmin, S.tmin64, S.tmax64, S.best, S.cull, S.R,
+        // the FP64 bounds are the FP32 ones widened; an any-hit query's R.t
+        // holds tmax until the query is answered
+        if (test_leaf<ANY, COUNT>(tris, first, count, S.o, S.d, S.tmin, double(S.tmin), double(S.R.t), S.best, S.cull, S.R,
                                   n_tri))
             return true;
         if (S.sp == 0) return true;
