@@ -13,7 +13,8 @@ throughput).
 Prints one JSON line.  Besides the headline Msamples/s it reports the
 roofline of the dominant kernel, k_extend_shade (closest-hit traversal fused
 with shading).  Its algorithmic bytes per frame are
-  32 * node records fetched + 48 * triangle records tested
+  16 * child records fetched (a 4-wide node is 64 bytes: 8-bit quantized
+    child boxes, counts and links) + 48 * triangle records tested
   + 96 per extension ray (ray 32 + path state 16 + shading record 16 in,
     radiance accumulator 16 read + 16 written)
   + 48 per continuation ray written (ray 32 + state 16)
@@ -25,6 +26,10 @@ the FP64 oracle (restatement of the reference's OpenMP CPU path, built -O3
 -ffast-math) on a seeded half of the pixels.
 """
 from __future__ import annotations
+
+# bytes of node data per child of a visited 4-wide node: the library's 64-byte
+# nodes (the 128-byte float-box records of RC_QNODE=0 would be 32)
+NODE_BYTES_PER_CHILD = 16
 
 import argparse
 import json
@@ -276,8 +281,8 @@ def main() -> int:
     ctx.set_counting(False)
     ext_rays = cst.extension_rays
     camera_rays = cfg.width * cfg.height * cfg.spp
-    alg_bytes = (32 * cst.node_records + 48 * cst.tri_tests + 96 * ext_rays + 48 * (ext_rays - camera_rays)
-                 + 48 * cst.shadow_rays)
+    alg_bytes = (NODE_BYTES_PER_CHILD * cst.node_records + 48 * cst.tri_tests + 96 * ext_rays
+                 + 48 * (ext_rays - camera_rays) + 48 * cst.shadow_rays)
     log(f"counting frame: {ext_rays} extension rays, {cst.node_records / ext_rays:.1f} records/ray, "
         f"{cst.tri_tests / ext_rays:.2f} triangles/ray, {cst.shadow_rays} shadow rays")
 
