@@ -187,6 +187,42 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
     return out
 
 
+def phong_c2(ctx, reps: int = 10) -> dict:
+    """Primary ray + one point-light shadow ray per pixel with Phong shading
+    (rc_raycast_phong) on the seeded sphere + ground scene (655,360 + 524,288
+    triangles), 1920x1080, 1 jittered sample.  The device time is the sum of
+    the kernels' HIP-event times; the wall time includes copying the hit
+    indices, distances and RGBA image back to the host (the entry point's
+    contract)."""
+    import raycer_amd as ra
+    from raycer_amd import api
+
+    sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_SPHERE_GROUND, seed=1))
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    s = ctx.upload(sd, nodes, ordered)
+    w, h = sd.desc.cfg.width, sd.desc.cfg.height
+    try:
+        ctx.raycast_phong(s, sd.desc.camera, w, h, 1, sd.desc.cfg.ray_start_offset, sd.desc.light)  # warm-up
+        ctx.set_profiling(True)
+        ctx.stats()
+        dev_ms = 0.0
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.raycast_phong(s, sd.desc.camera, w, h, 1, sd.desc.cfg.ray_start_offset, sd.desc.light)
+            st = ctx.stats()
+            dev_ms += st.extend_ms + st.shadow_ms + st.shade_ms + st.other_ms
+        wall = (time.perf_counter() - t0) / reps
+        ctx.set_profiling(False)
+    finally:
+        s.close()
+    rays = 2 * w * h
+    return {"workload": f"sphere-ground-{sd.ntri}tri-{w}x{h}-1spp-primary+shadow", "triangles": sd.ntri,
+            "device_ms": round(dev_ms / reps, 3), "wall_ms_with_host_copies": round(wall * 1e3, 3),
+            "grays_per_s_device": round(rays / (dev_ms / reps * 1e-3) / 1e9, 3),
+            "mpixels_per_s_device": round(w * h / (dev_ms / reps * 1e-3) / 1e6, 1)}
+
+
 def batch_c4(ctx, n_scenes: int = 64) -> dict:
     """Many-scene workload: soup scenes with seeds 0..n-1 (1k-500k triangles,
     log-uniform; 128^2 to 1024^2; 32 jittered spp; depth 5), autotuned samples
@@ -243,6 +279,7 @@ def main() -> int:
     ap.add_argument("--grid", type=int, default=1024, help="terrain vertices per side (1024 = 2,093,058 triangles)")
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c2", choices=("on", "off"), default="on", help="the 1080p Phong raycast line")
     ap.add_argument("--c4", choices=("on", "off"), default="on",
                     help="also time the 64-scene batch (autotuned, one captured graph)")
     ap.add_argument("--mse", choices=("full", "small", "off"), default="full",
@@ -357,6 +394,7 @@ def main() -> int:
             "bytes_per_ray": round(bytes_per_ray, 1),
             "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
             "achievable_copy_GBs": round(copy_gbs, 1),
+            "frac_of_achievable_copy": round(achieved / copy_gbs, 4),
         },
         "gsample_bounces_per_s": round(total_ext / elapsed / 1e9, 4),
         "shadow_rays_per_s_G": round(total_shadow / elapsed / 1e9, 4),
@@ -369,6 +407,10 @@ def main() -> int:
         "bvh_build_ms": round(build_ms, 1),
     }
     film.close()
+    if a.c2 == "on":
+        log("1080p primary + shadow Phong raycast (1.18M triangles)")
+        out["phong_1080p"] = phong_c2(ctx)
+        log(f"phong: {json.dumps(out['phong_1080p'])}")
     if a.c4 == "on":
         log("64-scene batch (autotuned, one captured graph)")
         out["batch_64_scenes"] = batch_c4(ctx)

@@ -980,6 +980,8 @@ extern "C" int rc_raycast_phong(rc_ctx* c, const rc_scene* s, const rc_camera* c
     if (!cam || !light) return rc_host_fail(RC_ERR_INVALID, "rc_raycast_phong: null camera or light");
     if (w < 2 || h < 2 || size_t(w) * h > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_raycast_phong: bad size");
     RC_TRY_HIP(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    c->stats = rc_stats{};  // this call's statistics only
     const uint32_t npix = w * h;
     RC_TRY(alloc_pass(c, npix));
     CamBasis B = make_basis(*cam, w, h);
@@ -1024,5 +1026,7 @@ extern "C" int rc_raycast_phong(rc_ctx* c, const rc_scene* s, const rc_camera* c
     RC_TRY(c->prof_collect());
     c->stats.extension_rays = npix;
     c->stats.shadow_rays = npix;
+    c->stats.samples = npix;
+    c->stats.render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RC_OK;
 }
