@@ -74,7 +74,7 @@ void sah_split(const std::vector<Prim>& p, const Box& node, const BuildEntry& e,
 }  // namespace
 
 extern "C" int or_bvh_build(const float* vertices, size_t ntri, uint64_t max_leaf, or_bvh** out) {
-    if (!out || (ntri && !vertices) || ntri == 0) return 1;
+    if (!out || (ntri && !vertices)) return 1;  // no triangles: one empty leaf, as the reference builds
     std::vector<Prim> prims(ntri);
     for (uint64_t i = 0; i < ntri; ++i) {
         Tri t = load_tri(vertices, i);

@@ -262,7 +262,9 @@ struct rch_bvh {
 };
 
 extern "C" int rch_bvh_build(const float* v, size_t ntri, const rch_bvh_build_info* info, rch_bvh** out) {
-    if (!out || !v || ntri == 0) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_build: empty input");
+    // no triangles: one leaf with AABB()'s empty box and no primitives, as
+    // FlatBVH::build makes over an empty list (FlatBVH.cpp:112-139)
+    if (!out || (!v && ntri != 0)) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_build: null argument");
     if (ntri > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_build: more than 2^31-1 triangles");
     auto t0 = std::chrono::steady_clock::now();
     rch_bvh* b = new (std::nothrow) rch_bvh();
@@ -356,7 +358,7 @@ struct FileHeader {
 
 extern "C" int rch_bvh_save(const char* path, const float* vertices, size_t ntri, const rc_flat_node* nodes,
                             size_t nnodes, const uint64_t* ordered) {
-    if (!path || !vertices || !nodes || !ordered || ntri == 0 || nnodes == 0)
+    if (!path || !nodes || nnodes == 0 || (ntri != 0 && (!vertices || !ordered)))
         return rc_host_fail(RC_ERR_INVALID, "rch_bvh_save: null or empty argument");
     FILE* f = std::fopen(path, "wb");
     if (!f) return rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_save: cannot write ") + path);
@@ -394,7 +396,8 @@ extern "C" int rch_bvh_file_counts(const char* path, size_t* nnodes, size_t* ntr
 
 extern "C" int rch_bvh_load(const char* path, const float* vertices, size_t ntri, rc_flat_node* nodes,
                             uint64_t* ordered) {
-    if (!path || !vertices || !nodes || !ordered) return rc_host_fail(RC_ERR_INVALID, "rch_bvh_load: null argument");
+    if (!path || !nodes || (ntri != 0 && (!vertices || !ordered)))
+        return rc_host_fail(RC_ERR_INVALID, "rch_bvh_load: null argument");
     FILE* f = std::fopen(path, "rb");
     if (!f) return rc_host_fail(RC_ERR_INVALID, std::string("rch_bvh_load: cannot open ") + path);
     FileHeader h;

@@ -287,8 +287,14 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
                                const rc_material* mats, size_t nmat, const rc_flat_node* nodes, size_t nnodes,
                                const uint64_t* ordered, rc_scene** out) {
     RC_TRY(rc_check_ctx(c, "rc_scene_upload"));
-    if (!out || !v || !tri_mat || !mats || !nodes || !ordered || ntri == 0 || nnodes == 0 || nmat == 0)
+    if (!out || !mats || !nodes || nnodes == 0 || nmat == 0 || (ntri != 0 && (!v || !tri_mat || !ordered)))
         return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: null or empty input");
+    // A scene without triangles has the single empty leaf FlatBVH::build makes
+    // over no primitives (FlatBVH.cpp:112-139); it uploads as one wide node of
+    // empty slots, so every ray misses.
+    const bool empty = ntri == 0;
+    if (empty && (nnodes != 1 || nodes[0].right_offset != 0 || nodes[0].primitive_count != 0))
+        return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: a scene without triangles needs the single empty leaf");
     if (ntri > kMaxTriangles)
         return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: more than 2^26 triangles");
     *out = nullptr;
@@ -303,7 +309,7 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
         }
         for (size_t i = 0; i < ntri; ++i)
             if (tri_mat[i] >= nmat) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: material index out of range");
-        for (size_t i = 0; i < nnodes; ++i) {
+        for (size_t i = 0; i < (empty ? 0 : nnodes); ++i) {
             const rc_flat_node& n = nodes[i];
             if (n.right_offset == 0) {
                 if (n.primitive_count == 0 || n.start_offset + n.primitive_count > ntri)
@@ -413,7 +419,16 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
         }
         return base;
     };
-    emit(0);
+    if (empty) {
+        recs.resize(kWidth);
+        for (NodeRec& r : recs) {
+            for (int q = 0; q < 3; ++q) { r.lo[q] = 0.0f; r.hi[q] = 0.0f; }
+            r.link = 0;
+            r.count = kEmptyChild;
+        }
+    } else {
+        emit(0);
+    }
     while (!todo.empty()) {
         Pending p = todo.back();
         todo.pop_back();
@@ -503,9 +518,16 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     s->ntri = ntri;
     s->nmat = nmat;
     s->depth = depth;
+    if (empty)
+        for (int k = 0; k < 3; ++k) bmin[k] = bmax[k] = 0.0f;
     std::memcpy(s->bmin, bmin, sizeof(bmin));
     std::memcpy(s->bmax, bmax, sizeof(bmax));
-    size_t b_tris = ntri * sizeof(TriRec), b_shade = ntri * sizeof(float4), b_mats = nmat * sizeof(DevMaterial);
+    // no zero-byte allocations: an empty scene still holds one (never read)
+    // triangle record
+    const size_t nalloc = std::max<size_t>(ntri, 1);
+    tris.resize(nalloc);
+    shade.resize(nalloc);
+    size_t b_tris = nalloc * sizeof(TriRec), b_shade = nalloc * sizeof(float4), b_mats = nmat * sizeof(DevMaterial);
     s->bytes = b_nodes + b_tris + b_shade + b_mats;
     auto cleanup = [&](int code, const std::string& msg) {
         if (s->nodes) (void)hipFree(s->nodes);

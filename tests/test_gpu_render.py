@@ -370,3 +370,43 @@ def test_phong_raycast_matches_oracle(ctx):
     assert np.allclose(t[same & (hit >= 0)], t_o[same & (hit >= 0)], rtol=1e-5)
     err = np.abs(rgba[same, :3] - rgb[same])
     assert np.quantile(err, 0.999) <= 1e-3 and (err <= 1e-3).mean() > 0.995
+
+
+def test_empty_scene_renders_the_background(ctx):
+    """No triangles (a scene file whose primitives are all non-triangles loads
+    this way): the single empty leaf uploads, every ray misses, and each
+    renderer shows what it shows on a miss -- sky and sun for the path tracer,
+    the background colour for the Whitted tracer -- exactly as the oracle."""
+    tsd, _, _ = scene(ctx, "terrain")
+    v = np.zeros((0, 9), np.float32)
+    sd = ra.SceneData(v, np.zeros(0, np.uint32), tsd.materials[:1].copy(), tsd.desc)
+    nodes, ordered, _, _ = ra.build_bvh(v)
+    s = ctx.upload(sd, nodes, ordered)
+    ob = orc.BVH(v)
+    try:
+        rays = api.make_rays(np.float32([0.0, 1.0, 0.0]), np.float32([[0, -1, 0], [1, 0, 0], [0, 0, 1]]))
+        h = ctx.trace_closest(s, rays)
+        assert h["tri"].tolist() == [-1, -1, -1] and np.isinf(h["t"]).all()
+        assert ctx.trace_any(s, rays).tolist() == [0, 0, 0]
+        cfg = cfg_for(sd, 32, 24, 4)
+        lin, _ = render_gpu(ctx, s, sd.desc.camera, cfg)
+        st = oracle_stats(ob, sd, sd.desc.camera, cfg)
+        replay_check(lin, st, min_equal=1.0)
+        assert ctx.stats().extension_rays == 32 * 24 * 4
+        rs = api.ray_settings(background=(0.1, 0.2, 0.3), ambient_intensity=0.5)
+        rm = api.ray_materials(1, ray_reflectance=0.5)
+        lights = [api.light(api.RC_LIGHT_POINT, position=(0.0, 5.0, 0.0), max_distance=10.0)]
+        film = ctx.film(32, 24)
+        ctx.raytrace(s, film, sd.desc.camera, cfg, rs, rm, lights)
+        np.testing.assert_allclose(film.linear()[:, :3], np.tile(np.float32([0.1, 0.2, 0.3]), (32 * 24, 1)),
+                                   rtol=1e-6)
+        film.close()
+        ids, t, rgba = ctx.raycast_phong(s, sd.desc.camera, 32, 24, 1, 1e-5, sd.desc.light)
+        assert (ids == -1).all() and np.isinf(t).all()
+        light = sd.desc.light
+        lt = [*light.position, *light.color, light.intensity, light.max_distance, light.attenuation]
+        amb = [*light.ambient_color, light.ambient_intensity]
+        hit, _, rgb = orc.raycast_phong(ob, sd.tri_material, sd.materials, sd.desc.camera, 32, 24, 1, 1e-5, lt, amb)
+        assert (hit == -1).all() and np.array_equal(rgba[:, :3], rgb.astype(np.float32))
+    finally:
+        s.close()

@@ -100,9 +100,23 @@ def test_binned_sah_bvh_is_valid_and_cheaper(name):
     assert np.array_equal(again[0], nodes) and np.array_equal(again[1], ordered)
 
 
-def test_bvh_rejects_empty_input():
+@pytest.mark.parametrize("method", [api.RCH_BVH_REFERENCE, api.RCH_BVH_BINNED])
+def test_empty_scene_bvh(tmp_path, method):
+    """No triangles: FlatBVH::build makes one leaf holding nothing, with
+    AABB()'s box (min = DBL_MAX, max = lowest); the oracle builds the same."""
+    v = np.zeros((0, 9), np.float32)
+    nodes, ordered, _, depth = ra.build_bvh(v, method=method)
+    assert len(nodes) == 1 and len(ordered) == 0 and depth == 1
+    n = nodes[0]
+    assert (n["right_offset"], n["start_offset"], n["primitive_count"]) == (0, 0, 0)
+    assert np.all(n["aabb_min"] == np.finfo(np.float64).max) and np.all(n["aabb_max"] == np.finfo(np.float64).min)
+    assert nodes.tobytes() == orc.BVH(v).nodes().tobytes()
+    path = tmp_path / "empty.bvh"
+    ra.save_bvh(path, v, nodes, ordered)
+    n2, o2 = ra.load_bvh(path, v)
+    assert n2.tobytes() == nodes.tobytes() and len(o2) == 0
     with pytest.raises(ra.RaycerError):
-        ra.build_bvh(np.zeros((0, 9), np.float32))
+        ra.load_bvh(path, np.zeros((1, 9), np.float32))
 
 
 def test_workload_triangle_counts():
