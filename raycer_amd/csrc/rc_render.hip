@@ -37,15 +37,19 @@ constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
 #endif
-// Giving each XCD a contiguous slice of the grid halves throughput here: the
-// grid is sized for the queue's capacity while only a prefix is live after
-// the first bounce, so the live rays land on one or two XCDs.  Off by default.
+// 1: each XCD takes a contiguous slice of the live queue entries (the grid is
+// sized for the queue's capacity; only a prefix is live).  3% slower on the
+// 2M-triangle terrain: the eight XCDs then sweep eight separate regions of the
+// image, whose nodes compete for the shared Infinity Cache, while dealt
+// round-robin they all sweep one band.  Off by default.
 #ifndef RC_XCD_REMAP
 #define RC_XCD_REMAP 0
 #endif
 
-__device__ __forceinline__ uint32_t trace_block() {
-    return RC_XCD_REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+__device__ __forceinline__ uint32_t trace_block(uint32_t n) {
+    if (!RC_XCD_REMAP) return blockIdx.x;
+    const uint32_t live = min((n + kTraceBlock - 1) / kTraceBlock, gridDim.x);  // blocks holding queue entries
+    return blockIdx.x < live ? xcd_block(blockIdx.x, live) : blockIdx.x;
 }
 
 struct CamBasis {
@@ -361,7 +365,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
     DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[cin];
-    const uint32_t i = trace_block() * kTraceBlock + threadIdx.x;
+    const uint32_t i = trace_block(n) * kTraceBlock + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const Node
                                                        uint8_t* __restrict__ occ_out) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[2];
-    const uint32_t i = trace_block() * kTraceBlock + threadIdx.x;
+    const uint32_t i = trace_block(n) * kTraceBlock + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     if (i < n) {
