@@ -278,6 +278,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=1024, help="terrain vertices per side (1024 = 2,093,058 triangles)")
     ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--spp-per-pass", type=int, default=64,
+                    help="samples per pixel per wavefront pass (64: the whole frame at once, 132.7 M paths, "
+                         "~27 GB of path queues; 0: the library's automatic choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2", choices=("on", "off"), default="on", help="the 1080p Phong raycast line")
     ap.add_argument("--c4", choices=("on", "off"), default="on",
@@ -308,6 +311,7 @@ def main() -> int:
     film = ctx.film(cfg.width, cfg.height)
     log(f"device {name} ({cus} CUs), setup {time.perf_counter() - t0:.1f} s")
 
+    ctx.set_tuning(spp_per_pass=a.spp_per_pass)
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
     log(f"streaming copy {copy_gbs:.0f} GB/s")
 
@@ -381,6 +385,7 @@ def main() -> int:
             "max_path_length": cfg.max_path_length,
             "estimator": "reference (2 rho |cos|)",
             "bvh": "32-bin SAH, collapsed to 4-wide",
+            "samples_per_pass": min(a.spp_per_pass, cfg.spp) if a.spp_per_pass else "auto",
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -407,6 +412,7 @@ def main() -> int:
         "bvh_build_ms": round(build_ms, 1),
     }
     film.close()
+    ctx.set_tuning()  # the other lines use the library's defaults
     if a.c2 == "on":
         log("1080p primary + shadow Phong raycast (1.18M triangles)")
         out["phong_1080p"] = phong_c2(ctx)

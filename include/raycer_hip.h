@@ -170,7 +170,9 @@ typedef struct rc_point_light {
 } rc_point_light;
 
 /* Launch tuning: threads per block of the ray-query kernels, pass size in
- * samples per pixel, grid limit of the ray-query kernels in blocks per CU. */
+ * samples per pixel (0, the default: each frame takes as many samples per pass
+ * as keep a pass within 2^26 paths, 2^21 for rc_raytrace; else 1..64), grid
+ * limit of the ray-query kernels in blocks per CU.  Defaults 256, 0, 64. */
 typedef struct rc_tuning {
     uint32_t block;
     uint32_t spp_per_pass;

@@ -587,7 +587,9 @@ class Context:
     def set_counting(self, on: bool) -> None:
         check(lib().rc_set_counting(self.h, int(on)))
 
-    def set_tuning(self, block=128, spp_per_pass=1, blocks_per_cu=8) -> None:
+    def set_tuning(self, block=256, spp_per_pass=0, blocks_per_cu=64) -> None:
+        """rc_set_tuning; the defaults are the context's own (spp_per_pass 0:
+        each frame chooses its pass size)."""
         t = Tuning(block, spp_per_pass, blocks_per_cu, 0)
         check(lib().rc_set_tuning(self.h, C.byref(t)))
 

@@ -38,7 +38,7 @@ int render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film* const* films
     RC_TRY(rc_check_ctx(c, fn));
     if (n == 0) return RC_OK;
     if (!scenes || !films || !cams || !cfgs) return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": null array");
-    std::vector<uint32_t> sp(n, std::max(1u, spp_all));
+    std::vector<uint32_t> sp(n);
     for (size_t i = 0; i < n; ++i) {
         RC_TRY(rc_check_scene(c, scenes[i], fn));
         RC_TRY(rc_validate_cfg(&cfgs[i], fn));
@@ -48,6 +48,10 @@ int render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film* const* films
             if (spp_pass[i] == 0 || spp_pass[i] > 64)
                 return rc_host_fail(RC_ERR_INVALID, std::string(fn) + ": spp_per_pass must be in [1, 64]");
             sp[i] = spp_pass[i];
+        } else {
+            rc_tuning t{};
+            t.spp_per_pass = spp_all;
+            sp[i] = pass_samples(t, cfgs[i].width * cfgs[i].height, cfgs[i].spp, kPassPaths);
         }
     }
     RC_TRY_HIP(hipSetDevice(c->device));

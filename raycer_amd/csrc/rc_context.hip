@@ -208,8 +208,8 @@ extern "C" int rc_set_tuning(rc_ctx* c, const rc_tuning* t) {
     if (!t) return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: null tuning");
     if (t->block != 64 && t->block != 128 && t->block != 256)
         return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: block must be 64, 128 or 256");
-    if (t->spp_per_pass == 0 || t->spp_per_pass > 64)
-        return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: spp_per_pass must be in [1, 64]");
+    if (t->spp_per_pass > 64)
+        return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: spp_per_pass must be 0 (automatic) or in [1, 64]");
     if (t->blocks_per_cu == 0 || t->blocks_per_cu > 64)
         return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: blocks_per_cu must be in [1, 64]");
     c->tuning = *t;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,20 @@ struct DevCounters {
 
 enum ProfKind { PK_EXTEND = 0, PK_SHADOW = 1, PK_SHADE = 2, PK_OTHER = 3, PK_CACHE = 4 };
 
+// Paths one pass of the path tracer may hold when the samples per pass are
+// chosen automatically (about 13 GB of queues at 200 bytes per path); the
+// Whitted tracer's per-level ray buffers are far larger per path.
+constexpr size_t kPassPaths = size_t(1) << 26;
+constexpr size_t kWhittedPassPaths = size_t(1) << 21;
+
+// Samples per pass of a frame: the tuning's spp_per_pass, or with 0 (the
+// default) as many as keep a pass within `budget` paths; at least one, at most
+// the frame's.
+inline uint32_t pass_samples(const rc_tuning& t, uint32_t npix, uint32_t spp, size_t budget) {
+    const size_t sp = t.spp_per_pass ? t.spp_per_pass : budget / std::max<size_t>(npix, 1);
+    return uint32_t(std::max<size_t>(1, std::min<size_t>(sp, spp)));
+}
+
 struct rc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -53,7 +68,7 @@ struct rc_ctx {
     std::string name;
     bool profiling = false;
     bool counting = false;
-    rc_tuning tuning{256, 1, 64, 0};
+    rc_tuning tuning{256, 0, 64, 0};  // samples per pass chosen per frame
     rc_stats stats{};
     DevCounters* counters = nullptr;  // device
     DevBuf trace_rays, trace_hits, trace_occ, ovf_list, deep_stack;

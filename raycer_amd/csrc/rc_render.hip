@@ -846,13 +846,11 @@ int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
     c->stats.cache_queries = c->stats.train_samples = 0;
     k_zero_counters<<<1, 1, 0, c->stream>>>(c->counters);
     CacheIO io{};
-    if (cache) {
-        const uint32_t spp_pass = std::max(1u, std::min(c->tuning.spp_per_pass, cfg->spp));
+    const uint32_t spp_pass = pass_samples(c->tuning, cfg->width * cfg->height, cfg->spp, kPassPaths);
+    if (cache)
         RC_TRY(rc_cache_frame_io(c, cache, cfg->cache_vertex, size_t(cfg->width) * cfg->height * spp_pass,
                                  uint64_t(cfg->width) * cfg->height * cfg->spp, &io));
-    }
-    int rc = rc_enqueue_frame(c, s, f, cam, cfg, c->tuning.spp_per_pass, progress, user, cache ? &io : nullptr,
-                              cache);
+    int rc = rc_enqueue_frame(c, s, f, cam, cfg, spp_pass, progress, user, cache ? &io : nullptr, cache);
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     if (rc != RC_OK) return rc;
     DevCounters h;

@@ -845,7 +845,7 @@ extern "C" int rc_raytrace(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_ca
     P.seed = cfg->seed;
 
     // one pass per tuning.spp_per_pass samples of every pixel
-    const uint32_t spp_pass = std::max(1u, std::min(c->tuning.spp_per_pass, cfg->spp));
+    const uint32_t spp_pass = pass_samples(c->tuning, P.npix, cfg->spp, kWhittedPassPaths);
     const size_t n0max = size_t(P.npix) * spp_pass;
     if (n0max > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_raytrace: pass too large");
     const size_t cap = std::max<size_t>(2 * n0max, size_t(1) << 20);

@@ -78,7 +78,7 @@ def test_batch_graph_equals_individual_renders(ctx):
         ctx.render(scenes[i], f2, cams[i], cfgs[i])
         assert np.array_equal(f2.linear(), batch[i]), i
         f2.close()
-    ctx.set_tuning(spp_per_pass=1)
+    ctx.set_tuning()
     # replaying the instantiated graph is not supported: a second batch call
     # re-captures; the films accumulate
     api.check(L.rc_render_batch(ctx.h, sarr, farr, carr, garr, n, C.byref(tun)))
@@ -132,7 +132,7 @@ def test_autotuned_batch_plan_equals_individual_renders(ctx, tmp_path):
             assert np.array_equal(f2.linear(), batch[i]), i
             f2.close()
     finally:
-        ctx.set_tuning(spp_per_pass=1)
+        ctx.set_tuning()
         for f in films:
             f.close()
         for s in scenes:

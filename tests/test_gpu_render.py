@@ -293,13 +293,35 @@ def test_film_sums_are_compensated(ctx):
     sd, s, ob = scene(ctx, "box")
     cfg = cfg_for(sd, 8, 8, 1024)
     ctx.set_tuning(spp_per_pass=1)
-    lin, sums = render_gpu(ctx, s, sd.desc.camera, cfg)
+    try:
+        lin, sums = render_gpu(ctx, s, sd.desc.camera, cfg)
+    finally:
+        ctx.set_tuning()
     st = oracle_stats(ob, sd, sd.desc.camera, cfg)
     same = np.abs(lin[:, :3] - st[:, 2:5] / st[:, 0:1]) <= 1e-4 * np.maximum(1, st[:, 2:5] / st[:, 0:1])
     rows = same.all(axis=1)
     rel = np.abs(sums[rows, :3] - st[rows, 2:5]) / np.maximum(st[rows, 2:5], 1e-6)
     assert rows.mean() > 0.5
     assert np.median(rel) < 2e-6, np.median(rel)
+
+
+def test_pass_size_does_not_change_the_image(ctx):
+    """Samples are keyed by pixel and sample index, so one sample per pass, four,
+    and the automatic choice (the whole 16-spp frame in one pass) trace the same
+    paths; only the grouping of the film's compensated sums differs."""
+    sd, s, _ = scene(ctx, "terrain")
+    cfg = cfg_for(sd, 96, 54, 16)
+    imgs = []
+    try:
+        for sp in (1, 4, 0):
+            ctx.set_tuning(spp_per_pass=sp)
+            lin, _ = render_gpu(ctx, s, sd.desc.camera, cfg)
+            imgs.append(lin)
+            assert ctx.stats().samples == 96 * 54 * 16
+    finally:
+        ctx.set_tuning()
+    for other in imgs[1:]:
+        np.testing.assert_allclose(other, imgs[0], rtol=1e-6, atol=1e-7)
 
 
 def test_interrupt_flag(ctx):
