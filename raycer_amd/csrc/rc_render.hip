@@ -114,6 +114,14 @@ struct PassParams {
     uint32_t has_sun, pad;
 };
 
+// The pixel and sample a pass's queue slot traces.
+__device__ __forceinline__ uint32_t slot_pixel(const PassParams& P, uint32_t slot) {
+    return order_to_pixel(P.order, slot_index(slot, P.npix, P.spp_pass));
+}
+__device__ __forceinline__ uint32_t slot_sample(const PassParams& P, uint32_t slot) {
+    return P.sample0 + slot_sample_in_pass(slot, P.npix, P.spp_pass);
+}
+
 // Filter::getWeightX of each FilterType (src/Rendering/Filters/*.cpp), in
 // FP32; r is the filter's radius.
 __device__ __forceinline__ float filter_1d(const PassParams& P, float x) {
@@ -167,8 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
         ctr->queue_count[2] = 0;
     }
     if (slot >= n) return;
-    uint32_t j = slot / P.npix, pixel = order_to_pixel(P.order, slot - j * P.npix);
-    uint32_t s = P.sample0 + j;
+    const uint32_t pixel = slot_pixel(P, slot), s = slot_sample(P, slot);
     float px = float(pixel % P.width), py = float(pixel / P.width);
     float w = 1.0f;
     if (!(P.sampler != RC_SAMPLER_RANDOM && P.spp == 1)) {
@@ -232,7 +239,7 @@ constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 // Training paths: a fixed-rate, frame-varying subset of the slots that run
 // to the end instead of stopping into the cache.
 __device__ __forceinline__ bool training_path(const PassParams& P, uint32_t slot, uint32_t stride) {
-    uint32_t pixel = order_to_pixel(P.order, slot % P.npix), sample = P.sample0 + slot / P.npix;
+    uint32_t pixel = slot_pixel(P, slot), sample = slot_sample(P, slot);
     uint32_t h = pixel * 2654435761u ^ (sample * 40503u + P.frame * 2246822519u);
     h ^= h >> 15;
     h *= 0x2C1B3C6Du;
@@ -272,7 +279,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
     float3 p = f3(r0) + (t * f3(r1));
     float3 rho = f3(dif);
     if (!more) return O;
-    float4 u = draw4(P.seed, order_to_pixel(P.order, slot % P.npix), P.sample0 + slot / P.npix, P.frame, k);
+    float4 u = draw4(P.seed, slot_pixel(P, slot), slot_sample(P, slot), P.frame, k);
     if (P.has_sun) {
         float cs = dot3(nrm, P.sun);
         if (cs > 0.0f) {
@@ -544,7 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_film_add(PixelOrder order, uint32_t 
     const uint32_t q = pixel_to_order(order, p);
     float4 s = sum[p], c = comp[p];
     for (uint32_t j = 0; j < spp_pass; ++j) {
-        uint32_t slot = j * npix + q;
+        uint32_t slot = RC_PIXEL_MAJOR ? q * spp_pass + j : j * npix + q;
         float w = W[slot];
         float4 l = L[slot];
         kahan(s.x, c.x, w * l.x);
@@ -554,6 +561,64 @@ __global__ __launch_bounds__(kBlock) void k_film_add(PixelOrder order, uint32_t 
     }
     sum[p] = s;
     comp[p] = c;
+}
+
+// The same for pixel-major slots: a pixel's samples are adjacent, so a block
+// of kFilmBlock pixels (consecutive in tile order) owns one contiguous run of
+// slots, staged through LDS kFilmChunk samples at a time so the loads stay
+// coalesced; each pixel still sums its samples in slot order.
+constexpr uint32_t kFilmBlock = 256, kFilmChunk = 8;
+__global__ __launch_bounds__(kFilmBlock) void k_film_add_pm(PixelOrder order, uint32_t npix, uint32_t spp_pass,
+                                                            const float4* __restrict__ L,
+                                                            const float* __restrict__ W, float4* __restrict__ sum,
+                                                            float4* __restrict__ comp) {
+    __shared__ float4 sl[kFilmBlock * kFilmChunk];
+    __shared__ float sw[kFilmBlock * kFilmChunk];
+    const uint32_t q0 = blockIdx.x * kFilmBlock, q = q0 + threadIdx.x;
+    const uint32_t nq = min(kFilmBlock, npix - q0);
+    const bool own = q < npix;
+    uint32_t p = 0;
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c = s;
+    if (own) {
+        p = order_to_pixel(order, q);
+        s = sum[p];
+        c = comp[p];
+    }
+    for (uint32_t j0 = 0; j0 < spp_pass; j0 += kFilmChunk) {
+        const uint32_t ch = min(kFilmChunk, spp_pass - j0);
+        for (uint32_t e = threadIdx.x; e < nq * ch; e += kFilmBlock) {
+            const uint32_t i = e / ch, k = e - i * ch;
+            const size_t slot = size_t(q0 + i) * spp_pass + j0 + k;
+            sl[i * kFilmChunk + k] = L[slot];
+            sw[i * kFilmChunk + k] = W[slot];
+        }
+        __syncthreads();
+        if (own) {
+            for (uint32_t k = 0; k < ch; ++k) {
+                const float w = sw[threadIdx.x * kFilmChunk + k];
+                const float4 l = sl[threadIdx.x * kFilmChunk + k];
+                kahan(s.x, c.x, w * l.x);
+                kahan(s.y, c.y, w * l.y);
+                kahan(s.z, c.z, w * l.z);
+                kahan(s.w, c.w, w);
+            }
+        }
+        __syncthreads();
+    }
+    if (own) {
+        sum[p] = s;
+        comp[p] = c;
+    }
+}
+
+// Film accumulation of one pass in the slot layout the kernels use.
+void launch_film_add(hipStream_t st, PixelOrder order, uint32_t npix, uint32_t spp_pass, const float4* L,
+                     const float* W, float4* sum, float4* comp) {
+    if (RC_PIXEL_MAJOR)
+        k_film_add_pm<<<(npix + kFilmBlock - 1) / kFilmBlock, kFilmBlock, 0, st>>>(order, npix, spp_pass, L, W, sum,
+                                                                                    comp);
+    else
+        k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, st>>>(order, npix, spp_pass, L, W, sum, comp);
 }
 
 int alloc_pass(rc_ctx* c, size_t npaths) {
@@ -690,8 +755,7 @@ int rc_launch_camera(rc_ctx* c, const rc_camera* cam, const rc_render_cfg* cfg, 
 int rc_launch_film_add(rc_ctx* c, rc_film* f, const rc_render_cfg* cfg, uint32_t spp_pass, const float4* L,
                        const float* W) {
     const uint32_t npix = cfg->width * cfg->height;
-    k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(make_order(cfg->width, cfg->height), npix,
-                                                                      spp_pass, L, W, f->sum, f->comp);
+    launch_film_add(c->stream, make_order(cfg->width, cfg->height), npix, spp_pass, L, W, f->sum, f->comp);
     RC_TRY_HIP(hipGetLastError());
     return RC_OK;
 }
@@ -793,9 +857,8 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
                                                                   CI.batch_cap);
         }
         RC_TRY(c->prof_begin(&ev));
-        k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(P.order, npix, this_pass,
-                                                                          c->slotL.as<float4>(), c->slotW.as<float>(),
-                                                                          f->sum, f->comp);
+        launch_film_add(c->stream, P.order, npix, this_pass, c->slotL.as<float4>(), c->slotW.as<float>(), f->sum,
+                        f->comp);
         RC_TRY(c->prof_end(ev, PK_OTHER));
         RC_TRY_HIP(hipGetLastError());
     }

@@ -78,7 +78,7 @@ struct WParams {
     float fog_dist, fog_steep, fog_h, fog_hsteep;
     uint32_t nlights;
     PixelOrder order;
-    uint32_t npix, sample0, frame;
+    uint32_t npix, sample0, spp_pass, frame;
     uint64_t seed;
 };
 
@@ -99,7 +99,8 @@ __device__ __forceinline__ uint32_t child_key(uint32_t key, uint32_t kind, uint3
 // Draw s of purpose p of a node: Philox counter (pixel, sample, frame, dim)
 // with dim = hash(key ^ hash(p << 16 | s)).
 __device__ __forceinline__ uint4 node_bits(const WParams& P, uint32_t slot, uint32_t key, uint32_t p, uint32_t s) {
-    const uint32_t pixel = order_to_pixel(P.order, slot % P.npix), sample = P.sample0 + slot / P.npix;
+    const uint32_t pixel = order_to_pixel(P.order, slot_index(slot, P.npix, P.spp_pass));
+    const uint32_t sample = P.sample0 + slot_sample_in_pass(slot, P.npix, P.spp_pass);
     const uint32_t dim = wt_hash(key ^ wt_hash((p << 16) | s));
     return philox(make_uint4(pixel, sample, P.frame, dim), make_uint2(uint32_t(P.seed), uint32_t(P.seed >> 32)));
 }
@@ -870,6 +871,7 @@ extern "C" int rc_raytrace(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_ca
         const uint32_t this_pass = std::min(spp_pass, cfg->spp - s0);
         const uint32_t n0 = P.npix * this_pass;
         P.sample0 = s0;
+        P.spp_pass = this_pass;
         WhittedBufs::Level& L0 = W->levels[0];
         RC_TRY(rc_launch_camera(c, cam, cfg, s0, this_pass, L0.rays.as<rc_ray>(), L0.A.as<float4>(), run.L,
                                 c->slotW.as<float>()));
