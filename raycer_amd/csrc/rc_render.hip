@@ -30,7 +30,7 @@ constexpr float kPi = 3.14159265358979323846f;
 #define RC_TRACE_BLOCK 128
 #endif
 #ifndef RC_LDS_DEPTH
-#define RC_LDS_DEPTH 24
+#define RC_LDS_DEPTH 20  // 1% faster than 24 on the terrain at 64 samples per pass; 12 overflows too often
 #endif
 constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the traversal kernels
 constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread

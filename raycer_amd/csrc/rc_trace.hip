@@ -22,7 +22,7 @@ namespace {
 #define RC_TRACE_BLOCK 256
 #endif
 #ifndef RC_LDS_DEPTH
-#define RC_LDS_DEPTH 24
+#define RC_LDS_DEPTH 20  // 1% faster than 24 on the terrain at 64 samples per pass; 12 overflows too often
 #endif
 constexpr int kBlock = RC_TRACE_BLOCK;
 constexpr int kLdsDepth = RC_LDS_DEPTH;
