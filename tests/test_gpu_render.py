@@ -358,6 +358,28 @@ def test_render_rejects_bad_config(ctx):
     film.close()
 
 
+def test_tuning_validation(ctx):
+    """spp_per_pass: 0 (automatic) and 1..64 are accepted, 65 is not; block and
+    blocks_per_cu keep their ranges; a rejected call leaves the tuning alone."""
+    try:
+        for sp in (0, 1, 64):
+            ctx.set_tuning(spp_per_pass=sp)
+        for kw in (dict(spp_per_pass=65), dict(block=96), dict(blocks_per_cu=0), dict(blocks_per_cu=65)):
+            with pytest.raises(ra.RaycerError):
+                ctx.set_tuning(**kw)
+        # the last accepted value (64) still holds: a 16-spp frame is one pass
+        sd, s, _ = scene(ctx, "box")
+        cfg = cfg_for(sd, 16, 16, 16)
+        ctx.set_profiling(True)
+        ctx.stats()
+        render_gpu(ctx, s, sd.desc.camera, cfg)
+        st = ctx.stats()
+        ctx.set_profiling(False)
+        assert st.extend_launches <= cfg.max_path_length
+    finally:
+        ctx.set_tuning()
+
+
 def test_reinhard_matches_oracle(ctx):
     sd, s, _ = scene(ctx, "box")
     cfg = cfg_for(sd, 64, 64, 4)
