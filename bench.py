@@ -223,6 +223,45 @@ def phong_c2(ctx, reps: int = 10) -> dict:
             "mpixels_per_s_device": round(w * h / (dev_ms / reps * 1e-3) / 1e6, 1)}
 
 
+def whitted_line(ctx, reps: int = 5) -> dict:
+    """The Whitted ray tracer (rc_raytrace) on the seeded sphere + ground scene
+    (655,360 + 524,288 triangles), 1920x1080, one sample per pixel: every
+    material reflects 0.5 with up to three reflection levels, one point light
+    with its shadow rays, ambient 0.1.  Wall time of the synchronous call (ray
+    tree levels, scans, shading, the film add; no copies to the host)."""
+    import raycer_amd as ra
+    from raycer_amd import api
+
+    sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_SPHERE_GROUND, seed=1))
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    s = ctx.upload(sd, nodes, ordered)
+    cfg = api.RenderCfg.from_buffer_copy(sd.desc.cfg)
+    cfg.spp = 1
+    L = sd.desc.light
+    lights = [api.light(api.RC_LIGHT_POINT, position=tuple(L.position), intensity=L.intensity,
+                        max_distance=L.max_distance, attenuation=L.attenuation)]
+    rs = api.ray_settings(max_ray_iterations=3, ambient_intensity=0.1)
+    rm = api.ray_materials(len(sd.materials), ray_reflectance=0.5)
+    film = ctx.film(cfg.width, cfg.height)
+    try:
+        ctx.raytrace(s, film, sd.desc.camera, cfg, rs, rm, lights)  # warm-up: sizes the level buffers
+        ext = shadow = 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.raytrace(s, film, sd.desc.camera, cfg, rs, rm, lights)
+            st = ctx.stats()
+            ext += st.extension_rays
+            shadow += st.shadow_rays
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        film.close()
+        s.close()
+    return {"workload": f"sphere-ground-{sd.ntri}tri-{cfg.width}x{cfg.height}-1spp-mirror0.5-depth3-pointlight",
+            "ms_per_frame": round(dt * 1e3, 3), "closest_hit_rays_per_frame": ext // reps,
+            "shadow_rays_per_frame": shadow // reps,
+            "grays_per_s": round((ext + shadow) / reps / dt / 1e9, 3)}
+
+
 def batch_c4(ctx, n_scenes: int = 64) -> dict:
     """Many-scene workload: soup scenes with seeds 0..n-1 (1k-500k triangles,
     log-uniform; 128^2 to 1024^2; 32 jittered spp; depth 5), autotuned samples
@@ -283,6 +322,7 @@ def main() -> int:
                          "~27 GB of path queues; 0: the library's automatic choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2", choices=("on", "off"), default="on", help="the 1080p Phong raycast line")
+    ap.add_argument("--whitted", choices=("on", "off"), default="on", help="the 1080p Whitted ray tracer line")
     ap.add_argument("--c4", choices=("on", "off"), default="on",
                     help="also time the 64-scene batch (autotuned, one captured graph)")
     ap.add_argument("--mse", choices=("full", "small", "off"), default="full",
@@ -417,6 +457,10 @@ def main() -> int:
         log("1080p primary + shadow Phong raycast (1.18M triangles)")
         out["phong_1080p"] = phong_c2(ctx)
         log(f"phong: {json.dumps(out['phong_1080p'])}")
+    if a.whitted == "on":
+        log("1080p Whitted ray tracer (1.18M triangles, mirrors, point light)")
+        out["whitted_1080p"] = whitted_line(ctx)
+        log(f"whitted: {json.dumps(out['whitted_1080p'])}")
     if a.c4 == "on":
         log("64-scene batch (autotuned, one captured graph)")
         out["batch_64_scenes"] = batch_c4(ctx)
