@@ -119,7 +119,7 @@ inline PixelOrder make_order(uint32_t w, uint32_t h) {
     return o;
 }
 
-__device__ __forceinline__ uint32_t order_to_pixel(const PixelOrder& o, uint32_t q) {
+__host__ __device__ __forceinline__ uint32_t order_to_pixel(const PixelOrder& o, uint32_t q) {
     if (q < o.a) {
         const uint32_t t = q >> 6, w = q & 63u, ty = t / o.tiles_x, tx = t - ty * o.tiles_x;
         return (ty * 8 + (w >> 3)) * o.width + tx * 8 + (w & 7u);
@@ -134,7 +134,7 @@ __device__ __forceinline__ uint32_t order_to_pixel(const PixelOrder& o, uint32_t
     return o.y0 * o.width + r;
 }
 
-__device__ __forceinline__ uint32_t pixel_to_order(const PixelOrder& o, uint32_t p) {
+__host__ __device__ __forceinline__ uint32_t pixel_to_order(const PixelOrder& o, uint32_t p) {
     const uint32_t y = p / o.width, x = p - y * o.width;
     if (x < o.x0 && y < o.y0) return ((((y >> 3) * o.tiles_x + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u));
     if (y < o.y0) return o.a + y * (o.width - o.x0) + (x - o.x0);
@@ -150,10 +150,10 @@ __device__ __forceinline__ uint32_t pixel_to_order(const PixelOrder& o, uint32_t
 #ifndef RC_PIXEL_MAJOR
 #define RC_PIXEL_MAJOR 1
 #endif
-__device__ __forceinline__ uint32_t slot_index(uint32_t slot, uint32_t npix, uint32_t spp_pass) {
+__host__ __device__ __forceinline__ uint32_t slot_index(uint32_t slot, uint32_t npix, uint32_t spp_pass) {
     return RC_PIXEL_MAJOR ? slot / spp_pass : slot % npix;
 }
-__device__ __forceinline__ uint32_t slot_sample_in_pass(uint32_t slot, uint32_t npix, uint32_t spp_pass) {
+__host__ __device__ __forceinline__ uint32_t slot_sample_in_pass(uint32_t slot, uint32_t npix, uint32_t spp_pass) {
     return RC_PIXEL_MAJOR ? slot % spp_pass : slot / npix;
 }
 
