@@ -1,8 +1,8 @@
 """ctypes wrapper of the FP64 oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker, never as the thing
-measured or shipped.  See oracle.h for the reference functions each entry
+cpu_baseline leg of bench.py, always as the reference results are compared
+with, never as the thing measured or shipped.  See oracle.h for the reference functions each entry
 point restates.
 """
 from __future__ import annotations

@@ -1,7 +1,7 @@
 /*
  * FP64 CPU oracle for the raycer triangle path-tracing hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only the test suite,
+ * TEST INFRASTRUCTURE ONLY.  Results are compared against this library: only the test suite,
  * the smoke check and the cpu_baseline leg of bench.py load it.  The product
  * library never links, calls or falls back to it.
  *

@@ -8,7 +8,7 @@ ROOT = Path(__file__).resolve().parent.parent
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
-# Worker threads for the oracle: the GPU box grants 16 cores to a job.
+# Worker threads for the oracle: a one-GPU job is granted 16 host cores.
 ORACLE_THREADS = int(os.environ.get("RAYCER_ORACLE_THREADS", "16"))
 
 
