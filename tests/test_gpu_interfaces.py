@@ -79,6 +79,21 @@ def test_batch_graph_equals_individual_renders(ctx):
         assert np.array_equal(f2.linear(), batch[i]), i
         f2.close()
     ctx.set_tuning()
+    # the context's default tuning: each scene's pass size chosen for it (all
+    # 8 samples in one pass here), the same as rendering it alone
+    for f in films:
+        f.clear()
+    api.check(L.rc_render_batch(ctx.h, sarr, farr, carr, garr, n, None))
+    batch_auto = [f.linear() for f in films]
+    for i in range(n):
+        f2 = ctx.film(cfgs[i].width, cfgs[i].height)
+        ctx.render(scenes[i], f2, cams[i], cfgs[i])
+        assert np.array_equal(f2.linear(), batch_auto[i]), i
+        np.testing.assert_allclose(batch_auto[i], batch[i], rtol=1e-6, atol=1e-7)  # same paths, other grouping
+        f2.close()
+    for f in films:
+        f.clear()
+    api.check(L.rc_render_batch(ctx.h, sarr, farr, carr, garr, n, C.byref(tun)))
     # replaying the instantiated graph is not supported: a second batch call
     # re-captures; the films accumulate
     api.check(L.rc_render_batch(ctx.h, sarr, farr, carr, garr, n, C.byref(tun)))
