@@ -32,7 +32,11 @@ constexpr float kPi = 3.14159265358979323846f;
 #ifndef RC_LDS_DEPTH
 #define RC_LDS_DEPTH 20  // 1% faster than 24 on the terrain at 64 samples per pass; 12 overflows too often
 #endif
-constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the traversal kernels
+constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the shadow kernel
+#ifndef RC_EXT_BLOCK
+#define RC_EXT_BLOCK 512  // 8 waves: the continuation bins span eight pixels' rays (1024 halves occupancy)
+#endif
+constexpr int kExtBlock = RC_EXT_BLOCK;      // threads per block of the extension kernel (its bins span a block)
 constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
@@ -46,9 +50,10 @@ constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
 #define RC_XCD_REMAP 0
 #endif
 
+template <int BLOCK>
 __device__ __forceinline__ uint32_t trace_block(uint32_t n) {
     if (!RC_XCD_REMAP) return blockIdx.x;
-    const uint32_t live = min((n + kTraceBlock - 1) / kTraceBlock, gridDim.x);  // blocks holding queue entries
+    const uint32_t live = min((n + BLOCK - 1) / BLOCK, gridDim.x);  // blocks holding queue entries
     return blockIdx.x < live ? xcd_block(blockIdx.x, live) : blockIdx.x;
 }
 
@@ -216,6 +221,58 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
     return base + prefix;
 }
 
+// Continuation rays are queued per traversal block in RC_BINS bins of their
+// angle to the normal (1: no bins), steep ones first: a steep ray tends to
+// leave the scene after a few nodes, a grazing one to run along the surface,
+// so binned waves hold rays of like length instead of waiting for their
+// longest.  Bin edges split the cosine-weighted rays evenly in cos^2 =
+// 1 - u (u the hemisphere draw), at RC_BIN_SPLIT for two bins.
+#ifndef RC_BINS
+#define RC_BINS 2
+#endif
+#ifndef RC_BIN_SPLIT
+#define RC_BIN_SPLIT 0.5f
+#endif
+__device__ __forceinline__ uint32_t continuation_bin(float u) {
+    if (RC_BINS <= 1) return 0;
+    if (RC_BINS == 2) return u < RC_BIN_SPLIT ? 0u : 1u;
+    return min(uint32_t(RC_BINS - 1), uint32_t(u * float(RC_BINS)));
+}
+
+// Reserves queue positions for a block's entries, bin 0 first, then bin 1,
+// ...; inside a bin in wave then lane order.  Per-wave ballots, the counts
+// through LDS (scratch: NB * waves + 1 words), one atomic per block.  Every
+// thread of the block calls it.
+template <int BLOCK, int NB>
+__device__ __forceinline__ uint32_t block_append_bins(bool pred, uint32_t bin, uint32_t* counter, uint32_t* scratch) {
+    constexpr uint32_t kW = BLOCK / 64;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < uint32_t(NB); ++b) {
+        const uint64_t m = __ballot(pred && bin == b);
+        if (lane == 0) scratch[b * kW + wave] = uint32_t(__popcll(m));
+        mine = bin == b ? m : mine;
+    }
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < uint32_t(NB); ++b) {
+#pragma unroll
+        for (uint32_t w = 0; w < kW; ++w) {
+            const uint32_t c = scratch[b * kW + w];
+            before += (b < bin || (b == bin && w < wave)) ? c : 0u;
+            total += c;
+        }
+    }
+    if (threadIdx.x == 0) scratch[NB * kW] = total ? atomicAdd(counter, total) : 0u;
+    __syncthreads();
+    const uint32_t base = scratch[NB * kW];
+    __syncthreads();  // the scratch words may be reused by the next call
+    return base + before + uint32_t(__popcll(mine & below));
+}
+
 // mapToHemisphere with distribution 1 (Sampler.cpp:50-64) in the ONB of a
 // triangle without texcoords (Triangle.cpp:40-44: T = n x ALMOST_UP, B = T x n).
 __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
@@ -231,6 +288,7 @@ __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
 
 struct ShadeOut {
     bool alive, shadow;
+    uint32_t bin = 0;  // continuation_bin of the continuation ray
     float3 thr, newo, newd, contrib, so;
     uint32_t rec;  // radiance-cache training record written at this vertex, or kNoRecord
 };
@@ -334,6 +392,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
     O.thr = thr;
     O.newo = p + w * P.offset;
     O.newd = w;
+    O.bin = continuation_bin(u.y);  // hemisphere: cos^2 = 1 - u.y
     return O;
 }
 
@@ -342,8 +401,12 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
 __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut& O, uint32_t slot,
                                             rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
-                                            DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI) {
-    uint32_t pos = wave_append(O.alive, &ctr->queue_count[cout]);
+                                            DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI,
+                                            uint32_t* bin_scratch = nullptr) {
+    // a traversal block queues its continuations by bin (RC_BINS)
+    uint32_t pos = (RC_BINS > 1 && bin_scratch)
+                       ? block_append_bins<kExtBlock, RC_BINS>(O.alive, O.bin, &ctr->queue_count[cout], bin_scratch)
+                       : wave_append(O.alive, &ctr->queue_count[cout]);
     if (O.alive) {
         float4* ro = reinterpret_cast<float4*>(rays_out + pos);
         ro[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f);
@@ -364,15 +427,15 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
 // thread, the shading of its hit.  A ray whose traversal stack outgrows LDS is
 // listed for the deep-stack kernel and shaded afterwards by k_shade_listed.
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
+__global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
     PassParams P, uint32_t k, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
     const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
     const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
     DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI) {
-    __shared__ uint32_t stack[kTraceLds * kTraceBlock];
+    __shared__ uint32_t stack[kTraceLds * kExtBlock];
     const uint32_t n = ctr->queue_count[cin];
-    const uint32_t i = trace_block(n) * kTraceBlock + threadIdx.x;
+    const uint32_t i = trace_block<kExtBlock>(n) * kExtBlock + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
@@ -385,11 +448,12 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_extend_shade(
         float4 s = st[i];
         slot = __float_as_uint(s.w);
         TraceResult R = traverse<false, COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
-                                               kTraceBlock, kTraceLds, n_rec, n_tri);
+                                               kExtBlock, kTraceLds, n_rec, n_tri);
         if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
     }
-    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI);
+    __shared__ uint32_t bin_scratch[RC_BINS * (kExtBlock / 64) + 1];
+    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, bin_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
@@ -435,7 +499,7 @@ __global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const Node
                                                        uint8_t* __restrict__ occ_out) {
     __shared__ uint32_t stack[kTraceLds * kTraceBlock];
     const uint32_t n = ctr->queue_count[2];
-    const uint32_t i = trace_block(n) * kTraceBlock + threadIdx.x;
+    const uint32_t i = trace_block<kTraceBlock>(n) * kTraceBlock + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     if (i < n) {
@@ -798,18 +862,18 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
         rc_ray* rout = rb;
         float4* sin_ = sa;
         float4* sout = sb;
-        const uint32_t grid_t = (n + kTraceBlock - 1) / kTraceBlock;
+        const uint32_t grid_t = (n + kTraceBlock - 1) / kTraceBlock, grid_e = (n + kExtBlock - 1) / kExtBlock;
         uint32_t* ovf = c->ovf_list.as<uint32_t>();
         for (uint32_t k = 0; k < cfg->max_path_length; ++k) {
             uint32_t cin = k & 1u, cout = cin ^ 1u;
             k_prep<<<1, 1, 0, c->stream>>>(c->counters, cout);
             RC_TRY(c->prof_begin(&ev));
             if (c->counting) {
-                k_extend_shade<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
+                k_extend_shade<true><<<grid_e, kExtBlock, 0, c->stream>>>(
                     P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
                     c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
             } else {
-                k_extend_shade<false><<<grid_t, kTraceBlock, 0, c->stream>>>(
+                k_extend_shade<false><<<grid_e, kExtBlock, 0, c->stream>>>(
                     P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
                     c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
             }
