@@ -233,6 +233,14 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
 #ifndef RC_BIN_SPLIT
 #define RC_BIN_SPLIT 0.5f
 #endif
+// Sun shadow rays likewise in two bins of the sun's height over the surface
+// (cos below RC_SHADOW_SPLIT: grazing, more often occluded nearby).
+#ifndef RC_SHADOW_BINS
+#define RC_SHADOW_BINS 2
+#endif
+#ifndef RC_SHADOW_SPLIT
+#define RC_SHADOW_SPLIT 0.5f
+#endif
 __device__ __forceinline__ uint32_t continuation_bin(float u) {
     if (RC_BINS <= 1) return 0;
     if (RC_BINS == 2) return u < RC_BIN_SPLIT ? 0u : 1u;
@@ -288,7 +296,8 @@ __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
 
 struct ShadeOut {
     bool alive, shadow;
-    uint32_t bin = 0;  // continuation_bin of the continuation ray
+    uint32_t bin = 0;   // continuation_bin of the continuation ray
+    uint32_t sbin = 0;  // 1: the sun ray grazes the surface (RC_SHADOW_BINS)
     float3 thr, newo, newd, contrib, so;
     uint32_t rec;  // radiance-cache training record written at this vertex, or kNoRecord
 };
@@ -345,6 +354,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
             O.contrib = mul3(thr, mul3(rho, P.sun_e)) * f;
             O.so = p + P.sun * P.offset;
             O.shadow = true;
+            O.sbin = cs < RC_SHADOW_SPLIT ? 1u : 0u;
         }
     }
     if (CI.cache_vertex > 0 && k == CI.cache_vertex) {
@@ -413,7 +423,9 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
         ro[1] = make_float4(O.newd.x, O.newd.y, O.newd.z, __builtin_inff());
         st_out[pos] = make_float4(O.thr.x, O.thr.y, O.thr.z, __uint_as_float(slot));
     }
-    uint32_t spos = wave_append(O.shadow, &ctr->queue_count[2]);
+    uint32_t spos = (RC_SHADOW_BINS > 1 && bin_scratch)
+                        ? block_append_bins<kExtBlock, 2>(O.shadow, O.sbin, &ctr->queue_count[2], bin_scratch)
+                        : wave_append(O.shadow, &ctr->queue_count[2]);
     if (O.shadow) {
         float4* ro = reinterpret_cast<float4*>(srays + spos);
         ro[0] = make_float4(O.so.x, O.so.y, O.so.z, 0.0f);
@@ -452,7 +464,7 @@ __global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
         if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
     }
-    __shared__ uint32_t bin_scratch[RC_BINS * (kExtBlock / 64) + 1];
+    __shared__ uint32_t bin_scratch[(RC_BINS > 2 ? RC_BINS : 2) * (kExtBlock / 64) + 1];
     emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, bin_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
