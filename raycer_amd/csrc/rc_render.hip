@@ -363,7 +363,8 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
         const float3 wo = make_float3(-r1.x, -r1.y, -r1.z);
         if (!training_path(P, slot, CI.train_stride)) {
             // the path ends here: its indirect radiance comes from the cache
-            uint32_t q = atomicAdd(&CI.counters[0], 1u);
+            // (one reservation per wave for the lanes that end here)
+            uint32_t q = wave_append(true, &CI.counters[0]);
             if (q < CI.cap) {
                 float4* Q = CI.queries + 4 * size_t(q);
                 Q[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
@@ -378,7 +379,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
         // the record's "before" value gets that ray's contribution added once
         // its occlusion is known (k_cache_nee_fix), and its queue position
         // once it is appended (emit_vertex)
-        uint32_t r = atomicAdd(&CI.counters[1], 1u);
+        uint32_t r = wave_append(true, &CI.counters[1]);
         if (r < CI.cap) {
             float4 lb = L[slot];
             float4* R = CI.records + 5 * size_t(r);
