@@ -440,6 +440,12 @@ def main() -> int:
             "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
             "achievable_copy_GBs": round(copy_gbs, 1),
             "frac_of_achievable_copy": round(achieved / copy_gbs, 4),
+            "hbm_measured_GBs": (round(traffic / (ext_ms / max(ext_launches, 1) * 1e-3) / 1e9, 1)
+                                 if traffic else None),
+            "note": "achieved counts the algorithmic node, triangle and queue bytes; L2 and the 256 MB "
+                    "Infinity Cache serve most node and triangle fetches, so it can exceed the HBM peak; "
+                    "hbm_measured_GBs is the rocprofv3 FETCH_SIZE + WRITE_SIZE traffic per launch over "
+                    "the same launch time",
         },
         "gsample_bounces_per_s": round(total_ext / elapsed / 1e9, 4),
         "shadow_rays_per_s_G": round(total_shadow / elapsed / 1e9, 4),

@@ -34,9 +34,9 @@ constexpr float kPi = 3.14159265358979323846f;
 #endif
 constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the shadow kernel
 #ifndef RC_EXT_BLOCK
-#define RC_EXT_BLOCK 512  // 8 waves: the continuation bins span eight pixels' rays (1024 halves occupancy)
+#define RC_EXT_BLOCK 256  // 4 waves per queue reservation: 3% faster than 512 and 128
 #endif
-constexpr int kExtBlock = RC_EXT_BLOCK;      // threads per block of the extension kernel (its bins span a block)
+constexpr int kExtBlock = RC_EXT_BLOCK;      // threads per block of the extension kernel
 constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
@@ -221,64 +221,31 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
     return base + prefix;
 }
 
-// Continuation rays are queued per traversal block in RC_BINS bins of their
-// angle to the normal (1: no bins), steep ones first: a steep ray tends to
-// leave the scene after a few nodes, a grazing one to run along the surface,
-// so binned waves hold rays of like length instead of waiting for their
-// longest.  Bin edges split the cosine-weighted rays evenly in cos^2 =
-// 1 - u (u the hemisphere draw), at RC_BIN_SPLIT for two bins.
-#ifndef RC_BINS
-#define RC_BINS 2
-#endif
-#ifndef RC_BIN_SPLIT
-#define RC_BIN_SPLIT 0.5f
-#endif
-// Sun shadow rays likewise in two bins of the sun's height over the surface
-// (cos below RC_SHADOW_SPLIT: grazing, more often occluded nearby).
-#ifndef RC_SHADOW_BINS
-#define RC_SHADOW_BINS 2
-#endif
-#ifndef RC_SHADOW_SPLIT
-#define RC_SHADOW_SPLIT 0.5f
-#endif
-__device__ __forceinline__ uint32_t continuation_bin(float u) {
-    if (RC_BINS <= 1) return 0;
-    if (RC_BINS == 2) return u < RC_BIN_SPLIT ? 0u : 1u;
-    return min(uint32_t(RC_BINS - 1), uint32_t(u * float(RC_BINS)));
-}
-
-// Reserves queue positions for a block's entries, bin 0 first, then bin 1,
-// ...; inside a bin in wave then lane order.  Per-wave ballots, the counts
-// through LDS (scratch: NB * waves + 1 words), one atomic per block.  Every
-// thread of the block calls it.
-template <int BLOCK, int NB>
-__device__ __forceinline__ uint32_t block_append_bins(bool pred, uint32_t bin, uint32_t* counter, uint32_t* scratch) {
+// Reserves `pred ? 1 : 0` entries of a queue for a whole block with one
+// atomic: per-wave ballots, the wave counts through LDS (scratch: BLOCK / 64
+// + 1 words), positions in wave then lane order.  Every queue reservation is
+// an atomic on one address; at one per 64-lane wave they serialized the
+// extension kernel (the camera bounce took 37 ms instead of 16.5 at 132.7 M
+// rays).  Every thread of the block calls it.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_append(bool pred, uint32_t* counter, uint32_t* scratch) {
     constexpr uint32_t kW = BLOCK / 64;
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    const uint64_t below = (1ull << lane) - 1ull;
-    uint64_t mine = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < uint32_t(NB); ++b) {
-        const uint64_t m = __ballot(pred && bin == b);
-        if (lane == 0) scratch[b * kW + wave] = uint32_t(__popcll(m));
-        mine = bin == b ? m : mine;
-    }
+    const uint64_t mask = __ballot(pred);
+    if (lane == 0) scratch[wave] = uint32_t(__popcll(mask));
     __syncthreads();
     uint32_t before = 0, total = 0;
 #pragma unroll
-    for (uint32_t b = 0; b < uint32_t(NB); ++b) {
-#pragma unroll
-        for (uint32_t w = 0; w < kW; ++w) {
-            const uint32_t c = scratch[b * kW + w];
-            before += (b < bin || (b == bin && w < wave)) ? c : 0u;
-            total += c;
-        }
+    for (uint32_t w = 0; w < kW; ++w) {
+        const uint32_t c = scratch[w];
+        before += w < wave ? c : 0u;
+        total += c;
     }
-    if (threadIdx.x == 0) scratch[NB * kW] = total ? atomicAdd(counter, total) : 0u;
+    if (threadIdx.x == 0) scratch[kW] = total ? atomicAdd(counter, total) : 0u;
     __syncthreads();
-    const uint32_t base = scratch[NB * kW];
+    const uint32_t base = scratch[kW];
     __syncthreads();  // the scratch words may be reused by the next call
-    return base + before + uint32_t(__popcll(mine & below));
+    return base + before + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
 // mapToHemisphere with distribution 1 (Sampler.cpp:50-64) in the ONB of a
@@ -296,8 +263,6 @@ __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
 
 struct ShadeOut {
     bool alive, shadow;
-    uint32_t bin = 0;   // continuation_bin of the continuation ray
-    uint32_t sbin = 0;  // 1: the sun ray grazes the surface (RC_SHADOW_BINS)
     float3 thr, newo, newd, contrib, so;
     uint32_t rec;  // radiance-cache training record written at this vertex, or kNoRecord
 };
@@ -354,7 +319,6 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
             O.contrib = mul3(thr, mul3(rho, P.sun_e)) * f;
             O.so = p + P.sun * P.offset;
             O.shadow = true;
-            O.sbin = cs < RC_SHADOW_SPLIT ? 1u : 0u;
         }
     }
     if (CI.cache_vertex > 0 && k == CI.cache_vertex) {
@@ -403,7 +367,6 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
     O.thr = thr;
     O.newo = p + w * P.offset;
     O.newd = w;
-    O.bin = continuation_bin(u.y);  // hemisphere: cos^2 = 1 - u.y
     return O;
 }
 
@@ -413,20 +376,18 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
                                             rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
                                             DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI,
-                                            uint32_t* bin_scratch = nullptr) {
-    // a traversal block queues its continuations by bin (RC_BINS)
-    uint32_t pos = (RC_BINS > 1 && bin_scratch)
-                       ? block_append_bins<kExtBlock, RC_BINS>(O.alive, O.bin, &ctr->queue_count[cout], bin_scratch)
-                       : wave_append(O.alive, &ctr->queue_count[cout]);
+                                            uint32_t* append_scratch = nullptr) {
+    // the extension kernel reserves once per block, k_shade_listed once per wave
+    uint32_t pos = append_scratch ? block_append<kExtBlock>(O.alive, &ctr->queue_count[cout], append_scratch)
+                                  : wave_append(O.alive, &ctr->queue_count[cout]);
     if (O.alive) {
         float4* ro = reinterpret_cast<float4*>(rays_out + pos);
         ro[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f);
         ro[1] = make_float4(O.newd.x, O.newd.y, O.newd.z, __builtin_inff());
         st_out[pos] = make_float4(O.thr.x, O.thr.y, O.thr.z, __uint_as_float(slot));
     }
-    uint32_t spos = (RC_SHADOW_BINS > 1 && bin_scratch)
-                        ? block_append_bins<kExtBlock, 2>(O.shadow, O.sbin, &ctr->queue_count[2], bin_scratch)
-                        : wave_append(O.shadow, &ctr->queue_count[2]);
+    uint32_t spos = append_scratch ? block_append<kExtBlock>(O.shadow, &ctr->queue_count[2], append_scratch)
+                                   : wave_append(O.shadow, &ctr->queue_count[2]);
     if (O.shadow) {
         float4* ro = reinterpret_cast<float4*>(srays + spos);
         ro[0] = make_float4(O.so.x, O.so.y, O.so.z, 0.0f);
@@ -465,8 +426,8 @@ __global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
         if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
     }
-    __shared__ uint32_t bin_scratch[(RC_BINS > 2 ? RC_BINS : 2) * (kExtBlock / 64) + 1];
-    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, bin_scratch);
+    __shared__ uint32_t append_scratch[kExtBlock / 64 + 1];
+    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
