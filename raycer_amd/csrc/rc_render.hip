@@ -168,18 +168,9 @@ __device__ __forceinline__ float filter_1d(const PassParams& P, float x) {
     }
 }
 
-// Tracer::generateMultiSamples (Tracer.cpp:115-144) for one slot.
-__global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restrict__ rays, float4* __restrict__ st,
-                                                   float4* __restrict__ L, float* __restrict__ W,
-                                                   DevCounters* __restrict__ ctr) {
-    uint32_t n = P.npix * P.spp_pass;
-    uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot == 0) {
-        ctr->queue_count[0] = n;
-        ctr->queue_count[1] = 0;
-        ctr->queue_count[2] = 0;
-    }
-    if (slot >= n) return;
+// Tracer::generateMultiSamples (Tracer.cpp:115-144) for one slot: its camera
+// ray (origin + tmin, direction + tmax, as rc_ray) and its filter weight.
+__device__ __forceinline__ float camera_ray(const PassParams& P, uint32_t slot, float4& r0, float4& r1) {
     const uint32_t pixel = slot_pixel(P, slot), s = slot_sample(P, slot);
     float px = float(pixel % P.width), py = float(pixel / P.width);
     float w = 1.0f;
@@ -194,16 +185,45 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
         px += ox;
         py += oy;
     }
-    float3 d = cam_dir(P.cam, px, py);
-    rc_ray r;
-    r.org[0] = P.cam.pos.x; r.org[1] = P.cam.pos.y; r.org[2] = P.cam.pos.z;
-    r.tmin = 0.0f;
-    r.dir[0] = d.x; r.dir[1] = d.y; r.dir[2] = d.z;
-    r.tmax = __builtin_inff();
-    rays[slot] = r;
+    const float3 d = cam_dir(P.cam, px, py);
+    r0 = make_float4(P.cam.pos.x, P.cam.pos.y, P.cam.pos.z, 0.0f);
+    r1 = make_float4(d.x, d.y, d.z, __builtin_inff());
+    return w;
+}
+
+// The pass's camera rays, path states, zeroed slot radiance and weights.
+__global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restrict__ rays, float4* __restrict__ st,
+                                                   float4* __restrict__ L, float* __restrict__ W,
+                                                   DevCounters* __restrict__ ctr) {
+    uint32_t n = P.npix * P.spp_pass;
+    uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot == 0) {
+        ctr->queue_count[0] = n;
+        ctr->queue_count[1] = 0;
+        ctr->queue_count[2] = 0;
+    }
+    if (slot >= n) return;
+    float4 r0, r1;
+    const float w = camera_ray(P, slot, r0, r1);
+    float4* rp = reinterpret_cast<float4*>(rays + slot);
+    rp[0] = r0;
+    rp[1] = r1;
     st[slot] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(slot));
     L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W[slot] = w;
+}
+
+// 1: the camera bounce makes its camera rays in the extension kernel (no
+// k_camera pass over the queues).
+#ifndef RC_FUSED_CAMERA
+#define RC_FUSED_CAMERA 1
+#endif
+// Start of a pass whose camera bounce makes its own rays (RC_FUSED_CAMERA):
+// the camera queue holds all n slots.
+__global__ void k_pass_start(DevCounters* ctr, uint32_t n) {
+    ctr->queue_count[0] = n;
+    ctr->queue_count[1] = 0;
+    ctr->queue_count[2] = 0;
 }
 
 // Reserves `pred ? 1 : 0` entries of a queue: one atomic per wave, positions
@@ -400,13 +420,19 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
 // Extend + shade: closest-hit traversal of queue entry i and, in the same
 // thread, the shading of its hit.  A ray whose traversal stack outgrows LDS is
 // listed for the deep-stack kernel and shaded afterwards by k_shade_listed.
-template <bool COUNT>
+// CAM: the camera bounce of a pass, whose entry i is camera slot i: the
+// thread makes the ray itself (k_camera's work) instead of loading it, writes
+// the slot's weight and zeroed radiance, and stores the ray and state only
+// when it has to be listed for the deep-stack kernel (cam_rays, cam_st: the
+// buffers `rays` and `st` name, written only in this mode).
+template <bool COUNT, bool CAM>
 __global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
     PassParams P, uint32_t k, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
     const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
     const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
-    DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI) {
+    DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI, float* __restrict__ W,
+    rc_ray* __restrict__ cam_rays, float4* __restrict__ cam_st) {
     __shared__ uint32_t stack[kTraceLds * kExtBlock];
     const uint32_t n = ctr->queue_count[cin];
     const uint32_t i = trace_block<kExtBlock>(n) * kExtBlock + threadIdx.x;
@@ -417,14 +443,31 @@ __global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
     O.rec = kNoRecord;
     uint32_t slot = 0;
     if (i < n) {
-        const float4* rp = reinterpret_cast<const float4*>(rays + i);
-        float4 r0 = rp[0], r1 = rp[1];
-        float4 s = st[i];
+        float4 r0, r1, s;
+        if (CAM) {
+            W[i] = camera_ray(P, i, r0, r1);
+            s = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(i));
+            L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            r0 = rp[0];
+            r1 = rp[1];
+            s = st[i];
+        }
         slot = __float_as_uint(s.w);
         TraceResult R = traverse<false, COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
                                                kExtBlock, kTraceLds, n_rec, n_tri);
-        if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
-        else O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
+        if (R.overflow) {
+            if (CAM) {  // the deep-stack kernel and k_shade_listed read the entry
+                float4* rp = reinterpret_cast<float4*>(cam_rays + i);
+                rp[0] = r0;
+                rp[1] = r1;
+                cam_st[i] = s;
+            }
+            ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+        } else {
+            O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
+        }
     }
     __shared__ uint32_t append_scratch[kExtBlock / 64 + 1];
     emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
@@ -829,9 +872,14 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
         const uint32_t grid_n = (n + kBlock - 1) / kBlock;
         hipEvent_t ev;
         if (CI.cache_vertex) k_cache_reset<<<1, 1, 0, c->stream>>>(CI.counters, 0);
-        RC_TRY(c->prof_begin(&ev));
-        k_camera<<<grid_n, kBlock, 0, c->stream>>>(P, ra, sa, c->slotL.as<float4>(), c->slotW.as<float>(), c->counters);
-        RC_TRY(c->prof_end(ev, PK_OTHER));
+        if (RC_FUSED_CAMERA) {
+            k_pass_start<<<1, 1, 0, c->stream>>>(c->counters, n);
+        } else {
+            RC_TRY(c->prof_begin(&ev));
+            k_camera<<<grid_n, kBlock, 0, c->stream>>>(P, ra, sa, c->slotL.as<float4>(), c->slotW.as<float>(),
+                                                       c->counters);
+            RC_TRY(c->prof_end(ev, PK_OTHER));
+        }
         rc_ray* rin = ra;
         rc_ray* rout = rb;
         float4* sin_ = sa;
@@ -842,15 +890,20 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
             uint32_t cin = k & 1u, cout = cin ^ 1u;
             k_prep<<<1, 1, 0, c->stream>>>(c->counters, cout);
             RC_TRY(c->prof_begin(&ev));
+            const bool cam = RC_FUSED_CAMERA && k == 0;
+#define RC_EXTEND(C, CM)                                                                                       \
+    k_extend_shade<C, CM><<<grid_e, kExtBlock, 0, c->stream>>>(                                                \
+        P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),              \
+        c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI, c->slotW.as<float>(), \
+        rin, sin_)
             if (c->counting) {
-                k_extend_shade<true><<<grid_e, kExtBlock, 0, c->stream>>>(
-                    P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
-                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
+                if (cam) RC_EXTEND(true, true);
+                else RC_EXTEND(true, false);
             } else {
-                k_extend_shade<false><<<grid_e, kExtBlock, 0, c->stream>>>(
-                    P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),
-                    c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI);
+                if (cam) RC_EXTEND(false, true);
+                else RC_EXTEND(false, false);
             }
+#undef RC_EXTEND
             RC_TRY(c->prof_end(ev, PK_EXTEND));
             // cold path: rays whose stack outgrew LDS
             RC_TRY(rc_launch_deep(c, s, false, rin, c->hits.as<rc_hit>(), nullptr));
