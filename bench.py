@@ -362,8 +362,9 @@ def main() -> int:
     ctx.set_counting(False)
     ext_rays = cst.extension_rays
     camera_rays = cfg.width * cfg.height * cfg.spp
+    # a camera ray is made in the kernel: no ray and state read (-48), its weight written (+4)
     alg_bytes = (NODE_BYTES_PER_CHILD * cst.node_records + 48 * cst.tri_tests + 96 * ext_rays
-                 + 48 * (ext_rays - camera_rays) + 48 * cst.shadow_rays)
+                 - 44 * camera_rays + 48 * (ext_rays - camera_rays) + 48 * cst.shadow_rays)
     log(f"counting frame: {ext_rays} extension rays, {cst.node_records / ext_rays:.1f} records/ray, "
         f"{cst.tri_tests / ext_rays:.2f} triangles/ray, {cst.shadow_rays} shadow rays")
 
