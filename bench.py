@@ -52,77 +52,99 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sd, cam, cfg, frac: int = 2, seed: int = 7) -> dict:
-    """FP64 oracle (-O3 -ffast-math, OpenMP) on a seeded 1/frac pixel sample."""
+def cpu_reference(sd, cam, cfg, subset: int = 64, seed: int = 7, ref_spp: int = 4096, split_spp: int = 64,
+                  cache_vertex: int = 2) -> dict:
+    """The CPU leg: the FP64 oracle (restatement of the reference's OpenMP CPU
+    path, built -O3 -ffast-math, unordered traversal as the reference) renders
+    a seeded 1/subset of the pixels at ref_spp samples (n x n jittered, n =
+    sqrt(ref_spp)).  Timed, that is
+    the CPU baseline; the image is the reference the GPU images are scored
+    against.  Then the split-tail estimate on the same pixels at split_spp:
+    the share of a split_spp-sample pixel estimate's variance that the light
+    gathered after vertex cache_vertex's next-event ray carries, which is
+    what a perfect radiance cache at that vertex would remove."""
     import oracle as orc
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     npix = cfg.width * cfg.height
     rng = np.random.default_rng(seed)
-    ids = np.sort(rng.choice(npix, npix // frac, replace=False)).astype(np.uint32)
+    ids = np.sort(rng.choice(npix, npix // subset, replace=False)).astype(np.uint32)
     ob = orc.BVH(sd.vertices, fast=True)
     ocfg = orc.render_cfg(cfg, traversal=0)
+    ocfg.spp = ref_spp
+    ocfg.frame = 500000
     t0 = time.perf_counter()
     st = orc.render(ob, sd.tri_material, sd.materials, cam, ocfg, pixel_ids=ids, threads=threads)
     dt = time.perf_counter() - t0
-    samples = len(ids) * cfg.spp
-    return {
+    samples = len(ids) * ref_spp
+    ref = st[:, 2:5] / st[:, 0:1]
+    scfg = orc.render_cfg(cfg, traversal=0)
+    scfg.spp = split_spp
+    scfg.frame = 600000
+    scfg.cache_vertex = cache_vertex
+    t1 = time.perf_counter()
+    sp = orc.render_split_tail(ob, sd.tri_material, sd.materials, cam, scfg, ids, threads=threads)
+    dt_split = time.perf_counter() - t1
+    tail_var = float(np.mean(sp[:, 4:7] / (2.0 * sp[:, 0:1] ** 2)))
+    base = {
         "value": round(samples / dt / 1e6, 4),
         "unit": "Msamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(ids)} seeded pixels (1/{frac} of {cfg.width}x{cfg.height}) x {cfg.spp} spp, "
-                  f"{dt:.1f} s, unordered FP64 traversal as the reference CPU path, extension rays/sample "
-                  f"{st[:, 11].sum() / samples:.2f}",
+        "sample": f"{len(ids)} seeded pixels (1/{subset} of {cfg.width}x{cfg.height}) x {ref_spp} spp "
+                  f"(the 4096-spp reference image), {dt:.1f} s, unordered FP64 traversal as the reference CPU path, "
+                  f"extension rays/sample {st[:, 11].sum() / samples:.2f}",
     }
+    return {"baseline": base, "ids": ids, "ref": ref, "tail_var": tail_var, "split_spp": split_spp,
+            "split_s": round(dt_split, 1)}
 
 
 def mse_pair(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
 
 
-def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, cached_spp: int = 1,
-                   plain_spp: int = 64, ref_spp: int = 4096, warmup_frames: int = 16,
-                   pretrain_frames: int = 256, train_steps: int = 1) -> dict:
-    """Plain and radiance-cache renders given the same wall time, against a
-    ref_spp plain render.
+# FLOPs of the radiance-cache MLP (81 -> 64 x 4 -> 3): 2 x 17,664 weights per
+# inference query; a training sample adds the backward pass through the
+# hidden layers (2 x 12,480) and the weight gradients (2 x 17,664)
+MLP_INFER_FLOP = 35328
+MLP_TRAIN_FLOP = 35328 + 24960 + 35328
+BF16_PEAK_TFLOPS = 2500.0
 
-    The budget is the time of one plain frame of plain_spp samples.  Cold: a
-    new cache spends the budget on warmup_frames training frames (discarded)
-    and then on as many training frames of cached_spp samples as fit,
-    accumulated.  Warm: the same cache after pretrain_frames more untimed
-    training frames (a cache carried over from earlier frames, as in an
-    animation) renders and trains within the budget.  Frames of one sample use
-    random sub-pixel positions under the same filter (one jittered sample would
-    be the unfiltered pixel centre).  Paths end into the cache
-    at vertex cache_vertex (after two bounces for 2); every frame ends with
-    train_steps Adam steps on its training samples.  MSE is over linear RGB
-    and over the Reinhard-mapped RGB (key 0.18, gamma 2.2)."""
+
+def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2, plain_spp: int = 64,
+                 ref_spp: int = 4096, pretrain_frames: int = 128, pretrain_spp: int = 4, pretrain_steps: int = 4,
+                 train_steps: int = 1) -> dict:
+    """The plain renderer and the radiance-cached renderer given the same wall
+    time.  The budget is one plain frame of plain_spp jittered samples.  A
+    cached frame is the same jittered plain_spp-sample pattern as one
+    wavefront pass whose paths end into the cache at vertex cache_vertex (after
+    two bounces for 2), followed by train_steps Adam steps on its training
+    samples, which overlap the next frame's traversal.  Cached frames are
+    rendered and accumulated while the projected time of the next one stays
+    within the budget (at least one frame).  Warm: the cache was trained on
+    pretrain_frames earlier frames (as in an animation); cold: a new cache.
+    Returns the linear images (scored against the CPU reference later), the
+    times, and a full-image MSE against a ref_spp GPU render."""
     from raycer_amd import api
 
     cfg = api.RenderCfg.from_buffer_copy(base_cfg)
     cfg.width, cfg.height = base_cfg.width // scale, base_cfg.height // scale
+    cfg.spp = plain_spp
     cam = sd.desc.camera
     film = ctx.film(cfg.width, cfg.height)
 
-    def image():
-        return film.linear()[:, :3].astype(np.float64), film.tonemap()[0][:, :3].astype(np.float64)
+    def lin():
+        return film.linear()[:, :3].astype(np.float32)
 
-    def score(lin, tm, ref_lin, ref_tm):
-        return {"mse_linear": mse_pair(lin, ref_lin), "mse_tonemapped": mse_pair(tm, ref_tm),
-                "mean_ratio": round(float(lin.mean() / ref_lin.mean()), 5)}
-
-    out = {"resolution": f"{cfg.width}x{cfg.height}", "reference_spp": ref_spp, "plain_spp": plain_spp,
-           "cache_vertex": cache_vertex, "train_steps_per_frame": train_steps}
+    out = {"resolution": f"{cfg.width}x{cfg.height}", "plain_spp": plain_spp, "cache_vertex": cache_vertex,
+           "train_steps_per_frame": train_steps}
+    images = {}
     try:
-        # reference: ref_spp / plain_spp frames of plain_spp samples
-        cfg.spp = plain_spp
         film.clear()
         for f in range(ref_spp // plain_spp):
             cfg.frame = 100000 + f
             ctx.render(scene, film, cam, cfg)
-        ref_lin, ref_tm = image()
-        # plain at the budget
+        images["gpu_ref"] = lin()
         film.clear()
         cfg.frame = 7
         ctx.synchronize()
@@ -131,60 +153,112 @@ def equal_time_mse(ctx, scene, sd, base_cfg, scale: int, cache_vertex: int = 2, 
         ctx.synchronize()
         budget = time.perf_counter() - t0
         out["budget_ms"] = round(budget * 1e3, 2)
-        out["plain"] = score(*image(), ref_lin, ref_tm)
-        cache = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, lr=1e-3, seed=1, max_batch=65536)
-        try:
-            ccfg = api.RenderCfg.from_buffer_copy(cfg)
-            ccfg.spp = cached_spp
-            ccfg.cache_vertex = cache_vertex
-            if cached_spp == 1:
-                # one jittered sample is the pixel centre with weight 1 (no
-                # filtering); random positions keep the filter's footprint
-                ccfg.sampler = api.RC_SAMPLER_RANDOM
-            frame = [200000]
+        images["plain"] = lin()
+        ccfg = api.RenderCfg.from_buffer_copy(cfg)
+        ccfg.cache_vertex = cache_vertex
+        frame = [200000]
 
-            def run(warmup):
-                frames = accumulated = 0
-                ctx.synchronize()
-                t0 = time.perf_counter()
-                film.clear()
-                while True:
-                    if frames == warmup:
-                        film.clear()
-                    ccfg.frame = frame[0]
-                    frame[0] += 1
-                    loss = cache.render(scene, film, cam, ccfg, train=train_steps)
-                    frames += 1
-                    if frames > warmup:
-                        accumulated += 1
-                    elapsed = time.perf_counter() - t0
-                    if frames > warmup and elapsed * (frames + 1) / frames > budget:
-                        break
-                ctx.synchronize()
-                st = ctx.stats()
-                r = score(*image(), ref_lin, ref_tm)
-                r.update({"spp": accumulated * cached_spp, "ms": round((time.perf_counter() - t0) * 1e3, 2),
-                          "train_steps": frames, "train_samples_last_frame": int(st.train_samples),
-                          "cache_queries_last_frame": int(st.cache_queries), "last_loss": round(loss, 5)})
-                return r
-
-            out["cached_cold"] = run(warmup_frames)
-            out["cached_cold"]["warmup_frames"] = warmup_frames
-            for f in range(pretrain_frames):
+        def run(cache):
+            frames = 0
+            ctx.synchronize()
+            film.clear()
+            t0 = time.perf_counter()
+            while True:
                 ccfg.frame = frame[0]
                 frame[0] += 1
-                film.clear()
-                cache.render(scene, film, cam, ccfg, train=train_steps)
-            out["cached_warm"] = run(0)
-            out["cached_warm"]["pretrain_frames"] = pretrain_frames
+                cache.render(scene, film, cam, ccfg, train=train_steps, want_loss=False)
+                frames += 1
+                elapsed = time.perf_counter() - t0
+                if elapsed * (frames + 1) / frames > budget:
+                    break
+            ctx.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            st = ctx.stats()
+            return {"frames": frames, "spp": frames * plain_spp, "ms": round(ms, 2),
+                    "ms_per_frame": round(ms / frames, 2), "train_samples_last_frame": int(st.train_samples),
+                    "cache_queries_last_frame": int(st.cache_queries)}, lin()
+
+        cold = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, lr=1e-3, seed=1, max_batch=65536)
+        try:
+            out["cached_cold"], images["cached_cold"] = run(cold)
         finally:
-            cache.close()
+            cold.close()
+        warm = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, lr=1e-3, seed=1, max_batch=65536)
+        try:
+            pcfg = api.RenderCfg.from_buffer_copy(ccfg)
+            pcfg.spp = pretrain_spp
+            pcfg.sampler = api.RC_SAMPLER_RANDOM
+            t0 = time.perf_counter()
+            for f in range(pretrain_frames):
+                pcfg.frame = 300000 + f
+                film.clear()
+                loss = warm.render(scene, film, cam, pcfg, train=pretrain_steps, want_loss=f == pretrain_frames - 1)
+            out["pretrain"] = {"frames": pretrain_frames, "spp": pretrain_spp, "steps_per_frame": pretrain_steps,
+                               "s": round(time.perf_counter() - t0, 2), "last_loss": round(loss, 5)}
+            out["cached_warm"], images["cached_warm"] = run(warm)
+            # one more warm frame with the kernel timers on: the learned
+            # kernels' time and FLOP rates (BF16 MFMA, FP32 accumulation)
+            ctx.set_profiling(True)
+            ctx.stats()
+            film.clear()
+            ccfg.frame = frame[0]
+            warm.render(scene, film, cam, ccfg, train=train_steps, want_loss=False)
+            st = ctx.stats()
+            ctx.set_profiling(False)
+            infer_flop = st.cache_queries * MLP_INFER_FLOP
+            train_flop = train_steps * st.train_samples * MLP_TRAIN_FLOP
+            out["mlp_kernels"] = {
+                "infer_queries": int(st.cache_queries), "infer_ms": round(st.cache_ms, 4),
+                "infer_tflops": round(infer_flop / (st.cache_ms * 1e-3) / 1e12, 2) if st.cache_ms else None,
+                "train_samples": int(st.train_samples), "train_ms": round(st.train_ms, 4),
+                "train_tflops": round(train_flop / (st.train_ms * 1e-3) / 1e12, 2) if st.train_ms else None,
+                "peak_bf16_tflops": BF16_PEAK_TFLOPS,
+                "frame_ms": round(st.render_ms, 2),
+                "note": "algorithmic FLOPs (35,328 per query; 95,616 per training sample) over the kernels' HIP-event "
+                        "time; MFMA busy and BF16 MOPS counters are in profiles/"}
+        finally:
+            warm.close()
     finally:
         film.close()
+    ref = images["gpu_ref"].astype(np.float64)
+    for k in ("plain", "cached_cold", "cached_warm"):
+        out.setdefault(k, {})
+        out[k]["mse_linear_vs_gpu_ref"] = mse_pair(images[k], ref)
     for k in ("cached_cold", "cached_warm"):
-        out[k]["ratio_linear"] = round(out[k]["mse_linear"] / out["plain"]["mse_linear"], 4)
-        out[k]["ratio_tonemapped"] = round(out[k]["mse_tonemapped"] / out["plain"]["mse_tonemapped"], 4)
-    return out
+        out[k]["ratio_vs_gpu_ref"] = round(out[k]["mse_linear_vs_gpu_ref"] / out["plain"]["mse_linear_vs_gpu_ref"], 4)
+    return {"summary": out, "images": images, "budget_s": budget}
+
+
+def score_against_cpu(forms: dict, cpu: dict) -> None:
+    """Adds the MSE of each image against the CPU reference on its pixels, the
+    cached/plain ratios, and the bound the split-tail variance puts on them."""
+    ids, ref = cpu["ids"], cpu["ref"]
+    out = forms["summary"]
+    m = {k: mse_pair(img[ids], ref) for k, img in forms["images"].items()}
+    plain = m["plain"]
+    out["mse_vs_cpu_ref"] = {
+        "reference": f"FP64 CPU oracle, {len(ids)} seeded pixels, 4096 spp",
+        "plain": plain, "cached_warm": m["cached_warm"], "cached_cold": m["cached_cold"],
+        "gpu_4096spp_vs_cpu_4096spp": m["gpu_ref"],
+        "gpu_4096spp_expected_from_noise": 2.0 * plain * out["plain_spp"] / 4096.0,
+        "ratio_warm": round(m["cached_warm"] / plain, 4),
+        "ratio_cold": round(m["cached_cold"] / plain, 4),
+    }
+    tail = cpu["tail_var"] * cpu["split_spp"] / out["plain_spp"]  # the same per-sample variance at plain_spp
+    warm = out["cached_warm"]
+    time_ratio = warm["ms_per_frame"] / out["budget_ms"]
+    out["variance_bound"] = {
+        "tail_variance": tail,
+        "tail_share_of_plain_mse": round(tail / plain, 4),
+        "ratio_bound_equal_spp": round(1.0 - tail / plain, 4),
+        "cached_frame_time_ratio": round(time_ratio, 4),
+        "ratio_bound_equal_time": round((1.0 - tail / plain) * time_ratio, 4),
+        "ratio_warm_time_scaled": round(m["cached_warm"] / plain * time_ratio, 4),
+        "note": "a perfect cache removes the variance of the light gathered after the cache vertex's next-event "
+                "ray (split-tail estimate on the CPU reference's pixels); at equal samples the cached MSE is at "
+                "least (1 - share) x plain; at equal time the cached form renders 1 / frame-time-ratio as many "
+                "samples",
+    }
 
 
 def phong_c2(ctx, reps: int = 10) -> dict:
@@ -320,6 +394,8 @@ def main() -> int:
     ap.add_argument("--spp-per-pass", type=int, default=64,
                     help="samples per pixel per wavefront pass (64: the whole frame at once, 132.7 M paths, "
                          "~27 GB of path queues; 0: the library's automatic choice)")
+    ap.add_argument("--hit", choices=("fp32", "fp64"), default="fp32",
+                    help="triangle decision: FP32 Möller–Trumbore (default) or FP32 filter + FP64 decision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2", choices=("on", "off"), default="on", help="the 1080p Phong raycast line")
     ap.add_argument("--whitted", choices=("on", "off"), default="on", help="the 1080p Whitted ray tracer line")
@@ -347,13 +423,16 @@ def main() -> int:
     cam = sd.desc.camera
     ctx = ra.Context(0)
     name, cus = ctx.device_name()
+    ctx.set_hit_decision(a.hit)
     scene = ctx.upload(sd, nodes, ordered)
     film = ctx.film(cfg.width, cfg.height)
     log(f"device {name} ({cus} CUs), setup {time.perf_counter() - t0:.1f} s")
 
     ctx.set_tuning(spp_per_pass=a.spp_per_pass)
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
-    log(f"streaming copy {copy_gbs:.0f} GB/s")
+    probes = {"l2_4MiB": ctx.read_bandwidth(4 << 20, 512), "infinity_cache_128MiB": ctx.read_bandwidth(128 << 20, 16),
+              "hbm_4GiB": ctx.read_bandwidth(4 << 30, 1)}
+    log(f"streaming copy {copy_gbs:.0f} GB/s, reads {json.dumps({k: round(v) for k, v in probes.items()})} GB/s")
 
     # untimed counting frame: node/triangle fetches of the same rays
     ctx.set_counting(True)
@@ -376,7 +455,7 @@ def main() -> int:
     ctx.synchronize()
     ext_ms = shd_ms = shade_ms = other_ms = 0.0
     ext_launches = 0
-    total_ext = total_shadow = 0
+    total_ext = total_shadow = overflow = 0
     t_start = time.perf_counter()
     for i in range(a.steps):
         cfg.frame = i
@@ -389,6 +468,7 @@ def main() -> int:
         ext_launches += st.extend_launches
         total_ext += st.extension_rays
         total_shadow += st.shadow_rays
+        overflow += st.overflow_rays
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
     ctx.set_profiling(False)
@@ -397,13 +477,18 @@ def main() -> int:
     # algorithmic bytes per frame scale with the frame's rays; the counting
     # frame used frame index 0 like the first timed frame
     bytes_per_ray = alg_bytes / ext_rays
-    achieved = bytes_per_ray * total_ext / (ext_ms * 1e-3) / 1e9
+    avg_launch_s = ext_ms / max(ext_launches, 1) * 1e-3
+    demand = bytes_per_ray * total_ext / (ext_ms * 1e-3) / 1e9
     traffic = None
     if PROFILE_TRAFFIC.exists():
         try:
             traffic = json.loads(PROFILE_TRAFFIC.read_text()).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
+    # achieved: the HBM bytes rocprofv3 counts per launch of this kernel
+    # (FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size) over the
+    # launch time measured here; without that profile, the algorithmic bytes
+    achieved = traffic / avg_launch_s / 1e9 if traffic else demand
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -426,6 +511,7 @@ def main() -> int:
             "max_path_length": cfg.max_path_length,
             "estimator": "reference (2 rho |cos|)",
             "bvh": "32-bin SAH, collapsed to 4-wide",
+            "hit_decision": a.hit,
             "samples_per_pass": min(a.spp_per_pass, cfg.spp) if a.spp_per_pass else "auto",
             "parallelism": "single GPU",
         },
@@ -437,19 +523,25 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "bytes_per_ray": round(bytes_per_ray, 1),
-            "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
+            "achieved_basis": "rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch (profiles/) over the live launch "
+                              "time" if traffic else "algorithmic bytes (no counter profile found)",
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "achievable_copy_GBs": round(copy_gbs, 1),
             "frac_of_achievable_copy": round(achieved / copy_gbs, 4),
-            "hbm_measured_GBs": (round(traffic / (ext_ms / max(ext_launches, 1) * 1e-3) / 1e9, 1)
-                                 if traffic else None),
-            "note": "achieved counts the algorithmic node, triangle and queue bytes; L2 and the 256 MB "
-                    "Infinity Cache serve most node and triangle fetches, so it can exceed the HBM peak; "
-                    "hbm_measured_GBs is the rocprofv3 FETCH_SIZE + WRITE_SIZE traffic per launch over "
-                    "the same launch time",
+            "demand": {
+                "bytes_per_ray": round(bytes_per_ray, 1),
+                "bytes_per_launch": round(bytes_per_ray * total_ext / max(ext_launches, 1)),
+                "GBs": round(demand, 1),
+                "frac_of_hbm_peak": round(demand / HBM_PEAK_GBS, 4),
+                "read_probe_GBs": {k: round(v, 1) for k, v in probes.items()},
+                "frac_of_infinity_cache_read": round(demand / probes["infinity_cache_128MiB"], 4),
+                "note": "node, triangle and queue bytes the traversal asks for (the algorithmic bytes per ray); "
+                        "L2 and the 256 MB Infinity Cache, which holds the whole scene, serve most of them",
+            },
         },
         "gsample_bounces_per_s": round(total_ext / elapsed / 1e9, 4),
         "shadow_rays_per_s_G": round(total_shadow / elapsed / 1e9, 4),
+        "overflow_rays_per_step": overflow // a.steps,
         "kernel_ms_per_step": {
             "extend": round(ext_ms / a.steps, 2),
             "shadow": round(shd_ms / a.steps, 2),
@@ -472,26 +564,38 @@ def main() -> int:
         log("64-scene batch (autotuned, one captured graph)")
         out["batch_64_scenes"] = batch_c4(ctx)
         log(f"batch: {json.dumps(out['batch_64_scenes'])}")
+    forms = None
     if a.mse != "off":
-        log(f"equal-time MSE of the radiance cache ({a.mse})")
-        out["equal_time_mse"] = equal_time_mse(ctx, scene, sd, cfg, 1 if a.mse == "full" else 4)
-        log(f"equal-time MSE: {json.dumps(out['equal_time_mse'])}")
-        # the same comparison on the closed box (emitter-lit, up to 8 rays per
+        log(f"plain and radiance-cached forms at equal time ({a.mse})")
+        scale = 1 if a.mse == "full" else 4
+        forms = cached_forms(ctx, scene, sd, cfg, scale=scale)
+        out["equal_time_mse"] = forms["summary"]
+        log(f"cached forms: {json.dumps(forms['summary'])}")
+        # the same comparison in the closed box (emitter-lit, up to 8 rays per
         # path), where every path reaches the cache vertex
         bd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_BOX))
         bn, bo, _, _ = ra.build_bvh(bd.vertices, method=api.RCH_BVH_BINNED)
         box = ctx.upload(bd, bn, bo)
         try:
-            out["equal_time_mse_box"] = equal_time_mse(ctx, box, bd, bd.desc.cfg, 1 if a.mse == "full" else 2,
-                                                       cached_spp=16, warmup_frames=4, pretrain_frames=64)
+            out["equal_time_mse_box"] = cached_forms(ctx, box, bd, bd.desc.cfg, scale=1 if a.mse == "full" else 2,
+                                                     plain_spp=16, pretrain_frames=64)["summary"]
         finally:
             box.close()
-        log(f"equal-time MSE (box): {json.dumps(out['equal_time_mse_box'])}")
+        log(f"cached forms (box): {json.dumps(out['equal_time_mse_box'])}")
     scene.close()
     ctx.close()
     if not a.no_cpu_baseline:
-        log("cpu baseline (oracle, half of the pixels)")
-        out["cpu_baseline"] = cpu_baseline(sd, cam, cfg)
+        log("cpu leg: oracle 4096-spp reference on 1/64 of the pixels (the CPU baseline) and the split-tail bound")
+        ocfg = api.RenderCfg.from_buffer_copy(cfg)
+        if forms is not None:
+            ocfg.width, ocfg.height = (cfg.width // (1 if a.mse == "full" else 4),
+                                       cfg.height // (1 if a.mse == "full" else 4))
+        cpu = cpu_reference(sd, cam, ocfg)
+        out["cpu_baseline"] = cpu["baseline"]
+        if forms is not None:
+            score_against_cpu(forms, cpu)
+            log(f"vs CPU reference: {json.dumps(forms['summary']['mse_vs_cpu_ref'])}")
+            log(f"variance bound: {json.dumps(forms['summary']['variance_bound'])}")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -172,12 +172,17 @@ typedef struct rc_point_light {
 /* Launch tuning: threads per block of the ray-query kernels, pass size in
  * samples per pixel (0, the default: each frame takes as many samples per pass
  * as keep a pass within 2^26 paths, 2^21 for rc_raytrace; else 1..64), grid
- * limit of the ray-query kernels in blocks per CU.  Defaults 256, 0, 64. */
+ * limit of the ray-query kernels in blocks per CU, and the traversal stack
+ * entries held on chip (registers and LDS) before a ray is handed to the
+ * deep-stack kernel, which walks it again with a stack in global memory
+ * (0, the default: all the build holds; 1..that depth: fewer, which sends
+ * more rays down the deep-stack path and changes no result).
+ * Defaults 256, 0, 64, 0. */
 typedef struct rc_tuning {
     uint32_t block;
     uint32_t spp_per_pass;
     uint32_t blocks_per_cu;
-    uint32_t pad;
+    uint32_t stack_limit;
 } rc_tuning;
 
 typedef struct rc_stats {
@@ -186,7 +191,7 @@ typedef struct rc_stats {
     uint64_t samples;            /* camera samples of the last render */
     uint64_t node_records;       /* 32-byte node records fetched (counting runs only) */
     uint64_t tri_tests;          /* 48-byte triangle records tested (counting runs only) */
-    uint64_t overflow_rays;      /* rays re-traced with the deep-stack kernel */
+    uint64_t overflow_rays;      /* rays re-walked by the deep-stack kernel, summed over all launches of the call */
     uint64_t extend_launches;
     double extend_ms;            /* sum of closest-hit kernel durations (profiling on) */
     double shadow_ms;
@@ -210,12 +215,28 @@ int rc_device_name(rc_ctx* ctx, char* buf, size_t len, int* cu_count);
 int rc_set_profiling(rc_ctx* ctx, int enable);
 /* counting: 1 = use the traversal variant that counts node/triangle fetches */
 int rc_set_counting(rc_ctx* ctx, int enable);
+/* How a ray-triangle test is decided (Triangle::intersect, Triangle.cpp:51-121,
+ * and its FP32 OpenCL form, data/opencl/intersections.cl:27-68):
+ * RC_HIT_FP32 (the default): Möller–Trumbore in FP32 with inclusive
+ * barycentric bounds and det == 0 rejected; RC_HIT_FP64: an FP32 filter with
+ * widened bounds, then the candidates decided in FP64 on the same FP32 inputs
+ * (distances to FP64 accuracy at grazing angles).  Applies to every traversal
+ * of the context. */
+enum { RC_HIT_FP32 = 0, RC_HIT_FP64 = 1 };
+int rc_set_hit_decision(rc_ctx* ctx, int mode);
 int rc_set_tuning(rc_ctx* ctx, const rc_tuning* t);
 int rc_get_stats(rc_ctx* ctx, rc_stats* out);
 int rc_synchronize(rc_ctx* ctx);
 /* Achievable-bandwidth probe (no reference counterpart): a float4 streaming
- * copy of `bytes` bytes, `iters` times; returns read+write GB/s. */
+ * copy of `bytes` bytes, `iters` times, at four grid sizes; returns the best
+ * read+write GB/s. */
 int rc_copy_bandwidth(rc_ctx* ctx, size_t bytes, int iters, double* gbps);
+/* Read-bandwidth probe of one cache level (no reference counterpart): a
+ * working set of `bytes` read `iters` times per launch by threads that
+ * re-read the same addresses, so a set that fits L2 (4 MiB per XCD) or the
+ * Infinity Cache (256 MiB) is served from it; returns the best read GB/s over
+ * three grid sizes. */
+int rc_read_bandwidth(rc_ctx* ctx, size_t bytes, int iters, double* gbps);
 
 /* ------------------------------------------------------------ scene */
 /* CLTracer::initializeBuffers / CLScene::readSceneFull (CLScene.cpp:72-263):

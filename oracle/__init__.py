@@ -60,10 +60,12 @@ def lib(fast: bool = False) -> C.CDLL:
         L.or_bvh_free.argtypes = [vp]
         L.or_intersect_triangle.argtypes = [vp, vp, vp, d, d, vp, vp, vp]
         L.or_trace_closest_ref.argtypes = [vp, vp, vp, sz, vp, vp, C.c_int]
-        L.or_trace_closest_diag.argtypes = [vp, vp, vp, sz, d, d, vp, vp, vp, C.c_int]
+        L.or_trace_closest_diag.argtypes = [vp, vp, vp, sz, d, d, vp, vp, vp, vp, C.c_int]
         L.or_trace_closest_brute.argtypes = [vp, sz, vp, sz, vp, vp, C.c_int]
         L.or_trace_any.argtypes = [vp, vp, vp, sz, d, vp, vp, C.c_int]
         L.or_camera_ray.argtypes = [vp, C.c_uint32, C.c_uint32, d, d, vp, vp]
+        L.or_render_split_tail.argtypes = [vp, vp, sz, vp, vp, vp, C.POINTER(RenderCfg), vp, sz, vp, C.c_int]
+        L.or_render_split_tail.restype = C.c_int
         L.or_render.argtypes = [vp, vp, sz, vp, vp, sz, vp, C.POINTER(RenderCfg), vp, sz, vp, C.c_int,
                                 C.POINTER(C.c_int)]
         L.or_trace_path.argtypes = [vp, vp, sz, vp, vp, vp, C.POINTER(RenderCfg), C.c_uint32, C.c_uint32, vp,
@@ -182,12 +184,15 @@ class BVH:
         lib().or_trace_closest_ref(self.h, _p(self.vertices), _p(rays), n, _p(t), _p(i), threads)
         return t, i
 
-    def trace_diag(self, rays: np.ndarray, rel_tie=1e-6, band=2e-5, threads: int = 0):
+    def trace_diag(self, rays: np.ndarray, rel_tie=1e-6, band=2e-5, threads: int = 0, with_cond=False):
+        """Closest hit (t, id) with near-tie flags; with_cond also returns the
+        conditioning |det| / (|e1| |e2| |d|) of each best hit (1 for a miss)."""
         rays = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
         n = rays.shape[0]
-        t, i, f = np.empty(n), np.empty(n, np.int64), np.empty(n, np.uint32)
-        lib().or_trace_closest_diag(self.h, _p(self.vertices), _p(rays), n, rel_tie, band, _p(t), _p(i), _p(f), threads)
-        return t, i, f
+        t, i, f, c = np.empty(n), np.empty(n, np.int64), np.empty(n, np.uint32), np.empty(n)
+        lib().or_trace_closest_diag(self.h, _p(self.vertices), _p(rays), n, rel_tie, band, _p(t), _p(i), _p(f), _p(c),
+                                    threads)
+        return (t, i, f, c) if with_cond else (t, i, f)
 
     def trace_any(self, rays: np.ndarray, band=2e-5, threads: int = 0):
         rays = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
@@ -236,6 +241,22 @@ def render(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, pixel_ids=None, threads
     ca = camera_array(cam)
     L.or_render(bvh.h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), ma.shape[0],
                 _p(ca), C.byref(cfg), _p(ids), npix, _p(out), threads, C.byref(stop))
+    return out
+
+
+def render_split_tail(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, pixel_ids, threads: int = 0):
+    """or_render_split_tail over the listed pixels (cfg.cache_vertex > 0):
+    per pixel (sum_w, sum_w La[3], sum_w^2 (La - Lb)^2 [3]) for two paths of
+    each sample that share everything up to that vertex's next-event ray."""
+    L = lib(bvh.fast)
+    tm = np.ascontiguousarray(tri_mat, np.uint32)
+    ma = np.ascontiguousarray(materials_array(mats) if mats.dtype.names else mats, np.float64)
+    ids = np.ascontiguousarray(pixel_ids, np.uint32)
+    out = np.empty((ids.shape[0], 7))
+    rc = L.or_render_split_tail(bvh.h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), _p(camera_array(cam)),
+                                C.byref(cfg), _p(ids), ids.shape[0], _p(out), threads)
+    if rc != 0:
+        raise ValueError("render_split_tail needs cfg.cache_vertex > 0")
     return out
 
 

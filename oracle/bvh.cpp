@@ -273,7 +273,7 @@ extern "C" void or_trace_closest_ref(const or_bvh* b, const float* vertices, con
 
 extern "C" void or_trace_closest_diag(const or_bvh* b, const float* vertices, const float* rays, size_t n,
                                       double rel_tie, double band, double* t_out, int64_t* id_out,
-                                      uint32_t* flags_out, int nthreads) {
+                                      uint32_t* flags_out, double* cond_out, int nthreads) {
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < int64_t(n); ++i) {
@@ -283,6 +283,7 @@ extern "C" void or_trace_closest_diag(const or_bvh* b, const float* vertices, co
         t_out[i] = D.t;
         id_out[i] = D.id;
         if (flags_out) flags_out[i] = D.flags;
+        if (cond_out) cond_out[i] = D.id >= 0 ? D.cond : 1.0;
     }
 }
 

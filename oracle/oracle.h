@@ -86,10 +86,12 @@ void or_trace_closest_ref(const or_bvh* b, const float* vertices, const float* r
  * flags bit0: near tie (another candidate within rel_tie of the best t, or a
  * candidate inside the barycentric band closer than the best);
  * bit1: best hit inside the barycentric band (grazes an edge);
- * bit2: |det| of the best hit inside the FP32 rounding band. */
+ * bit2: |det| of the best hit inside the FP32 rounding band.
+ * cond_out (may be NULL): conditioning |det| / (|e1| |e2| |d|) of the best
+ * hit (1 for a miss), which bounds the FP32 error of its decision. */
 void or_trace_closest_diag(const or_bvh* b, const float* vertices, const float* rays, size_t n,
                            double rel_tie, double bary_band,
-                           double* t_out, int64_t* id_out, uint32_t* flags_out, int nthreads);
+                           double* t_out, int64_t* id_out, uint32_t* flags_out, double* cond_out, int nthreads);
 /* Brute force over every triangle (no BVH): the BVH-free pin. */
 void or_trace_closest_brute(const float* vertices, size_t ntri, const float* rays, size_t n,
                             double* t_out, int64_t* id_out, int nthreads);
@@ -138,6 +140,15 @@ int or_render(const or_bvh* b, const float* vertices, size_t ntri, const uint32_
               const double* materials, size_t nmat, const double cam[7],
               const or_render_cfg* cfg, const uint32_t* pixel_ids, size_t npix,
               double* stats_out, int nthreads, volatile int* interrupt);
+
+/* Split-tail estimate of the variance a radiance cache at cfg->cache_vertex
+ * (> 0) could remove, over the listed pixels: per pixel 7 doubles
+ *   sum_w, sum_w La[3], sum_w^2 (La - Lb)^2 [3]
+ * where La and Lb are two paths of the same sample sharing everything up to
+ * that vertex's next-event ray.  Returns nonzero for cache_vertex 0. */
+int or_render_split_tail(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,
+                         const double* materials, const double cam[7], const or_render_cfg* cfg,
+                         const uint32_t* pixel_ids, size_t npix, double* out, int nthreads);
 
 /* One path (pixel, sample) with a trace of every vertex, for replay checks.
  * vertices_out: max_path_length * 8 doubles (hit t, tri id, thr.rgb, L.rgb).

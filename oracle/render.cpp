@@ -240,8 +240,12 @@ struct PathOut {
 // Sun NEE uses the BRDF each estimator implies: (2 rho / pi) cos_i (reference)
 // or rho / pi (lambert), times E cos_s; it happens at vertex k only when a
 // continuation ray k + 1 would still be allowed.
+// tail_vertex / tail_frame: the draws of vertices k >= tail_vertex (the
+// bounce from vertex tail_vertex on) use frame tail_frame instead of c.frame,
+// so two calls that differ only in tail_frame share the path up to and
+// including that vertex's next-event ray and continue independently.
 PathOut trace_path(const Scene& s, const or_render_cfg& c, uint32_t pixel, uint32_t sample, V3 o, V3 d,
-                   double* trace_out) {
+                   double* trace_out, uint32_t tail_vertex = 0xFFFFFFFFu, uint32_t tail_frame = 0) {
     PathOut P;
     V3 thr(1, 1, 1);
     V3 L(0, 0, 0);
@@ -276,7 +280,7 @@ PathOut trace_path(const Scene& s, const or_render_cfg& c, uint32_t pixel, uint3
         V3 p = r.o + (t * r.d);
         V3 rho(m[0], m[1], m[2]);
         double u[4];
-        draw4(c.seed, pixel, sample, c.frame, k, u);
+        draw4(c.seed, pixel, sample, k >= tail_vertex ? tail_frame : c.frame, k, u);
         bool more = (k + 1) < c.max_path_length;
         if (has_sun && more) {
             double cs = dot(n, sun);
@@ -402,6 +406,47 @@ extern "C" int or_render(const or_bvh* b, const float* vertices, size_t ntri, co
         }
     }
     return stopped;
+}
+
+// Variance a radiance cache at vertex c.cache_vertex could remove: each
+// sample's path is traced twice, sharing everything up to that vertex's
+// next-event ray and with independent tails after it (tail frames frame' =
+// frame ^ 0x9E3779B9 for the second).  Per listed pixel, 7 doubles:
+// sum_w, sum_w La[3] (the first path's estimate), sum_w^2 (La - Lb)^2 [3].
+// E[(La - Lb)^2] = 2 Var(tail | prefix), so sum_w^2 (La - Lb)^2 / (2 sum_w^2)
+// is the tail's share of the pixel estimate's variance (law of total
+// variance), which a perfect cache replaces by its expectation.
+extern "C" int or_render_split_tail(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,
+                                    const double* mats, const double cam[7], const or_render_cfg* cfg,
+                                    const uint32_t* pixel_ids, size_t npix, double* out, int nthreads) {
+    Scene s{b, vertices, tri_mat, mats, {}};
+    s.tri_flags.resize(ntri);
+    for (size_t i = 0; i < ntri; ++i) s.tri_flags[i] = uint32_t(mats[OR_MAT_STRIDE * tri_mat[i] + 13]);
+    const or_render_cfg c = *cfg;
+    if (c.cache_vertex == 0) return 1;
+    Cam k = make_cam(cam, c.width, c.height);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const uint32_t tf = c.frame ^ 0x9E3779B9u;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < int64_t(npix); ++i) {
+        uint32_t pixel = pixel_ids[i];
+        double* q = out + 7 * i;
+        for (int j = 0; j < 7; ++j) q[j] = 0.0;
+        for (uint32_t smp = 0; smp < c.spp; ++smp) {
+            V3 o, d;
+            double w;
+            sample_camera(k, c, pixel, smp, o, d, w);
+            PathOut A = trace_path(s, c, pixel, smp, o, d, nullptr, c.cache_vertex, c.frame);
+            PathOut B = trace_path(s, c, pixel, smp, o, d, nullptr, c.cache_vertex, tf);
+            const double la[3] = {A.L.x, A.L.y, A.L.z}, lb[3] = {B.L.x, B.L.y, B.L.z};
+            q[0] += w;
+            for (int ch = 0; ch < 3; ++ch) {
+                q[1 + ch] += w * la[ch];
+                q[4 + ch] += w * w * (la[ch] - lb[ch]) * (la[ch] - lb[ch]);
+            }
+        }
+    }
+    return 0;
 }
 
 extern "C" int or_trace_path(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,

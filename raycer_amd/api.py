@@ -25,6 +25,7 @@ RC_FILTER_BOX, RC_FILTER_TENT, RC_FILTER_BELL, RC_FILTER_GAUSSIAN, RC_FILTER_MIT
  RC_SAMPLER_POISSON_DISC) = range(6)
 RC_TONEMAP_REINHARD, RC_TONEMAP_LINEAR, RC_TONEMAP_SIMPLE, RC_TONEMAP_PASSTHROUGH = range(4)
 RC_EST_REFERENCE, RC_EST_LAMBERT = 0, 1
+RC_HIT_FP32, RC_HIT_FP64 = 0, 1
 RCH_SCENE_BOX, RCH_SCENE_SPHERE_GROUND, RCH_SCENE_TERRAIN, RCH_SCENE_SOUP = 1, 2, 3, 4
 RCH_BVH_REFERENCE, RCH_BVH_BINNED = 0, 1
 
@@ -160,7 +161,7 @@ class PointLight(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_uint32), ("spp_per_pass", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("stack_limit", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -242,6 +243,7 @@ def lib() -> C.CDLL:
             "rc_device_name": [vp, C.c_char_p, sz, C.POINTER(i32)],
             "rc_set_profiling": [vp, i32],
             "rc_set_counting": [vp, i32],
+            "rc_set_hit_decision": [vp, i32],
             "rc_set_tuning": [vp, C.POINTER(Tuning)],
             "rc_get_stats": [vp, C.POINTER(Stats)],
             "rc_synchronize": [vp],
@@ -272,6 +274,7 @@ def lib() -> C.CDLL:
             "rc_render_batch_plan": [vp, vp, vp, vp, vp, sz, vp],
             "rc_autotune_batch": [vp, vp, vp, vp, sz, C.c_char_p, vp],
             "rc_copy_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
+            "rc_read_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
             "rc_render_progress": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), vp, vp],
             "rc_cache_create": [vp, C.POINTER(CacheCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(vp)],
             "rc_cache_free": [vp],
@@ -587,10 +590,15 @@ class Context:
     def set_counting(self, on: bool) -> None:
         check(lib().rc_set_counting(self.h, int(on)))
 
-    def set_tuning(self, block=256, spp_per_pass=0, blocks_per_cu=64) -> None:
+    def set_hit_decision(self, mode: str) -> None:
+        """"fp32" (RC_HIT_FP32, the default) or "fp64" (RC_HIT_FP64)."""
+        check(lib().rc_set_hit_decision(self.h, {"fp32": RC_HIT_FP32, "fp64": RC_HIT_FP64}[mode]))
+
+    def set_tuning(self, block=256, spp_per_pass=0, blocks_per_cu=64, stack_limit=0) -> None:
         """rc_set_tuning; the defaults are the context's own (spp_per_pass 0:
-        each frame chooses its pass size)."""
-        t = Tuning(block, spp_per_pass, blocks_per_cu, 0)
+        each frame chooses its pass size; stack_limit 0: the whole on-chip
+        traversal stack before the deep-stack kernel takes a ray over)."""
+        t = Tuning(block, spp_per_pass, blocks_per_cu, stack_limit)
         check(lib().rc_set_tuning(self.h, C.byref(t)))
 
     def stats(self) -> Stats:
@@ -604,6 +612,12 @@ class Context:
     def copy_bandwidth(self, nbytes: int = 1 << 30, iters: int = 20) -> float:
         g = C.c_double()
         check(lib().rc_copy_bandwidth(self.h, nbytes, iters, C.byref(g)))
+        return g.value
+
+    def read_bandwidth(self, nbytes: int, iters: int) -> float:
+        """rc_read_bandwidth: read GB/s of a working set of nbytes."""
+        g = C.c_double()
+        check(lib().rc_read_bandwidth(self.h, nbytes, iters, C.byref(g)))
         return g.value
 
     def upload(self, sd: SceneData, nodes=None, ordered=None) -> "Scene":
@@ -716,13 +730,16 @@ class RadianceCache:
         check(lib().rc_cache_train_step(self.ctx.h, self.h, _ptr(q), _ptr(t), q.shape[0], C.byref(loss)))
         return loss.value
 
-    def render(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, train: int = 1) -> float:
+    def render(self, scene: "Scene", film: "Film", cam: Camera, cfg: RenderCfg, train: int = 1,
+               want_loss: bool = True) -> float:
         """One frame with paths ending into the cache at cfg.cache_vertex; then
         `train` Adam steps (0 = none) on interleaved slices of the frame's
-        training samples (returns their mean loss)."""
+        training samples, queued behind the frame.  With want_loss the call
+        waits for them and returns their mean loss; without, it returns 0 and
+        the steps overlap whatever the caller does next."""
         loss = C.c_float()
         check(lib().rc_render_cached(self.ctx.h, scene.h, film.h, C.byref(cam), C.byref(cfg), self.h, int(train),
-                                     C.byref(loss)))
+                                     C.byref(loss) if want_loss else None))
         return loss.value
 
     def close(self) -> None:

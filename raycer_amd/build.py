@@ -52,26 +52,33 @@ def _run(cmd: list[str]) -> None:
         print(out)
 
 
-def _objects() -> list[tuple[list[str], Path]]:
+def _objects(build_dir: Path, defines: list[str]) -> list[tuple[list[str], Path]]:
     jobs = []
+    dflags = [f"-D{d}" for d in defines]
     for src in sorted(CSRC.glob("*.hip")):
-        obj = BUILD / (src.stem + ".hip.o")
-        jobs.append(([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)], obj))
+        obj = build_dir / (src.stem + ".hip.o")
+        jobs.append(([HIPCC, *HIP_FLAGS, *dflags, "-c", str(src), "-o", str(obj)], obj))
     for src in sorted((CSRC / "host").glob("*.cpp")):
-        obj = BUILD / (src.stem + ".cpp.o")
-        jobs.append((["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)], obj))
+        obj = build_dir / (src.stem + ".cpp.o")
+        jobs.append((["g++", *CXX_FLAGS, *dflags, "-c", str(src), "-o", str(obj)], obj))
     return jobs
 
 
-def build_library(jobs: int = 8) -> Path:
-    BUILD.mkdir(exist_ok=True)
-    work = _objects()
+def build_library(jobs: int = 8, defines: list[str] | None = None, out_dir: Path | None = None) -> Path:
+    """Builds libraycer.so; with defines (NAME=VALUE, the kernels' compile-time
+    switches such as RC_REG_STACK) into out_dir, for A/B measurements (load
+    it with RAYCER_LIBRARY=<out_dir>/libraycer.so)."""
+    defines = list(defines or [])
+    lib = (out_dir / "libraycer.so") if out_dir else LIB
+    build_dir = (out_dir / "_build") if out_dir else BUILD
+    build_dir.mkdir(parents=True, exist_ok=True)
+    work = _objects(build_dir, defines)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda j: _run(j[0]), work))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *[str(o) for _, o in work]])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_runner() -> Path:
@@ -90,10 +97,18 @@ def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle", action="store_true")
     ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--define", action="append", default=[], help="NAME=VALUE for a variant build (with --out)")
+    ap.add_argument("--out", type=Path, default=None, help="directory of a variant build's libraycer.so")
     a = ap.parse_args(argv)
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
         print("hipcc not found", file=sys.stderr)
         return 1
+    if a.define and not a.out:
+        print("--define needs --out (the in-tree library is the default build)", file=sys.stderr)
+        return 1
+    if a.out:
+        print(f"built {build_library(a.j, a.define, a.out.resolve())}")
+        return 0
     build_library(a.j)
     build_runner()
     if a.oracle:

@@ -56,7 +56,7 @@ int render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film* const* films
     }
     RC_TRY_HIP(hipSetDevice(c->device));
     // size every scratch buffer for the largest frame before capturing
-    for (size_t i = 0; i < n; ++i) RC_TRY(rc_prepare_frame(c, &cfgs[i], sp[i]));
+    for (size_t i = 0; i < n; ++i) RC_TRY(c->fb.ensure(rc_pass_paths(&cfgs[i], sp[i])));
     bool was_profiling = c->profiling;
     c->profiling = false;
     auto t0 = std::chrono::steady_clock::now();
@@ -64,8 +64,11 @@ int render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film* const* films
     RC_TRY_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     k_batch_zero<<<1, 1, 0, c->stream>>>(c->counters);
     int rc = RC_OK;
-    for (size_t i = 0; i < n && rc == RC_OK; ++i)
-        rc = rc_enqueue_frame(c, scenes[i], films[i], &cams[i], &cfgs[i], sp[i], nullptr, nullptr);
+    for (size_t i = 0; i < n && rc == RC_OK; ++i) {
+        FrameShape shape;
+        shape.spp_pass = sp[i];
+        rc = rc_enqueue_frame(c, c->fb, c->stream, scenes[i], films[i], &cams[i], &cfgs[i], shape, nullptr, nullptr);
+    }
     hipError_t ec = hipStreamEndCapture(c->stream, &graph);
     c->profiling = was_profiling;
     if (rc != RC_OK) {

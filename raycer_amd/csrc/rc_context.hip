@@ -155,6 +155,7 @@ extern "C" int rc_create(int device, rc_ctx** out) {
         return rc_host_fail(RC_ERR_ALLOC, std::string("hipMalloc(counters): ") + hipGetErrorString(me));
     }
     (void)hipMemset(c->counters, 0, sizeof(DevCounters));
+    c->fb.counters = c->counters;  // not owned by the frame buffers
     *out = c;
     return RC_OK;
 }
@@ -165,10 +166,16 @@ extern "C" int rc_destroy(rc_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->batch_graph) (void)hipGraphExecDestroy(c->batch_graph);
     rc_whitted_release(c);
-    DevBuf* bufs[] = {&c->trace_rays, &c->trace_hits, &c->trace_occ, &c->ovf_list, &c->deep_stack, &c->ray_a,
-                      &c->ray_b, &c->st_a, &c->st_b, &c->hits, &c->srays, &c->scontrib, &c->socc, &c->slotL,
-                      &c->slotW};
+    DevBuf* bufs[] = {&c->trace_rays, &c->trace_hits, &c->trace_occ};
     for (DevBuf* b : bufs) b->release();
+    c->fb.release();
+    for (FrameBuffers* b : c->branches) {
+        b->release();
+        delete b;
+    }
+    c->branches.clear();
+    for (hipStream_t b : c->branch_streams) (void)hipStreamDestroy(b);
+    c->branch_streams.clear();
     for (auto& p : c->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -203,6 +210,14 @@ extern "C" int rc_set_counting(rc_ctx* c, int enable) {
     return RC_OK;
 }
 
+extern "C" int rc_set_hit_decision(rc_ctx* c, int mode) {
+    RC_TRY(rc_check_ctx(c, "rc_set_hit_decision"));
+    if (mode != RC_HIT_FP32 && mode != RC_HIT_FP64)
+        return rc_host_fail(RC_ERR_INVALID, "rc_set_hit_decision: mode must be RC_HIT_FP32 or RC_HIT_FP64");
+    c->hit_fp64 = mode == RC_HIT_FP64;
+    return RC_OK;
+}
+
 extern "C" int rc_set_tuning(rc_ctx* c, const rc_tuning* t) {
     RC_TRY(rc_check_ctx(c, "rc_set_tuning"));
     if (!t) return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: null tuning");
@@ -212,6 +227,9 @@ extern "C" int rc_set_tuning(rc_ctx* c, const rc_tuning* t) {
         return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: spp_per_pass must be 0 (automatic) or in [1, 64]");
     if (t->blocks_per_cu == 0 || t->blocks_per_cu > 64)
         return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: blocks_per_cu must be in [1, 64]");
+    if (t->stack_limit > uint32_t(rcd::kOnChipStack))
+        return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: stack_limit must be 0 (all) or in [1, " +
+                                                std::to_string(rcd::kOnChipStack) + "]");
     c->tuning = *t;
     return RC_OK;
 }
@@ -220,6 +238,12 @@ extern "C" int rc_get_stats(rc_ctx* c, rc_stats* out) {
     RC_TRY(rc_check_ctx(c, "rc_get_stats"));
     if (!out) return rc_host_fail(RC_ERR_INVALID, "rc_get_stats: null out");
     RC_TRY(c->prof_collect());
+    if (c->train_ev1) {  // the last cached frame's optimizer steps, timed on the cache's stream
+        RC_TRY_HIP(hipEventSynchronize(c->train_ev1));
+        float ms = 0.0f;
+        RC_TRY_HIP(hipEventElapsedTime(&ms, c->train_ev0, c->train_ev1));
+        c->stats.train_ms = ms;
+    }
     *out = c->stats;
     return RC_OK;
 }
@@ -718,10 +742,61 @@ extern "C" int rc_memcpy_d2h(rc_ctx* c, void* dst, const void* src, size_t bytes
     return RC_OK;
 }
 
-// ---------------------------------------------------------------- bandwidth probe
+// ---------------------------------------------------------------- bandwidth probes
 namespace {
+// Streaming copy, four 16-byte loads in flight per thread before their stores.
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
-    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+    const size_t stride = size_t(gridDim.x) * 256;
+    size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// Reads n float4 `iters` times (each thread the same addresses every time, so
+// a working set that fits a cache level stays in it) and folds them into one
+// value per thread, which is stored only if it is impossible (keeps the loads).
+__global__ __launch_bounds__(256) void k_read(const float4* __restrict__ src, size_t n, int iters,
+                                              float* __restrict__ sink) {
+    const size_t stride = size_t(gridDim.x) * 256;
+    float acc = 0.0f;
+    for (int it = 0; it < iters; ++it) {
+        size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+        for (; i + 3 * stride < n; i += 4 * stride) {
+            const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+            acc += (a.x + b.y) + (c.z + d.w);
+        }
+        for (; i < n; i += stride) acc += src[i].x;
+    }
+    if (acc == -1.0f) sink[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+__global__ void k_fill(float4* p, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256)
+        p[i] = make_float4(1.0f, 2.0f, 3.0f, 4.0f);
+}
+
+// Times `iters` launches of f(grid) with HIP events on the context stream (ms).
+template <class F>
+int time_launches(rc_ctx* c, int iters, F f, float* ms) {
+    hipEvent_t e0, e1;
+    RC_TRY_HIP(hipEventCreate(&e0));
+    RC_TRY_HIP(hipEventCreate(&e1));
+    f();  // warm-up
+    RC_TRY_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i) f();
+    RC_TRY_HIP(hipEventRecord(e1, c->stream));
+    RC_TRY_HIP(hipEventSynchronize(e1));
+    RC_TRY_HIP(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    RC_TRY_HIP(hipGetLastError());
+    return RC_OK;
 }
 }  // namespace
 
@@ -734,21 +809,48 @@ extern "C" int rc_copy_bandwidth(rc_ctx* c, size_t bytes, int iters, double* gbp
     RC_TRY(a.ensure(n * sizeof(float4)));
     int rc = b.ensure(n * sizeof(float4));
     if (rc != RC_OK) { a.release(); return rc; }
-    hipEvent_t e0, e1;
-    RC_TRY_HIP(hipEventCreate(&e0));
-    RC_TRY_HIP(hipEventCreate(&e1));
-    unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, size_t(c->cu_count) * 16));
-    k_copy<<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n);  // warm-up
-    RC_TRY_HIP(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < iters; ++i) k_copy<<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n);
-    RC_TRY_HIP(hipEventRecord(e1, c->stream));
-    RC_TRY_HIP(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    RC_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    double best = 0.0;
+    // the best of four grid sizes: the achievable rate, not one launch shape's
+    for (unsigned per_cu : {4u, 8u, 16u, 32u}) {
+        if (rc != RC_OK) break;
+        const unsigned grid = unsigned(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, size_t(c->cu_count) * per_cu)));
+        float ms = 0.0f;
+        rc = time_launches(c, iters, [&]() { k_copy<<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n); },
+                           &ms);
+        if (rc == RC_OK) best = std::max(best, 2.0 * double(n * sizeof(float4)) * iters / (double(ms) * 1e-3) / 1e9);
+    }
     a.release();
     b.release();
-    *gbps = 2.0 * double(n * sizeof(float4)) * iters / (double(ms) * 1e-3) / 1e9;
+    if (rc != RC_OK) return rc;
+    *gbps = best;
+    return RC_OK;
+}
+
+extern "C" int rc_read_bandwidth(rc_ctx* c, size_t bytes, int iters, double* gbps) {
+    RC_TRY(rc_check_ctx(c, "rc_read_bandwidth"));
+    if (!gbps || bytes < 4096 || iters <= 0 || iters > 4096)
+        return rc_host_fail(RC_ERR_INVALID, "rc_read_bandwidth: bad argument");
+    RC_TRY_HIP(hipSetDevice(c->device));
+    const size_t n = bytes / sizeof(float4);
+    DevBuf a, sink;
+    RC_TRY(a.ensure(n * sizeof(float4)));
+    const unsigned grid_max = unsigned(c->cu_count) * 32u;
+    int rc = sink.ensure(size_t(grid_max) * 256 * sizeof(float));
+    if (rc != RC_OK) { a.release(); return rc; }
+    k_fill<<<grid_max, 256, 0, c->stream>>>(a.as<float4>(), n);
+    double best = 0.0;
+    for (unsigned per_cu : {8u, 16u, 32u}) {
+        if (rc != RC_OK) break;
+        const unsigned grid = unsigned(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, size_t(c->cu_count) * per_cu)));
+        float ms = 0.0f;
+        // one launch reads the set `iters` times, so launch overhead does not count
+        rc = time_launches(c, 3, [&]() { k_read<<<grid, 256, 0, c->stream>>>(a.as<float4>(), n, iters, sink.as<float>()); },
+                           &ms);
+        if (rc == RC_OK) best = std::max(best, double(n * sizeof(float4)) * iters * 3 / (double(ms) * 1e-3) / 1e9);
+    }
+    a.release();
+    sink.release();
+    if (rc != RC_OK) return rc;
+    *gbps = best;
     return RC_OK;
 }

@@ -73,7 +73,24 @@ struct alignas(16) DevMaterial {
 };
 
 constexpr uint32_t kNoHit = 0x7fffffffu;
-constexpr int kDeepStack = 256;  // stack entries of the overflow kernel
+constexpr int kDeepStack = 256;  // stack entries of the overflow kernel (memory part)
+
+// Traversal stack of one lane: the top kRegStack entries in registers, as a
+// shift register (no dynamic register indexing), deeper entries in LDS laid
+// out [depth][lane] so a wave's accesses hit 64 distinct banks.  A pop takes
+// its entry from a register, so the next node fetch never waits on an LDS
+// read; the LDS refill of the bottom register runs off the critical path.
+// A ray whose stack would outgrow kOnChipStack entries is handed to the
+// deep-stack kernel.
+#ifndef RC_REG_STACK
+#define RC_REG_STACK 4
+#endif
+#ifndef RC_LDS_DEPTH
+#define RC_LDS_DEPTH 16
+#endif
+constexpr int kRegStack = RC_REG_STACK;
+constexpr int kLdsStack = RC_LDS_DEPTH;
+constexpr int kOnChipStack = kRegStack + kLdsStack;
 constexpr int kWidth = 4;        // children per wide node
 constexpr uint32_t kEmptyChild = 0xffffffffu;
 
@@ -319,8 +336,67 @@ __device__ __forceinline__ bool mt_hit64(const TriRec& T, float3 of, float3 df, 
     return true;
 }
 
+// The FP32 decision (the default, RC_HIT_FP32): Triangle::intersect's
+// Möller–Trumbore (Triangle.cpp:61-91) and its OpenCL form
+// (data/opencl/intersections.cl:27-68) in FP32, double-sided, inclusive
+// barycentric bounds, with the scale-free det == 0 instead of |det| <
+// epsilon.  The bounds are tested without a division: u = U/det and v =
+// V/det lie in [0, 1] with u + v <= 1 exactly when sign(det) U, sign(det) V
+// >= 0 and their sum <= |det|; t = T/det is one IEEE division for the
+// triangles that pass.
+__device__ __forceinline__ bool mt_dec32(const TriRec& T, float3 o, float3 d, float& t, float& u, float& v) {
+    const float3 v0 = f3(T.v0);
+    const float3 e1 = f3(T.v1) - v0, e2 = f3(T.v2) - v0;
+    const float3 pv = cross3(d, e2);
+    const float det = dot3(e1, pv);
+    if (det == 0.0f) return false;
+    const float sg = det < 0.0f ? -1.0f : 1.0f, adet = fabsf(det);
+    const float3 tv = o - v0;
+    const float U = sg * dot3(tv, pv);
+    if (U < 0.0f || U > adet) return false;
+    const float3 qv = cross3(tv, e1);
+    const float V = sg * dot3(d, qv);
+    if (V < 0.0f || U + V > adet) return false;
+    t = dot3(e2, qv) / det;
+    u = U / adet;
+    v = V / adet;
+    return true;
+}
+
 // Relative window inside which an FP32 distance defers to the FP64 one.
 constexpr float kRefineWindow = 1e-3f;
+// FP32 decision: nodes are culled a hair beyond the best distance so a
+// triangle at exactly that distance (an exact tie) is still reached.
+constexpr float kTieWindow = 4.0f / 16777216.0f;
+
+// Stack push of x (false: the stack already holds `limit` entries).  sp
+// counts all entries; the first kRegStack live in reg[0..] (reg[0] the top),
+// entry k >= kRegStack at mem[(k - kRegStack) * stride].
+__device__ __forceinline__ bool stack_push(uint32_t* reg, int& sp, uint32_t x, uint32_t* mem, int stride, int limit) {
+    if (sp >= limit) return false;
+    if (kRegStack == 0) {
+        mem[sp * stride] = x;
+    } else {
+        if (sp >= kRegStack) mem[(sp - kRegStack) * stride] = reg[kRegStack - 1];
+#pragma unroll
+        for (int k = kRegStack - 1; k > 0; --k) reg[k] = reg[k - 1];
+        reg[0] = x;
+    }
+    ++sp;
+    return true;
+}
+
+// Stack pop (sp > 0): the top register, the registers shifted up and the
+// bottom one refilled from memory while entries remain there.
+__device__ __forceinline__ uint32_t stack_pop(uint32_t* reg, int& sp, const uint32_t* mem, int stride) {
+    --sp;
+    if (kRegStack == 0) return mem[sp * stride];
+    const uint32_t x = reg[0];
+#pragma unroll
+    for (int k = 0; k < kRegStack - 1; ++k) reg[k] = reg[k + 1];
+    if (sp >= kRegStack) reg[kRegStack - 1] = mem[(sp - kRegStack) * stride];
+    return x;
+}
 
 struct TraceResult {
     float t, u, v;
@@ -347,14 +423,25 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
 }
 
-// Tests the triangles tris[first, first + count).  Candidates pass the FP32
-// filter within kRefineWindow of the current best and are then decided in
-// FP64.  Closest hit keeps the smallest t; an exact tie goes to the lower
-// original triangle id, so the answer does not depend on traversal order.
-// Returns true when an any-hit query is answered.
-template <bool ANY, bool COUNT>
+// Tests the triangles tris[first, first + count).  DEC64: candidates pass
+// the FP32 filter within kRefineWindow of the current best and are then
+// decided in FP64 on the exact FP32 inputs (RC_HIT_FP64); otherwise the FP32
+// test decides (mt_dec32).  Closest hit keeps the smallest t; an exact tie
+// goes to the lower original triangle id, so the answer does not depend on
+// traversal order.  Returns true when an any-hit query is answered (tmax_any:
+// its tmax).
+template <bool DEC64>
+struct BestT {
+    using T = float;
+};
+template <>
+struct BestT<true> {
+    using T = double;
+};
+
+template <bool ANY, bool COUNT, bool DEC64>
 __device__ __forceinline__ bool test_leaf(const TriRec* __restrict__ tris, uint32_t first, uint32_t count, float3 o,
-                                          float3 d, float tmin, double tmin64, double tmax64, double& best,
+                                          float3 d, float tmin, float tmax_any, typename BestT<DEC64>::T& best,
                                           float& cull, TraceResult& R, uint32_t& n_tri) {
     for (uint32_t j = first; j < first + count; ++j) {
         const float4* q = reinterpret_cast<const float4*>(tris + j);
@@ -363,29 +450,56 @@ __device__ __forceinline__ bool test_leaf(const TriRec* __restrict__ tris, uint3
         T.v1 = q[1];
         T.v2 = q[2];
         if (COUNT) n_tri += 1;
-        float t32;
-        if (!mt_hit(T, o, d, t32)) continue;
-        if (t32 > cull || t32 < tmin * (1.0f - kRefineWindow) - 1e-30f) continue;
-        double t64, u64, v64;
-        if (!mt_hit64(T, o, d, t64, u64, v64)) continue;
-        if (t64 < tmin64) continue;
-        uint32_t id = __float_as_uint(T.v0.w);
-        if (ANY) {
-            if (t64 <= tmax64 && !(__float_as_uint(T.v1.w) & RC_MAT_NON_SHADOWING)) {
+        if (DEC64) {
+            float t32;
+            if (!mt_hit(T, o, d, t32)) continue;
+            if (t32 > cull || t32 < tmin * (1.0f - kRefineWindow) - 1e-30f) continue;
+            double t64, u64, v64;
+            if (!mt_hit64(T, o, d, t64, u64, v64)) continue;
+            if (t64 < double(tmin)) continue;
+            uint32_t id = __float_as_uint(T.v0.w);
+            if (ANY) {
+                if (t64 <= double(tmax_any) && !(__float_as_uint(T.v1.w) & RC_MAT_NON_SHADOWING)) {
+                    R.t = float(t64);
+                    R.id = id;
+                    return true;
+                }
+            } else if (t64 < best || (t64 == best && id < R.id)) {
+                best = t64;
                 R.t = float(t64);
+                R.u = float(u64);
+                R.v = float(v64);
                 R.id = id;
-                return true;
+                cull = R.t * (1.0f + kRefineWindow);
             }
-        } else if (t64 < best || (t64 == best && id < R.id)) {
-            best = t64;
-            R.t = float(t64);
-            R.u = float(u64);
-            R.v = float(v64);
-            R.id = id;
-            cull = R.t * (1.0f + kRefineWindow);
+        } else {
+            float t, u, v;
+            if (!mt_dec32(T, o, d, t, u, v)) continue;
+            if (t < tmin || t > cull) continue;
+            uint32_t id = __float_as_uint(T.v0.w);
+            if (ANY) {
+                if (t <= tmax_any && !(__float_as_uint(T.v1.w) & RC_MAT_NON_SHADOWING)) {
+                    R.t = t;
+                    R.id = id;
+                    return true;
+                }
+            } else if (t < best || (t == best && id < R.id)) {
+                best = t;
+                R.t = t;
+                R.u = u;
+                R.v = v;
+                R.id = id;
+                cull = t * (1.0f + kTieWindow);
+            }
         }
     }
     return false;
+}
+
+// Cull distance of a query with no hit yet.
+template <bool DEC64>
+__device__ __forceinline__ float initial_cull(float tmax) {
+    return tmax * (1.0f + (DEC64 ? kRefineWindow : kTieWindow));
 }
 
 // Compare-exchange of two children by entry distance, as selects (no branch).
@@ -415,23 +529,23 @@ constexpr size_t kMaxTriangles = size_t(1) << 26;
 // Ordered, t-culled traversal over 4-wide nodes.  The four child boxes are
 // tested and sorted by entry distance (a 5-comparator network); leaf children
 // are tested in that order at once, the nearest interior child is visited
-// next and the others are pushed far-to-near on a per-thread stack whose
-// entries live at stack[k * stride] (LDS for the main kernels, global memory
-// for the deep-stack kernel).  ANY: stop at the first triangle (not
-// nonShadowing) with t in [tmin, tmax].
-template <bool ANY, bool COUNT>
+// next and the others are pushed far-to-near on the lane's stack (registers,
+// then stack[k * stride]: LDS for the main kernels, global memory for the
+// deep-stack kernel), at most max_depth entries.  ANY: stop at the first
+// triangle (not nonShadowing) with t in [tmin, tmax].
+template <bool ANY, bool COUNT, bool DEC64>
 __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                 float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                 int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
     const float INF = __builtin_inff();
     TraceResult R{tmax, 0.0f, 0.0f, kNoHit, false};
     tmin = fmaxf(tmin, 0.0f);
-    const double tmin64 = tmin, tmax64 = tmax;
-    double best = tmax64;
-    float cull = tmax * (1.0f + kRefineWindow);
+    typename BestT<DEC64>::T best = tmax;
+    float cull = initial_cull<DEC64>(tmax);
     float3 inv = safe_inv(d);
     uint32_t cur = 0;
     int sp = 0;
+    uint32_t reg[kRegStack > 0 ? kRegStack : 1];
     while (true) {
         const float4* p = reinterpret_cast<const float4*>(nodes + cur);
         float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6], q7 = p[7];
@@ -451,28 +565,24 @@ __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict
         order2(t1, l1, c1, t2, l2, c2);
         // leaf children, nearest first
         if (c0 != 0 && t0 != INF && t0 <= cull)
-            if (test_leaf<ANY, COUNT>(tris, l0, c0, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+            if (test_leaf<ANY, COUNT, DEC64>(tris, l0, c0, o, d, tmin, tmax, best, cull, R, n_tri)) return R;
         if (c1 != 0 && t1 != INF && t1 <= cull)
-            if (test_leaf<ANY, COUNT>(tris, l1, c1, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+            if (test_leaf<ANY, COUNT, DEC64>(tris, l1, c1, o, d, tmin, tmax, best, cull, R, n_tri)) return R;
         if (c2 != 0 && t2 != INF && t2 <= cull)
-            if (test_leaf<ANY, COUNT>(tris, l2, c2, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+            if (test_leaf<ANY, COUNT, DEC64>(tris, l2, c2, o, d, tmin, tmax, best, cull, R, n_tri)) return R;
         if (c3 != 0 && t3 != INF && t3 <= cull)
-            if (test_leaf<ANY, COUNT>(tris, l3, c3, o, d, tmin, tmin64, tmax64, best, cull, R, n_tri)) return R;
+            if (test_leaf<ANY, COUNT, DEC64>(tris, l3, c3, o, d, tmin, tmax, best, cull, R, n_tri)) return R;
         // interior children: farthest pushed first, nearest visited next
         bool have = false;
         uint32_t next = 0;
-#define RC_INTERIOR(t, l, c)                                   \
-        if (c == 0 && t != INF && t <= cull) {                 \
-            if (have) {                                        \
-                if (sp >= max_depth) {                         \
-                    R.overflow = true;                         \
-                    return R;                                  \
-                }                                              \
-                stack[sp * stride] = next;                     \
-                ++sp;                                          \
-            }                                                  \
-            next = l;                                          \
-            have = true;                                       \
+#define RC_INTERIOR(t, l, c)                                            \
+        if (c == 0 && t != INF && t <= cull) {                          \
+            if (have && !stack_push(reg, sp, next, stack, stride, max_depth)) { \
+                R.overflow = true;                                      \
+                return R;                                               \
+            }                                                           \
+            next = l;                                                   \
+            have = true;                                                \
         }
         RC_INTERIOR(t3, l3, c3)
         RC_INTERIOR(t2, l2, c2)
@@ -483,8 +593,7 @@ __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict
             cur = next;
         } else {
             if (sp == 0) break;
-            --sp;
-            cur = stack[sp * stride];
+            cur = stack_pop(reg, sp, stack, stride);
         }
     }
     return R;
@@ -495,33 +604,36 @@ __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict
 // then does at most one node visit and one leaf, so the lanes of a wave stay
 // in step instead of waiting while one lane tests up to four leaves.  The
 // state is explicit (trav_begin / trav_step).
+template <bool DEC64>
 struct Trav {
     float3 o, d, inv;
     float tmin, cull;
-    double best;
+    typename BestT<DEC64>::T best;
     TraceResult R;
     uint32_t cur;
     int sp;
+    uint32_t reg[kRegStack > 0 ? kRegStack : 1];  // top of the stack
 };
 
-__device__ __forceinline__ void trav_begin(Trav& S, float3 o, float3 d, float tmin, float tmax) {
+template <bool DEC64>
+__device__ __forceinline__ void trav_begin(Trav<DEC64>& S, float3 o, float3 d, float tmin, float tmax) {
     S.o = o;
     S.d = d;
     S.inv = safe_inv(d);
     S.tmin = fmaxf(tmin, 0.0f);
     S.best = tmax;
-    S.cull = tmax * (1.0f + kRefineWindow);
+    S.cull = initial_cull<DEC64>(tmax);
     S.R = TraceResult{tmax, 0.0f, 0.0f, kNoHit, false};
     S.cur = 0;  // root: interior record 0
     S.sp = 0;
 }
 
 // One step; true when the traversal has ended (S.R holds the result,
-// S.R.overflow when the stack would have outgrown max_depth).
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trav_step(Trav& S, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
-                                          uint32_t* stack, int stride, int max_depth, uint32_t& n_rec,
-                                          uint32_t& n_tri) {
+// S.R.overflow when the stack would have outgrown max_depth entries).
+template <bool ANY, bool COUNT, bool DEC64>
+__device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restrict__ nodes,
+                                          const TriRec* __restrict__ tris, uint32_t* stack, int stride, int max_depth,
+                                          uint32_t& n_rec, uint32_t& n_tri) {
     const float INF = __builtin_inff();
     if (!(S.cur & kLeafTag)) {
 #if RC_QNODE
@@ -590,13 +702,9 @@ __device__ __forceinline__ bool trav_step(Trav& S, const NodeT* __restrict__ nod
 #define RC_CHILD(t, l, c)                                                                                      \
         if (t != INF && t <= S.cull) {                                                                         \
             const uint32_t e = c != 0 ? (kLeafTag | (c << 26) | l) : l;                                       \
-            if (have) {                                                                                        \
-                if (S.sp >= max_depth) {                                                                       \
-                    S.R.overflow = true;                                                                       \
-                    return true;                                                                               \
-                }                                                                                              \
-                stack[S.sp * stride] = next;                                                                   \
-                ++S.sp;                                                                                        \
+            if (have && !stack_push(S.reg, S.sp, next, stack, stride, max_depth)) {                            \
+                S.R.overflow = true;                                                                           \
+                return true;                                                                                   \
             }                                                                                                  \
             next = e;                                                                                          \
             have = true;                                                                                       \
@@ -610,45 +718,45 @@ __device__ __forceinline__ bool trav_step(Trav& S, const NodeT* __restrict__ nod
             S.cur = next;
         } else {
             if (S.sp == 0) return true;
-            --S.sp;
-            S.cur = stack[S.sp * stride];
+            S.cur = stack_pop(S.reg, S.sp, stack, stride);
         }
     }
     if (S.cur & kLeafTag) {
         const uint32_t first = S.cur & ((1u << 26) - 1u), count = (S.cur >> 26) & 31u;
-        // the FP64 bounds are the FP32 ones widened; an any-hit query's R.t
-        // holds tmax until the query is answered
-        if (test_leaf<ANY, COUNT>(tris, first, count, S.o, S.d, S.tmin, double(S.tmin), double(S.R.t), S.best, S.cull, S.R,
-                                  n_tri))
+        // an any-hit query's R.t holds tmax until the query is answered
+        if (test_leaf<ANY, COUNT, DEC64>(tris, first, count, S.o, S.d, S.tmin, S.R.t, S.best, S.cull, S.R, n_tri))
             return true;
         if (S.sp == 0) return true;
-        --S.sp;
-        S.cur = stack[S.sp * stride];
+        S.cur = stack_pop(S.reg, S.sp, stack, stride);
     }
     return false;
 }
 
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, bool DEC64>
 __device__ __forceinline__ TraceResult traverse_ifif(const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                      float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                      int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
-    Trav S;
+    Trav<DEC64> S;
     trav_begin(S, o, d, tmin, tmax);
-    while (!trav_step<ANY, COUNT>(S, nodes, tris, stack, stride, max_depth, n_rec, n_tri)) {
+    while (!trav_step<ANY, COUNT, DEC64>(S, nodes, tris, stack, stride, max_depth, n_rec, n_tri)) {
     }
     return S.R;
 }
 
-template <bool ANY, bool COUNT>
+// Closest hit (ANY = false) or any hit of one ray.  stack: the lane's memory
+// part of the stack, entry k at stack[k * stride]; max_depth: entries the
+// stack may hold in all (registers and memory).
+template <bool ANY, bool COUNT, bool DEC64>
 __device__ __forceinline__ TraceResult traverse(const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                 float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                 int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
 #if RC_QNODE
-    return traverse_ifif<ANY, COUNT>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
+    return traverse_ifif<ANY, COUNT, DEC64>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
 #else
     if (RC_TRAV_IFIF)
-        return traverse_ifif<ANY, COUNT>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
-    return traverse_inline<ANY, COUNT>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
+        return traverse_ifif<ANY, COUNT, DEC64>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec,
+                                                n_tri);
+    return traverse_inline<ANY, COUNT, DEC64>(nodes, tris, o, d, tmin, tmax, stack, stride, max_depth, n_rec, n_tri);
 #endif
 }
 

@@ -40,7 +40,8 @@ struct DevCounters {
     unsigned long long tri_tests;
     unsigned long long ext_rays;
     unsigned long long shadow_rays;
-    uint32_t ovf_count;
+    unsigned long long ovf_total;  // rays the deep-stack kernel walked, over every launch since the last reset
+    uint32_t ovf_count;            // rays listed for the next deep-stack launch
     uint32_t queue_count[4];   // wavefront queue sizes (ping, pong, shadow, scratch)
     uint32_t error;
 };
@@ -61,6 +62,25 @@ inline uint32_t pass_samples(const rc_tuning& t, uint32_t npix, uint32_t spp, si
     return uint32_t(std::max<size_t>(1, std::min<size_t>(sp, spp)));
 }
 
+// Device buffers of one frame's wavefront passes (path and shadow queues,
+// slot radiance and weights, the cold path's overflow list and deep stack)
+// and its counters.  The context owns one set (fb, whose counters are the
+// context's); a batch owns one more per concurrent branch.
+struct FrameBuffers {
+    DevBuf ray_a, ray_b, st_a, st_b, hits, srays, scontrib, socc, slotL, slotW, ovf_list, deep_stack;
+    DevCounters* counters = nullptr;
+    bool own_counters = false;
+    int ensure(size_t npaths);  // every buffer sized for a pass of npaths paths
+    void release();
+};
+
+// Samples per pass and threads per block of one frame's passes.
+struct FrameShape {
+    uint32_t spp_pass = 1;
+    uint32_t ext_block = 256;     // extension kernel: 128 or 256
+    uint32_t shadow_block = 128;  // shadow kernel: 64, 128 or 256
+};
+
 struct rc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -68,12 +88,17 @@ struct rc_ctx {
     std::string name;
     bool profiling = false;
     bool counting = false;
-    rc_tuning tuning{256, 0, 64, 0};  // samples per pass chosen per frame
+    bool hit_fp64 = false;  // RC_HIT_FP64: FP32 filter, FP64 decision of triangle hits
+    rc_tuning tuning{256, 0, 64, 0};  // samples per pass chosen per frame, the whole on-chip stack
+
+    // traversal stack entries held on chip before a ray goes to the deep-stack kernel
+    int stack_depth() const {
+        return tuning.stack_limit ? int(tuning.stack_limit) : rcd::kOnChipStack;
+    }
     rc_stats stats{};
     DevCounters* counters = nullptr;  // device
-    DevBuf trace_rays, trace_hits, trace_occ, ovf_list, deep_stack;
-    // wavefront state
-    DevBuf ray_a, ray_b, st_a, st_b, hits, srays, scontrib, socc, slotL, slotW;
+    DevBuf trace_rays, trace_hits, trace_occ;
+    FrameBuffers fb;  // wavefront state of rc_render and the ray queries' cold path (fb.counters == counters)
     // profiling events
     struct Pending {
         hipEvent_t a, b;
@@ -81,7 +106,10 @@ struct rc_ctx {
     };
     std::vector<hipEvent_t> ev_free;
     std::vector<Pending> pending;
+    hipEvent_t train_ev0 = nullptr, train_ev1 = nullptr;  // the last cached frame's optimizer steps (timing)
     hipGraphExec_t batch_graph = nullptr;
+    std::vector<FrameBuffers*> branches;      // a batch's concurrent branches
+    std::vector<hipStream_t> branch_streams;
     struct WhittedBufs* whitted = nullptr;  // the Whitted tracer's ray-tree levels
 
     int prof_begin(hipEvent_t* a);
@@ -113,11 +141,38 @@ struct rc_film {
 };
 
 // Launches the closest-hit (ANY = false) or any-hit traversal over a device
-// ray array.  n_dev (may be null) is a device count read by the kernel; n_bound
-// bounds it and sizes the grid.
+// ray array, then the deep-stack kernel over the rays whose stack outgrew the
+// on-chip depth.  n_dev (may be null) is a device count read by the kernel;
+// n_bound bounds it and sizes the grid.
 int rc_launch_trace(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, const uint32_t* n_dev,
                     uint32_t n_bound, rc_hit* hits, uint8_t* occ);
 int rc_launch_deep(rc_ctx* ctx, const rc_scene* s, bool any, const rc_ray* rays, rc_hit* hits, uint8_t* occ);
+
+// One traversal launch with its scratch given explicitly: the overflow list
+// (n_bound entries), the deep-stack kernel's stack (rc_deep_stack_bytes())
+// and the counters.  rc_launch_trace fills it from the context's buffers.
+struct TraceLaunch {
+    const rc_scene* scene = nullptr;
+    bool any = false;
+    const rc_ray* rays = nullptr;
+    const uint32_t* n_dev = nullptr;
+    uint32_t n_bound = 0;
+    rc_hit* hits = nullptr;
+    uint8_t* occ = nullptr;
+    uint32_t* ovf_list = nullptr;
+    uint32_t* deep_stack = nullptr;
+    DevCounters* ctr = nullptr;
+    hipStream_t stream = nullptr;
+    int max_depth = rcd::kOnChipStack;  // on-chip stack entries before the deep-stack kernel
+    uint32_t max_blocks = 1;            // grid limit; the kernel's grid-stride loop covers the rest
+    bool counting = false;
+    bool dec64 = false;
+};
+size_t rc_deep_stack_bytes();
+// Clears the overflow count and launches the traversal kernel.
+int rc_launch_trace_with(const TraceLaunch& L);
+// The deep-stack kernel over the rays the last traversal launch listed.
+int rc_launch_deep_with(const TraceLaunch& L);
 // Camera rays of one pass of Tracer::generateMultiSamples (slot = j * npix + q
 // for sample sample0 + j of the pixel at position q of the tiled order): ray,
 // state (1, 1, 1, slot), L = 0 and the filter weight W.
@@ -138,29 +193,39 @@ struct CacheIO {
     float* batch_t;         // 3 floats per training sample
     uint32_t batch_cap;
     uint32_t cache_vertex;  // 0 = off
-    uint32_t train_stride;  // one training path per train_stride slots
+    uint32_t train_stride;  // one training path per train_stride slots; 0: the adaptive stride *stride_dev
+    const uint32_t* stride_dev;
     uint32_t cap;           // record / query capacity of a pass
 };
 
 int rc_validate_cfg(const rc_render_cfg* cfg, const char* fn);
-// Sizes every buffer a frame of this config needs (outside any capture).
-int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass);
-// Enqueues the passes of one frame on the context stream; cache_io / cache are
-// null for the plain renderer.
-int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                     uint32_t spp_pass, rc_progress_fn progress, void* user, const CacheIO* cache_io = nullptr,
-                     rc_cache* cache = nullptr);
+// Paths of one pass of a frame (pixels x samples per pass, clamped to the spp).
+size_t rc_pass_paths(const rc_render_cfg* cfg, uint32_t spp_pass);
+// Enqueues the passes of one frame on stream st with the buffers B (sized
+// beforehand for the frame's passes); cache_io / cache are null for the plain
+// renderer.  Profiling events are recorded only when st is the context stream.
+int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene* s, rc_film* f, const rc_camera* cam,
+                     const rc_render_cfg* cfg, const FrameShape& shape, rc_progress_fn progress, void* user,
+                     const CacheIO* cache_io = nullptr, rc_cache* cache = nullptr);
+// Zeroes a frame's statistics counters (a kernel, so it may be captured).
+void rc_zero_counters(DevCounters* ctr, hipStream_t st);
 // Radiance cache, device side (rc_mlp.hip).
-// L[slot] += thr * cache(query) for the *n_dev (<= n_bound) queries.
-int rc_cache_apply_device(rc_ctx* c, rc_cache* k, const float4* queries, const uint32_t* n_dev, uint32_t n_bound,
-                          float4* L);
+// L[slot] += thr * cache(query) for the *n_dev (<= n_bound) queries, on
+// stream st once the optimizer steps in flight are done.
+int rc_cache_apply_device(rc_ctx* c, hipStream_t st, rc_cache* k, const float4* queries, const uint32_t* n_dev,
+                          uint32_t n_bound, float4* L);
 // Sizes the cache's frame buffers for passes of pass_paths slots and a frame
-// of frame_samples samples, fills io and enqueues the clearing of its counters.
-int rc_cache_frame_io(rc_ctx* c, rc_cache* k, uint32_t cache_vertex, size_t pass_paths, uint64_t frame_samples,
-                      CacheIO* io);
-// After a frame: the frame's query and training-sample counts and `steps`
-// optimizer steps over interleaved slices of the training samples it gathered.
-int rc_cache_train_frame(rc_ctx* c, rc_cache* k, uint32_t steps, float* loss, uint32_t* samples, uint32_t* queries);
+// of frame_samples samples, fills io and enqueues the clearing of its
+// per-frame counters on st.
+int rc_cache_frame_io(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t cache_vertex, size_t pass_paths,
+                      uint64_t frame_samples, CacheIO* io);
+// After a frame's passes on st: freezes the frame's training samples, steers
+// the adaptive stride and queues `steps` optimizer steps on the cache's own
+// stream behind the frame; synchronises st and returns the frame's cache
+// queries (counts[0]) and training samples (counts[1]).
+int rc_cache_frame_end(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t steps, uint32_t counts[2]);
+// Mean loss of the last frame's optimizer steps (waits for them).
+int rc_cache_last_loss(rc_ctx* c, rc_cache* k, float* loss);
 rc_ctx* rc_cache_owner(const rc_cache* k);
 int rc_check_ctx(const rc_ctx* ctx, const char* fn);
 int rc_check_scene(const rc_ctx* ctx, const rc_scene* s, const char* fn);

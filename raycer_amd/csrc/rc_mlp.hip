@@ -231,12 +231,23 @@ __device__ __forceinline__ size_t quad_index(uint32_t s, int f, int F) {
 
 // Forward + backward of 16 samples per wave.  Relative L2 loss
 // (y - t)^2 / (sg(y)^2 + eps) averaged over samples and channels.
+// Samples of slice j of `steps` interleaved slices of a batch whose size is
+// min(*n_dev, n_host) (n_host alone without a device count): samples j,
+// j + steps, j + 2 steps, ...
+__device__ __forceinline__ uint32_t slice_count(uint32_t n_host, const uint32_t* n_dev, uint32_t steps, uint32_t j) {
+    const uint32_t total = n_dev ? min(*n_dev, n_host) : n_host;
+    return total > j ? (total - j + steps - 1) / steps : 0u;
+}
+
 __global__ __launch_bounds__(kThreads) void k_cache_fwd_bwd(const v4s* __restrict__ wt_g, const v4s* __restrict__ wtb_g,
                                                             const float* __restrict__ params, Bounds B,
                                                             const float* __restrict__ queries,
-                                                            const float* __restrict__ targets, uint32_t n, uint32_t stride, uint32_t offset,
-                                                            uint32_t n_pad, float loss_eps, short* __restrict__ act,
+                                                            const float* __restrict__ targets, uint32_t n_host,
+                                                            const uint32_t* __restrict__ n_dev, uint32_t stride,
+                                                            uint32_t offset, float loss_eps, short* __restrict__ act,
                                                             short* __restrict__ dz, float* __restrict__ loss_out) {
+    const uint32_t n = slice_count(n_host, n_dev, stride, offset), n_pad = (n + 15) / 16 * 16;
+    if (blockIdx.x * (kThreads / 64) * 16 >= n_pad) return;  // no sample group for this block (uniform)
     __shared__ v4s wt[kFwdTiles * 64];
     __shared__ v4s wtb[kBwdTiles * 64];
     load_tiles(wt, wt_g, kFwdTiles * 64);
@@ -346,8 +357,10 @@ __device__ __forceinline__ WgTile wg_tile(int t) {
 // The waves of input tile 0 also sum their split's dz rows: the bias
 // gradient partials (partial_b[split][dz feature]).
 __global__ __launch_bounds__(kThreads) void k_cache_wgrad(const short* __restrict__ act, const short* __restrict__ dz,
-                                                          uint32_t n_pad, float* __restrict__ partial,
+                                                          uint32_t n_host, const uint32_t* __restrict__ n_dev,
+                                                          uint32_t stride, uint32_t offset, float* __restrict__ partial,
                                                           float* __restrict__ partial_b) {
+    const uint32_t n_pad = (slice_count(n_host, n_dev, stride, offset) + 15) / 16 * 16;
     const int lane = threadIdx.x & 63;
     const uint32_t wave = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     if (wave >= uint32_t(kWgTiles * kSplits)) return;
@@ -400,12 +413,17 @@ __global__ __launch_bounds__(kThreads) void k_cache_reduce(const float* __restri
     grad[p] = g;
 }
 
-// Adam (Kingma & Ba) on the FP32 masters.
+// Adam (Kingma & Ba) on the FP32 masters; step t = *step + 1 (bias
+// corrections 1 - beta^t).  A slice without samples takes no step.
 __global__ void k_cache_adam(float* __restrict__ params, float* __restrict__ m, float* __restrict__ v,
-                             const float* __restrict__ grad, float lr, float b1, float b2, float eps, float c1,
-                             float c2) {
+                             const float* __restrict__ grad, float lr, float b1, float b2, float eps, uint32_t n_host,
+                             const uint32_t* __restrict__ n_dev, uint32_t stride, uint32_t offset,
+                             const uint32_t* __restrict__ step) {
+    if (slice_count(n_host, n_dev, stride, offset) == 0) return;
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= uint32_t(kParams)) return;
+    const float t = float(*step + 1u);
+    const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
     float g = grad[p];
     float mm = b1 * m[p] + (1.0f - b1) * g;
     float vv = b2 * v[p] + (1.0f - b2) * g * g;
@@ -448,6 +466,28 @@ __global__ void k_cache_pack(const float* __restrict__ params, v4s* __restrict__
     }
 }
 
+// Counts the Adam step just taken (after k_cache_adam, which read the count).
+__global__ void k_cache_step_done(uint32_t n_host, const uint32_t* __restrict__ n_dev, uint32_t stride, uint32_t offset,
+                                  uint32_t* __restrict__ step) {
+    if (slice_count(n_host, n_dev, stride, offset) > 0) *step += 1u;
+}
+
+// End of a rendered frame: the training-sample count is frozen for the
+// optimizer steps (counters[4]; the next frame clears counters[2]) and, when
+// adaptive, the training-path stride is steered towards 0.9 max_batch
+// samples per frame (counters[5]).
+__global__ void k_cache_frame_end(uint32_t* __restrict__ ctr, uint32_t max_batch, uint32_t adaptive) {
+    const uint32_t got = ctr[2];
+    ctr[4] = min(got, max_batch);
+    if (adaptive) {
+        const float cur = float(ctr[5]);
+        const float next = got ? cur * float(got) / (0.9f * float(max_batch)) : cur * 0.5f;
+        ctr[5] = uint32_t(fminf(fmaxf(next, 1.0f), float(1u << 20)));
+    }
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
 __global__ void k_zero_f(float* p, uint32_t n) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i < n) p[i] = 0.0f;
@@ -458,6 +498,12 @@ __global__ void k_zero_u(uint32_t* p, uint32_t n) {
 }
 
 }  // namespace
+
+// Device counters of a cache (uint32): [0] queries and [1] training records of
+// the current pass, [2] training samples and [3] cache queries of the current
+// frame, [4] training samples of the last finished frame, [5] adaptive
+// training-path stride, [6] Adam steps taken.
+constexpr int kCacheCounters = 8;
 
 struct rc_cache {
     rc_ctx* ctx = nullptr;
@@ -470,11 +516,17 @@ struct rc_cache {
     v4s* fwd = nullptr;
     v4s* bwd = nullptr;
     float* loss = nullptr;
-    uint32_t step = 0;
-    uint32_t* counters = nullptr;  // frame counters read by the renderer kernels
-    uint32_t auto_stride = 0;      // training-path stride when cfg.train_stride == 0 (0: not yet chosen)
+    uint32_t* counters = nullptr;  // kCacheCounters words, read and written by the kernels
+    bool stride_set = false;       // counters[5] holds an adaptive stride
     DevBuf act, dz, partial, queries, targets, out;
     DevBuf pass_q, pass_r;  // queries and training records of one render pass
+    // The optimizer steps after a rendered frame run on their own stream,
+    // while the host goes on and the next frame traverses; the next cache
+    // inference (or any call touching the weights) waits for ev_trained.
+    hipStream_t train_stream = nullptr;
+    hipEvent_t ev_frame = nullptr, ev_t0 = nullptr, ev_trained = nullptr;
+    bool pending = false;     // steps queued since the last wait
+    uint32_t last_steps = 0;  // steps of the last frame (for its mean loss)
 };
 
 namespace {
@@ -491,66 +543,71 @@ uint32_t waves_grid(uint32_t n_pad) {
     return std::max(1u, std::min(2048u, (groups + 3) / 4));
 }
 
+// Makes stream st wait for the optimizer steps in flight (GPU-side wait).
+int wait_training(rc_cache* k, hipStream_t st) {
+    if (k->pending) RC_TRY_HIP(hipStreamWaitEvent(st, k->ev_trained, 0));
+    return RC_OK;
+}
+
+// Gradient of the loss over slice j of `steps` slices of the batch in
+// (d_queries, d_targets), whose size is min(*n_dev, n_host); on stream st.
+int grad_device(rc_ctx* c, hipStream_t st, rc_cache* k, const float* d_queries, const float* d_targets,
+                uint32_t n_host, const uint32_t* n_dev, uint32_t steps, uint32_t j) {
+    const uint32_t n_max = (n_host - std::min(n_host, j) + steps - 1) / steps, n_pad = (n_max + 15) / 16 * 16;
+    RC_TRY(k->act.ensure(size_t(std::max(n_pad, 16u)) * kActF * sizeof(short)));
+    RC_TRY(k->dz.ensure(size_t(std::max(n_pad, 16u)) * kDzF * sizeof(short)));
+    RC_TRY(k->partial.ensure(size_t(kSplits) * (kWgTiles * 256 + kDzF) * sizeof(float)));
+    float* partial_b = k->partial.as<float>() + size_t(kSplits) * kWgTiles * 256;
+    k_cache_fwd_bwd<<<waves_grid(std::max(n_pad, 16u)), kThreads, 0, st>>>(
+        k->fwd, k->bwd, k->params, k->bounds, d_queries, d_targets, n_host, n_dev, steps, j, k->cfg.loss_eps,
+        k->act.as<short>(), k->dz.as<short>(), k->loss);
+    const uint32_t wg_waves = kWgTiles * kSplits;
+    k_cache_wgrad<<<(wg_waves + 3) / 4, kThreads, 0, st>>>(k->act.as<short>(), k->dz.as<short>(), n_host, n_dev,
+                                                          steps, j, k->partial.as<float>(), partial_b);
+    k_cache_reduce<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, st>>>(k->partial.as<float>(), partial_b,
+                                                                             k->grad);
+    RC_TRY_HIP(hipGetLastError());
+    return RC_OK;
+}
+
+// One Adam step on the gradient of slice j (skipped when it has no samples).
+int adam_device(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t n_host, const uint32_t* n_dev, uint32_t steps,
+                uint32_t j) {
+    k_cache_adam<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, st>>>(
+        k->params, k->m, k->v, k->grad, k->cfg.lr, k->cfg.beta1, k->cfg.beta2, k->cfg.adam_eps, n_host, n_dev, steps, j,
+        k->counters + 6);
+    k_cache_step_done<<<1, 1, 0, st>>>(n_host, n_dev, steps, j, k->counters + 6);
+    const uint32_t n = (kFwdTiles + kBwdTiles) * 64;
+    k_cache_pack<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>(k->params, k->fwd, k->bwd);
+    RC_TRY_HIP(hipGetLastError());
+    return RC_OK;
+}
+
 }  // namespace
 
 // Device-side entry points used by the renderer.
-int rc_cache_apply_device(rc_ctx* c, rc_cache* k, const float4* queries, const uint32_t* n_dev, uint32_t n_bound,
-                          float4* L) {
+int rc_cache_apply_device(rc_ctx* c, hipStream_t st, rc_cache* k, const float4* queries, const uint32_t* n_dev,
+                          uint32_t n_bound, float4* L) {
     if (n_bound == 0) return RC_OK;
+    RC_TRY(wait_training(k, st));
     uint32_t grid = waves_grid((n_bound + 15) / 16 * 16);
-    k_cache_infer<true><<<grid, kThreads, 0, c->stream>>>(k->fwd, k->params, k->bounds,
-                                                          reinterpret_cast<const float*>(queries), n_bound, n_dev,
-                                                          reinterpret_cast<float*>(L));
+    k_cache_infer<true><<<grid, kThreads, 0, st>>>(k->fwd, k->params, k->bounds, reinterpret_cast<const float*>(queries),
+                                                   n_bound, n_dev, reinterpret_cast<float*>(L));
     RC_TRY_HIP(hipGetLastError());
     return RC_OK;
 }
 
-// Gradient of the loss over n samples already on the device (n known on the host).
-int rc_cache_grad_device(rc_ctx* c, rc_cache* k, const float* d_queries, const float* d_targets, uint32_t n,
-                         uint32_t stride = 1, uint32_t offset = 0) {
-    uint32_t n_pad = (n + 15) / 16 * 16;
-    RC_TRY(k->act.ensure(size_t(n_pad) * kActF * sizeof(short)));
-    RC_TRY(k->dz.ensure(size_t(n_pad) * kDzF * sizeof(short)));
-    RC_TRY(k->partial.ensure(size_t(kSplits) * (kWgTiles * 256 + kDzF) * sizeof(float)));
-    float* partial_b = k->partial.as<float>() + size_t(kSplits) * kWgTiles * 256;
-    k_zero_f<<<1, kThreads, 0, c->stream>>>(k->loss, 1);
-    k_cache_fwd_bwd<<<waves_grid(n_pad), kThreads, 0, c->stream>>>(
-        k->fwd, k->bwd, k->params, k->bounds, d_queries, d_targets, n, stride, offset, n_pad, k->cfg.loss_eps,
-        k->act.as<short>(),
-        k->dz.as<short>(), k->loss);
-    const uint32_t wg_waves = kWgTiles * kSplits;
-    k_cache_wgrad<<<(wg_waves + 3) / 4, kThreads, 0, c->stream>>>(k->act.as<short>(), k->dz.as<short>(), n_pad,
-                                                                   k->partial.as<float>(), partial_b);
-    k_cache_reduce<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, c->stream>>>(k->partial.as<float>(),
-                                                                                    partial_b, k->grad);
-    RC_TRY_HIP(hipGetLastError());
-    return RC_OK;
-}
-
-int rc_cache_adam_device(rc_ctx* c, rc_cache* k) {
-    ++k->step;
-    float c1 = 1.0f - std::pow(k->cfg.beta1, float(k->step));
-    float c2 = 1.0f - std::pow(k->cfg.beta2, float(k->step));
-    k_cache_adam<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, c->stream>>>(
-        k->params, k->m, k->v, k->grad, k->cfg.lr, k->cfg.beta1, k->cfg.beta2, k->cfg.adam_eps, c1, c2);
-    RC_TRY_HIP(hipGetLastError());
-    return cache_pack(c, k);
-}
-
-int rc_cache_frame_io(rc_ctx* c, rc_cache* k, uint32_t cache_vertex, size_t pass_paths, uint64_t frame_samples,
-                      CacheIO* io) {
-    RC_TRY(k->pass_q.ensure(pass_paths * 4 * sizeof(float4)));
-    RC_TRY(k->pass_r.ensure(pass_paths * 5 * sizeof(float4)));
-    RC_TRY(k->queries.ensure(size_t(k->cfg.max_batch) * 12 * 4));
-    RC_TRY(k->targets.ensure(size_t(k->cfg.max_batch) * 3 * 4));
-    uint32_t stride = k->cfg.train_stride;
-    if (stride == 0) {
-        // adaptive: first frame as if every training path reached the cache
-        // vertex; later frames steer the record count to ~0.9 max_batch
-        if (k->auto_stride == 0)
-            k->auto_stride = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(frame_samples / k->cfg.max_batch, 1u << 20)));
-        stride = k->auto_stride;
-    }
+int rc_cache_frame_io(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t cache_vertex, size_t pass_paths,
+                      uint64_t frame_samples, CacheIO* io) {
+    const size_t need_q = pass_paths * 4 * sizeof(float4), need_r = pass_paths * 5 * sizeof(float4);
+    const size_t need_bq = size_t(k->cfg.max_batch) * 12 * 4, need_bt = size_t(k->cfg.max_batch) * 3 * 4;
+    if (k->pending && (k->pass_q.bytes < need_q || k->pass_r.bytes < need_r || k->queries.bytes < need_bq ||
+                       k->targets.bytes < need_bt))
+        RC_TRY_HIP(hipEventSynchronize(k->ev_trained));  // buffers the steps in flight read are about to move
+    RC_TRY(k->pass_q.ensure(need_q));
+    RC_TRY(k->pass_r.ensure(need_r));
+    RC_TRY(k->queries.ensure(need_bq));
+    RC_TRY(k->targets.ensure(need_bt));
     io->queries = k->pass_q.as<float4>();
     io->records = k->pass_r.as<float4>();
     io->counters = k->counters;
@@ -558,42 +615,58 @@ int rc_cache_frame_io(rc_ctx* c, rc_cache* k, uint32_t cache_vertex, size_t pass
     io->batch_t = k->targets.as<float>();
     io->batch_cap = k->cfg.max_batch;
     io->cache_vertex = cache_vertex;
-    io->train_stride = stride;
+    io->train_stride = k->cfg.train_stride;
+    io->stride_dev = k->counters + 5;
     io->cap = uint32_t(pass_paths);
-    k_zero_u<<<1, 64, 0, c->stream>>>(k->counters, 4);
+    k_zero_u<<<1, 64, 0, st>>>(k->counters, 4);
+    if (k->cfg.train_stride == 0 && !k->stride_set) {
+        // adaptive: the first frame as if every training path reached the
+        // cache vertex; later frames steer the count (k_cache_frame_end)
+        const uint32_t first =
+            uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(frame_samples / k->cfg.max_batch, 1u << 20)));
+        k_set_u32<<<1, 1, 0, st>>>(k->counters + 5, first);
+        k->stride_set = true;
+    }
     RC_TRY_HIP(hipGetLastError());
     return RC_OK;
 }
 
 rc_ctx* rc_cache_owner(const rc_cache* k) { return k->ctx; }
 
-int rc_cache_train_frame(rc_ctx* c, rc_cache* k, uint32_t steps, float* loss, uint32_t* samples, uint32_t* queries) {
-    uint32_t h[4] = {0, 0, 0, 0};
-    RC_TRY_HIP(hipMemcpyAsync(h, k->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    RC_TRY_HIP(hipStreamSynchronize(c->stream));
-    uint32_t n = std::min(h[2], k->cfg.max_batch);
-    *samples = n;
-    *queries = h[3];
-    if (k->cfg.train_stride == 0 && k->auto_stride) {
-        double want = 0.9 * double(k->cfg.max_batch);
-        double next = h[2] ? double(k->auto_stride) * double(h[2]) / want : double(k->auto_stride) * 0.5;
-        k->auto_stride = uint32_t(std::max(1.0, std::min(next, double(1u << 20))));
+int rc_cache_frame_end(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t steps, uint32_t counts[2]) {
+    k_cache_frame_end<<<1, 1, 0, st>>>(k->counters, k->cfg.max_batch, k->cfg.train_stride == 0 ? 1u : 0u);
+    uint32_t h[2];  // training samples, cache queries of the frame
+    RC_TRY_HIP(hipMemcpyAsync(h, k->counters + 2, sizeof(h), hipMemcpyDeviceToHost, st));
+    k->last_steps = steps;
+    if (steps > 0) {
+        // the optimizer steps wait for the frame's samples on the cache's stream
+        RC_TRY_HIP(hipEventRecord(k->ev_frame, st));
+        RC_TRY_HIP(hipStreamWaitEvent(k->train_stream, k->ev_frame, 0));
+        RC_TRY_HIP(hipEventRecord(k->ev_t0, k->train_stream));
+        k_zero_f<<<1, kThreads, 0, k->train_stream>>>(k->loss, 1);
+        for (uint32_t j = 0; j < steps; ++j) {
+            RC_TRY(grad_device(c, k->train_stream, k, k->queries.as<float>(), k->targets.as<float>(), k->cfg.max_batch,
+                               k->counters + 4, steps, j));
+            RC_TRY(adam_device(c, k->train_stream, k, k->cfg.max_batch, k->counters + 4, steps, j));
+        }
+        RC_TRY_HIP(hipEventRecord(k->ev_trained, k->train_stream));
+        k->pending = true;
+        c->train_ev0 = k->ev_t0;
+        c->train_ev1 = k->ev_trained;
     }
+    RC_TRY_HIP(hipStreamSynchronize(st));
+    counts[0] = h[1];
+    counts[1] = std::min(h[0], k->cfg.max_batch);
+    return RC_OK;
+}
+
+int rc_cache_last_loss(rc_ctx* c, rc_cache* k, float* loss) {
     *loss = 0.0f;
-    if (n == 0 || steps == 0) return RC_OK;
-    steps = std::min(steps, n);
-    // steps Adam steps on interleaved slices j, j + steps, j + 2 steps, ...
-    float sum = 0.0f;
-    for (uint32_t j = 0; j < steps; ++j) {
-        const uint32_t nj = (n - j + steps - 1) / steps;
-        RC_TRY(rc_cache_grad_device(c, k, k->queries.as<float>(), k->targets.as<float>(), nj, steps, j));
-        RC_TRY(rc_cache_adam_device(c, k));
-        float l = 0.0f;
-        RC_TRY_HIP(hipMemcpyAsync(&l, k->loss, 4, hipMemcpyDeviceToHost, c->stream));
-        RC_TRY_HIP(hipStreamSynchronize(c->stream));
-        sum += l;
-    }
-    *loss = sum / float(steps);
+    if (k->last_steps == 0) return RC_OK;
+    RC_TRY_HIP(hipEventSynchronize(k->ev_trained));
+    float l = 0.0f;
+    RC_TRY_HIP(hipMemcpy(&l, k->loss, 4, hipMemcpyDeviceToHost));
+    *loss = l / float(k->last_steps);
     return RC_OK;
 }
 
@@ -621,8 +694,13 @@ extern "C" int rc_cache_create(rc_ctx* c, const rc_cache_cfg* cfg, const float b
         hipMalloc(&k->v, kParams * 4) != hipSuccess || hipMalloc(&k->grad, kParams * 4) != hipSuccess ||
         hipMalloc(&k->fwd, kFwdTiles * 64 * sizeof(v4s)) != hipSuccess ||
         hipMalloc(&k->bwd, kBwdTiles * 64 * sizeof(v4s)) != hipSuccess || hipMalloc(&k->loss, 4) != hipSuccess ||
-        hipMalloc(&k->counters, 4 * sizeof(uint32_t)) != hipSuccess)
+        hipMalloc(&k->counters, kCacheCounters * sizeof(uint32_t)) != hipSuccess)
         return fail("device allocation failed");
+    if (hipStreamCreateWithFlags(&k->train_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&k->ev_frame, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&k->ev_t0) != hipSuccess || hipEventCreate(&k->ev_trained) != hipSuccess)
+        return fail("stream or event creation failed");
+    k_zero_u<<<1, 64, 0, c->stream>>>(k->counters, kCacheCounters);
     // He-uniform hidden layers, small output layer with the output starting near 0.5
     std::vector<float> p(kParams, 0.0f);
     std::mt19937 rng(cfg->seed);
@@ -641,9 +719,25 @@ extern "C" int rc_cache_create(rc_ctx* c, const rc_cache_cfg* cfg, const float b
     return RC_OK;
 }
 
+namespace {
+// Host-side wait for the optimizer steps in flight (the synchronous API calls
+// below read or replace what they write).
+int settle(rc_cache* k) {
+    if (k->pending) RC_TRY_HIP(hipEventSynchronize(k->ev_trained));
+    k->pending = false;
+    return RC_OK;
+}
+}  // namespace
+
 extern "C" int rc_cache_free(rc_cache* k) {
     if (!k) return RC_OK;
     (void)hipStreamSynchronize(k->ctx->stream);
+    if (k->train_stream) (void)hipStreamSynchronize(k->train_stream);
+    if (k->ctx->train_ev1 == k->ev_trained) k->ctx->train_ev0 = k->ctx->train_ev1 = nullptr;
+    if (k->ev_frame) (void)hipEventDestroy(k->ev_frame);
+    if (k->ev_t0) (void)hipEventDestroy(k->ev_t0);
+    if (k->ev_trained) (void)hipEventDestroy(k->ev_trained);
+    if (k->train_stream) (void)hipStreamDestroy(k->train_stream);
     float* fp[] = {k->params, k->m, k->v, k->grad, k->loss};
     for (float* q : fp)
         if (q) (void)hipFree(q);
@@ -659,10 +753,11 @@ extern "C" int rc_cache_free(rc_cache* k) {
 extern "C" int rc_cache_set_params(rc_cache* k, const float* params) {
     if (!k || !params) return rc_host_fail(RC_ERR_INVALID, "rc_cache_set_params: null argument");
     rc_ctx* c = k->ctx;
+    RC_TRY(settle(k));
     RC_TRY_HIP(hipMemcpyAsync(k->params, params, kParams * 4, hipMemcpyHostToDevice, c->stream));
     k_zero_f<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, c->stream>>>(k->m, kParams);
     k_zero_f<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, c->stream>>>(k->v, kParams);
-    k->step = 0;
+    k_set_u32<<<1, 1, 0, c->stream>>>(k->counters + 6, 0u);  // Adam step count
     RC_TRY(cache_pack(c, k));
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     return RC_OK;
@@ -670,6 +765,7 @@ extern "C" int rc_cache_set_params(rc_cache* k, const float* params) {
 
 extern "C" int rc_cache_get_params(rc_cache* k, float* params) {
     if (!k || !params) return rc_host_fail(RC_ERR_INVALID, "rc_cache_get_params: null argument");
+    RC_TRY(settle(k));
     RC_TRY_HIP(hipMemcpyAsync(params, k->params, kParams * 4, hipMemcpyDeviceToHost, k->ctx->stream));
     RC_TRY_HIP(hipStreamSynchronize(k->ctx->stream));
     return RC_OK;
@@ -692,6 +788,7 @@ extern "C" int rc_cache_infer(rc_ctx* c, rc_cache* k, const float* q, size_t n, 
     if (!k || k->ctx != c || (n && (!q || !out))) return rc_host_fail(RC_ERR_INVALID, "rc_cache_infer: bad argument");
     if (n == 0) return RC_OK;
     if (n > (1u << 28)) return rc_host_fail(RC_ERR_INVALID, "rc_cache_infer: too many queries");
+    RC_TRY(settle(k));
     RC_TRY(stage(c, k, q, nullptr, n));
     RC_TRY(k->out.ensure(n * 3 * 4));
     uint32_t n_pad = uint32_t((n + 15) / 16 * 16);
@@ -709,8 +806,10 @@ extern "C" int rc_cache_gradient(rc_ctx* c, rc_cache* k, const float* q, const f
     RC_TRY(rc_check_ctx(c, "rc_cache_gradient"));
     if (!k || k->ctx != c || !q || !t || n == 0) return rc_host_fail(RC_ERR_INVALID, "rc_cache_gradient: bad argument");
     if (n > k->cfg.max_batch) return rc_host_fail(RC_ERR_INVALID, "rc_cache_gradient: batch larger than max_batch");
+    RC_TRY(settle(k));
     RC_TRY(stage(c, k, q, t, n));
-    RC_TRY(rc_cache_grad_device(c, k, k->queries.as<float>(), k->targets.as<float>(), uint32_t(n)));
+    k_zero_f<<<1, kThreads, 0, c->stream>>>(k->loss, 1);
+    RC_TRY(grad_device(c, c->stream, k, k->queries.as<float>(), k->targets.as<float>(), uint32_t(n), nullptr, 1, 0));
     if (grad) RC_TRY_HIP(hipMemcpyAsync(grad, k->grad, kParams * 4, hipMemcpyDeviceToHost, c->stream));
     float l = 0.0f;
     RC_TRY_HIP(hipMemcpyAsync(&l, k->loss, 4, hipMemcpyDeviceToHost, c->stream));
@@ -721,7 +820,7 @@ extern "C" int rc_cache_gradient(rc_ctx* c, rc_cache* k, const float* q, const f
 
 extern "C" int rc_cache_train_step(rc_ctx* c, rc_cache* k, const float* q, const float* t, size_t n, float* loss) {
     RC_TRY(rc_cache_gradient(c, k, q, t, n, nullptr, loss));
-    RC_TRY(rc_cache_adam_device(c, k));
+    RC_TRY(adam_device(c, c->stream, k, uint32_t(n), nullptr, 1, 0));
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     return RC_OK;
 }

@@ -26,18 +26,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr float kPi = 3.14159265358979323846f;
-#ifndef RC_TRACE_BLOCK
-#define RC_TRACE_BLOCK 128
-#endif
-#ifndef RC_LDS_DEPTH
-#define RC_LDS_DEPTH 20  // 1% faster than 24 on the terrain at 64 samples per pass; 12 overflows too often
-#endif
-constexpr int kTraceBlock = RC_TRACE_BLOCK;  // threads per block of the shadow kernel
-#ifndef RC_EXT_BLOCK
-#define RC_EXT_BLOCK 256  // 4 waves per queue reservation: 3% faster than 512 and 128
-#endif
-constexpr int kExtBlock = RC_EXT_BLOCK;      // threads per block of the extension kernel
-constexpr int kTraceLds = RC_LDS_DEPTH;      // LDS stack entries per thread
+// Threads per block of the extension kernel (4 waves per queue reservation:
+// 3% faster than 512 and 128 on the 2M-triangle terrain) and of the shadow
+// kernel, unless a launch shape says otherwise (the batch autotuner).
+constexpr uint32_t kExtBlockDefault = 256;
+constexpr uint32_t kShadowBlockDefault = 128;
+constexpr int kLdsRows = kLdsStack > 0 ? kLdsStack : 1;  // LDS stack entries per thread (at least one row)
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
 #endif
@@ -116,7 +110,8 @@ struct PassParams {
     uint32_t max_len, rr_start, estimator, frame;
     float3 sky, sun, sun_e;
     uint64_t seed;
-    uint32_t has_sun, pad;
+    uint32_t has_sun;
+    int stack_limit;  // on-chip traversal stack entries before the deep-stack kernel
 };
 
 // The pixel and sample a pass's queue slot traces.
@@ -290,7 +285,8 @@ constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 
 // Training paths: a fixed-rate, frame-varying subset of the slots that run
 // to the end instead of stopping into the cache.
-__device__ __forceinline__ bool training_path(const PassParams& P, uint32_t slot, uint32_t stride) {
+__device__ __forceinline__ bool training_path(const PassParams& P, uint32_t slot, const CacheIO& CI) {
+    const uint32_t stride = max(1u, CI.train_stride ? CI.train_stride : *CI.stride_dev);
     uint32_t pixel = slot_pixel(P, slot), sample = slot_sample(P, slot);
     uint32_t h = pixel * 2654435761u ^ (sample * 40503u + P.frame * 2246822519u);
     h ^= h >> 15;
@@ -345,7 +341,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
         // The cache holds the indirect part of the radiance leaving this
         // vertex (the direct sun is the next-event ray above, for every path).
         const float3 wo = make_float3(-r1.x, -r1.y, -r1.z);
-        if (!training_path(P, slot, CI.train_stride)) {
+        if (!training_path(P, slot, CI)) {
             // the path ends here: its indirect radiance comes from the cache
             // (one reservation per wave for the lanes that end here)
             uint32_t q = wave_append(true, &CI.counters[0]);
@@ -391,23 +387,25 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
 }
 
 // Appends the continuation and shadow ray of a shaded vertex (every lane of
-// the wave calls this at the same point).
+// the wave calls this at the same point; with BLOCK > 0 every thread of the
+// block, which reserves once per block, else once per wave).
+template <int BLOCK>
 __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut& O, uint32_t slot,
                                             rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
                                             DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI,
                                             uint32_t* append_scratch = nullptr) {
     // the extension kernel reserves once per block, k_shade_listed once per wave
-    uint32_t pos = append_scratch ? block_append<kExtBlock>(O.alive, &ctr->queue_count[cout], append_scratch)
-                                  : wave_append(O.alive, &ctr->queue_count[cout]);
+    uint32_t pos = BLOCK ? block_append<BLOCK>(O.alive, &ctr->queue_count[cout], append_scratch)
+                         : wave_append(O.alive, &ctr->queue_count[cout]);
     if (O.alive) {
         float4* ro = reinterpret_cast<float4*>(rays_out + pos);
         ro[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f);
         ro[1] = make_float4(O.newd.x, O.newd.y, O.newd.z, __builtin_inff());
         st_out[pos] = make_float4(O.thr.x, O.thr.y, O.thr.z, __uint_as_float(slot));
     }
-    uint32_t spos = append_scratch ? block_append<kExtBlock>(O.shadow, &ctr->queue_count[2], append_scratch)
-                                   : wave_append(O.shadow, &ctr->queue_count[2]);
+    uint32_t spos = BLOCK ? block_append<BLOCK>(O.shadow, &ctr->queue_count[2], append_scratch)
+                          : wave_append(O.shadow, &ctr->queue_count[2]);
     if (O.shadow) {
         float4* ro = reinterpret_cast<float4*>(srays + spos);
         ro[0] = make_float4(O.so.x, O.so.y, O.so.z, 0.0f);
@@ -425,17 +423,21 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
 // the slot's weight and zeroed radiance, and stores the ray and state only
 // when it has to be listed for the deep-stack kernel (cam_rays, cam_st: the
 // buffers `rays` and `st` name, written only in this mode).
-template <bool COUNT, bool CAM>
-__global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
+// Blocks past the live entries return at once (the grid is sized for the
+// queue's capacity; the test is uniform over the block, before any barrier).
+// DEC64: the triangle decision mode; BLOCK: threads per block.
+template <bool COUNT, bool CAM, bool DEC64, int BLOCK>
+__global__ __launch_bounds__(BLOCK, RC_MIN_WAVES) void k_extend_shade(
     PassParams P, uint32_t k, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
     const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
     const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
     DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI, float* __restrict__ W,
     rc_ray* __restrict__ cam_rays, float4* __restrict__ cam_st) {
-    __shared__ uint32_t stack[kTraceLds * kExtBlock];
+    __shared__ uint32_t stack[kLdsRows * BLOCK];
     const uint32_t n = ctr->queue_count[cin];
-    const uint32_t i = trace_block<kExtBlock>(n) * kExtBlock + threadIdx.x;
+    if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t i = trace_block<BLOCK>(n) * BLOCK + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
@@ -455,8 +457,8 @@ __global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
             s = st[i];
         }
         slot = __float_as_uint(s.w);
-        TraceResult R = traverse<false, COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
-                                               kExtBlock, kTraceLds, n_rec, n_tri);
+        TraceResult R = traverse<false, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
+                                                      BLOCK, P.stack_limit, n_rec, n_tri);
         if (R.overflow) {
             if (CAM) {  // the deep-stack kernel and k_shade_listed read the entry
                 float4* rp = reinterpret_cast<float4*>(cam_rays + i);
@@ -469,8 +471,8 @@ __global__ __launch_bounds__(kExtBlock, RC_MIN_WAVES) void k_extend_shade(
             O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
         }
     }
-    __shared__ uint32_t append_scratch[kExtBlock / 64 + 1];
-    emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
+    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
+    emit_vertex<BLOCK>(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
@@ -501,29 +503,30 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
             slot = __float_as_uint(s.w);
             O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L, CI);
         }
-        emit_vertex(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI);
+        emit_vertex<0>(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI);
     }
 }
 
 // Any hit of a shadow ray; an unoccluded ray adds its contribution to its
 // slot.  Overflowing rays are listed for the deep-stack kernel and applied by
 // k_shadow_apply_listed.
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock, RC_MIN_WAVES) void k_shadow(const NodeT* __restrict__ nodes,
+template <bool COUNT, bool DEC64, int BLOCK>
+__global__ __launch_bounds__(BLOCK, RC_MIN_WAVES) void k_shadow(const NodeT* __restrict__ nodes,
                                                        const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
                                                        const float4* __restrict__ scontrib, float4* __restrict__ L,
                                                        uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr,
-                                                       uint8_t* __restrict__ occ_out) {
-    __shared__ uint32_t stack[kTraceLds * kTraceBlock];
+                                                       uint8_t* __restrict__ occ_out, int stack_limit) {
+    __shared__ uint32_t stack[kLdsRows * BLOCK];
     const uint32_t n = ctr->queue_count[2];
-    const uint32_t i = trace_block<kTraceBlock>(n) * kTraceBlock + threadIdx.x;
+    if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t i = trace_block<BLOCK>(n) * BLOCK + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     if (i < n) {
         const float4* rp = reinterpret_cast<const float4*>(srays + i);
         float4 r0 = rp[0], r1 = rp[1];
-        TraceResult R = traverse<true, COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
-                                              kTraceBlock, kTraceLds, n_rec, n_tri);
+        TraceResult R = traverse<true, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
+                                                     BLOCK, stack_limit, n_rec, n_tri);
         if (R.overflow) {
             ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         } else {
@@ -702,20 +705,87 @@ void launch_film_add(hipStream_t st, PixelOrder order, uint32_t npix, uint32_t s
         k_film_add<<<(npix + kBlock - 1) / kBlock, kBlock, 0, st>>>(order, npix, spp_pass, L, W, sum, comp);
 }
 
-int alloc_pass(rc_ctx* c, size_t npaths) {
-    RC_TRY(c->ray_a.ensure(npaths * sizeof(rc_ray)));
-    RC_TRY(c->ray_b.ensure(npaths * sizeof(rc_ray)));
-    RC_TRY(c->st_a.ensure(npaths * sizeof(float4)));
-    RC_TRY(c->st_b.ensure(npaths * sizeof(float4)));
-    RC_TRY(c->hits.ensure(npaths * sizeof(rc_hit)));
-    RC_TRY(c->srays.ensure(npaths * sizeof(rc_ray)));
-    RC_TRY(c->scontrib.ensure(npaths * sizeof(float4)));
-    RC_TRY(c->socc.ensure(npaths));
-    RC_TRY(c->slotL.ensure(npaths * sizeof(float4)));
-    RC_TRY(c->slotW.ensure(npaths * sizeof(float)));
-    RC_TRY(c->ovf_list.ensure(npaths * sizeof(uint32_t)));
-    RC_TRY(c->deep_stack.ensure(size_t(2048) * kDeepStack * sizeof(uint32_t)));
-    return RC_OK;
+// Launches of the extension and shadow kernels in every template variant:
+// counting (default block only), camera bounce, decision mode, block size.
+struct ExtendArgs {
+    PassParams P;
+    uint32_t k;
+    const rc_scene* s;
+    rc_ray* rin;
+    float4* sin;
+    rc_ray* rout;
+    float4* sout;
+    rc_ray* srays;
+    float4* scontrib;
+    float4* L;
+    uint32_t* ovf;
+    DevCounters* ctr;
+    uint32_t cin, cout;
+    CacheIO CI;
+    float* W;
+};
+
+template <bool COUNT, bool CAM, bool DEC64, int BLOCK>
+void extend_t(const ExtendArgs& a, uint32_t n, hipStream_t st) {
+    k_extend_shade<COUNT, CAM, DEC64, BLOCK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
+        a.P, a.k, a.s->nodes, a.s->tris, a.rin, a.sin, a.s->shade, a.s->mats, a.rout, a.sout, a.srays, a.scontrib, a.L,
+        a.ovf, a.ctr, a.cin, a.cout, a.CI, a.W, a.rin, a.sin);
+}
+
+template <bool COUNT, bool CAM, bool DEC64>
+void extend_b(const ExtendArgs& a, uint32_t n, hipStream_t st, uint32_t block) {
+    if constexpr (!COUNT) {
+        if (block == 128) return extend_t<COUNT, CAM, DEC64, 128>(a, n, st);
+    }
+    extend_t<COUNT, CAM, DEC64, 256>(a, n, st);
+}
+
+void launch_extend(const ExtendArgs& a, uint32_t n, hipStream_t st, bool count, bool cam, bool dec64, uint32_t block) {
+#define RC_EXT(C, CM, D) extend_b<C, CM, D>(a, n, st, block)
+    if (count) {
+        if (cam) { if (dec64) RC_EXT(true, true, true); else RC_EXT(true, true, false); }
+        else { if (dec64) RC_EXT(true, false, true); else RC_EXT(true, false, false); }
+    } else {
+        if (cam) { if (dec64) RC_EXT(false, true, true); else RC_EXT(false, true, false); }
+        else { if (dec64) RC_EXT(false, false, true); else RC_EXT(false, false, false); }
+    }
+#undef RC_EXT
+}
+
+struct ShadowArgs {
+    const rc_scene* s;
+    const rc_ray* srays;
+    const float4* scontrib;
+    float4* L;
+    uint32_t* ovf;
+    DevCounters* ctr;
+    uint8_t* occ_out;
+    int stack_limit;
+};
+
+template <bool COUNT, bool DEC64, int BLOCK>
+void shadow_t(const ShadowArgs& a, uint32_t n, hipStream_t st) {
+    k_shadow<COUNT, DEC64, BLOCK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
+        a.s->nodes, a.s->tris, a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit);
+}
+
+template <bool COUNT, bool DEC64>
+void shadow_b(const ShadowArgs& a, uint32_t n, hipStream_t st, uint32_t block) {
+    if constexpr (!COUNT) {
+        if (block == 64) return shadow_t<COUNT, DEC64, 64>(a, n, st);
+        if (block == 256) return shadow_t<COUNT, DEC64, 256>(a, n, st);
+    }
+    shadow_t<COUNT, DEC64, 128>(a, n, st);
+}
+
+void launch_shadow(const ShadowArgs& a, uint32_t n, hipStream_t st, bool count, bool dec64, uint32_t block) {
+    if (count) {
+        if (dec64) shadow_b<true, true>(a, n, st, block);
+        else shadow_b<true, false>(a, n, st, block);
+    } else {
+        if (dec64) shadow_b<false, true>(a, n, st, block);
+        else shadow_b<false, false>(a, n, st, block);
+    }
 }
 
 uint32_t isqrt_exact(uint32_t v) {
@@ -730,11 +800,43 @@ __global__ void k_zero_counters(DevCounters* ctr) {
     ctr->tri_tests = 0;
     ctr->ext_rays = 0;
     ctr->shadow_rays = 0;
+    ctr->ovf_total = 0;
     ctr->ovf_count = 0;
     ctr->error = 0;
 }
 
 }  // namespace
+
+void rc_zero_counters(DevCounters* ctr, hipStream_t st) { k_zero_counters<<<1, 1, 0, st>>>(ctr); }
+
+int FrameBuffers::ensure(size_t npaths) {
+    if (!counters) {
+        RC_TRY_HIP(hipMalloc(&counters, sizeof(DevCounters)));
+        own_counters = true;
+        RC_TRY_HIP(hipMemset(counters, 0, sizeof(DevCounters)));
+    }
+    RC_TRY(ray_a.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(ray_b.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(st_a.ensure(npaths * sizeof(float4)));
+    RC_TRY(st_b.ensure(npaths * sizeof(float4)));
+    RC_TRY(hits.ensure(npaths * sizeof(rc_hit)));
+    RC_TRY(srays.ensure(npaths * sizeof(rc_ray)));
+    RC_TRY(scontrib.ensure(npaths * sizeof(float4)));
+    RC_TRY(socc.ensure(npaths));
+    RC_TRY(slotL.ensure(npaths * sizeof(float4)));
+    RC_TRY(slotW.ensure(npaths * sizeof(float)));
+    RC_TRY(ovf_list.ensure(npaths * sizeof(uint32_t)));
+    RC_TRY(deep_stack.ensure(rc_deep_stack_bytes()));
+    return RC_OK;
+}
+
+void FrameBuffers::release() {
+    DevBuf* b[] = {&ray_a, &ray_b, &st_a, &st_b, &hits, &srays, &scontrib, &socc, &slotL, &slotW, &ovf_list, &deep_stack};
+    for (DevBuf* x : b) x->release();
+    if (own_counters && counters) (void)hipFree(counters);
+    if (own_counters) counters = nullptr;
+    own_counters = false;
+}
 
 // The filter's radius and shape constants, in double as the reference's
 // constructors compute them (GaussianFilter.cpp:15-23), rounded to float.
@@ -785,15 +887,9 @@ int rc_validate_cfg(const rc_render_cfg* cfg, const char* fn) {
     return RC_OK;
 }
 
-// Sizes every buffer a frame of this config needs (outside any capture).
-int rc_prepare_frame(rc_ctx* c, const rc_render_cfg* cfg, uint32_t spp_pass) {
+size_t rc_pass_paths(const rc_render_cfg* cfg, uint32_t spp_pass) {
     spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
-    const size_t npaths = size_t(cfg->width) * cfg->height * spp_pass;
-    if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
-    RC_TRY(alloc_pass(c, npaths));
-    RC_TRY(c->ovf_list.ensure(npaths * sizeof(uint32_t)));
-    RC_TRY(c->deep_stack.ensure(size_t(2048) * kDeepStack * sizeof(uint32_t)));
-    return RC_OK;
+    return size_t(cfg->width) * cfg->height * spp_pass;
 }
 
 // The parameters of a frame's passes (the pass fields are set per pass).
@@ -841,27 +937,46 @@ int rc_launch_film_add(rc_ctx* c, rc_film* f, const rc_render_cfg* cfg, uint32_t
     return RC_OK;
 }
 
-// One frame of a scene into a film: the passes of rc_render, enqueued on the
-// context stream.  Used by rc_render and the batched graph.
-int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                     uint32_t spp_pass, rc_progress_fn progress, void* user, const CacheIO* cache_io,
-                     rc_cache* cache) {
+// One frame of a scene into a film: the passes of rc_render, enqueued on
+// stream st with the buffers B.  Used by rc_render, the cached renderer and
+// the batched graph.
+int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene* s, rc_film* f,
+                     const rc_camera* cam, const rc_render_cfg* cfg, const FrameShape& shape, rc_progress_fn progress,
+                     void* user, const CacheIO* cache_io, rc_cache* cache) {
     CacheIO CI{};
     if (cache_io) CI = *cache_io;
     const uint32_t npix = cfg->width * cfg->height;
-    spp_pass = std::max(1u, std::min(spp_pass, cfg->spp));
+    const uint32_t spp_pass = std::max(1u, std::min(shape.spp_pass, cfg->spp));
     const size_t npaths = size_t(npix) * spp_pass;
     if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
-    RC_TRY(alloc_pass(c, npaths));
+    if (B.slotW.bytes < npaths * sizeof(float) || B.ovf_list.bytes < npaths * sizeof(uint32_t) || !B.counters)
+        return rc_host_fail(RC_ERR_INVALID, "rc_render: frame buffers are smaller than the pass");
+    const bool timed = st == c->stream;  // profiling events live on the context stream
     PassParams P = base_params(cam, cfg);
+    P.stack_limit = c->stack_depth();
     const uint32_t grid_small = uint32_t(c->cu_count) * 2u;  // list-driven cold-path kernels
-    rc_ray* ra = c->ray_a.as<rc_ray>();
-    rc_ray* rb = c->ray_b.as<rc_ray>();
-    float4* sa = c->st_a.as<float4>();
-    float4* sb = c->st_b.as<float4>();
+    rc_ray* ra = B.ray_a.as<rc_ray>();
+    rc_ray* rb = B.ray_b.as<rc_ray>();
+    float4* sa = B.st_a.as<float4>();
+    float4* sb = B.st_b.as<float4>();
+    float4* L = B.slotL.as<float4>();
+    float* W = B.slotW.as<float>();
+    rc_ray* srays = B.srays.as<rc_ray>();
+    float4* scontrib = B.scontrib.as<float4>();
+    uint8_t* socc = B.socc.as<uint8_t>();
+    uint32_t* ovf = B.ovf_list.as<uint32_t>();
+    DevCounters* ctr = B.counters;
+    TraceLaunch deep;
+    deep.scene = s;
+    deep.ovf_list = ovf;
+    deep.deep_stack = B.deep_stack.as<uint32_t>();
+    deep.ctr = ctr;
+    deep.stream = st;
+    deep.counting = c->counting;
+    deep.dec64 = c->hit_fp64;
     for (uint32_t s0 = 0; s0 < cfg->spp; s0 += spp_pass) {
         if (progress) {
-            RC_TRY_HIP(hipStreamSynchronize(c->stream));
+            RC_TRY_HIP(hipStreamSynchronize(st));
             uint64_t done = uint64_t(double(npix) * double(s0) / double(cfg->spp));
             if (progress(user, done)) return rc_host_fail(RC_ERR_INTERRUPTED, "rc_render: interrupted");
         }
@@ -870,87 +985,67 @@ int rc_enqueue_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
         P.sample0 = s0;
         const uint32_t n = npix * this_pass;
         const uint32_t grid_n = (n + kBlock - 1) / kBlock;
-        hipEvent_t ev;
-        if (CI.cache_vertex) k_cache_reset<<<1, 1, 0, c->stream>>>(CI.counters, 0);
+        hipEvent_t ev = nullptr;
+        if (CI.cache_vertex) k_cache_reset<<<1, 1, 0, st>>>(CI.counters, 0);
         if (RC_FUSED_CAMERA) {
-            k_pass_start<<<1, 1, 0, c->stream>>>(c->counters, n);
+            k_pass_start<<<1, 1, 0, st>>>(ctr, n);
         } else {
-            RC_TRY(c->prof_begin(&ev));
-            k_camera<<<grid_n, kBlock, 0, c->stream>>>(P, ra, sa, c->slotL.as<float4>(), c->slotW.as<float>(),
-                                                       c->counters);
-            RC_TRY(c->prof_end(ev, PK_OTHER));
+            if (timed) RC_TRY(c->prof_begin(&ev));
+            k_camera<<<grid_n, kBlock, 0, st>>>(P, ra, sa, L, W, ctr);
+            if (timed) RC_TRY(c->prof_end(ev, PK_OTHER));
         }
         rc_ray* rin = ra;
         rc_ray* rout = rb;
         float4* sin_ = sa;
         float4* sout = sb;
-        const uint32_t grid_t = (n + kTraceBlock - 1) / kTraceBlock, grid_e = (n + kExtBlock - 1) / kExtBlock;
-        uint32_t* ovf = c->ovf_list.as<uint32_t>();
         for (uint32_t k = 0; k < cfg->max_path_length; ++k) {
             uint32_t cin = k & 1u, cout = cin ^ 1u;
-            k_prep<<<1, 1, 0, c->stream>>>(c->counters, cout);
-            RC_TRY(c->prof_begin(&ev));
-            const bool cam = RC_FUSED_CAMERA && k == 0;
-#define RC_EXTEND(C, CM)                                                                                       \
-    k_extend_shade<C, CM><<<grid_e, kExtBlock, 0, c->stream>>>(                                                \
-        P, k, s->nodes, s->tris, rin, sin_, s->shade, s->mats, rout, sout, c->srays.as<rc_ray>(),              \
-        c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf, c->counters, cin, cout, CI, c->slotW.as<float>(), \
-        rin, sin_)
-            if (c->counting) {
-                if (cam) RC_EXTEND(true, true);
-                else RC_EXTEND(true, false);
-            } else {
-                if (cam) RC_EXTEND(false, true);
-                else RC_EXTEND(false, false);
-            }
-#undef RC_EXTEND
-            RC_TRY(c->prof_end(ev, PK_EXTEND));
-            // cold path: rays whose stack outgrew LDS
-            RC_TRY(rc_launch_deep(c, s, false, rin, c->hits.as<rc_hit>(), nullptr));
-            k_shade_listed<<<grid_small, kBlock, 0, c->stream>>>(P, k, rin, sin_, c->hits.as<rc_hit>(), s->shade,
-                                                                 s->mats, rout, sout, c->srays.as<rc_ray>(),
-                                                                 c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
-                                                                 c->counters, cout, CI);
+            k_prep<<<1, 1, 0, st>>>(ctr, cout);
+            if (timed) RC_TRY(c->prof_begin(&ev));
+            const bool camb = RC_FUSED_CAMERA && k == 0;
+            ExtendArgs ea{P, k, s, rin, sin_, rout, sout, srays, scontrib, L, ovf, ctr, cin, cout, CI, W};
+            launch_extend(ea, n, st, c->counting, camb, c->hit_fp64, shape.ext_block);
+            if (timed) RC_TRY(c->prof_end(ev, PK_EXTEND));
+            // cold path: rays whose stack outgrew the on-chip depth
+            deep.any = false;
+            deep.rays = rin;
+            deep.hits = B.hits.as<rc_hit>();
+            deep.occ = nullptr;
+            RC_TRY(rc_launch_deep_with(deep));
+            k_shade_listed<<<grid_small, kBlock, 0, st>>>(P, k, rin, sin_, B.hits.as<rc_hit>(), s->shade, s->mats, rout,
+                                                          sout, srays, scontrib, L, ovf, ctr, cout, CI);
             if (P.has_sun && k + 1 < cfg->max_path_length) {
                 // at the cache vertex the shadow results are kept for the training records
                 const bool nee_fix = CI.cache_vertex && k == CI.cache_vertex;
-                uint8_t* occ_out = nee_fix ? c->socc.as<uint8_t>() : nullptr;
-                k_reset_ovf<<<1, 1, 0, c->stream>>>(c->counters);
-                RC_TRY(c->prof_begin(&ev));
-                if (c->counting) {
-                    k_shadow<true><<<grid_t, kTraceBlock, 0, c->stream>>>(
-                        s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
-                        c->counters, occ_out);
-                } else {
-                    k_shadow<false><<<grid_t, kTraceBlock, 0, c->stream>>>(
-                        s->nodes, s->tris, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), c->slotL.as<float4>(), ovf,
-                        c->counters, occ_out);
-                }
-                RC_TRY(c->prof_end(ev, PK_SHADOW));
-                RC_TRY(rc_launch_deep(c, s, true, c->srays.as<rc_ray>(), nullptr, c->socc.as<uint8_t>()));
-                k_shadow_apply_listed<<<grid_small, kBlock, 0, c->stream>>>(
-                    c->scontrib.as<float4>(), c->socc.as<uint8_t>(), ovf, c->slotL.as<float4>(), c->counters);
+                uint8_t* occ_out = nee_fix ? socc : nullptr;
+                k_reset_ovf<<<1, 1, 0, st>>>(ctr);
+                if (timed) RC_TRY(c->prof_begin(&ev));
+                ShadowArgs sa_{s, srays, scontrib, L, ovf, ctr, occ_out, P.stack_limit};
+                launch_shadow(sa_, n, st, c->counting, c->hit_fp64, shape.shadow_block);
+                if (timed) RC_TRY(c->prof_end(ev, PK_SHADOW));
+                deep.any = true;
+                deep.rays = srays;
+                deep.hits = nullptr;
+                deep.occ = socc;
+                RC_TRY(rc_launch_deep_with(deep));
+                k_shadow_apply_listed<<<grid_small, kBlock, 0, st>>>(scontrib, socc, ovf, L, ctr);
                 if (nee_fix)
-                    k_cache_nee_fix<<<grid_small, kBlock, 0, c->stream>>>(CI.records, CI.counters, CI.cap,
-                                                                         c->scontrib.as<float4>(),
-                                                                         c->socc.as<uint8_t>());
+                    k_cache_nee_fix<<<grid_small, kBlock, 0, st>>>(CI.records, CI.counters, CI.cap, scontrib, socc);
             }
             std::swap(rin, rout);
             std::swap(sin_, sout);
         }
         if (CI.cache_vertex) {
             // paths that stopped at the cache vertex: add thr * cache(x, w_o)
-            RC_TRY(c->prof_begin(&ev));
-            RC_TRY(rc_cache_apply_device(c, cache, CI.queries, &CI.counters[0], CI.cap, c->slotL.as<float4>()));
-            RC_TRY(c->prof_end(ev, PK_CACHE));
-            k_cache_targets<<<grid_small, kBlock, 0, c->stream>>>(CI.records, CI.counters, CI.cap,
-                                                                  c->slotL.as<float4>(), CI.batch_q, CI.batch_t,
-                                                                  CI.batch_cap);
+            if (timed) RC_TRY(c->prof_begin(&ev));
+            RC_TRY(rc_cache_apply_device(c, st, cache, CI.queries, &CI.counters[0], CI.cap, L));
+            if (timed) RC_TRY(c->prof_end(ev, PK_CACHE));
+            k_cache_targets<<<grid_small, kBlock, 0, st>>>(CI.records, CI.counters, CI.cap, L, CI.batch_q, CI.batch_t,
+                                                          CI.batch_cap);
         }
-        RC_TRY(c->prof_begin(&ev));
-        launch_film_add(c->stream, P.order, npix, this_pass, c->slotL.as<float4>(), c->slotW.as<float>(), f->sum,
-                        f->comp);
-        RC_TRY(c->prof_end(ev, PK_OTHER));
+        if (timed) RC_TRY(c->prof_begin(&ev));
+        launch_film_add(st, P.order, npix, this_pass, L, W, f->sum, f->comp);
+        if (timed) RC_TRY(c->prof_end(ev, PK_OTHER));
         RC_TRY_HIP(hipGetLastError());
     }
     return RC_OK;
@@ -989,22 +1084,30 @@ int check_frame_args(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
     return RC_OK;
 }
 
-// One frame, synchronously, with the statistics of rc_get_stats.
+// One frame, synchronously, with the statistics of rc_get_stats.  With a
+// cache, `train` optimizer steps on the frame's training samples are queued
+// on the cache's own stream behind the frame (the next frame's traversal runs
+// while they do; its cache inference waits for them).
 int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                 rc_progress_fn progress, void* user, rc_cache* cache) {
+                 rc_progress_fn progress, void* user, rc_cache* cache, uint32_t train) {
     RC_TRY_HIP(hipSetDevice(c->device));
     auto t0 = std::chrono::steady_clock::now();
     c->stats.extend_ms = c->stats.shadow_ms = c->stats.shade_ms = c->stats.other_ms = 0.0;
-    c->stats.cache_ms = c->stats.train_ms = 0.0;
+    c->stats.cache_ms = 0.0;
     c->stats.extend_launches = c->stats.shadow_launches = 0;
     c->stats.cache_queries = c->stats.train_samples = 0;
-    k_zero_counters<<<1, 1, 0, c->stream>>>(c->counters);
+    rc_zero_counters(c->counters, c->stream);
+    FrameShape shape;
+    shape.spp_pass = pass_samples(c->tuning, cfg->width * cfg->height, cfg->spp, kPassPaths);
+    RC_TRY(c->fb.ensure(rc_pass_paths(cfg, shape.spp_pass)));
     CacheIO io{};
-    const uint32_t spp_pass = pass_samples(c->tuning, cfg->width * cfg->height, cfg->spp, kPassPaths);
     if (cache)
-        RC_TRY(rc_cache_frame_io(c, cache, cfg->cache_vertex, size_t(cfg->width) * cfg->height * spp_pass,
+        RC_TRY(rc_cache_frame_io(c, c->stream, cache, cfg->cache_vertex, rc_pass_paths(cfg, shape.spp_pass),
                                  uint64_t(cfg->width) * cfg->height * cfg->spp, &io));
-    int rc = rc_enqueue_frame(c, s, f, cam, cfg, spp_pass, progress, user, cache ? &io : nullptr, cache);
+    int rc = rc_enqueue_frame(c, c->fb, c->stream, s, f, cam, cfg, shape, progress, user, cache ? &io : nullptr,
+                              cache);
+    uint32_t cache_counts[2] = {0, 0};
+    if (rc == RC_OK && cache) rc = rc_cache_frame_end(c, c->stream, cache, train, cache_counts);
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     if (rc != RC_OK) return rc;
     DevCounters h;
@@ -1017,7 +1120,11 @@ int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
     c->stats.samples = uint64_t(cfg->width) * cfg->height * cfg->spp;
     c->stats.node_records = h.node_records;
     c->stats.tri_tests = h.tri_tests;
-    c->stats.overflow_rays = h.ovf_count;
+    c->stats.overflow_rays = h.ovf_total;
+    if (cache) {
+        c->stats.cache_queries = cache_counts[0];
+        c->stats.train_samples = train ? cache_counts[1] : 0;
+    }
     if (progress) progress(user, uint64_t(cfg->width) * cfg->height);
     return RC_OK;
 }
@@ -1026,7 +1133,7 @@ int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
 extern "C" int rc_render_progress(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
                                   const rc_render_cfg* cfg, rc_progress_fn progress, void* user) {
     RC_TRY(check_frame_args(c, s, f, cam, cfg, "rc_render"));
-    return render_frame(c, s, f, cam, cfg, progress, user, nullptr);
+    return render_frame(c, s, f, cam, cfg, progress, user, nullptr, 0);
 }
 
 extern "C" int rc_render_cached(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
@@ -1036,15 +1143,9 @@ extern "C" int rc_render_cached(rc_ctx* c, const rc_scene* s, rc_film* f, const 
     if (cfg->cache_vertex == 0 || cfg->cache_vertex + 1 >= cfg->max_path_length)
         return rc_host_fail(RC_ERR_INVALID,
                             "rc_render_cached: cache_vertex must satisfy 1 <= cache_vertex < max_path_length - 1");
-    RC_TRY(render_frame(c, s, f, cam, cfg, nullptr, nullptr, cache));
-    float l = 0.0f;
-    uint32_t n = 0, q = 0;
-    auto t0 = std::chrono::steady_clock::now();
-    RC_TRY(rc_cache_train_frame(c, cache, train > 0 ? uint32_t(std::min(train, 16)) : 0u, &l, &n, &q));
-    if (train) c->stats.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    c->stats.train_samples = train ? n : 0;
-    c->stats.cache_queries = q;
-    if (loss) *loss = l;
+    if (train < 0 || train > 16) return rc_host_fail(RC_ERR_INVALID, "rc_render_cached: train must be in [0, 16]");
+    RC_TRY(render_frame(c, s, f, cam, cfg, nullptr, nullptr, cache, uint32_t(train)));
+    if (loss) RC_TRY(rc_cache_last_loss(c, cache, loss));
     return RC_OK;
 }
 
@@ -1118,10 +1219,19 @@ __global__ __launch_bounds__(kBlock) void k_phong_prepare(uint32_t npix, LightPa
     rgba[p] = make_float4(col.x, col.y, col.z, 1.0f);
 }
 
+// Adds the unoccluded direct light and splits the hit records into the
+// entry point's hit-index and distance arrays (either may be null).
 __global__ __launch_bounds__(kBlock) void k_phong_finish(uint32_t npix, const uint8_t* __restrict__ occ,
-                                                         const float4* __restrict__ direct, float4* __restrict__ rgba) {
+                                                         const float4* __restrict__ direct, float4* __restrict__ rgba,
+                                                         const rc_hit* __restrict__ hits, int32_t* __restrict__ ids,
+                                                         float* __restrict__ dist) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npix) return;
+    if (ids || dist) {
+        const float4 h = reinterpret_cast<const float4*>(hits)[p];
+        if (ids) ids[p] = __float_as_int(h.w);
+        if (dist) dist[p] = h.x;
+    }
     if (occ[p]) return;
     float4 c = rgba[p], dd = direct[p];
     rgba[p] = make_float4(c.x + dd.x, c.y + dd.y, c.z + dd.z, 1.0f);
@@ -1139,7 +1249,7 @@ extern "C" int rc_raycast_phong(rc_ctx* c, const rc_scene* s, const rc_camera* c
     auto t0 = std::chrono::steady_clock::now();
     c->stats = rc_stats{};  // this call's statistics only
     const uint32_t npix = w * h;
-    RC_TRY(alloc_pass(c, npix));
+    RC_TRY(c->fb.ensure(npix));
     CamBasis B = make_basis(*cam, w, h);
     LightParams Lp;
     Lp.pos = make_float3(light->position[0], light->position[1], light->position[2]);
@@ -1155,30 +1265,29 @@ extern "C" int rc_raycast_phong(rc_ctx* c, const rc_scene* s, const rc_camera* c
     k_zero_counters<<<1, 1, 0, c->stream>>>(c->counters);
     hipEvent_t ev;
     RC_TRY(c->prof_begin(&ev));
-    k_cast_camera<<<grid, kBlock, 0, c->stream>>>(B, w, npix, seed, c->ray_a.as<rc_ray>());
+    k_cast_camera<<<grid, kBlock, 0, c->stream>>>(B, w, npix, seed, c->fb.ray_a.as<rc_ray>());
     RC_TRY(c->prof_end(ev, PK_OTHER));
-    RC_TRY(rc_launch_trace(c, s, false, c->ray_a.as<rc_ray>(), nullptr, npix, c->hits.as<rc_hit>(), nullptr));
-    float4* rgba_d = c->slotL.as<float4>();
+    FrameBuffers& FB = c->fb;
+    RC_TRY(rc_launch_trace(c, s, false, FB.ray_a.as<rc_ray>(), nullptr, npix, FB.hits.as<rc_hit>(), nullptr));
+    float4* rgba_d = FB.slotL.as<float4>();
     RC_TRY(c->prof_begin(&ev));
-    k_phong_prepare<<<grid, kBlock, 0, c->stream>>>(npix, Lp, c->ray_a.as<rc_ray>(), c->hits.as<rc_hit>(), s->shade,
-                                                    s->mats, c->srays.as<rc_ray>(), c->scontrib.as<float4>(), rgba_d);
+    k_phong_prepare<<<grid, kBlock, 0, c->stream>>>(npix, Lp, FB.ray_a.as<rc_ray>(), FB.hits.as<rc_hit>(), s->shade,
+                                                    s->mats, FB.srays.as<rc_ray>(), FB.scontrib.as<float4>(), rgba_d);
     RC_TRY(c->prof_end(ev, PK_SHADE));
-    RC_TRY(rc_launch_trace(c, s, true, c->srays.as<rc_ray>(), nullptr, npix, nullptr, c->socc.as<uint8_t>()));
+    RC_TRY(rc_launch_trace(c, s, true, FB.srays.as<rc_ray>(), nullptr, npix, nullptr, FB.socc.as<uint8_t>()));
+    // the hit indices and distances go to two device arrays (the queue
+    // buffers, free by now) and from there straight to the caller's arrays
+    int32_t* ids_d = hit_ids ? reinterpret_cast<int32_t*>(FB.st_a.p) : nullptr;
+    float* t_d = t ? reinterpret_cast<float*>(FB.st_b.p) : nullptr;
     RC_TRY(c->prof_begin(&ev));
-    k_phong_finish<<<grid, kBlock, 0, c->stream>>>(npix, c->socc.as<uint8_t>(), c->scontrib.as<float4>(), rgba_d);
+    k_phong_finish<<<grid, kBlock, 0, c->stream>>>(npix, FB.socc.as<uint8_t>(), FB.scontrib.as<float4>(), rgba_d,
+                                                   FB.hits.as<rc_hit>(), ids_d, t_d);
     RC_TRY(c->prof_end(ev, PK_OTHER));
     RC_TRY_HIP(hipGetLastError());
-    std::vector<rc_hit> hh;
-    if (hit_ids || t) {
-        hh.resize(npix);
-        RC_TRY_HIP(hipMemcpyAsync(hh.data(), c->hits.p, npix * sizeof(rc_hit), hipMemcpyDeviceToHost, c->stream));
-    }
+    if (hit_ids) RC_TRY_HIP(hipMemcpyAsync(hit_ids, ids_d, npix * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (t) RC_TRY_HIP(hipMemcpyAsync(t, t_d, npix * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (rgba) RC_TRY_HIP(hipMemcpyAsync(rgba, rgba_d, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
-    for (uint32_t i = 0; i < npix && !hh.empty(); ++i) {
-        if (hit_ids) hit_ids[i] = hh[i].tri;
-        if (t) t[i] = hh[i].t;
-    }
     RC_TRY(c->prof_collect());
     c->stats.extension_rays = npix;
     c->stats.shadow_rays = npix;

@@ -651,7 +651,6 @@ struct Run {
     int trace_and_count(uint32_t k, uint32_t n) {
         WhittedBufs::Level& Lv = W->levels[k];
         RC_TRY(rc_launch_trace(c, s, false, Lv.rays.as<rc_ray>(), nullptr, n, Lv.hits.as<rc_hit>(), nullptr));
-        RC_TRY(rc_launch_deep(c, s, false, Lv.rays.as<rc_ray>(), Lv.hits.as<rc_hit>(), nullptr));
         ext_rays += n;
         Counts C{Lv.nc.as<uint32_t>(), Lv.ns.as<uint32_t>(), Lv.na.as<uint32_t>(), Lv.dec.as<uint4>()};
         k_wt_count<<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(P, k, n, level_in(k), s->shade, s->mats, wm,
@@ -714,7 +713,6 @@ struct Run {
         RC_TRY_HIP(hipGetLastError());
         if (tot[1]) {
             RC_TRY(rc_launch_trace(c, s, true, W->srays.as<rc_ray>(), nullptr, tot[1], nullptr, W->socc.as<uint8_t>()));
-            RC_TRY(rc_launch_deep(c, s, true, W->srays.as<rc_ray>(), nullptr, W->socc.as<uint8_t>()));
             k_wt_sum<<<(tot[1] + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(tot[1], W->sval.as<float4>(),
                                                                                W->socc.as<uint8_t>(), nullptr, 1, L);
             shadow_rays += tot[1];
@@ -722,7 +720,6 @@ struct Run {
         if (tot[2]) {
             // AO rays are not shadow rays: nonShadowing materials occlude them (Raytracer.cpp:470-489)
             RC_TRY(rc_launch_trace(c, s, false, W->arays.as<rc_ray>(), nullptr, tot[2], W->ahits.as<rc_hit>(), nullptr));
-            RC_TRY(rc_launch_deep(c, s, false, W->arays.as<rc_ray>(), W->ahits.as<rc_hit>(), nullptr));
             k_wt_sum<<<(tot[2] + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(tot[2], W->aval.as<float4>(), nullptr,
                                                                                W->ahits.as<rc_hit>(), 2, L);
             shadow_rays += tot[2];
@@ -790,6 +787,7 @@ extern "C" int rc_raytrace(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_ca
     if (!c->whitted) c->whitted = new WhittedBufs();
     WhittedBufs* W = c->whitted;
     auto t0 = std::chrono::steady_clock::now();
+    rc_zero_counters(c->counters, c->stream);
 
     // device copies of the materials and lights
     std::vector<WMaterial> wm(std::max<uint32_t>(nmat, 1));
@@ -861,11 +859,11 @@ extern "C" int rc_raytrace(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_ca
     RC_TRY(W->aval.ensure(cap * sizeof(float4)));
     RC_TRY(W->ahits.ensure(cap * sizeof(rc_hit)));
     RC_TRY(W->totals.ensure(4 * sizeof(uint32_t)));
-    RC_TRY(c->slotL.ensure(n0max * sizeof(float4)));
-    RC_TRY(c->slotW.ensure(n0max * sizeof(float)));
-    RC_TRY(c->ovf_list.ensure(cap * sizeof(uint32_t)));
-    RC_TRY(c->deep_stack.ensure(size_t(2048) * kDeepStack * sizeof(uint32_t)));
-    run.L = c->slotL.as<float4>();
+    RC_TRY(c->fb.slotL.ensure(n0max * sizeof(float4)));
+    RC_TRY(c->fb.slotW.ensure(n0max * sizeof(float)));
+    RC_TRY(c->fb.ovf_list.ensure(cap * sizeof(uint32_t)));
+    RC_TRY(c->fb.deep_stack.ensure(rc_deep_stack_bytes()));
+    run.L = c->fb.slotL.as<float4>();
 
     for (uint32_t s0 = 0; s0 < cfg->spp; s0 += spp_pass) {
         const uint32_t this_pass = std::min(spp_pass, cfg->spp - s0);
@@ -874,12 +872,12 @@ extern "C" int rc_raytrace(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_ca
         P.spp_pass = this_pass;
         WhittedBufs::Level& L0 = W->levels[0];
         RC_TRY(rc_launch_camera(c, cam, cfg, s0, this_pass, L0.rays.as<rc_ray>(), L0.A.as<float4>(), run.L,
-                                c->slotW.as<float>()));
+                                c->fb.slotW.as<float>()));
         k_wt_root<<<(n0 + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(n0, L0.keys.as<uint32_t>(), L0.B.as<float4>());
         RC_TRY_HIP(hipGetLastError());
         RC_TRY(run.trace_and_count(0, n0));
         RC_TRY(run.process(0, 0, n0));
-        RC_TRY(rc_launch_film_add(c, f, cfg, this_pass, run.L, c->slotW.as<float>()));
+        RC_TRY(rc_launch_film_add(c, f, cfg, this_pass, run.L, c->fb.slotW.as<float>()));
     }
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     DevCounters h;
@@ -889,6 +887,7 @@ extern "C" int rc_raytrace(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_ca
     c->stats.extension_rays = run.ext_rays;
     c->stats.shadow_rays = run.shadow_rays;
     c->stats.samples = uint64_t(P.npix) * cfg->spp;
+    c->stats.overflow_rays = h.ovf_total;
     c->stats.render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RC_OK;
 }

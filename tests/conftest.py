@@ -25,3 +25,12 @@ def ctx():
     c = ra.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(params=["fp32", "fp64"])
+def decision(request, ctx):
+    """Runs a test under each triangle decision mode (RC_HIT_FP32, the
+    default, and RC_HIT_FP64); yields True for FP32."""
+    ctx.set_hit_decision(request.param)
+    yield request.param == "fp32"
+    ctx.set_hit_decision("fp32")

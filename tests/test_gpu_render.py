@@ -18,7 +18,6 @@ import oracle as orc
 import raycer_amd as ra
 from raycer_amd import api
 from tests.conftest import ORACLE_THREADS
-from tests.helpers import check_closest
 
 pytestmark = pytest.mark.gpu
 
@@ -324,6 +323,31 @@ def test_pass_size_does_not_change_the_image(ctx):
         np.testing.assert_allclose(other, imgs[0], rtol=1e-6, atol=1e-7)
 
 
+def test_deep_stack_path_gives_the_same_image(ctx, decision):
+    """With the on-chip traversal stack limited to two entries most extension
+    and shadow rays overflow: the deep-stack kernel walks them again, the
+    listed-ray shading and shadow kernels finish them, and the camera bounce
+    stores the camera rays it made for them.  Samples are keyed by pixel,
+    sample and vertex, so the image must be bit-identical to the on-chip
+    render, and the overflow statistic counts the listed rays of every launch."""
+    sd, s, _ = scene(ctx, "terrain")
+    cfg = cfg_for(sd, 64, 48, 4)
+    lin0, sums0 = render_gpu(ctx, s, sd.desc.camera, cfg)
+    ovf0 = ctx.stats().overflow_rays
+    ctx.set_tuning(stack_limit=2)
+    try:
+        lin1, sums1 = render_gpu(ctx, s, sd.desc.camera, cfg)
+        st = ctx.stats()
+    finally:
+        ctx.set_tuning()
+    assert sums1.tobytes() == sums0.tobytes()
+    total = st.extension_rays + st.shadow_rays
+    print(f"overflow rays: {st.overflow_rays} of {total} (default stack: {ovf0})")
+    assert st.overflow_rays > 0.2 * st.extension_rays
+    assert st.overflow_rays <= total
+    assert ovf0 < st.overflow_rays
+
+
 def test_interrupt_flag(ctx):
     import ctypes
     sd, s, _ = scene(ctx, "box")
@@ -364,7 +388,11 @@ def test_tuning_validation(ctx):
     try:
         for sp in (0, 1, 64):
             ctx.set_tuning(spp_per_pass=sp)
-        for kw in (dict(spp_per_pass=65), dict(block=96), dict(blocks_per_cu=0), dict(blocks_per_cu=65)):
+        for sl in (0, 1, 20):
+            ctx.set_tuning(stack_limit=sl)
+        ctx.set_tuning(spp_per_pass=64)
+        for kw in (dict(spp_per_pass=65), dict(block=96), dict(blocks_per_cu=0), dict(blocks_per_cu=65),
+                   dict(stack_limit=21)):
             with pytest.raises(ra.RaycerError):
                 ctx.set_tuning(**kw)
         # the last accepted value (64) still holds: a 16-spp frame is one pass

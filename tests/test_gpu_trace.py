@@ -2,9 +2,13 @@
 
 Bar: the closest-hit triangle index is bit-exact except rays the oracle lists
 as near ties (another triangle within 1e-6 relative distance, an edge inside
-the FP32 barycentric band, or a grazing determinant); the GPU's distance is
-within 1e-5 relative of the FP64 Möller–Trumbore distance to the triangle it
-reports.  Any-hit decisions are exact except rays the oracle lists as fragile.
+the barycentric band, or a grazing determinant); the GPU's distance is within
+1e-5 relative of the FP64 Möller–Trumbore distance to the triangle it reports.
+Any-hit decisions are exact except rays the oracle lists as fragile.  Each
+parity test runs under both triangle decision modes: FP64 (the bar above
+as stated) and FP32 (the default), where the listing band is its rounding
+band, 1e-4, and the distance bound max(1e-5, 16 * 2^-24 / cond) for a hit of
+conditioning cond (tests/helpers.py).
 """
 import numpy as np
 import pytest
@@ -13,7 +17,7 @@ import oracle as orc
 import raycer_amd as ra
 from raycer_amd import api
 from tests.conftest import ORACLE_THREADS
-from tests.helpers import camera_rays, check_closest, mt_fp64, random_rays
+from tests.helpers import FP32_BAND, camera_rays, check_closest, closest_diag, mt_fp64, random_rays
 
 pytestmark = pytest.mark.gpu
 
@@ -35,28 +39,28 @@ def scene(ctx, name):
 
 
 @pytest.mark.parametrize("name", ["box", "sphere5", "terrain128", "soup"])
-def test_closest_hit_camera_rays(ctx, name):
+def test_closest_hit_camera_rays(ctx, decision, name):
     sd, s, ob = scene(ctx, name)
     w, h = 320, 240
     rng = np.random.default_rng(3)
     rays = camera_rays(sd.desc.camera, w, h, jitter=rng.random((w * h, 2)))
     hits = ctx.trace_closest(s, rays)
-    diag = ob.trace_diag(rays, threads=ORACLE_THREADS)
-    mism, listed = check_closest(hits, sd.vertices, rays, diag)
+    diag = closest_diag(ob, rays, decision, ORACLE_THREADS)
+    mism, listed = check_closest(hits, sd.vertices, rays, diag, fp32=decision)
     print(f"{name}: {mism} mismatches, {listed} listed near ties of {len(rays)}")
     assert (hits["tri"] >= 0).mean() > 0.3
 
 
 @pytest.mark.parametrize("name", ["box", "sphere5", "soup"])
-def test_closest_hit_random_rays(ctx, name):
+def test_closest_hit_random_rays(ctx, decision, name):
     sd, s, ob = scene(ctx, name)
     v = sd.vertices.reshape(-1, 3)
     rays = random_rays(60000, 7, v.min(0), v.max(0))
     hits = ctx.trace_closest(s, rays)
     # in the box the inner boxes' bottom faces are coplanar with the floor:
     # rays from below hit both at the same t, exact ties the oracle lists
-    check_closest(hits, sd.vertices, rays, ob.trace_diag(rays, threads=ORACLE_THREADS),
-                  max_listed=0.05 if name == "box" else 0.01)
+    check_closest(hits, sd.vertices, rays, closest_diag(ob, rays, decision, ORACLE_THREADS),
+                  max_listed=0.05 if name == "box" else 0.01, fp32=decision)
 
 
 @pytest.mark.parametrize("name,method,max_leaf", [("sphere5", api.RCH_BVH_BINNED, 5), ("soup", api.RCH_BVH_BINNED, 5),
@@ -84,18 +88,18 @@ def test_results_do_not_depend_on_the_bvh(ctx, name, method, max_leaf):
         s2.close()
 
 
-def test_closest_hit_matches_brute_force_without_bvh(ctx):
+def test_closest_hit_matches_brute_force_without_bvh(ctx, decision):
     sd, s, _ = scene(ctx, "box")
     rays = random_rays(5000, 9, [0, 0, 0], [1, 1, 1])
     hits = ctx.trace_closest(s, rays)
     t_bf, id_bf = orc.trace_brute(sd.vertices, rays, threads=ORACLE_THREADS)
     ob = orc.BVH(sd.vertices)
-    _, _, flags = ob.trace_diag(rays, threads=ORACLE_THREADS)
+    flags = closest_diag(ob, rays, decision, ORACLE_THREADS)[2]
     bad = (hits["tri"] != id_bf) & (flags == 0)
     assert not bad.any()
 
 
-def test_tmin_tmax_window(ctx):
+def test_tmin_tmax_window(ctx, decision):
     sd, s, ob = scene(ctx, "box")
     rays = camera_rays(sd.desc.camera, 64, 64)
     full = ctx.trace_closest(s, rays)
@@ -109,10 +113,11 @@ def test_tmin_tmax_window(ctx):
     assert np.all(h2["t"][hit] >= full["t"][hit] * 1.001)
     # past the first hit many rays run along the floor under the inner boxes,
     # whose bottom faces are coplanar with it (listed exact ties)
-    check_closest(h2, sd.vertices, win2, ob.trace_diag(win2, threads=ORACLE_THREADS), max_listed=0.05)
+    check_closest(h2, sd.vertices, win2, closest_diag(ob, win2, decision, ORACLE_THREADS), max_listed=0.05,
+                  fp32=decision)
 
 
-def test_axis_aligned_rays_and_miss(ctx):
+def test_axis_aligned_rays_and_miss(ctx, decision):
     sd, s, ob = scene(ctx, "box")
     g = np.linspace(0.05, 0.95, 17)
     org = np.array([[x, y, -1.0] for x in g for y in g], np.float32)
@@ -121,8 +126,8 @@ def test_axis_aligned_rays_and_miss(ctx):
     hits = ctx.trace_closest(s, rays)
     # the 17 rays with x == y hit the back wall exactly on the diagonal shared
     # by its two triangles: exact ties, resolved to the lower id
-    t_o, id_o, flags = ob.trace_diag(rays, threads=ORACLE_THREADS)
-    check_closest(hits, sd.vertices, rays, (t_o, id_o, flags), max_listed=0.1)
+    check_closest(hits, sd.vertices, rays, closest_diag(ob, rays, decision, ORACLE_THREADS), max_listed=0.1,
+                  fp32=decision)
     up =api.make_rays(np.float32([0.5, 2.0, 0.5]), np.float32([[0, 1, 0]]))
     assert ctx.trace_closest(s, up)["tri"][0] == -1
     assert np.isinf(ctx.trace_closest(s, up)["t"][0])
@@ -153,7 +158,7 @@ def test_exact_tie_goes_to_lower_original_id(ctx):
     s.close()
 
 
-def test_any_hit_parity(ctx):
+def test_any_hit_parity(ctx, decision):
     sd, s, ob = scene(ctx, "sphere5")
     w, h = 200, 150
     rays = camera_rays(sd.desc.camera, w, h)
@@ -167,7 +172,7 @@ def test_any_hit_parity(ctx):
     sr = api.make_rays(p + d * 1e-4, d.astype(np.float32), tmax=dist.astype(np.float32))
     sr["org"] = (p + d * 1e-4).astype(np.float32)
     occ = ctx.trace_any(s, sr)
-    o_occ, fragile = ob.trace_any(sr, threads=ORACLE_THREADS)
+    o_occ, fragile = ob.trace_any(sr, band=FP32_BAND if decision else 2e-5, threads=ORACLE_THREADS)
     bad = (occ != o_occ) & (fragile == 0)
     assert not bad.any(), np.nonzero(bad)[0][:10]
     assert 0.05 < occ.mean() < 0.95
@@ -223,7 +228,7 @@ def test_upload_rejects_corrupt_bvh(ctx):
 
 
 @pytest.mark.slow
-def test_full_size_sphere_ground_ray_set(ctx):
+def test_full_size_sphere_ground_ray_set(ctx, decision):
     """The 1,179,648-triangle scene at 1920x1080 with per-pixel jitter: every
     primary ray checked against the FP64 oracle."""
     p = ra.scene_params(api.RCH_SCENE_SPHERE_GROUND, seed=1)
@@ -235,7 +240,7 @@ def test_full_size_sphere_ground_ray_set(ctx):
     rays = camera_rays(sd.desc.camera, w, h, jitter=rng.random((w * h, 2)))
     hits = ctx.trace_closest(s, rays)
     ob = orc.BVH(sd.vertices)
-    mism, listed = check_closest(hits, sd.vertices, rays, ob.trace_diag(rays, threads=ORACLE_THREADS),
-                                 max_listed=0.005)
+    mism, listed = check_closest(hits, sd.vertices, rays, closest_diag(ob, rays, decision, ORACLE_THREADS),
+                                 max_listed=0.005 if not decision else 0.02, fp32=decision)
     print(f"full C2 ray set: {mism} mismatches, {listed} listed of {w * h}")
     s.close()

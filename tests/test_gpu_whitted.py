@@ -83,8 +83,20 @@ def test_whitted_replays_the_oracle(ctx, sampler):
     film2 = ctx.film(48, 48)
     ctx.raytrace(s, film2, sd.desc.camera, cfg, rs, rm, lights)
     assert film2.linear().tobytes() == lin.tobytes()
+    # every ray re-walked by the deep-stack kernel (on-chip stack of one
+    # entry): the same image, each listed ray walked and counted once
+    film3 = ctx.film(48, 48)
+    ctx.set_tuning(stack_limit=1)
+    try:
+        ctx.raytrace(s, film3, sd.desc.camera, cfg, rs, rm, lights)
+        st3 = ctx.stats()
+    finally:
+        ctx.set_tuning()
+    assert film3.linear().tobytes() == lin.tobytes()
+    assert 0.2 * st3.extension_rays < st3.overflow_rays <= st3.extension_rays + st3.shadow_rays
     film.close()
     film2.close()
+    film3.close()
     s.close()
 
 
