@@ -466,9 +466,10 @@ public:
         return s;
     }
 
-    // Scene::loadFromFile / saveToFile (Scene.cpp:67-183): cereal XML or JSON
-    // by the file's extension.  Triangles, materials, lights, settings, the
-    // model list and a built root BVH; other primitive types are skipped.
+    // Scene::loadFromFile / saveToFile (Scene.cpp:67-183): cereal XML, JSON or
+    // binary by the file's extension (.xml, .json, .bin).  Triangles,
+    // materials, lights, settings, the model list and a built root BVH; other
+    // primitive types are skipped (XML, JSON) or rejected (binary).
     static Scene loadFromFile(const std::string& path) {
         rch_scene_file_data* d = nullptr;
         check(rch_scene_file_load(path.c_str(), &d));

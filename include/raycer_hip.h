@@ -200,9 +200,11 @@ typedef struct rc_stats {
     double render_ms;            /* wall time of the last render call */
     uint64_t shadow_launches;
     double cache_ms;             /* radiance-cache inference kernels (profiling on) */
-    double train_ms;             /* radiance-cache training step of the last cached render */
+    double train_ms;             /* optimizer steps after the last cached frame (GPU time on the cache's stream) */
     uint64_t cache_queries;      /* paths that ended into the cache */
     uint64_t train_samples;      /* training samples of the last cached render */
+    double graph_ms;             /* last batch call: capture and instantiation or update (0: graph relaunched) */
+    uint64_t graph_relaunched;   /* last batch call: 1 when it relaunched the executable graph as it was */
 } rc_stats;
 
 /* ------------------------------------------------------------ context */
@@ -432,13 +434,32 @@ int rc_cache_gradient(rc_ctx* ctx, rc_cache* cache, const float* queries, const 
 int rc_render_cached(rc_ctx* ctx, const rc_scene* scene, rc_film* film, const rc_camera* cam,
                      const rc_render_cfg* cfg, rc_cache* cache, int train, float* loss);
 
-/* Many scenes in one captured HIP graph, launched once: scenes[i] is
- * rendered into films[i] with cams[i] and cfgs[i]. */
+/* Many scenes in one captured HIP graph: scenes[i] is rendered into
+ * films[i] with cams[i] and cfgs[i].  The executable graph is kept: a call
+ * with the same batch (scenes, films, cameras, configurations, plans, context
+ * settings) relaunches it without capturing; a changed batch is captured
+ * again and the executable graph updated in place (instantiated anew when its
+ * shape changed).  rc_stats.graph_ms / graph_relaunched tell which. */
 int rc_render_batch(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* const* films, const rc_camera* cams,
                     const rc_render_cfg* cfgs, size_t nscenes, const rc_tuning* tuning);
 /* The same with samples per pass chosen per scene (spp_per_pass[i] in [1, 64]). */
 int rc_render_batch_plan(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* const* films, const rc_camera* cams,
                          const rc_render_cfg* cfgs, size_t nscenes, const uint32_t* spp_per_pass);
+/* Launch plan of one scene of a batch: the wavefront queue length (samples
+ * per pass: a pass's queue holds pixels x spp_per_pass paths) and the block
+ * shapes of its traversal kernels. */
+typedef struct rc_batch_plan {
+    uint32_t spp_per_pass;   /* 1..64 */
+    uint32_t ext_block;      /* threads per block of the extension kernel: 128 or 256 (0 = 256) */
+    uint32_t shadow_block;   /* of the shadow kernel: 64, 128 or 256 (0 = 128) */
+    uint32_t pad;
+} rc_batch_plan;
+/* rc_render_batch with one plan per scene and the frames spread over
+ * `branches` (1..8) concurrent graph branches, each with its own wavefront
+ * buffers: frames are dealt longest first (estimated cost pixels x spp x
+ * log2 triangles) to the least loaded branch, and run in that order. */
+int rc_render_batch_tuned(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* const* films, const rc_camera* cams,
+                          const rc_render_cfg* cfgs, size_t nscenes, const rc_batch_plan* plans, uint32_t branches);
 /* Autotuner for rc_render_batch_plan: per size class (ceil(log2(pixels))) it
  * renders one scene of the class with 1, 2, 4, ... 64 samples per pass (at
  * most the frame's spp and 2^25 paths per pass) and keeps the fastest.  With a
@@ -446,6 +467,17 @@ int rc_render_batch_plan(rc_ctx* ctx, const rc_scene* const* scenes, rc_film* co
  * "<device>/<class>".  Writes one entry per scene into spp_per_pass. */
 int rc_autotune_batch(rc_ctx* ctx, const rc_scene* const* scenes, const rc_camera* cams, const rc_render_cfg* cfgs,
                       size_t nscenes, const char* cache_path, uint32_t* spp_per_pass);
+/* Autotuner for rc_render_batch_tuned: per size class, the samples per pass
+ * as rc_autotune_batch chooses them, then the block shapes (extension 128 or
+ * 256, shadow 64, 128 or 256 threads) timed at those samples per pass; then,
+ * for the whole batch, the branch count (1, 2, 4, 8) timed as relaunches of
+ * the captured graph.  A candidate replaces the default only when it is 2%
+ * faster.  Choices are read from and added to the JSON cache (keys
+ * "<device>/<class>/spp|ext|shadow" and "<device>/branches/<nscenes>").
+ * Writes plans[nscenes] and *branches. */
+int rc_autotune_batch_plan(rc_ctx* ctx, const rc_scene* const* scenes, const rc_camera* cams,
+                           const rc_render_cfg* cfgs, size_t nscenes, const char* cache_path, rc_batch_plan* plans,
+                           uint32_t* branches);
 
 #ifdef __cplusplus
 }

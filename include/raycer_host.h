@@ -117,11 +117,11 @@ int rch_obj_free(rch_obj* model);
 
 /* ------------------------------------------------------------ scene files
  * Scene::loadFromFile / saveToFile (src/Raytracing/Scene.cpp:67-183) for the
- * cereal XML (.xml) and JSON (.json) forms of Scene::serialize (Scene.h:330-345):
+ * cereal XML (.xml), JSON (.json) and binary (.bin) forms of Scene::serialize (Scene.h:330-345):
  * the settings, camera, tone mapper, simple fog, lights, materials, triangle
  * primitives, the model list and a built root BVH.  Other primitive types
  * (planes, spheres, boxes, ...) are counted and skipped, as are textures and
- * the volumetric fog; cereal's binary form is not read. */
+ * the volumetric fog.  The binary form (.bin) holds triangle scenes only. */
 typedef struct rch_scene_model {   /* ModelLoaderInfo (src/Utils/ModelLoader.h:27-85) */
     char path[1024];               /* modelFilePath */
     rch_obj_info info;             /* scale, rotate, translate, baseMaterial, ignoreMaterials */
@@ -155,11 +155,14 @@ typedef struct rch_scene_file_data {
     size_t skipped_primitives;     /* non-triangle primitives and invisible triangles in the file */
 } rch_scene_file_data;
 
-/* Reads a scene file; the result is freed with rch_scene_file_free. */
+/* Reads a scene file (.xml, .json or .bin); the result is freed with rch_scene_file_free. */
 int rch_scene_file_load(const char* path, rch_scene_file_data** out);
-/* Parses a scene from text (json != 0: JSON, else XML). */
-int rch_scene_file_parse(const char* text, size_t len, int json, rch_scene_file_data** out);
-/* Writes a scene (.xml or .json by the path's extension) in cereal's layout. */
+/* Parses a scene from memory: format RCH_SCENE_XML (0), RCH_SCENE_JSON (1)
+ * or RCH_SCENE_BINARY (2, cereal's BinaryOutputArchive: Scene.cpp:88-90,
+ * 144-146; files holding textures or non-triangle primitives are rejected). */
+enum { RCH_SCENE_XML = 0, RCH_SCENE_JSON = 1, RCH_SCENE_BINARY = 2 };
+int rch_scene_file_parse(const char* text, size_t len, int format, rch_scene_file_data** out);
+/* Writes a scene (.xml, .json or .bin by the path's extension) in cereal's layout. */
 int rch_scene_file_save(const char* path, const rch_scene_file_data* scene);
 void rch_scene_file_free(rch_scene_file_data* scene);
 

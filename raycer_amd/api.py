@@ -170,7 +170,14 @@ class Stats(C.Structure):
                 ("extend_launches", C.c_uint64), ("extend_ms", C.c_double), ("shadow_ms", C.c_double),
                 ("shade_ms", C.c_double), ("other_ms", C.c_double), ("render_ms", C.c_double),
                 ("shadow_launches", C.c_uint64), ("cache_ms", C.c_double), ("train_ms", C.c_double),
-                ("cache_queries", C.c_uint64), ("train_samples", C.c_uint64)]
+                ("cache_queries", C.c_uint64), ("train_samples", C.c_uint64), ("graph_ms", C.c_double),
+                ("graph_relaunched", C.c_uint64)]
+
+
+class BatchPlan(C.Structure):
+    """rc_batch_plan: one scene's samples per pass and kernel block shapes."""
+    _fields_ = [("spp_per_pass", C.c_uint32), ("ext_block", C.c_uint32), ("shadow_block", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class CacheCfg(C.Structure):
@@ -273,6 +280,8 @@ def lib() -> C.CDLL:
             "rc_render_batch": [vp, vp, vp, vp, vp, sz, C.POINTER(Tuning)],
             "rc_render_batch_plan": [vp, vp, vp, vp, vp, sz, vp],
             "rc_autotune_batch": [vp, vp, vp, vp, sz, C.c_char_p, vp],
+            "rc_render_batch_tuned": [vp, vp, vp, vp, vp, sz, vp, u32],
+            "rc_autotune_batch_plan": [vp, vp, vp, vp, sz, C.c_char_p, vp, C.POINTER(u32)],
             "rc_copy_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
             "rc_read_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
             "rc_render_progress": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), vp, vp],
@@ -493,22 +502,27 @@ def _scene_file(ptr) -> SceneFile:
         lib().rch_scene_file_free(ptr)
 
 
+RCH_SCENE_XML, RCH_SCENE_JSON, RCH_SCENE_BINARY = 0, 1, 2
+
+
 def load_scene_file(path) -> SceneFile:
-    """rch_scene_file_load: a cereal XML or JSON scene file (Scene::loadFromFile)."""
+    """rch_scene_file_load: a cereal XML, JSON or binary (.bin) scene file (Scene::loadFromFile)."""
     p = C.POINTER(SceneFileData)()
     check(lib().rch_scene_file_load(str(path).encode(), C.byref(p)))
     return _scene_file(p)
 
 
-def parse_scene_file(text: str, json: bool) -> SceneFile:
-    b = text.encode()
+def parse_scene_file(text, json: bool = False, binary: bool = False) -> SceneFile:
+    """rch_scene_file_parse from memory: text (XML or JSON) or bytes (cereal binary)."""
+    b = text if isinstance(text, (bytes, bytearray)) else text.encode()
+    fmt = RCH_SCENE_BINARY if binary else RCH_SCENE_JSON if json else RCH_SCENE_XML
     p = C.POINTER(SceneFileData)()
-    check(lib().rch_scene_file_parse(b, len(b), int(json), C.byref(p)))
+    check(lib().rch_scene_file_parse(bytes(b), len(b), fmt, C.byref(p)))
     return _scene_file(p)
 
 
 def save_scene_file(path, sf: SceneFile) -> None:
-    """rch_scene_file_save: writes the scene as cereal XML or JSON (by extension)."""
+    """rch_scene_file_save: writes the scene as cereal XML, JSON or binary (by extension)."""
     d = SceneFileData()
     d.tracer_type, d.camera, d.cfg, d.tonemap, d.ray = sf.tracer_type, sf.camera, sf.cfg, sf.tonemap, sf.ray
     mats = np.ascontiguousarray(sf.materials)
@@ -659,6 +673,23 @@ class Context:
             check(lib().rc_render_batch_plan(self.h, sarr, farr, carr, garr, n, _ptr(sp)))
         else:
             check(lib().rc_render_batch(self.h, sarr, farr, carr, garr, n, C.byref(tuning) if tuning else None))
+
+    def render_batch_tuned(self, scenes, films, cams, cfgs, plans, branches: int = 1) -> None:
+        """rc_render_batch_tuned: one BatchPlan per scene, frames over
+        `branches` concurrent graph branches."""
+        n, sarr, farr, carr, garr = self._batch_arrays(scenes, cams, cfgs, films)
+        parr = plans if isinstance(plans, C.Array) else (BatchPlan * n)(*plans)
+        check(lib().rc_render_batch_tuned(self.h, sarr, farr, carr, garr, n, C.addressof(parr), int(branches)))
+
+    def autotune_batch_plan(self, scenes, cams, cfgs, cache_path=None):
+        """rc_autotune_batch_plan: (plans, branches)."""
+        n, sarr, _, carr, garr = self._batch_arrays(scenes, cams, cfgs)
+        plans = (BatchPlan * n)()
+        br = C.c_uint32()
+        check(lib().rc_autotune_batch_plan(self.h, sarr, carr, garr, n,
+                                           str(cache_path).encode() if cache_path else None, C.addressof(plans),
+                                           C.byref(br)))
+        return plans, br.value
 
     def autotune_batch(self, scenes, cams, cfgs, cache_path=None) -> np.ndarray:
         """Samples per pass for each scene from rc_autotune_batch."""

@@ -89,6 +89,24 @@ def build_runner() -> Path:
     return RUNNER
 
 
+SANITIZE_FLAGS = ["-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+                  "-fno-omit-frame-pointer", "-fopenmp", "-pthread"]
+
+
+def build_sanitized(out_dir: Path) -> Path:
+    """The host half of the library (csrc/host: OBJ/PLY, scene files, BVH
+    builders and files, scene generators) and the FP64 oracle, built with
+    AddressSanitizer and UndefinedBehaviorSanitizer (g++, host code only) and
+    linked into the driver tests/native/host_sanitize.cpp."""
+    out_dir.mkdir(parents=True, exist_ok=True)
+    exe = out_dir / "host_sanitize"
+    srcs = [ROOT / "tests" / "native" / "host_sanitize.cpp", *sorted((CSRC / "host").glob("*.cpp")),
+            *(ROOT / "oracle" / f for f in ("bvh.cpp", "render.cpp", "mlp.cpp"))]
+    _run(["g++", *SANITIZE_FLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}", f"-I{CSRC / 'host'}", f"-I{ROOT / 'oracle'}",
+          *[str(x) for x in srcs], "-o", str(exe)])
+    return exe
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j4"])
 
@@ -99,10 +117,15 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("-j", type=int, default=8)
     ap.add_argument("--define", action="append", default=[], help="NAME=VALUE for a variant build (with --out)")
     ap.add_argument("--out", type=Path, default=None, help="directory of a variant build's libraycer.so")
+    ap.add_argument("--sanitize", type=Path, default=None,
+                    help="build the host half and the oracle with ASan + UBSan into this directory (host_sanitize)")
     a = ap.parse_args(argv)
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
         print("hipcc not found", file=sys.stderr)
         return 1
+    if a.sanitize:
+        print(f"built {build_sanitized(a.sanitize.resolve())}")
+        return 0
     if a.define and not a.out:
         print("--define needs --out (the in-tree library is the default build)", file=sys.stderr)
         return 1

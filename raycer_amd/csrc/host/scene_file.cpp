@@ -16,6 +16,7 @@
 #include <map>
 #include <new>
 #include <sstream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -45,10 +46,25 @@ struct ParseError {
     std::string msg;
 };
 
+// Nesting the readers accept: scene files nest a few levels; a hostile file
+// ("[[[[..." or "<a><a><a>...") is rejected instead of exhausting the stack.
+constexpr int kMaxNesting = 256;
+struct NestGuard {
+    int& depth;
+    explicit NestGuard(int& d) : depth(d) {
+        if (++depth > kMaxNesting) {
+            --depth;
+            throw ParseError{"nesting deeper than " + std::to_string(kMaxNesting) + " levels"};
+        }
+    }
+    ~NestGuard() { --depth; }
+};
+
 // ---------------------------------------------------------------- XML
 struct XmlParser {
     const char* p;
     const char* e;
+    int depth = 0;
     void ws() {
         while (p < e && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
     }
@@ -89,6 +105,7 @@ struct XmlParser {
         return o;
     }
     Node element() {
+        NestGuard guard(depth);
         skip_misc();
         if (p >= e || *p != '<') throw ParseError{"expected an element"};
         ++p;
@@ -161,6 +178,7 @@ struct XmlParser {
 struct JsonParser {
     const char* p;
     const char* e;
+    int depth = 0;
     void ws() {
         while (p < e && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
     }
@@ -199,12 +217,13 @@ struct JsonParser {
         return o;
     }
     void value(Node& n) {
+        NestGuard guard(depth);
         ws();
         if (p >= e) throw ParseError{"unexpected end"};
         if (*p == '{') {
             ++p;
             ws();
-            if (*p == '}') { ++p; return; }
+            if (p < e && *p == '}') { ++p; return; }
             for (;;) {
                 ws();
                 std::string k = str();
@@ -222,7 +241,7 @@ struct JsonParser {
             ++p;
             n.seq = true;
             ws();
-            if (*p == ']') { ++p; return; }
+            if (p < e && *p == ']') { ++p; return; }
             for (size_t i = 0;; ++i) {
                 Node& c = n.add("value" + std::to_string(i));
                 value(c);
@@ -954,6 +973,266 @@ void write_json(std::ostream& os, const Node& n, int depth, const std::string& l
     }
 }
 
+// ---------------------------------------------------------------- cereal binary
+// cereal's BinaryOutputArchive (the ".bin" scene files of Scene::saveToFile,
+// Scene.cpp:144-146, read back at :88-90) writes the values of
+// Scene::serialize in member order with no names: arithmetic values as their
+// little-endian bytes (bool one byte, enum classes as their int), std::string
+// and std::vector as a uint64 count then the elements, C arrays as their
+// elements alone.  The schema below follows the serialize functions it cites;
+// a binary file is read into the same name tree the XML and JSON readers
+// build (and written from the tree the XML and JSON writers use), so one
+// interpreter serves all three forms.  Lists of textures and of non-triangle
+// primitives have no schema here: a file holding any is rejected by name.
+enum class BK { U64, I64, I32, BOOL, F64, STR, STRUCT, VEC, ARR, NONE };
+struct BType {
+    BK kind;
+    std::vector<std::pair<std::string, const BType*>> fields;  // STRUCT
+    const BType* elem = nullptr;                               // VEC, ARR
+    size_t n = 0;                                              // ARR
+    std::string what;                                          // NONE: what the list holds
+    size_t min_bytes() const {
+        switch (kind) {
+        case BK::U64: case BK::I64: case BK::F64: case BK::STR: case BK::VEC: return 8;
+        case BK::I32: return 4;
+        case BK::BOOL: return 1;
+        case BK::ARR: return n * elem->min_bytes();
+        case BK::STRUCT: {
+            size_t b = 0;
+            for (const auto& f : fields) b += f.second->min_bytes();
+            return b;
+        }
+        default: return 1;
+        }
+    }
+};
+
+const BType* leaf(BK k) {
+    static const BType u64{BK::U64}, i64{BK::I64}, i32{BK::I32}, b{BK::BOOL}, f64{BK::F64}, str{BK::STR};
+    switch (k) {
+    case BK::U64: return &u64;
+    case BK::I64: return &i64;
+    case BK::I32: return &i32;
+    case BK::BOOL: return &b;
+    case BK::F64: return &f64;
+    default: return &str;
+    }
+}
+
+struct Schema {
+    std::vector<std::unique_ptr<BType>> pool;
+    const BType* st(std::vector<std::pair<std::string, const BType*>> f) {
+        pool.push_back(std::unique_ptr<BType>(new BType{BK::STRUCT, std::move(f)}));
+        return pool.back().get();
+    }
+    const BType* vec(const BType* e) {
+        pool.push_back(std::unique_ptr<BType>(new BType{BK::VEC, {}, e}));
+        return pool.back().get();
+    }
+    const BType* arr(const BType* e, size_t n) {
+        pool.push_back(std::unique_ptr<BType>(new BType{BK::ARR, {}, e, n}));
+        return pool.back().get();
+    }
+    const BType* none(const std::string& what) {
+        pool.push_back(std::unique_ptr<BType>(new BType{BK::NONE, {}, nullptr, 0, what}));
+        return vec(pool.back().get());
+    }
+    const BType* scene = nullptr;
+
+    Schema() {
+        const BType *U = leaf(BK::U64), *I = leaf(BK::I64), *E = leaf(BK::I32), *B = leaf(BK::BOOL),
+                    *D = leaf(BK::F64), *S = leaf(BK::STR);
+        const BType* color = st({{"r", D}, {"g", D}, {"b", D}, {"a", D}});            // Color.h:78-84
+        const BType* v3 = st({{"x", D}, {"y", D}, {"z", D}});                         // Vector3.h:74-79
+        const BType* v2 = st({{"x", D}, {"y", D}});                                   // Vector2.h:63-67
+        const BType* euler = st({{"pitch", D}, {"yaw", D}, {"roll", D}});             // EulerAngle.h:55-60
+        const BType* general = st({{"tracerType", E}, {"maxRayIterations", U}, {"maxPathLength", U},  // Scene.h:89-131
+                                   {"pathSampleCount", U}, {"rayStartOffset", D}, {"backgroundColor", color},
+                                   {"offLensColor", color}, {"multiSamplerType", E}, {"multiSamplerFilterType", E},
+                                   {"multiSampleCountSqrt", U}, {"timeSamplerType", E}, {"timeSampleCount", U},
+                                   {"cameraSamplerType", E}, {"cameraSampleCountSqrt", U}, {"visualizeDepth", B},
+                                   {"visualizeDepthMaxDistance", D}, {"enableNormalMapping", B}});
+        const BType* camera = st({{"position", v3}, {"orientation", euler}, {"projectionType", E}, {"fov", D},  // Camera.h:88-103
+                                  {"orthoSize", D}, {"fishEyeAngle", D}, {"apertureSize", D}, {"focalDistance", D},
+                                  {"isTimeVariant", B}, {"translateInTime", v3}, {"rotateInTime", euler}});
+        const BType* tone = st({{"type", E}, {"applyGamma", B}, {"shouldClamp", B}, {"gamma", D},  // Scene.h:134-160
+                                {"exposure", D}, {"key", D}, {"enableAveraging", B}, {"averagingAlpha", D}});
+        const BType* sfog = st({{"enabled", B}, {"color", color}, {"distance", D}, {"steepness", D},  // Scene.h:162-183
+                                {"heightDispersion", B}, {"height", D}, {"heightSteepness", D}});
+        const BType* vfog = st({{"enabled", B}, {"color", color}, {"density", D}, {"steps", U}});  // Scene.h:185-200
+        const BType* build = st({{"maxLeafSize", U}, {"useSAH", B}, {"regularSAHSplits", U},  // FlatBVH.h:19-36
+                                 {"axisSelection", E}, {"axisSplit", E}});
+        const BType* aabb = st({{"min", v3}, {"max", v3}});                            // AABB.h:44-48
+        const BType* fnode = st({{"aabb", aabb}, {"rightOffset", I}, {"startOffset", U},  // FlatBVH.h:38-53
+                                 {"primitiveCount", U}});
+        const BType* prim = st({{"id", U}, {"materialId", U}, {"invisible", B}});       // Primitive.h:46-51
+        const BType* bvh = st({{"primitive", prim}, {"hasBeenBuilt", B}, {"flatNodes", vec(fnode)},  // FlatBVH.h:99-105
+                               {"orderedPrimitiveIds", vec(U)}});
+        const BType* root_bvh = st({{"enabled", B}, {"buildInfo", build}, {"bvh", bvh}});  // Scene.h:202-216
+        const BType* material = st({{"id", U}, {"skipLighting", B}, {"nonShadowing", B},  // Material.h:80-120
+            {"normalInterpolation", B}, {"invertNormal", B}, {"enableCSG", B}, {"isEmissive", B},
+            {"texcoordScale", v2}, {"ambientReflectance", color}, {"ambientMapTextureId", U},
+            {"diffuseReflectance", color}, {"diffuseMapTextureId", U}, {"specularReflectance", color},
+            {"specularShininess", D}, {"specularMapTextureId", U}, {"emittance", color},
+            {"emittanceMapTextureId", U}, {"fresnelReflection", B}, {"refractiveIndex", D}, {"rayReflectance", D},
+            {"rayReflectanceMapTextureId", U}, {"rayReflectanceGlossinessSamplerType", E},
+            {"rayReflectanceGlossinessSampleCountSqrt", U}, {"rayReflectanceGlossiness", D},
+            {"rayTransmittance", D}, {"rayTransmittanceMapTextureId", U},
+            {"rayTransmittanceGlossinessSamplerType", E}, {"rayTransmittanceGlossinessSampleCountSqrt", U},
+            {"rayTransmittanceGlossiness", D}, {"enableRayTransmissionAttenuation", B},
+            {"rayTransmissionAttenuationFactor", D}, {"rayTransmissionAttenuationColor", color},
+            {"normalMapTextureId", U}, {"maskMapTextureId", U}, {"heightMapTextureId", U}});
+        const BType* boxes = st({{"enabled", B}, {"useDefaultMaterial", B}, {"material", material}});  // Scene.h:218-232
+        const BType* textures = st({{"colorTextures", none("color textures")},  // Scene.h:234-266
+            {"colorGradientTextures", none("color gradient textures")}, {"checkerTextures", none("checker textures")},
+            {"imageTextures", none("image textures")}, {"perlinNoiseTextures", none("Perlin noise textures")},
+            {"valueNoiseTextures", none("value noise textures")}, {"cellNoiseTextures", none("cell noise textures")},
+            {"marbleTextures", none("marble textures")}, {"woodTextures", none("wood textures")},
+            {"fireTextures", none("fire textures")}, {"atmosphereTextures", none("atmosphere textures")},
+            {"voronoiTextures", none("Voronoi textures")}});
+        const BType* light = st({{"color", color}, {"intensity", D}});                 // Lights.h:14-24
+        const BType* ambient = st({{"light", light}, {"enableAmbientOcclusion", B},    // Lights.h:26-44
+                                   {"ambientOcclusionSamplerType", E}, {"ambientOcclusionSampleCountSqrt", U},
+                                   {"ambientOcclusionMaxSampleDistance", D},
+                                   {"ambientOcclusionSampleDistribution", D}});
+        const BType* dir = st({{"light", light}, {"direction", v3}});                   // Lights.h:46-57
+        const BType* point = st({{"light", light}, {"position", v3}, {"maxDistance", D}, {"attenuation", D},  // Lights.h:59-81
+                                 {"enableAreaLight", B}, {"areaLightSamplerType", E}, {"areaLightSampleCountSqrt", U},
+                                 {"areaLightRadius", D}});
+        const BType* spot = st({{"light", point}, {"direction", v3}, {"angle", D}, {"sideAttenuation", D}});  // Lights.h:83-97
+        const BType* lights = st({{"ambientLight", ambient}, {"directionalLights", vec(dir)},  // Scene.h:272-287
+                                  {"pointLights", vec(point)}, {"spotLights", vec(spot)}});
+        const BType* model = st({{"modelFilePath", S}, {"scale", v3}, {"rotate", euler}, {"translate", v3},  // ModelLoader.h:66-84
+            {"baseMaterial", material}, {"invisibleTriangles", B}, {"ignoreMaterials", B}, {"enableGroups", B},
+            {"enableGroupsInstances", B}, {"enableCombinedGroup", B}, {"enableCombinedGroupInstance", B},
+            {"defaultMaterialId", U}, {"idStartOffset", U}, {"combinedGroupId", U}, {"combinedGroupInstanceId", U}});
+        const BType* tri = st({{"primitive", prim}, {"vertices", arr(v3, 3)}, {"normals", arr(v3, 3)},  // Triangle.h:45-51
+                               {"texcoords", arr(v2, 3)}});
+        const BType* prims = st({{"triangles", vec(tri)}, {"planes", none("planes")}, {"spheres", none("spheres")},  // Scene.h:291-320
+            {"boxes", none("boxes")}, {"cylinders", none("cylinders")}, {"toruses", none("toruses")},
+            {"blinnBlobs", none("blinn blobs")}, {"csgs", none("CSG primitives")},
+            {"primitiveGroups", none("primitive groups")}, {"instances", none("instances")}});
+        scene = st({{"general", general}, {"camera", camera}, {"toneMapper", tone}, {"simpleFog", sfog},  // Scene.h:331-348
+                    {"volumetricFog", vfog}, {"rootBVH", root_bvh}, {"boundingBoxes", boxes},
+                    {"textures", textures}, {"materials", vec(material)}, {"lights", lights},
+                    {"models", vec(model)}, {"primitives", prims}});
+    }
+};
+
+const Schema& schema() {
+    static const Schema s;
+    return s;
+}
+
+struct BinReader {
+    const uint8_t* p;
+    const uint8_t* e;
+    template <class T>
+    T get() {
+        if (size_t(e - p) < sizeof(T)) throw ParseError{"binary scene file ends early"};
+        T v;
+        std::memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+        return v;
+    }
+    void read(const BType& t, Node& n) {
+        switch (t.kind) {
+        case BK::U64: n.text = std::to_string(get<uint64_t>()); break;
+        case BK::I64: n.text = std::to_string(get<int64_t>()); break;
+        case BK::I32: n.text = std::to_string(get<int32_t>()); break;
+        case BK::BOOL: {
+            const uint8_t b = get<uint8_t>();
+            if (b > 1) throw ParseError{"binary scene file: a bool holds " + std::to_string(b)};
+            n.text = b ? "true" : "false";
+            break;
+        }
+        case BK::F64: n.text = fmt(get<double>()); break;
+        case BK::STR: {
+            const uint64_t len = get<uint64_t>();
+            if (len > uint64_t(e - p)) throw ParseError{"binary scene file: a string runs past the end"};
+            n.text.assign(reinterpret_cast<const char*>(p), size_t(len));
+            p += len;
+            break;
+        }
+        case BK::STRUCT:
+            for (const auto& f : t.fields) {
+                n.kids.push_back(Node{f.first, std::string(), false, {}});
+                read(*f.second, n.kids.back());
+            }
+            break;
+        case BK::VEC: {
+            const uint64_t count = get<uint64_t>();
+            n.seq = true;
+            if (count == 0) break;
+            if (t.elem->kind == BK::NONE)
+                throw ParseError{"binary scene files with " + t.elem->what + " are not supported"};
+            if (count > uint64_t(e - p) / std::max<size_t>(1, t.elem->min_bytes()))
+                throw ParseError{"binary scene file: a list of " + std::to_string(count) + " runs past the end"};
+            n.kids.reserve(size_t(count));
+            for (uint64_t i = 0; i < count; ++i) {
+                n.kids.push_back(Node{"value" + std::to_string(i), std::string(), false, {}});
+                read(*t.elem, n.kids.back());
+            }
+            break;
+        }
+        case BK::ARR:
+            for (size_t i = 0; i < t.n; ++i) {
+                n.kids.push_back(Node{"value" + std::to_string(i), std::string(), false, {}});
+                read(*t.elem, n.kids.back());
+            }
+            break;
+        default: throw ParseError{"binary scene file: no schema"};
+        }
+    }
+};
+
+Node parse_binary(const uint8_t* p, size_t len) {
+    BinReader R{p, p + len};
+    Node root{"cereal", std::string(), false, {}};
+    root.kids.push_back(Node{"scene", std::string(), false, {}});
+    R.read(*schema().scene, root.kids.back());
+    if (R.p != R.e) throw ParseError{"binary scene file: " + std::to_string(R.e - R.p) + " bytes after the scene"};
+    return root;
+}
+
+template <class T>
+void put_bytes(std::string& out, T v) {
+    out.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+
+// Writes the tree node n (null: the member is absent, its zero value is
+// written) as type t.
+void write_binary(const BType& t, const Node* n, std::string& out) {
+    const std::string empty;
+    const std::string& tx = n ? n->text : empty;
+    switch (t.kind) {
+    case BK::U64: put_bytes<uint64_t>(out, std::strtoull(tx.c_str(), nullptr, 10)); break;
+    case BK::I64: put_bytes<int64_t>(out, std::strtoll(tx.c_str(), nullptr, 10)); break;
+    case BK::I32: put_bytes<int32_t>(out, int32_t(std::strtol(tx.c_str(), nullptr, 10))); break;
+    case BK::BOOL: put_bytes<uint8_t>(out, tx == "true" || tx == "1" ? 1 : 0); break;
+    case BK::F64: put_bytes<double>(out, std::strtod(tx.c_str(), nullptr)); break;
+    case BK::STR:
+        put_bytes<uint64_t>(out, tx.size());
+        out += tx;
+        break;
+    case BK::STRUCT:
+        for (const auto& f : t.fields) write_binary(*f.second, n ? n->get(f.first.c_str()) : nullptr, out);
+        break;
+    case BK::VEC: {
+        const size_t count = n ? n->kids.size() : 0;
+        if (count && t.elem->kind == BK::NONE)
+            throw ParseError{"binary scene files with " + t.elem->what + " are not supported"};
+        put_bytes<uint64_t>(out, count);
+        for (size_t i = 0; i < count; ++i) write_binary(*t.elem, &n->kids[i], out);
+        break;
+    }
+    case BK::ARR:
+        for (size_t i = 0; i < t.n; ++i) write_binary(*t.elem, n && i < n->kids.size() ? &n->kids[i] : nullptr, out);
+        break;
+    default: throw ParseError{"binary scene file: no schema"};
+    }
+}
+
 bool ends_with(const std::string& s, const char* suf) {
     size_t n = std::strlen(suf);
     return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
@@ -961,13 +1240,16 @@ bool ends_with(const std::string& s, const char* suf) {
 
 }  // namespace
 
-extern "C" int rch_scene_file_parse(const char* text, size_t len, int json, rch_scene_file_data** out) {
-    if (!text || !out) return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_parse: null argument");
+extern "C" int rch_scene_file_parse(const char* text, size_t len, int format, rch_scene_file_data** out) {
+    if ((!text && len) || !out) return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_parse: null argument");
+    if (format < 0 || format > RCH_SCENE_BINARY) return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_parse: unknown format");
     try {
         // a NUL-terminated copy: the parsers may look one byte past the end
-        const std::string buf(text, len);
+        const std::string buf(text ? text : "", len);
         Node root;
-        if (json) {
+        if (format == RCH_SCENE_BINARY) {
+            root = parse_binary(reinterpret_cast<const uint8_t*>(buf.data()), buf.size());
+        } else if (format == RCH_SCENE_JSON) {
             JsonParser P{buf.c_str(), buf.c_str() + buf.size()};
             root = P.parse();
         } else {
@@ -986,15 +1268,16 @@ extern "C" int rch_scene_file_parse(const char* text, size_t len, int json, rch_
 extern "C" int rch_scene_file_load(const char* path, rch_scene_file_data** out) {
     if (!path || !out) return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_load: null argument");
     const std::string p(path);
-    const bool json = ends_with(p, ".json");
-    if (!json && !ends_with(p, ".xml"))
-        return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_load: unsupported scene file format (.xml or .json)");
+    const int format = ends_with(p, ".json") ? RCH_SCENE_JSON : ends_with(p, ".xml") ? RCH_SCENE_XML
+                       : ends_with(p, ".bin") ? RCH_SCENE_BINARY : -1;
+    if (format < 0)
+        return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_load: unsupported scene file format (.xml, .json or .bin)");
     std::ifstream f(p, std::ios::binary);
     if (!f.good()) return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_load: could not open " + p);
     std::stringstream ss;
     ss << f.rdbuf();
     const std::string text = ss.str();
-    return rch_scene_file_parse(text.data(), text.size(), json ? 1 : 0, out);
+    return rch_scene_file_parse(text.data(), text.size(), format, out);
 }
 
 extern "C" int rch_scene_file_save(const char* path, const rch_scene_file_data* d) {
@@ -1007,14 +1290,18 @@ extern "C" int rch_scene_file_save(const char* path, const rch_scene_file_data* 
     for (size_t i = 0; d->nnodes && i < d->ntri; ++i)
         if (d->ordered[i] >= d->ntri) return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_save: bad ordered index");
     const std::string p(path);
-    const bool json = ends_with(p, ".json");
-    if (!json && !ends_with(p, ".xml"))
-        return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_save: unsupported scene file format (.xml or .json)");
+    const bool json = ends_with(p, ".json"), bin = ends_with(p, ".bin");
+    if (!json && !bin && !ends_with(p, ".xml"))
+        return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_save: unsupported scene file format (.xml, .json or .bin)");
     try {
         Node root = scene_tree(*d);
+        std::string bytes;
+        if (bin) write_binary(*schema().scene, &root.kids.front(), bytes);
         std::ofstream f(p, std::ios::binary | std::ios::trunc);
         if (!f.good()) return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_save: could not open " + p);
-        if (json) {
+        if (bin) {
+            f.write(bytes.data(), std::streamsize(bytes.size()));
+        } else if (json) {
             f << "{\n    \"scene\": ";
             write_json(f, root.kids.front(), 1, "scene");
             f << "\n}\n";
@@ -1023,6 +1310,8 @@ extern "C" int rch_scene_file_save(const char* path, const rch_scene_file_data* 
             write_xml(f, root, 0);
         }
         return f.good() ? RC_OK : rc_host_fail(RC_ERR_INVALID, "rch_scene_file_save: write failed");
+    } catch (const ParseError& e) {
+        return rc_host_fail(RC_ERR_INVALID, "rch_scene_file_save: " + e.msg);
     } catch (const std::bad_alloc&) {
         return rc_host_fail(RC_ERR_ALLOC, "rch_scene_file_save: out of memory");
     }

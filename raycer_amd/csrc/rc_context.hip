@@ -27,22 +27,6 @@
 
 using namespace rcd;
 
-namespace {
-thread_local std::string g_last_error;
-}
-
-int rc_host_fail(int code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
-}
-
-extern "C" int rc_last_error(char* buf, size_t len) {
-    if (!buf || len == 0) return RC_ERR_INVALID;
-    size_t n = std::min(len - 1, g_last_error.size());
-    std::memcpy(buf, g_last_error.data(), n);
-    buf[n] = 0;
-    return RC_OK;
-}
 
 int DevBuf::ensure(size_t need) {
     if (need <= bytes) return RC_OK;
@@ -176,6 +160,8 @@ extern "C" int rc_destroy(rc_ctx* c) {
     c->branches.clear();
     for (hipStream_t b : c->branch_streams) (void)hipStreamDestroy(b);
     c->branch_streams.clear();
+    for (hipEvent_t e : c->branch_events) (void)hipEventDestroy(e);
+    c->branch_events.clear();
     for (auto& p : c->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);

@@ -108,8 +108,10 @@ struct rc_ctx {
     std::vector<Pending> pending;
     hipEvent_t train_ev0 = nullptr, train_ev1 = nullptr;  // the last cached frame's optimizer steps (timing)
     hipGraphExec_t batch_graph = nullptr;
-    std::vector<FrameBuffers*> branches;      // a batch's concurrent branches
+    std::vector<uint8_t> batch_key;           // every value the cached batch graph's kernels were captured with
+    std::vector<FrameBuffers*> branches;      // a batch's concurrent branches beyond the first (which uses fb)
     std::vector<hipStream_t> branch_streams;
+    std::vector<hipEvent_t> branch_events;    // fork / join of the branches under capture
     struct WhittedBufs* whitted = nullptr;  // the Whitted tracer's ray-tree levels
 
     int prof_begin(hipEvent_t* a);
@@ -207,6 +209,9 @@ size_t rc_pass_paths(const rc_render_cfg* cfg, uint32_t spp_pass);
 int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene* s, rc_film* f, const rc_camera* cam,
                      const rc_render_cfg* cfg, const FrameShape& shape, rc_progress_fn progress, void* user,
                      const CacheIO* cache_io = nullptr, rc_cache* cache = nullptr);
+// One frame with an explicit launch shape, synchronously (the autotuner).
+int rc_render_shaped(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                     const FrameShape& shape);
 // Zeroes a frame's statistics counters (a kernel, so it may be captured).
 void rc_zero_counters(DevCounters* ctr, hipStream_t st);
 // Radiance cache, device side (rc_mlp.hip).

@@ -1089,7 +1089,8 @@ int check_frame_args(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* 
 // on the cache's own stream behind the frame (the next frame's traversal runs
 // while they do; its cache inference waits for them).
 int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
-                 rc_progress_fn progress, void* user, rc_cache* cache, uint32_t train) {
+                 rc_progress_fn progress, void* user, rc_cache* cache, uint32_t train,
+                 const FrameShape* forced = nullptr) {
     RC_TRY_HIP(hipSetDevice(c->device));
     auto t0 = std::chrono::steady_clock::now();
     c->stats.extend_ms = c->stats.shadow_ms = c->stats.shade_ms = c->stats.other_ms = 0.0;
@@ -1099,6 +1100,7 @@ int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
     rc_zero_counters(c->counters, c->stream);
     FrameShape shape;
     shape.spp_pass = pass_samples(c->tuning, cfg->width * cfg->height, cfg->spp, kPassPaths);
+    if (forced) shape = *forced;
     RC_TRY(c->fb.ensure(rc_pass_paths(cfg, shape.spp_pass)));
     CacheIO io{};
     if (cache)
@@ -1129,6 +1131,12 @@ int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
     return RC_OK;
 }
 }  // namespace
+
+int rc_render_shaped(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
+                     const FrameShape& shape) {
+    RC_TRY(check_frame_args(c, s, f, cam, cfg, "rc_render"));
+    return render_frame(c, s, f, cam, cfg, nullptr, nullptr, nullptr, 0, &shape);
+}
 
 extern "C" int rc_render_progress(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
                                   const rc_render_cfg* cfg, rc_progress_fn progress, void* user) {

@@ -94,9 +94,10 @@ def test_batch_graph_equals_individual_renders(ctx):
     for f in films:
         f.clear()
     api.check(L.rc_render_batch(ctx.h, sarr, farr, carr, garr, n, C.byref(tun)))
-    # replaying the instantiated graph is not supported: a second batch call
-    # re-captures; the films accumulate
+    # the second call with the same batch relaunches the kept executable
+    # graph; the films accumulate
     api.check(L.rc_render_batch(ctx.h, sarr, farr, carr, garr, n, C.byref(tun)))
+    assert ctx.stats().graph_relaunched == 1
     again = films[0].sums()
     assert np.allclose(again[:, 3], 2 * 8, atol=1e-3)
     for f in films:

@@ -7,7 +7,17 @@ materials with ids 1..8, the 1.5-intensity point light at (8, 10, 5), ambient
 0.01 -- and its non-triangle primitives counted as skipped; XML and JSON
 round trips of a generated scene reproduce every array bit for bit, including
 a built BVH, which loads back as the same tree.
+
+The binary form (cereal's BinaryOutputArchive, Scene.cpp:88-90,144-146) is
+pinned two ways: tests/golden/cereal/cereal_binary_rules.bin, written by the
+cereal release the reference vendors (make_binary_rules.cpp there), is decoded
+by the encoding rules the library's schema uses (int32 enums, one-byte bools,
+uint64 counts, C arrays without a count, base-class members inline, no
+version words); and a saved binary scene has exactly the size its member
+lists give (Scene.h, Material.h, Lights.h, Triangle.h, FlatBVH.h).
 """
+import json
+import struct
 import ctypes as C
 
 import numpy as np
@@ -91,7 +101,7 @@ def light_fields(L):
     return f
 
 
-@pytest.mark.parametrize("ext", ["xml", "json"])
+@pytest.mark.parametrize("ext", ["xml", "json", "bin"])
 def test_round_trip(tmp_path, ext):
     sf, _ = generated_scene_file()
     path = tmp_path / f"scene.{ext}"
@@ -126,6 +136,114 @@ def test_json_and_xml_hold_the_same_scene(tmp_path):
     text = (tmp_path / "a.json").read_text()
     assert text.lstrip().startswith('{') and '"scene"' in text and '"flatNodes": []' in text
     assert (tmp_path / "a.xml").read_text().startswith('<?xml version="1.0" encoding="utf-8"?>\n<cereal>')
+
+
+def test_binary_holds_the_same_scene_as_xml(tmp_path):
+    sf, _ = generated_scene_file()
+    sf.models = [("meshes/cornellbox.obj", generated_scene_file()[1], 1)]
+    for ext in ("xml", "bin"):
+        api.save_scene_file(tmp_path / f"a.{ext}", sf)
+    x = api.load_scene_file(tmp_path / "a.xml")
+    b = api.load_scene_file(tmp_path / "a.bin")
+    for f in ("vertices", "tri_material", "tri_ids", "material_ids", "materials", "nodes", "ordered"):
+        assert getattr(x, f).tobytes() == getattr(b, f).tobytes(), f
+    assert bytes(x.ray_materials) == bytes(b.ray_materials)
+    assert struct_eq(x.ray, b.ray) and struct_eq(x.tonemap, b.tonemap) and struct_eq(x.camera, b.camera)
+    assert [light_fields(q) for q in x.lights] == [light_fields(q) for q in b.lights]
+    assert x.models[0][0] == b.models[0][0] and struct_eq(x.models[0][1], b.models[0][1])
+    # the sample scene through the binary form
+    sample = api.load_scene_file(SAMPLE)
+    api.save_scene_file(tmp_path / "sample.bin", sample)
+    sb = api.load_scene_file(tmp_path / "sample.bin")
+    assert sb.materials.tobytes() == sample.materials.tobytes() and struct_eq(sb.camera, sample.camera)
+
+
+# Encoded sizes (bytes) of the serialized types, from the member lists of
+# their serialize functions under the rules the fixture pins.
+COLOR, VEC3, VEC2, EULER = 32, 24, 16, 24
+MATERIAL = 8 + 6 + VEC2 + COLOR + 8 + COLOR + 8 + COLOR + 8 + 8 + COLOR + 8 + 1 + 8 + 8 + 8 + 4 + 8 + 8 + 8 + 8 + 4 \
+    + 8 + 8 + 1 + 8 + COLOR + 8 + 8 + 8                                        # Material.h:80-120
+GENERAL = 4 + 8 * 3 + 8 + 2 * COLOR + 4 + 4 + 8 + 4 + 8 + 4 + 8 + 1 + 8 + 1     # Scene.h:108-128
+CAMERA = VEC3 + EULER + 4 + 5 * 8 + 1 + VEC3 + EULER                            # Camera.h:88-103
+TONE, SFOG, VFOG = 4 + 2 + 4 * 8 + 1, 1 + COLOR + 2 * 8 + 1 + 2 * 8, 1 + COLOR + 8 + 8
+BUILD, PRIM, NODE = 8 + 1 + 8 + 4 + 4, 8 + 8 + 1, 2 * VEC3 + 3 * 8               # FlatBVH.h, Primitive.h
+LIGHT = COLOR + 8                                                               # Lights.h:14-24
+AMBIENT, DIRECTIONAL = LIGHT + 1 + 4 + 8 + 8 + 8, LIGHT + VEC3
+POINT = LIGHT + VEC3 + 8 + 8 + 1 + 4 + 8 + 8
+SPOT = POINT + VEC3 + 8 + 8
+TRIANGLE = PRIM + 3 * VEC3 + 3 * VEC3 + 3 * VEC2                                # Triangle.h:45-51
+MODEL_FIXED = 8 + VEC3 + EULER + VEC3 + MATERIAL + 6 + 4 * 8                    # ModelLoader.h:66-84
+
+
+def test_cereal_binary_rules_fixture():
+    raw = (ROOT / "tests" / "golden" / "cereal" / "cereal_binary_rules.bin").read_bytes()
+    want = {f[0]: f[2] for f in json.loads((ROOT / "tests" / "golden" / "cereal" / "cereal_binary_rules.json")
+                                            .read_text())["fields"]}
+    o = 0
+
+    def take(fmt):
+        nonlocal o
+        v = struct.unpack_from("<" + fmt, raw, o)
+        o += struct.calcsize("<" + fmt)
+        return v if len(v) > 1 else v[0]
+
+    assert take("i") == want["kind"]                      # enum class: int32
+    assert take("B") == 1 and want["flag"] is True        # bool: one byte
+    assert take("q") == want["offset"] and take("Q") == want["count"] and take("d") == want["value"]
+    n = take("Q")                                         # std::string: uint64 count, bytes
+    assert raw[o:o + n].decode() == want["path"]
+    o += n
+    assert take("Q") == len(want["tris"])                 # std::vector: uint64 count
+    for tid, mid, inv, verts in want["tris"]:
+        assert take("QQB") == (tid, mid, int(inv))        # base class members inline, no version word
+        for v in verts:                                   # C array: no count
+            assert list(take("ddd")) == [float(x) for x in v]
+    assert take("Q") == 3 and [take("Q") for _ in range(3)] == want["ids"]
+    assert take("Q") == 0 and o == len(raw)
+
+
+def test_binary_scene_size_follows_the_member_lists(tmp_path):
+    sf, _ = generated_scene_file()
+    sf.models = [("meshes/box.obj", generated_scene_file()[1], 0)]
+    api.save_scene_file(tmp_path / "s.bin", sf)
+    size = (tmp_path / "s.bin").stat().st_size
+    ndir = sum(1 for L in sf.lights if L.type == api.RC_LIGHT_DIRECTIONAL)
+    npt = sum(1 for L in sf.lights if L.type == api.RC_LIGHT_POINT)
+    nspot = sum(1 for L in sf.lights if L.type == api.RC_LIGHT_SPOT)
+    want = (GENERAL + CAMERA + TONE + SFOG + VFOG
+            + 1 + BUILD + PRIM + 1 + 8 + NODE * len(sf.nodes) + 8 + 8 * len(sf.ordered)   # rootBVH
+            + 2 + MATERIAL                                                                # boundingBoxes
+            + 12 * 8                                                                      # textures (empty)
+            + 8 + MATERIAL * len(sf.materials)
+            + AMBIENT + 8 + DIRECTIONAL * ndir + 8 + POINT * npt + 8 + SPOT * nspot
+            + 8 + MODEL_FIXED + len("meshes/box.obj")
+            + 8 + TRIANGLE * len(sf.tri_material) + 9 * 8)                                # primitives
+    assert size == want
+
+
+def test_binary_scene_errors(tmp_path):
+    sf, _ = generated_scene_file()
+    api.save_scene_file(tmp_path / "s.bin", sf)
+    raw = (tmp_path / "s.bin").read_bytes()
+    assert api.parse_scene_file(raw, binary=True).vertices.shape == sf.vertices.shape
+    # the last nine counts are the non-triangle primitive lists: a plane is rejected by name
+    planes = bytearray(raw)
+    planes[-72:-64] = struct.pack("<Q", 1)
+    with pytest.raises(ra.RaycerError, match="planes are not supported"):
+        api.parse_scene_file(bytes(planes), binary=True)
+    with pytest.raises(ra.RaycerError, match="ends early|runs past"):
+        api.parse_scene_file(raw[:-9], binary=True)
+    with pytest.raises(ra.RaycerError, match="bytes after the scene"):
+        api.parse_scene_file(raw + b"\0", binary=True)
+    bad_bool = bytearray(raw)
+    bad_bool[GENERAL - 1] = 7                             # enableNormalMapping
+    with pytest.raises(ra.RaycerError, match="bool"):
+        api.parse_scene_file(bytes(bad_bool), binary=True)
+    huge = bytearray(raw)
+    off = GENERAL + CAMERA + TONE + SFOG + VFOG + 1 + BUILD + PRIM + 1            # flatNodes count
+    huge[off:off + 8] = struct.pack("<Q", 1 << 60)
+    with pytest.raises(ra.RaycerError, match="runs past the end"):
+        api.parse_scene_file(bytes(huge), binary=True)
 
 
 def test_models_round_trip(tmp_path):
