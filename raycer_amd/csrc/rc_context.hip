@@ -730,18 +730,28 @@ extern "C" int rc_memcpy_d2h(rc_ctx* c, void* dst, const void* src, size_t bytes
 
 // ---------------------------------------------------------------- bandwidth probes
 namespace {
-// Streaming copy, four 16-byte loads in flight per thread before their stores.
+// Streaming copy, four 16-byte loads in flight per thread before their
+// stores; NT: nontemporal loads and stores (the data is touched once).
+template <bool NT>
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
     const size_t stride = size_t(gridDim.x) * 256;
     size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f* s4 = reinterpret_cast<const v4f*>(src);
+    v4f* d4 = reinterpret_cast<v4f*>(dst);
+    auto ld = [&](size_t k) { return NT ? __builtin_nontemporal_load(s4 + k) : s4[k]; };
+    auto st = [&](size_t k, v4f v) {
+        if (NT) __builtin_nontemporal_store(v, d4 + k);
+        else d4[k] = v;
+    };
     for (; i + 3 * stride < n; i += 4 * stride) {
-        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
+        const v4f a = ld(i), b = ld(i + stride), c = ld(i + 2 * stride), d = ld(i + 3 * stride);
+        st(i, a);
+        st(i + stride, b);
+        st(i + 2 * stride, c);
+        st(i + 3 * stride, d);
     }
-    for (; i < n; i += stride) dst[i] = src[i];
+    for (; i < n; i += stride) st(i, ld(i));
 }
 
 // Reads n float4 `iters` times (each thread the same addresses every time, so
@@ -796,15 +806,23 @@ extern "C" int rc_copy_bandwidth(rc_ctx* c, size_t bytes, int iters, double* gbp
     int rc = b.ensure(n * sizeof(float4));
     if (rc != RC_OK) { a.release(); return rc; }
     double best = 0.0;
-    // the best of four grid sizes: the achievable rate, not one launch shape's
-    for (unsigned per_cu : {4u, 8u, 16u, 32u}) {
-        if (rc != RC_OK) break;
-        const unsigned grid = unsigned(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, size_t(c->cu_count) * per_cu)));
-        float ms = 0.0f;
-        rc = time_launches(c, iters, [&]() { k_copy<<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n); },
-                           &ms);
-        if (rc == RC_OK) best = std::max(best, 2.0 * double(n * sizeof(float4)) * iters / (double(ms) * 1e-3) / 1e9);
-    }
+    // the best of four grid sizes, plain and nontemporal: the achievable rate,
+    // not one launch shape's
+    for (int nt = 0; nt < 2; ++nt)
+        for (unsigned per_cu : {4u, 8u, 16u, 32u}) {
+            if (rc != RC_OK) break;
+            const unsigned grid =
+                unsigned(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, size_t(c->cu_count) * per_cu)));
+            float ms = 0.0f;
+            rc = time_launches(
+                c, iters,
+                [&]() {
+                    if (nt) k_copy<true><<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n);
+                    else k_copy<false><<<grid, 256, 0, c->stream>>>(a.as<float4>(), b.as<float4>(), n);
+                },
+                &ms);
+            if (rc == RC_OK) best = std::max(best, 2.0 * double(n * sizeof(float4)) * iters / (double(ms) * 1e-3) / 1e9);
+        }
     a.release();
     b.release();
     if (rc != RC_OK) return rc;
