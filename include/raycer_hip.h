@@ -264,6 +264,15 @@ int rc_trace_any(rc_ctx* ctx, const rc_scene* scene, const rc_ray* rays, size_t 
  * context's stream). */
 int rc_trace_closest_device(rc_ctx* ctx, const rc_scene* scene, const rc_ray* d_rays, size_t n, rc_hit* d_hits);
 int rc_trace_any_device(rc_ctx* ctx, const rc_scene* scene, const rc_ray* d_rays, size_t n, uint8_t* d_occluded);
+/* rc_trace_closest_device without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory while
+ * they run (code under HIP graph capture, tight serving loops).  The kernels'
+ * scratch (overflow list, deep stack, counters) is a fixed 6.25 MiB array in
+ * device memory that every GPU the library runs on holds until the process
+ * ends; at most 2^20 rays per call (larger batches are rejected, split them).
+ * Calls that overlap on one GPU share that array and race: serialise them. */
+int rc_trace_closest_device_noalloc(rc_ctx* ctx, const rc_scene* scene, const rc_ray* d_rays, size_t n,
+                                    rc_hit* d_hits);
 /* Device buffers owned by the context's allocator, for the *_device calls. */
 int rc_device_alloc(rc_ctx* ctx, size_t bytes, void** dptr);
 int rc_device_free(rc_ctx* ctx, void* dptr);

@@ -140,6 +140,13 @@ extern "C" int rc_create(int device, rc_ctx** out) {
     }
     (void)hipMemset(c->counters, 0, sizeof(DevCounters));
     c->fb.counters = c->counters;  // not owned by the frame buffers
+    int se2 = rc_noalloc_setup(device);  // this device's copy of the allocation-free scratch
+    if (se2 != RC_OK) {
+        (void)hipFree(c->counters);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return se2;
+    }
     *out = c;
     return RC_OK;
 }

@@ -209,6 +209,9 @@ size_t rc_pass_paths(const rc_render_cfg* cfg, uint32_t spp_pass);
 int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene* s, rc_film* f, const rc_camera* cam,
                      const rc_render_cfg* cfg, const FrameShape& shape, rc_progress_fn progress, void* user,
                      const CacheIO* cache_io = nullptr, rc_cache* cache = nullptr);
+// Looks up the device address of the allocation-free entry point's scratch
+// array on `device` (current), once per device (rc_noalloc.hip).
+int rc_noalloc_setup(int device);
 // One frame with an explicit launch shape, synchronously (the autotuner).
 int rc_render_shaped(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam, const rc_render_cfg* cfg,
                      const FrameShape& shape);
