@@ -276,6 +276,11 @@ int rc_trace_closest_device_noalloc(rc_ctx* ctx, const rc_scene* scene, const rc
 /* Device buffers owned by the context's allocator, for the *_device calls. */
 int rc_device_alloc(rc_ctx* ctx, size_t bytes, void** dptr);
 int rc_device_free(rc_ctx* ctx, void* dptr);
+/* Page-locked host memory (hipHostMalloc): the entry points that return
+ * images or hit arrays to the host copy into such buffers at the host link's
+ * full rate, into pageable memory through a staging copy. */
+int rc_host_alloc(rc_ctx* ctx, size_t bytes, void** hptr);
+int rc_host_free(rc_ctx* ctx, void* hptr);
 int rc_memcpy_h2d(rc_ctx* ctx, void* dst, const void* src, size_t bytes);
 int rc_memcpy_d2h(rc_ctx* ctx, void* dst, const void* src, size_t bytes);
 

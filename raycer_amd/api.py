@@ -263,6 +263,8 @@ def lib() -> C.CDLL:
             "rc_trace_any_device": [vp, vp, vp, sz, vp],
             "rc_device_alloc": [vp, sz, C.POINTER(vp)],
             "rc_device_free": [vp, vp],
+            "rc_host_alloc": [vp, sz, C.POINTER(vp)],
+            "rc_host_free": [vp, vp],
             "rc_memcpy_h2d": [vp, vp, vp, sz],
             "rc_memcpy_d2h": [vp, vp, vp, sz],
             "rc_raytrace": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), C.POINTER(RaySettings), vp, u32,
@@ -707,15 +709,50 @@ class Context:
         check(lib().rc_raytrace(self.h, scene.h, film.h, C.byref(cam), C.byref(cfg), C.byref(settings),
                                 C.cast(mats, C.c_void_p), len(mats), C.cast(lts, C.c_void_p), len(lights)))
 
+    def pinned(self, shape, dtype) -> "PinnedArray":
+        """A numpy array in page-locked host memory (rc_host_alloc)."""
+        return PinnedArray(self, shape, dtype)
+
     def raycast_phong(self, scene: "Scene", cam: Camera, width: int, height: int, seed: int, offset: float,
-                      light: PointLight):
+                      light: PointLight, out=None):
+        """Returns (hit ids, t, rgba); out: the three arrays to fill (numpy
+        arrays of those shapes and types, pinned ones for the full copy rate)."""
         n = width * height
-        ids = np.empty(n, np.int32)
-        t = np.empty(n, np.float32)
-        rgba = np.empty((n, 4), np.float32)
+        if out is None:
+            ids = np.empty(n, np.int32)
+            t = np.empty(n, np.float32)
+            rgba = np.empty((n, 4), np.float32)
+        else:
+            ids, t, rgba = (np.asarray(a) for a in out)
+            if (ids.shape, ids.dtype, t.shape, t.dtype, rgba.shape, rgba.dtype) != (
+                    (n,), np.int32, (n,), np.float32, (n, 4), np.float32) or not all(
+                    a.flags.c_contiguous and a.flags.writeable for a in (ids, t, rgba)):
+                raise ValueError("raycast_phong: out must be writable contiguous int32 (n,), float32 (n,), "
+                                 "float32 (n, 4)")
         check(lib().rc_raycast_phong(self.h, scene.h, C.byref(cam), width, height, seed, offset, C.byref(light),
                                      _ptr(ids), _ptr(t), _ptr(rgba)))
         return ids, t, rgba
+
+
+class PinnedArray:
+    """Page-locked host memory (rc_host_alloc) viewed as a numpy array; freed
+    by close()."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        check(lib().rc_host_alloc(ctx.h, max(nbytes, 1), C.byref(p)))
+        self.p = p
+        buf = (C.c_uint8 * max(nbytes, 1)).from_address(p.value)
+        self.array = np.frombuffer(buf, np.uint8, nbytes).view(dt).reshape(shape)
+
+    def close(self) -> None:
+        if self.p:
+            self.array = None
+            check(lib().rc_host_free(self.ctx.h, self.p))
+            self.p = None
 
 
 class RadianceCache:

@@ -166,6 +166,8 @@ extern "C" int rc_destroy(rc_ctx* c) {
     }
     c->branches.clear();
     for (hipStream_t b : c->branch_streams) (void)hipStreamDestroy(b);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    if (c->copy_ev) (void)hipEventDestroy(c->copy_ev);
     c->branch_streams.clear();
     for (hipEvent_t e : c->branch_events) (void)hipEventDestroy(e);
     c->branch_events.clear();
@@ -719,6 +721,23 @@ extern "C" int rc_device_free(rc_ctx* c, void* p) {
     if (p) {
         RC_TRY_HIP(hipStreamSynchronize(c->stream));
         RC_TRY_HIP(hipFree(p));
+    }
+    return RC_OK;
+}
+extern "C" int rc_host_alloc(rc_ctx* c, size_t bytes, void** p) {
+    RC_TRY(rc_check_ctx(c, "rc_host_alloc"));
+    if (!p || bytes == 0) return rc_host_fail(RC_ERR_INVALID, "rc_host_alloc: bad argument");
+    RC_TRY_HIP(hipSetDevice(c->device));
+    hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return rc_host_fail(RC_ERR_ALLOC, std::string("rc_host_alloc: ") + hipGetErrorString(e));
+    return RC_OK;
+}
+extern "C" int rc_host_free(rc_ctx* c, void* p) {
+    RC_TRY(rc_check_ctx(c, "rc_host_free"));
+    if (p) {
+        RC_TRY_HIP(hipStreamSynchronize(c->stream));
+        if (c->copy_stream) RC_TRY_HIP(hipStreamSynchronize(c->copy_stream));
+        RC_TRY_HIP(hipHostFree(p));
     }
     return RC_OK;
 }

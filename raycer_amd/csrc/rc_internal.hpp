@@ -111,6 +111,8 @@ struct rc_ctx {
     std::vector<uint8_t> batch_key;           // every value the cached batch graph's kernels were captured with
     std::vector<FrameBuffers*> branches;      // a batch's concurrent branches beyond the first (which uses fb)
     std::vector<hipStream_t> branch_streams;
+    hipStream_t copy_stream = nullptr;  // device-to-host copies that overlap the main stream's kernels
+    hipEvent_t copy_ev = nullptr;
     std::vector<hipEvent_t> branch_events;    // fork / join of the branches under capture
     struct WhittedBufs* whitted = nullptr;  // the Whitted tracer's ray-tree levels
 

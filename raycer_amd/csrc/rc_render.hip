@@ -32,9 +32,19 @@ constexpr float kPi = 3.14159265358979323846f;
 constexpr uint32_t kExtBlockDefault = 256;
 constexpr uint32_t kShadowBlockDefault = 128;
 constexpr int kLdsRows = kLdsStack > 0 ? kLdsStack : 1;  // LDS stack entries per thread (at least one row)
+// Minimum waves per SIMD the traversal kernels are compiled for, by triangle
+// decision.  FP32: 8, which caps them at 64 VGPRs with no spills (59-62 used;
+// 7 waves at 70 without the cap), 11% faster on the 2M-triangle terrain at
+// 1080p x 64 spp (extension 59.5 ms against 66.9 per frame).  FP64: no
+// minimum (88-92 VGPRs, 5 waves); at 8 it spills 28-39 VGPRs to scratch.
 #ifndef RC_MIN_WAVES
-#define RC_MIN_WAVES 1  // minimum waves per SIMD the traversal kernels are compiled for
+#define RC_MIN_WAVES 8
 #endif
+#ifndef RC_MIN_WAVES_FP64
+#define RC_MIN_WAVES_FP64 1
+#endif
+template <bool DEC64>
+constexpr int kMinWaves = DEC64 ? RC_MIN_WAVES_FP64 : RC_MIN_WAVES;
 // 1: each XCD takes a contiguous slice of the live queue entries (the grid is
 // sized for the queue's capacity; only a prefix is live).  3% slower on the
 // 2M-triangle terrain: the eight XCDs then sweep eight separate regions of the
@@ -427,7 +437,7 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
 // queue's capacity; the test is uniform over the block, before any barrier).
 // DEC64: the triangle decision mode; BLOCK: threads per block.
 template <bool COUNT, bool CAM, bool DEC64, int BLOCK>
-__global__ __launch_bounds__(BLOCK, RC_MIN_WAVES) void k_extend_shade(
+__global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
     PassParams P, uint32_t k, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
     const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
     const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
@@ -511,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
 // slot.  Overflowing rays are listed for the deep-stack kernel and applied by
 // k_shadow_apply_listed.
 template <bool COUNT, bool DEC64, int BLOCK>
-__global__ __launch_bounds__(BLOCK, RC_MIN_WAVES) void k_shadow(const NodeT* __restrict__ nodes,
+__global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT* __restrict__ nodes,
                                                        const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
                                                        const float4* __restrict__ scontrib, float4* __restrict__ L,
                                                        uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr,
@@ -1184,11 +1194,14 @@ __global__ __launch_bounds__(kBlock) void k_phong_prepare(uint32_t npix, LightPa
                                                           const float4* __restrict__ shade,
                                                           const DevMaterial* __restrict__ mats,
                                                           rc_ray* __restrict__ srays, float4* __restrict__ direct,
-                                                          float4* __restrict__ rgba) {
+                                                          float4* __restrict__ rgba, int32_t* __restrict__ ids,
+                                                          float* __restrict__ hit_t) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npix) return;
     float4 h = reinterpret_cast<const float4*>(hits)[p];
     int tri = __float_as_int(h.w);
+    if (ids) ids[p] = tri;
+    if (hit_t) hit_t[p] = h.x;
     float4* sr = reinterpret_cast<float4*>(srays + p);
     if (tri < 0) {
         sr[0] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // empty interval: never occluded
@@ -1227,20 +1240,11 @@ __global__ __launch_bounds__(kBlock) void k_phong_prepare(uint32_t npix, LightPa
     rgba[p] = make_float4(col.x, col.y, col.z, 1.0f);
 }
 
-// Adds the unoccluded direct light and splits the hit records into the
-// entry point's hit-index and distance arrays (either may be null).
+// Adds the unoccluded direct light.
 __global__ __launch_bounds__(kBlock) void k_phong_finish(uint32_t npix, const uint8_t* __restrict__ occ,
-                                                         const float4* __restrict__ direct, float4* __restrict__ rgba,
-                                                         const rc_hit* __restrict__ hits, int32_t* __restrict__ ids,
-                                                         float* __restrict__ dist) {
+                                                         const float4* __restrict__ direct, float4* __restrict__ rgba) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= npix) return;
-    if (ids || dist) {
-        const float4 h = reinterpret_cast<const float4*>(hits)[p];
-        if (ids) ids[p] = __float_as_int(h.w);
-        if (dist) dist[p] = h.x;
-    }
-    if (occ[p]) return;
+    if (p >= npix || occ[p]) return;
     float4 c = rgba[p], dd = direct[p];
     rgba[p] = make_float4(c.x + dd.x, c.y + dd.y, c.z + dd.z, 1.0f);
 }
@@ -1278,23 +1282,40 @@ extern "C" int rc_raycast_phong(rc_ctx* c, const rc_scene* s, const rc_camera* c
     FrameBuffers& FB = c->fb;
     RC_TRY(rc_launch_trace(c, s, false, FB.ray_a.as<rc_ray>(), nullptr, npix, FB.hits.as<rc_hit>(), nullptr));
     float4* rgba_d = FB.slotL.as<float4>();
-    RC_TRY(c->prof_begin(&ev));
-    k_phong_prepare<<<grid, kBlock, 0, c->stream>>>(npix, Lp, FB.ray_a.as<rc_ray>(), FB.hits.as<rc_hit>(), s->shade,
-                                                    s->mats, FB.srays.as<rc_ray>(), FB.scontrib.as<float4>(), rgba_d);
-    RC_TRY(c->prof_end(ev, PK_SHADE));
-    RC_TRY(rc_launch_trace(c, s, true, FB.srays.as<rc_ray>(), nullptr, npix, nullptr, FB.socc.as<uint8_t>()));
     // the hit indices and distances go to two device arrays (the queue
-    // buffers, free by now) and from there straight to the caller's arrays
+    // buffers, free by now) as soon as the closest hits are known, and from
+    // there to the caller's arrays on the copy stream while the shadow rays
+    // are traced
     int32_t* ids_d = hit_ids ? reinterpret_cast<int32_t*>(FB.st_a.p) : nullptr;
     float* t_d = t ? reinterpret_cast<float*>(FB.st_b.p) : nullptr;
     RC_TRY(c->prof_begin(&ev));
-    k_phong_finish<<<grid, kBlock, 0, c->stream>>>(npix, FB.socc.as<uint8_t>(), FB.scontrib.as<float4>(), rgba_d,
-                                                   FB.hits.as<rc_hit>(), ids_d, t_d);
+    k_phong_prepare<<<grid, kBlock, 0, c->stream>>>(npix, Lp, FB.ray_a.as<rc_ray>(), FB.hits.as<rc_hit>(), s->shade,
+                                                    s->mats, FB.srays.as<rc_ray>(), FB.scontrib.as<float4>(), rgba_d,
+                                                    ids_d, t_d);
+    RC_TRY(c->prof_end(ev, PK_SHADE));
+    const bool side = hit_ids || t;
+    if (side) {
+        if (!c->copy_stream) RC_TRY_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        if (!c->copy_ev) RC_TRY_HIP(hipEventCreateWithFlags(&c->copy_ev, hipEventDisableTiming));
+        RC_TRY_HIP(hipEventRecord(c->copy_ev, c->stream));
+        RC_TRY_HIP(hipStreamWaitEvent(c->copy_stream, c->copy_ev, 0));
+        if (hit_ids)
+            RC_TRY_HIP(hipMemcpyAsync(hit_ids, ids_d, npix * sizeof(int32_t), hipMemcpyDeviceToHost, c->copy_stream));
+        if (t) RC_TRY_HIP(hipMemcpyAsync(t, t_d, npix * sizeof(float), hipMemcpyDeviceToHost, c->copy_stream));
+    }
+    RC_TRY(rc_launch_trace(c, s, true, FB.srays.as<rc_ray>(), nullptr, npix, nullptr, FB.socc.as<uint8_t>()));
+    RC_TRY(c->prof_begin(&ev));
+    k_phong_finish<<<grid, kBlock, 0, c->stream>>>(npix, FB.socc.as<uint8_t>(), FB.scontrib.as<float4>(), rgba_d);
     RC_TRY(c->prof_end(ev, PK_OTHER));
     RC_TRY_HIP(hipGetLastError());
-    if (hit_ids) RC_TRY_HIP(hipMemcpyAsync(hit_ids, ids_d, npix * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    if (t) RC_TRY_HIP(hipMemcpyAsync(t, t_d, npix * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (rgba) RC_TRY_HIP(hipMemcpyAsync(rgba, rgba_d, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    if (side) {
+        // the ids and t buffers are the frame's queues: the next call on the
+        // main stream must not overwrite them before the copies are done
+        RC_TRY_HIP(hipEventRecord(c->copy_ev, c->copy_stream));
+        RC_TRY_HIP(hipStreamWaitEvent(c->stream, c->copy_ev, 0));
+        RC_TRY_HIP(hipStreamSynchronize(c->copy_stream));
+    }
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     RC_TRY(c->prof_collect());
     c->stats.extension_rays = npix;

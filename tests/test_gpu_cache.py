@@ -158,6 +158,95 @@ def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx):
     np.testing.assert_array_equal(cached, plain)
 
 
+@pytest.fixture(scope="module")
+def workload_terrain(ctx):
+    """The 2,093,058-triangle terrain of the 1080p workload (32-bin SAH BVH)."""
+    import raycer_amd as ra
+
+    sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_TERRAIN, seed=1, grid=1024))
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices, method=api.RCH_BVH_BINNED)
+    s = ctx.upload(sd, nodes, ordered)
+    yield sd, s
+    s.close()
+
+
+@pytest.mark.slow
+def test_zero_cache_identity_at_the_workload(ctx, workload_terrain):
+    """The zero-cache identity on the cached workload itself: the
+    2,093,058-triangle terrain at 1920 x 1080, cache vertex 2, the 65,536-sample
+    training batch.  Without training paths the cached film equals the plain
+    render truncated after 4 rays bit for bit; with them (one Adam step of
+    65,536 samples per frame) the frame takes 65,536 training samples, its
+    loss is finite and the next frame's cache has moved."""
+    sd, s = workload_terrain
+    cfg = cfg_for(sd, 1920, 1080, 16, cache_vertex=2)
+    cfg.sky[0] = cfg.sky[1] = cfg.sky[2] = 0.0
+    assert sd.ntri == 2093058
+    b0, b1 = np.asarray(sd.desc.bounds_min), np.asarray(sd.desc.bounds_max)
+    film = ctx.film(cfg.width, cfg.height)
+    zero = api.RadianceCache(ctx, b0, b1, train_stride=0xFFFFFFFF, max_batch=65536)
+    trained = api.RadianceCache(ctx, b0, b1, max_batch=65536)
+    try:
+        for cache in (zero, trained):
+            p = cache.params()
+            p[cr.OFF_W[4]:cr.OFF_B[4]] = 0.0
+            p[cr.OFF_B[4]:] = -200.0  # exp underflows to exactly 0 in FP32
+            cache.set_params(p)
+        zero.render(s, film, sd.desc.camera, cfg, train=False)
+        st = ctx.stats()
+        cached = film.sums()
+        film.clear()
+        p0 = trained.params()
+        loss = trained.render(s, film, sd.desc.camera, cfg, train=1)
+        st2 = ctx.stats()
+        p1 = trained.params()
+    finally:
+        film.close()
+        zero.close()
+        trained.close()
+    plain_cfg = cfg_for(sd, 1920, 1080, 16, max_path_length=4)
+    plain_cfg.sky[0] = plain_cfg.sky[1] = plain_cfg.sky[2] = 0.0
+    _, plain = render_gpu(ctx, s, sd.desc.camera, plain_cfg)
+    print(f"1080p x 16 spp: {st.cache_queries} cache queries; trained frame: {st2.train_samples} samples, "
+          f"loss {loss:.4f}")
+    assert st.cache_queries > 1920 * 1080 and st.train_samples == 0
+    np.testing.assert_array_equal(cached, plain)
+    assert st2.train_samples == 65536 and np.isfinite(loss) and loss > 0
+    assert np.isfinite(p1).all() and (p1 != p0).any()
+
+
+@pytest.mark.slow
+def test_gradient_at_the_training_batch(ctx):
+    """The gradient at the cached workload's 65,536-sample training batch (the
+    weight-gradient kernel's full split-K depth) against the BF16 model and
+    the FP64 oracle, as at 4,093 samples."""
+    p = cr.random_params(7)
+    n = 65536
+    q = cr.random_queries(n, LO, HI, 2)
+    t = (np.exp(np.random.default_rng(10).normal(size=(n, 3))) * 0.5).astype(np.float32)
+    cache = make_cache(ctx, max_batch=n)
+    try:
+        cache.set_params(p)
+        g, loss = cache.gradient(q, t)
+        g2, _ = cache.gradient(q, t)
+    finally:
+        cache.close()
+    np.testing.assert_array_equal(g, g2)
+    enc = cr.encode(q, LO, HI)
+    lb, gb = cr.loss_grad_bf16(p, enc, t)
+    l64, g64 = orc.mlp_loss_grad(p.astype(np.float64), enc, t.astype(np.float64), 81, 64, 4, 3, rel=True, eps=1e-2)
+    assert abs(loss - lb) < 2e-3 * lb and abs(loss - l64) < 1e-2 * l64
+    for l in range(5):
+        for name, sl in (("W", slice(cr.OFF_W[l], cr.OFF_B[l])),
+                         ("b", slice(cr.OFF_B[l], cr.OFF_B[l] + cr.FAN_OUT[l]))):
+            rm = np.linalg.norm(g[sl] - gb[sl]) / np.linalg.norm(gb[sl])
+            ro = np.linalg.norm(g[sl] - g64[sl]) / np.linalg.norm(g64[sl])
+            cos = float(g[sl] @ g64[sl] / (np.linalg.norm(g[sl]) * np.linalg.norm(g64[sl])))
+            print(f"n = {n}, layer {l} {name}: rel L2 vs bf16 model {rm:.2e}, vs fp64 {ro:.2e}, cos {cos:.6f}")
+            assert rm < 1e-2, (l, name, rm)
+            assert ro < 0.06 and cos > 0.998, (l, name, ro, cos)
+
+
 def test_cached_render_rejects_bad_cache_vertex(ctx):
     sd, s, _ = scene(ctx, "terrain")
     cache = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max)
