@@ -46,6 +46,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Msamples/s at 1080p PT; equal-time MSE vs 4096-spp ref, cached vs plain"
 HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = ROOT / "profiles" / "extend_traffic_terrain1080p.json"
+PROFILE_MFMA = ROOT / "profiles" / "mfma_cache_terrain1080p.json"
 
 
 def log(msg: str) -> None:
@@ -97,6 +98,54 @@ def cpu_reference(sd, cam, cfg, subset: int = 64, seed: int = 7, ref_spp: int = 
     }
     return {"baseline": base, "ids": ids, "ref": ref, "tail_var": tail_var, "split_spp": split_spp,
             "split_s": round(dt_split, 1)}
+
+
+def c1_gpu_image(ctx) -> tuple:
+    """C1: the closed box at 256 x 256, 16 jittered samples, box filter, up to
+    8 rays with Russian roulette from vertex 3, one frame on the GPU."""
+    import raycer_amd as ra
+    from raycer_amd import api
+
+    sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_BOX))
+    nodes, ordered, _, _ = ra.build_bvh(sd.vertices)
+    s = ctx.upload(sd, nodes, ordered)
+    cfg = api.RenderCfg.from_buffer_copy(sd.desc.cfg)
+    cfg.width, cfg.height, cfg.spp, cfg.frame = 256, 256, 16, 0
+    film = ctx.film(cfg.width, cfg.height)
+    try:
+        ctx.render(s, film, sd.desc.camera, cfg)
+        img = film.linear()[:, :3].astype(np.float64)
+    finally:
+        film.close()
+        s.close()
+    return sd, cfg, img
+
+
+def c1_vs_cpu(sd, cfg, img, subset: int = 4, ref_spp: int = 4096, seed: int = 11) -> dict:
+    """The C1 image's MSE against a ref_spp-sample CPU oracle reference on a
+    seeded 1/subset of its pixels, and the MSE the two images' noise alone
+    predicts (from the oracle's per-sample moments; an upper estimate for
+    jittered sampling)."""
+    import oracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    npix = cfg.width * cfg.height
+    ids = np.sort(np.random.default_rng(seed).choice(npix, npix // subset, replace=False)).astype(np.uint32)
+    ocfg = orc.render_cfg(cfg, traversal=0)
+    ocfg.spp = ref_spp
+    ocfg.frame = 700000
+    t0 = time.perf_counter()
+    st = orc.render(orc.BVH(sd.vertices), sd.tri_material, sd.materials, sd.desc.camera, ocfg, pixel_ids=ids,
+                    threads=threads)
+    dt = time.perf_counter() - t0
+    sw, sw2 = st[:, 0:1], st[:, 1:2]
+    ref = st[:, 2:5] / sw
+    var_ref = np.maximum((st[:, 8:11] - 2 * ref * st[:, 5:8] + ref * ref * sw2) / (sw * sw), 0)
+    mse = mse_pair(img[ids], ref)
+    expected = float(np.mean(var_ref * (1.0 + ref_spp / cfg.spp)))
+    return {"workload": "box-256x256-16spp-boxfilter-depth8-rr3", "reference": f"FP64 CPU oracle, {len(ids)} seeded "
+            f"pixels, {ref_spp} spp, {dt:.1f} s", "gpu_mse_vs_cpu_ref": mse, "expected_from_noise": expected,
+            "ratio": round(mse / expected, 4)}
 
 
 def mse_pair(a: np.ndarray, b: np.ndarray) -> float:
@@ -210,8 +259,12 @@ def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2
             out["mlp_kernels"] = {
                 "infer_queries": int(st.cache_queries), "infer_ms": round(st.cache_ms, 4),
                 "infer_tflops": round(infer_flop / (st.cache_ms * 1e-3) / 1e12, 2) if st.cache_ms else None,
+                "infer_frac_of_bf16_peak": round(infer_flop / (st.cache_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)
+                if st.cache_ms else None,
                 "train_samples": int(st.train_samples), "train_ms": round(st.train_ms, 4),
                 "train_tflops": round(train_flop / (st.train_ms * 1e-3) / 1e12, 2) if st.train_ms else None,
+                "train_frac_of_bf16_peak": round(train_flop / (st.train_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)
+                if st.train_ms else None,
                 "peak_bf16_tflops": BF16_PEAK_TFLOPS,
                 "frame_ms": round(st.render_ms, 2),
                 "note": "algorithmic FLOPs (35,328 per query; 95,616 per training sample) over the kernels' HIP-event "
@@ -588,6 +641,11 @@ def main() -> int:
         scale = 1 if a.mse == "full" else 4
         forms = cached_forms(ctx, scene, sd, cfg, scale=scale)
         out["equal_time_mse"] = forms["summary"]
+        if PROFILE_MFMA.exists():
+            try:  # the MFMA counters of the same kernels (rocprofv3, profiles/)
+                out["equal_time_mse"]["mlp_kernels"]["mfma_counters"] = json.loads(PROFILE_MFMA.read_text())["kernels"]
+            except (OSError, ValueError, KeyError):
+                pass
         log(f"cached forms: {json.dumps(forms['summary'])}")
         # the same comparison in the closed box (emitter-lit, up to 8 rays per
         # path), where every path reaches the cache vertex
@@ -601,6 +659,7 @@ def main() -> int:
             box.close()
         log(f"cached forms (box): {json.dumps(out['equal_time_mse_box'])}")
     scene.close()
+    c1 = c1_gpu_image(ctx) if not a.no_cpu_baseline else None
     ctx.close()
     if not a.no_cpu_baseline:
         log("cpu leg: oracle 4096-spp reference on 1/64 of the pixels (the CPU baseline) and the split-tail bound")
@@ -614,6 +673,8 @@ def main() -> int:
             score_against_cpu(forms, cpu)
             log(f"vs CPU reference: {json.dumps(forms['summary']['mse_vs_cpu_ref'])}")
             log(f"variance bound: {json.dumps(forms['summary']['variance_bound'])}")
+        out["c1_mse_vs_cpu_ref"] = c1_vs_cpu(*c1)
+        log(f"C1 vs CPU reference: {json.dumps(out['c1_mse_vs_cpu_ref'])}")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

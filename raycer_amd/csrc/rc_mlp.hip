@@ -1,7 +1,8 @@
 // Radiance cache: an MLP 81 -> 64 -> 64 -> 64 -> 64 -> 3 (ReLU hidden, exp
 // output, 17,923 parameters) evaluated and trained with BF16 MFMA
-// (v_mfma_f32_16x16x16_bf16) and FP32 accumulation.  No counterpart in the
-// reference.
+// (v_mfma_f32_16x16x32_bf16, two 16-wide k-tiles per product, and
+// v_mfma_f32_16x16x16_bf16 for single tiles) and FP32 accumulation.  No
+// counterpart in the reference.
 //
 // Every GEMM is computed transposed, D = W . X^T, with the 16 samples of a
 // wave as the N dimension: the D layout of a 16x16 output tile (lane l holds
@@ -84,6 +85,19 @@ __device__ __forceinline__ v4f relu4(v4f x) {
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// Two k-tiles in one v_mfma_f32_16x16x32_bf16 (gfx950; twice the work of the
+// 16x16x16 form per issue): each lane passes its values of k-tile 0 followed
+// by those of k-tile 1 in both operands.  A and B put the same (lane group,
+// element) at the same k index, so the product sums every k of both tiles
+// once, whatever k each (lane group, element) stands for.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ v4f mfma2(v4s a0, v4s a1, v4s b0, v4s b1, v4f c) {
+    const v8s a = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    const v8s b = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
+                                                   0);
+}
 
 // Input k-tile kt of the 96-wide padded encoding for this lane: features
 // f = 16 kt + 4 (lane / 16) + e.  Features 0..71 are sin/cos(2^k pi p_a) at
@@ -127,12 +141,17 @@ __device__ __forceinline__ void layer0(const v4s* __restrict__ wt, const float* 
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) acc[nt] = bias_tile(params, 0, nt, lane);
 #pragma unroll
-    for (int kt = 0; kt < 6; ++kt) {
+    for (int kt = 0; kt < 6; kt += 2) {
         __builtin_amdgcn_sched_barrier(0);  // keep one k-step of weights in flight
-        v4s x = encode_tile(kt, p01, q, valid, lane);
-        if (x_out) x_out[kt] = x;
+        const v4s x0 = encode_tile(kt, p01, q, valid, lane), x1 = encode_tile(kt + 1, p01, q, valid, lane);
+        if (x_out) {
+            x_out[kt] = x0;
+            x_out[kt + 1] = x1;
+        }
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[nt] = mfma(wt[fwd_tile(0, nt, kt) * 64 + lane], x, acc[nt]);
+        for (int nt = 0; nt < 4; ++nt)
+            acc[nt] = mfma2(wt[fwd_tile(0, nt, kt) * 64 + lane], wt[fwd_tile(0, nt, kt + 1) * 64 + lane], x0, x1,
+                            acc[nt]);
     }
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) h0[nt] = pack4(relu4(acc[nt]));
@@ -144,10 +163,12 @@ __device__ __forceinline__ void hidden(const v4s* __restrict__ wt, const float* 
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) acc[nt] = bias_tile(params, l, nt, lane);
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) {
+    for (int kt = 0; kt < 4; kt += 2) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[nt] = mfma(wt[fwd_tile(l, nt, kt) * 64 + lane], in[kt], acc[nt]);
+        for (int nt = 0; nt < 4; ++nt)
+            acc[nt] = mfma2(wt[fwd_tile(l, nt, kt) * 64 + lane], wt[fwd_tile(l, nt, kt + 1) * 64 + lane], in[kt],
+                            in[kt + 1], acc[nt]);
     }
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) out[nt] = pack4(relu4(acc[nt]));
@@ -157,9 +178,10 @@ __device__ __forceinline__ v4f output_layer(const v4s* __restrict__ wt, const fl
                                             const v4s in[4]) {
     v4f acc = bias_tile(params, 4, 0, lane);
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) {
+    for (int kt = 0; kt < 4; kt += 2) {
         __builtin_amdgcn_sched_barrier(0);
-        acc = mfma(wt[fwd_tile(4, 0, kt) * 64 + lane], in[kt], acc);
+        acc = mfma2(wt[fwd_tile(4, 0, kt) * 64 + lane], wt[fwd_tile(4, 0, kt + 1) * 64 + lane], in[kt], in[kt + 1],
+                    acc);
     }
     return acc;
 }
@@ -324,9 +346,10 @@ __global__ __launch_bounds__(kThreads) void k_cache_fwd_bwd(const v4s* __restric
             for (int nt = 0; nt < 4; ++nt) {
                 v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int kt = 0; kt < 4; ++kt) {
+                for (int kt = 0; kt < 4; kt += 2) {
                     __builtin_amdgcn_sched_barrier(0);
-                    acc = mfma(wtb[bwd_tile(l, nt, kt) * 64 + lane], dcur[kt], acc);
+                    acc = mfma2(wtb[bwd_tile(l, nt, kt) * 64 + lane], wtb[bwd_tile(l, nt, kt + 1) * 64 + lane],
+                                dcur[kt], dcur[kt + 1], acc);
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e) acc[e] = bf2f(h[l - 1][nt][e]) > 0.0f ? acc[e] : 0.0f;
@@ -373,10 +396,21 @@ __global__ __launch_bounds__(kThreads) void k_cache_wgrad(const short* __restric
     const int fi = kActBase[T.l] + 16 * T.it + (lane & 15);
     v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
     float bsum = 0.0f;
-    for (uint32_t k = k0; k < k1; ++k) {
-        uint32_t quad = k * 4 + (lane >> 4);  // samples 4 quad .. 4 quad + 3
-        v4s a = *reinterpret_cast<const v4s*>(dz + ((size_t(quad) * kDzF + fo) << 2));
-        v4s b = *reinterpret_cast<const v4s*>(act + ((size_t(quad) * kActF + fi) << 2));
+    uint32_t k = k0;
+    for (; k + 1 < k1; k += 2) {  // two 16-sample steps per 16x16x32 product
+        const uint32_t q0 = k * 4 + (lane >> 4), q1 = q0 + 4;  // samples 4 q .. 4 q + 3
+        const v4s a0 = *reinterpret_cast<const v4s*>(dz + ((size_t(q0) * kDzF + fo) << 2));
+        const v4s b0 = *reinterpret_cast<const v4s*>(act + ((size_t(q0) * kActF + fi) << 2));
+        const v4s a1 = *reinterpret_cast<const v4s*>(dz + ((size_t(q1) * kDzF + fo) << 2));
+        const v4s b1 = *reinterpret_cast<const v4s*>(act + ((size_t(q1) * kActF + fi) << 2));
+        acc = mfma2(a0, a1, b0, b1, acc);
+        bsum += (bf2f(a0[0]) + bf2f(a0[1])) + (bf2f(a0[2]) + bf2f(a0[3]));
+        bsum += (bf2f(a1[0]) + bf2f(a1[1])) + (bf2f(a1[2]) + bf2f(a1[3]));
+    }
+    if (k < k1) {
+        const uint32_t quad = k * 4 + (lane >> 4);
+        const v4s a = *reinterpret_cast<const v4s*>(dz + ((size_t(quad) * kDzF + fo) << 2));
+        const v4s b = *reinterpret_cast<const v4s*>(act + ((size_t(quad) * kActF + fi) << 2));
         acc = mfma(a, b, acc);
         bsum += (bf2f(a[0]) + bf2f(a[1])) + (bf2f(a[2]) + bf2f(a[3]));
     }
