@@ -12,7 +12,11 @@ throughput).
 
 Prints one JSON line.  Besides the headline Msamples/s it reports the
 roofline of the dominant kernel, k_extend_shade (closest-hit traversal fused
-with shading).  Its algorithmic bytes per frame are
+with shading): `achieved` is the HBM bytes rocprofv3 counts per launch
+(FETCH_SIZE + WRITE_SIZE, calibrated on a copy of known size, kept in
+profiles/extend_traffic_terrain1080p.json) over the average launch time
+measured here with HIP events on the renderer's stream.  `demand` holds the
+algorithmic bytes per frame,
   16 * child records fetched (a 4-wide node is 64 bytes: 8-bit quantized
     child boxes, counts and links) + 48 * triangle records tested
   + 96 per extension ray (ray 32 + path state 16 + shading record 16 in,
@@ -20,10 +24,15 @@ with shading).  Its algorithmic bytes per frame are
   + 48 per continuation ray written (ray 32 + state 16)
   + 48 per shadow ray written (ray 32 + contribution 16),
 counted by the counting variant on one untimed frame (materials, a few
-64-byte records, are left out) and divided by the kernel time measured with HIP
-events on the renderer's stream during the timed frames.  The CPU baseline is
-the FP64 oracle (restatement of the reference's OpenMP CPU path, built -O3
--ffast-math) on a seeded half of the pixels.
+64-byte records, are left out), against the library's L2, Infinity Cache and
+HBM read probes.  Further lines: the 1080p Phong raycast (C2), the Whitted
+tracer, the 64-scene batch graph (C4) and the radiance-cached renderer at
+equal time (C5).  The CPU leg runs the FP64 oracle (restatement of the
+reference's OpenMP CPU path, built -O3 -ffast-math) on the host cores: a
+4096-spp reference on a seeded 1/64 of the 1080p pixels, timed as the CPU
+baseline and used to score the plain and cached images, the split-tail
+variance bound of the cache, and the C1 image against its own 4096-spp
+reference.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 """MI355X (gfx950) triangle path tracer with raycer's Scene/Tracer/Film interface.
 
 The product is libraycer.so (HIP kernels + C ABI, include/raycer_hip.h and
-include/raycer_host.h) with the C++ interface in csrc/cpp/raycer.hpp.  This
+include/raycer_host.h) with the C++ interface in include/raycer.hpp.  This
 package holds the build script and a ctypes binding used by the tests and
 the benchmark.
 """
