@@ -474,6 +474,9 @@ def main() -> int:
     ap.add_argument("--spp-per-pass", type=int, default=64,
                     help="samples per pixel per wavefront pass (64: the whole frame at once, 132.7 M paths, "
                          "~27 GB of path queues; 0: the library's automatic choice)")
+    ap.add_argument("--park-steps", type=int, default=None,
+                    help="traversal steps before an extension ray is parked for the resume kernel (0: never; "
+                         "default: the library's)")
     ap.add_argument("--hit", choices=("fp32", "fp64"), default="fp32",
                     help="triangle decision: FP32 Möller–Trumbore (default) or FP32 filter + FP64 decision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -508,7 +511,7 @@ def main() -> int:
     film = ctx.film(cfg.width, cfg.height)
     log(f"device {name} ({cus} CUs), setup {time.perf_counter() - t0:.1f} s")
 
-    ctx.set_tuning(spp_per_pass=a.spp_per_pass)
+    ctx.set_tuning(spp_per_pass=a.spp_per_pass, park_steps=a.park_steps)
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
     probes = {"l2_4MiB": ctx.read_bandwidth(4 << 20, 512), "infinity_cache_128MiB": ctx.read_bandwidth(128 << 20, 16),
               "hbm_4GiB": ctx.read_bandwidth(4 << 30, 1)}
@@ -535,7 +538,7 @@ def main() -> int:
     ctx.synchronize()
     ext_ms = shd_ms = shade_ms = other_ms = 0.0
     ext_launches = 0
-    total_ext = total_shadow = overflow = 0
+    total_ext = total_shadow = overflow = parked = 0
     t_start = time.perf_counter()
     for i in range(a.steps):
         cfg.frame = i
@@ -549,6 +552,7 @@ def main() -> int:
         total_ext += st.extension_rays
         total_shadow += st.shadow_rays
         overflow += st.overflow_rays
+        parked += st.parked_rays
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
     ctx.set_profiling(False)
@@ -593,6 +597,7 @@ def main() -> int:
             "bvh": "32-bin SAH, collapsed to 4-wide",
             "hit_decision": a.hit,
             "samples_per_pass": min(a.spp_per_pass, cfg.spp) if a.spp_per_pass else "auto",
+            "park_steps": a.park_steps if a.park_steps is not None else api.PARK_STEPS_DEFAULT,
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -622,6 +627,7 @@ def main() -> int:
         "gsample_bounces_per_s": round(total_ext / elapsed / 1e9, 4),
         "shadow_rays_per_s_G": round(total_shadow / elapsed / 1e9, 4),
         "overflow_rays_per_step": overflow // a.steps,
+        "parked_rays_per_step": parked // a.steps,
         "kernel_ms_per_step": {
             "extend": round(ext_ms / a.steps, 2),
             "shadow": round(shd_ms / a.steps, 2),

@@ -172,17 +172,22 @@ typedef struct rc_point_light {
 /* Launch tuning: threads per block of the ray-query kernels, pass size in
  * samples per pixel (0, the default: each frame takes as many samples per pass
  * as keep a pass within 2^26 paths, 2^21 for rc_raytrace; else 1..64), grid
- * limit of the ray-query kernels in blocks per CU, and the traversal stack
+ * limit of the ray-query kernels in blocks per CU, the traversal stack
  * entries held on chip (registers and LDS) before a ray is handed to the
  * deep-stack kernel, which walks it again with a stack in global memory
  * (0, the default: all the build holds; 1..that depth: fewer, which sends
- * more rays down the deep-stack path and changes no result).
- * Defaults 256, 0, 64, 0. */
+ * more rays down the deep-stack path and changes no result), and the
+ * traversal steps a path tracer's extension ray takes in the first kernel
+ * of a bounce before it is parked and finished by a second kernel whose
+ * waves hold only such long traversals (0: never park; 1..1024; the results
+ * are the same either way).  Defaults 256, 0, 64, 0, RC_PARK_STEPS_DEFAULT. */
+#define RC_PARK_STEPS_DEFAULT 28
 typedef struct rc_tuning {
     uint32_t block;
     uint32_t spp_per_pass;
     uint32_t blocks_per_cu;
     uint32_t stack_limit;
+    uint32_t park_steps;
 } rc_tuning;
 
 typedef struct rc_stats {
@@ -205,6 +210,7 @@ typedef struct rc_stats {
     uint64_t train_samples;      /* training samples of the last cached render */
     double graph_ms;             /* last batch call: capture and instantiation or update (0: graph relaunched) */
     uint64_t graph_relaunched;   /* last batch call: 1 when it relaunched the executable graph as it was */
+    uint64_t parked_rays;        /* extension traversals parked and finished by the resume kernel (rc_tuning.park_steps) */
 } rc_stats;
 
 /* ------------------------------------------------------------ context */

@@ -159,9 +159,12 @@ class PointLight(C.Structure):
                 ("ambient_intensity", C.c_float)]
 
 
+PARK_STEPS_DEFAULT = 28  # RC_PARK_STEPS_DEFAULT in include/raycer_hip.h
+
+
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_uint32), ("spp_per_pass", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("stack_limit", C.c_uint32)]
+                ("stack_limit", C.c_uint32), ("park_steps", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -171,7 +174,7 @@ class Stats(C.Structure):
                 ("shade_ms", C.c_double), ("other_ms", C.c_double), ("render_ms", C.c_double),
                 ("shadow_launches", C.c_uint64), ("cache_ms", C.c_double), ("train_ms", C.c_double),
                 ("cache_queries", C.c_uint64), ("train_samples", C.c_uint64), ("graph_ms", C.c_double),
-                ("graph_relaunched", C.c_uint64)]
+                ("graph_relaunched", C.c_uint64), ("parked_rays", C.c_uint64)]
 
 
 class BatchPlan(C.Structure):
@@ -610,11 +613,13 @@ class Context:
         """"fp32" (RC_HIT_FP32, the default) or "fp64" (RC_HIT_FP64)."""
         check(lib().rc_set_hit_decision(self.h, {"fp32": RC_HIT_FP32, "fp64": RC_HIT_FP64}[mode]))
 
-    def set_tuning(self, block=256, spp_per_pass=0, blocks_per_cu=64, stack_limit=0) -> None:
+    def set_tuning(self, block=256, spp_per_pass=0, blocks_per_cu=64, stack_limit=0, park_steps=None) -> None:
         """rc_set_tuning; the defaults are the context's own (spp_per_pass 0:
         each frame chooses its pass size; stack_limit 0: the whole on-chip
-        traversal stack before the deep-stack kernel takes a ray over)."""
-        t = Tuning(block, spp_per_pass, blocks_per_cu, stack_limit)
+        traversal stack before the deep-stack kernel takes a ray over;
+        park_steps None: the library's default)."""
+        t = Tuning(block, spp_per_pass, blocks_per_cu, stack_limit,
+                   PARK_STEPS_DEFAULT if park_steps is None else park_steps)
         check(lib().rc_set_tuning(self.h, C.byref(t)))
 
     def stats(self) -> Stats:

@@ -41,7 +41,7 @@ void put(std::vector<uint8_t>& k, const T& v) {
 
 void put_buffers(std::vector<uint8_t>& k, const FrameBuffers& B) {
     const DevBuf* b[] = {&B.ray_a, &B.ray_b, &B.st_a, &B.st_b, &B.hits, &B.srays, &B.scontrib, &B.socc, &B.slotL,
-                         &B.slotW, &B.ovf_list, &B.deep_stack};
+                         &B.slotW, &B.ovf_list, &B.deep_stack, &B.park};
     for (const DevBuf* x : b) put(k, x->p);
     put(k, B.counters);
 }
@@ -89,6 +89,7 @@ __global__ void k_batch_zero(DevCounters* ctr) {
     ctr->ext_rays = 0;
     ctr->shadow_rays = 0;
     ctr->ovf_total = 0;
+    ctr->park_total = 0;
     ctr->ovf_count = 0;
     ctr->error = 0;
 }
@@ -178,6 +179,7 @@ int render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film* const* films
     put(key, c->counting);
     put(key, c->hit_fp64);
     put(key, c->stack_depth());
+    put(key, c->tuning.park_steps);
     for (uint32_t b = 0; b < branches; ++b) {
         put_buffers(key, branch_buffers(c, b));
         put(key, assign[b].size());
@@ -235,7 +237,7 @@ int render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film* const* films
     RC_TRY_HIP(hipGraphLaunch(c->batch_graph, c->stream));
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     c->stats.render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    uint64_t ext = 0, shadow = 0, ovf = 0;
+    uint64_t ext = 0, shadow = 0, ovf = 0, parked = 0;
     for (uint32_t b = 0; b < branches; ++b) {
         DevCounters h;
         RC_TRY_HIP(hipMemcpy(&h, branch_buffers(c, b).counters, sizeof(DevCounters), hipMemcpyDeviceToHost));
@@ -243,10 +245,12 @@ int render_batch(rc_ctx* c, const rc_scene* const* scenes, rc_film* const* films
         ext += h.ext_rays;
         shadow += h.shadow_rays;
         ovf += h.ovf_total;
+        parked += h.park_total;
     }
     c->stats.extension_rays = ext;
     c->stats.shadow_rays = shadow;
     c->stats.overflow_rays = ovf;
+    c->stats.parked_rays = parked;
     uint64_t samples = 0;
     for (size_t i = 0; i < n; ++i) samples += uint64_t(cfgs[i].width) * cfgs[i].height * cfgs[i].spp;
     c->stats.samples = samples;

@@ -225,6 +225,8 @@ extern "C" int rc_set_tuning(rc_ctx* c, const rc_tuning* t) {
     if (t->stack_limit > uint32_t(rcd::kOnChipStack))
         return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: stack_limit must be 0 (all) or in [1, " +
                                                 std::to_string(rcd::kOnChipStack) + "]");
+    if (t->park_steps > 1024)
+        return rc_host_fail(RC_ERR_INVALID, "rc_set_tuning: park_steps must be 0 (never park) or in [1, 1024]");
     c->tuning = *t;
     return RC_OK;
 }

@@ -732,6 +732,92 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
     return false;
 }
 
+// A traversal parked mid-way (the extension kernel's step budget ran out):
+// 32 words per ray, written and read as eight 16-byte vectors.  Words: 0 the
+// queue entry, 1 the current entry, 2 the stack size, 3 the best hit's id,
+// 4..6 its t, u, v, 7 the cull distance, 8..9 the best distance (a double
+// with the FP64 decision, else a float in word 8), 10.. the register stack
+// (top first), then the LDS rows in their order (only the first sp - kRegStack
+// are meaningful).
+constexpr int kParkWords = 32;
+static_assert(10 + kRegStack + kLdsStack <= kParkWords, "a parked traversal does not fit its record");
+
+// Word w of a parked record (compile-time w after unrolling).
+template <bool DEC64>
+__device__ __forceinline__ uint32_t park_word(const Trav<DEC64>& S, uint32_t entry, const uint32_t* stack, int stride,
+                                              int w) {
+    if (w == 0) return entry;
+    if (w == 1) return S.cur;
+    if (w == 2) return uint32_t(S.sp);
+    if (w == 3) return S.R.id;
+    if (w == 4) return __float_as_uint(S.R.t);
+    if (w == 5) return __float_as_uint(S.R.u);
+    if (w == 6) return __float_as_uint(S.R.v);
+    if (w == 7) return __float_as_uint(S.cull);
+    if (w == 8 || w == 9) {
+        if constexpr (DEC64) {
+            const unsigned long long b = __double_as_longlong(S.best);
+            return w == 8 ? uint32_t(b) : uint32_t(b >> 32);
+        } else {
+            return w == 8 ? __float_as_uint(S.best) : 0u;
+        }
+    }
+    if (w < 10 + kRegStack) return S.reg[w - 10];
+    const int j = w - 10 - kRegStack;  // LDS row
+    return j < kLdsStack && j < S.sp - kRegStack ? stack[j * stride] : 0u;
+}
+
+template <bool DEC64>
+__device__ __forceinline__ void trav_park(const Trav<DEC64>& S, uint32_t entry, const uint32_t* stack, int stride,
+                                          uint32_t* __restrict__ rec) {
+    uint4* r = reinterpret_cast<uint4*>(rec);
+#pragma unroll
+    for (int q = 0; q < kParkWords / 4; ++q)
+        r[q] = make_uint4(park_word(S, entry, stack, stride, 4 * q), park_word(S, entry, stack, stride, 4 * q + 1),
+                          park_word(S, entry, stack, stride, 4 * q + 2), park_word(S, entry, stack, stride, 4 * q + 3));
+}
+
+template <bool DEC64>
+__device__ __forceinline__ void unpark_word(Trav<DEC64>& S, uint32_t* stack, int stride, int w, uint32_t v,
+                                            uint32_t& best_lo) {
+    if (w == 1) S.cur = v;
+    else if (w == 2) S.sp = int(v);
+    else if (w == 3) S.R.id = v;
+    else if (w == 4) S.R.t = __uint_as_float(v);
+    else if (w == 5) S.R.u = __uint_as_float(v);
+    else if (w == 6) S.R.v = __uint_as_float(v);
+    else if (w == 7) S.cull = __uint_as_float(v);
+    else if (w == 8) {
+        best_lo = v;
+        if constexpr (!DEC64) S.best = __uint_as_float(v);
+    } else if (w == 9) {
+        if constexpr (DEC64) S.best = __longlong_as_double((long long)(uint64_t(best_lo) | (uint64_t(v) << 32)));
+    } else if (w >= 10 && w < 10 + kRegStack) {
+        S.reg[w - 10] = v;
+    } else if (w >= 10 + kRegStack) {
+        const int j = w - 10 - kRegStack;
+        if (j < kLdsStack && j < S.sp - kRegStack) stack[j * stride] = v;
+    }
+}
+
+// Resumes a parked traversal: S must have been begun on the same ray
+// (trav_begin), which restores the ray's own fields; this restores the rest
+// and the lane's LDS rows (word 0, the queue entry, is the caller's).
+template <bool DEC64>
+__device__ __forceinline__ void trav_unpark(Trav<DEC64>& S, const uint32_t* __restrict__ rec, uint32_t* stack,
+                                            int stride) {
+    const uint4* r = reinterpret_cast<const uint4*>(rec);
+    uint32_t best_lo = 0;
+#pragma unroll
+    for (int q = 0; q < kParkWords / 4; ++q) {
+        const uint4 v = r[q];
+        unpark_word(S, stack, stride, 4 * q, v.x, best_lo);
+        unpark_word(S, stack, stride, 4 * q + 1, v.y, best_lo);
+        unpark_word(S, stack, stride, 4 * q + 2, v.z, best_lo);
+        unpark_word(S, stack, stride, 4 * q + 3, v.w, best_lo);
+    }
+}
+
 template <bool ANY, bool COUNT, bool DEC64>
 __device__ __forceinline__ TraceResult traverse_ifif(const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
                                                      float3 o, float3 d, float tmin, float tmax, uint32_t* stack,

@@ -41,8 +41,10 @@ struct DevCounters {
     unsigned long long ext_rays;
     unsigned long long shadow_rays;
     unsigned long long ovf_total;  // rays the deep-stack kernel walked, over every launch since the last reset
+    unsigned long long park_total; // traversals parked for the resume kernel since the last reset
     uint32_t ovf_count;            // rays listed for the next deep-stack launch
     uint32_t queue_count[4];   // wavefront queue sizes (ping, pong, shadow, scratch)
+    uint32_t park_count[2];    // traversals parked in the two park regions (ping-pong between phases)
     uint32_t error;
 };
 
@@ -67,7 +69,7 @@ inline uint32_t pass_samples(const rc_tuning& t, uint32_t npix, uint32_t spp, si
 // and its counters.  The context owns one set (fb, whose counters are the
 // context's); a batch owns one more per concurrent branch.
 struct FrameBuffers {
-    DevBuf ray_a, ray_b, st_a, st_b, hits, srays, scontrib, socc, slotL, slotW, ovf_list, deep_stack;
+    DevBuf ray_a, ray_b, st_a, st_b, hits, srays, scontrib, socc, slotL, slotW, ovf_list, deep_stack, park;
     DevCounters* counters = nullptr;
     bool own_counters = false;
     int ensure(size_t npaths);  // every buffer sized for a pass of npaths paths
@@ -89,7 +91,7 @@ struct rc_ctx {
     bool profiling = false;
     bool counting = false;
     bool hit_fp64 = false;  // RC_HIT_FP64: FP32 filter, FP64 decision of triangle hits
-    rc_tuning tuning{256, 0, 64, 0};  // samples per pass chosen per frame, the whole on-chip stack
+    rc_tuning tuning{256, 0, 64, 0, RC_PARK_STEPS_DEFAULT};  // samples per pass chosen per frame, the whole on-chip stack
 
     // traversal stack entries held on chip before a ray goes to the deep-stack kernel
     int stack_depth() const {

@@ -40,6 +40,20 @@ constexpr int kLdsRows = kLdsStack > 0 ? kLdsStack : 1;  // LDS stack entries pe
 #ifndef RC_MIN_WAVES
 #define RC_MIN_WAVES 8
 #endif
+// Park records per pass: 1 / RC_PARK_DIV of its paths (128 bytes each).
+// Kernels per parked extension launch (the extension kernel and
+// RC_PARK_PHASES - 1 resume kernels) and the iteration budget of the
+// resume kernels that park again.
+#ifndef RC_PARK_DIV
+#define RC_PARK_DIV 2
+#endif
+#ifndef RC_PARK_PHASES
+#define RC_PARK_PHASES 3
+#endif
+#ifndef RC_PARK_STEPS2
+#define RC_PARK_STEPS2 20
+#endif
+static_assert(RC_PARK_PHASES >= 2, "a parking extension launch needs a resume kernel");
 #ifndef RC_MIN_WAVES_FP64
 #define RC_MIN_WAVES_FP64 1
 #endif
@@ -436,15 +450,33 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
 // Blocks past the live entries return at once (the grid is sized for the
 // queue's capacity; the test is uniform over the block, before any barrier).
 // DEC64: the triangle decision mode; BLOCK: threads per block.
-template <bool COUNT, bool CAM, bool DEC64, int BLOCK>
+// Parking: an extension ray still traversing after `steps` loop iterations
+// (the if-if loop's, one node visit and at most one leaf each) writes its
+// traversal state to a park region and is continued by k_extend_resume,
+// whose waves hold only such rays; a resume kernel may park its own long
+// traversals again for the next one, the last runs them to the end.  A wave
+// otherwise runs until its longest ray ends: in the diffuse bounces most rays
+// leave for the sky after a few nodes while a few graze the terrain for many,
+// and the waves' lanes idle.  Records beyond a region's `cap` are not
+// written: those rays finish in place.
+struct ParkIO {
+    uint32_t* rec;   // kParkWords words per parked traversal
+    uint32_t cap;    // records of the region
+    uint32_t steps;  // budget of the kernel writing the region (0: no parking)
+    uint32_t which;  // the region's counter: DevCounters::park_count[which]
+};
+
+template <bool COUNT, bool CAM, bool DEC64, int BLOCK, bool PARK>
 __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
     PassParams P, uint32_t k, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
     const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
     const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
     DevCounters* __restrict__ ctr, uint32_t cin, uint32_t cout, CacheIO CI, float* __restrict__ W,
-    rc_ray* __restrict__ cam_rays, float4* __restrict__ cam_st) {
+    rc_ray* __restrict__ cam_rays, float4* __restrict__ cam_st, ParkIO PK) {
+    static_assert(!(PARK && CAM), "the camera bounce does not park");
     __shared__ uint32_t stack[kLdsRows * BLOCK];
+    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
     const uint32_t n = ctr->queue_count[cin];
     if (blockIdx.x * uint32_t(BLOCK) >= n) return;
     const uint32_t i = trace_block<BLOCK>(n) * BLOCK + threadIdx.x;
@@ -454,7 +486,45 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
     O.alive = O.shadow = false;
     O.rec = kNoRecord;
     uint32_t slot = 0;
-    if (i < n) {
+    if constexpr (PARK) {
+        // the path state is read after the traversal, and the ray kept only
+        // in S, so that fewer values live across the traversal loop
+        Trav<DEC64> S;
+        bool done = true;
+        if (i < n) {
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            const float4 r0 = rp[0], r1 = rp[1];
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+            uint32_t steps = 0;
+            do {
+                done = trav_step<false, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, P.stack_limit,
+                                                      n_rec, n_tri);
+            } while (!done && ++steps < PK.steps);
+        }
+        // every thread of the block reserves (one atomic per block)
+        const uint32_t pos = block_append<BLOCK>(!done, &ctr->park_count[PK.which], append_scratch);
+        bool parked = false;
+        if (!done) {
+            if (pos < PK.cap) {
+                trav_park(S, i, stack + threadIdx.x, BLOCK, PK.rec + size_t(pos) * kParkWords);
+                parked = true;
+            } else {
+                while (!trav_step<false, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, P.stack_limit,
+                                                       n_rec, n_tri)) {
+                }
+            }
+        }
+        if (i < n && !parked) {
+            if (S.R.overflow) {
+                ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+            } else {
+                const float4 s = st[i];
+                slot = __float_as_uint(s.w);
+                O = shade_vertex(P, k, make_float4(S.o.x, S.o.y, S.o.z, 0.0f), make_float4(S.d.x, S.d.y, S.d.z, 0.0f),
+                                 s, S.R.t, S.R.id == kNoHit ? -1 : int(S.R.id), shade, mats, L, CI);
+            }
+        }
+    } else if (i < n) {
         float4 r0, r1, s;
         if (CAM) {
             W[i] = camera_ray(P, i, r0, r1);
@@ -481,13 +551,88 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
             O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
         }
     }
-    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
     emit_vertex<BLOCK>(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
     }
 }
+
+// The parked traversals of region `in`, continued and shaded as
+// k_extend_shade would have (same rays, same steps, same hit), then their
+// continuation and shadow rays appended; with PARK those still traversing
+// after out.steps more iterations are parked again in region `out`.  One
+// record per thread; the grid covers the region's capacity and blocks past
+// the parked count return at once (uniform over the block, before any
+// barrier).
+template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
+__global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
+    PassParams P, uint32_t k, const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris,
+    const rc_ray* __restrict__ rays, const float4* __restrict__ st, const float4* __restrict__ shade,
+    const DevMaterial* __restrict__ mats, rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
+    rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
+    DevCounters* __restrict__ ctr, uint32_t cout, CacheIO CI, ParkIO in, ParkIO out) {
+    __shared__ uint32_t stack[kLdsRows * BLOCK];
+    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
+    const uint32_t n = min(ctr->park_count[in.which], in.cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&ctr->park_total, (unsigned long long)n);
+    if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t q = blockIdx.x * uint32_t(BLOCK) + threadIdx.x;
+    uint32_t n_rec = 0, n_tri = 0;
+    ShadeOut O;
+    O.alive = O.shadow = false;
+    O.rec = kNoRecord;
+    uint32_t slot = 0;
+    Trav<DEC64> S;
+    uint32_t i = 0;
+    bool done = true;
+    if (q < n) {
+        const uint32_t* rec = in.rec + size_t(q) * kParkWords;
+        i = rec[0];  // the queue entry (word 0)
+        {
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            const float4 r0 = rp[0], r1 = rp[1];
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+        }
+        trav_unpark(S, rec, stack + threadIdx.x, BLOCK);
+        uint32_t steps = 0;
+        do {
+            done = trav_step<false, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, P.stack_limit, n_rec,
+                                                  n_tri);
+        } while (!done && (!PARK || ++steps < out.steps));
+    }
+    bool parked = false;
+    if constexpr (PARK) {
+        const uint32_t pos = block_append<BLOCK>(!done, &ctr->park_count[out.which], append_scratch);
+        if (!done) {
+            if (pos < out.cap) {
+                trav_park(S, i, stack + threadIdx.x, BLOCK, out.rec + size_t(pos) * kParkWords);
+                parked = true;
+            } else {
+                while (!trav_step<false, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, P.stack_limit,
+                                                       n_rec, n_tri)) {
+                }
+            }
+        }
+    }
+    if (q < n && !parked) {
+        if (S.R.overflow) {
+            ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+        } else {
+            const float4 s = st[i];
+            slot = __float_as_uint(s.w);
+            O = shade_vertex(P, k, make_float4(S.o.x, S.o.y, S.o.z, 0.0f), make_float4(S.d.x, S.d.y, S.d.z, 0.0f), s,
+                             S.R.t, S.R.id == kNoHit ? -1 : int(S.R.id), shade, mats, L, CI);
+        }
+    }
+    emit_vertex<BLOCK>(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
+    if (COUNT) {
+        wave_add(&ctr->node_records, n_rec);
+        wave_add(&ctr->tri_tests, n_tri);
+    }
+}
+
+__global__ void k_zero_park(DevCounters* ctr, uint32_t which) { ctr->park_count[which] = 0; }
 
 // Shading of the listed queue entries (rays the deep-stack kernel traced).
 __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t k, const rc_ray* __restrict__ rays,
@@ -624,6 +769,8 @@ __global__ void k_prep(DevCounters* ctr, uint32_t cout) {
     ctr->queue_count[cout] = 0;
     ctr->queue_count[2] = 0;
     ctr->ovf_count = 0;
+    ctr->park_count[0] = 0;
+    ctr->park_count[1] = 0;
 }
 
 __global__ void k_reset_ovf(DevCounters* ctr) { ctr->ovf_count = 0; }
@@ -733,21 +880,30 @@ struct ExtendArgs {
     uint32_t cin, cout;
     CacheIO CI;
     float* W;
+    ParkIO PK;  // PK.steps > 0: the non-camera bounces park long traversals
 };
 
-template <bool COUNT, bool CAM, bool DEC64, int BLOCK>
+template <bool COUNT, bool CAM, bool DEC64, int BLOCK, bool PARK>
 void extend_t(const ExtendArgs& a, uint32_t n, hipStream_t st) {
-    k_extend_shade<COUNT, CAM, DEC64, BLOCK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
+    k_extend_shade<COUNT, CAM, DEC64, BLOCK, PARK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
         a.P, a.k, a.s->nodes, a.s->tris, a.rin, a.sin, a.s->shade, a.s->mats, a.rout, a.sout, a.srays, a.scontrib, a.L,
-        a.ovf, a.ctr, a.cin, a.cout, a.CI, a.W, a.rin, a.sin);
+        a.ovf, a.ctr, a.cin, a.cout, a.CI, a.W, a.rin, a.sin, a.PK);
 }
 
 template <bool COUNT, bool CAM, bool DEC64>
 void extend_b(const ExtendArgs& a, uint32_t n, hipStream_t st, uint32_t block) {
-    if constexpr (!COUNT) {
-        if (block == 128) return extend_t<COUNT, CAM, DEC64, 128>(a, n, st);
+    if constexpr (!CAM) {
+        if (a.PK.steps) {
+            if constexpr (!COUNT) {
+                if (block == 128) return extend_t<COUNT, CAM, DEC64, 128, true>(a, n, st);
+            }
+            return extend_t<COUNT, CAM, DEC64, 256, true>(a, n, st);
+        }
     }
-    extend_t<COUNT, CAM, DEC64, 256>(a, n, st);
+    if constexpr (!COUNT) {
+        if (block == 128) return extend_t<COUNT, CAM, DEC64, 128, false>(a, n, st);
+    }
+    extend_t<COUNT, CAM, DEC64, 256, false>(a, n, st);
 }
 
 void launch_extend(const ExtendArgs& a, uint32_t n, hipStream_t st, bool count, bool cam, bool dec64, uint32_t block) {
@@ -760,6 +916,40 @@ void launch_extend(const ExtendArgs& a, uint32_t n, hipStream_t st, bool count, 
         else { if (dec64) RC_EXT(false, false, true); else RC_EXT(false, false, false); }
     }
 #undef RC_EXT
+}
+
+// Parked traversals: a grid over the input region's capacity (blocks past
+// the parked count return at once).
+template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
+void resume_t(const ExtendArgs& a, const ParkIO& in, const ParkIO& out, hipStream_t st) {
+    k_extend_resume<COUNT, DEC64, BLOCK, PARK><<<std::max(1u, (in.cap + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(
+        a.P, a.k, a.s->nodes, a.s->tris, a.rin, a.sin, a.s->shade, a.s->mats, a.rout, a.sout, a.srays, a.scontrib, a.L,
+        a.ovf, a.ctr, a.cout, a.CI, in, out);
+}
+
+template <bool COUNT, bool DEC64, bool PARK>
+void resume_b(const ExtendArgs& a, const ParkIO& in, const ParkIO& out, hipStream_t st, uint32_t block) {
+    if constexpr (!COUNT) {
+        if (block == 128) return resume_t<COUNT, DEC64, 128, PARK>(a, in, out, st);
+    }
+    resume_t<COUNT, DEC64, 256, PARK>(a, in, out, st);
+}
+
+// The resume kernels of an extension launch that parked into region 0:
+// `phases` - 1 of them, alternating between the two regions; all but the
+// last park again after `steps2` iterations.
+void launch_resumes(const ExtendArgs& a, const ParkIO regions[2], uint32_t phases, uint32_t steps2, hipStream_t st,
+                    bool count, bool dec64, uint32_t block) {
+    for (uint32_t ph = 1; ph < phases; ++ph) {
+        ParkIO in = regions[(ph - 1) & 1u], out = regions[ph & 1u];
+        const bool park = ph + 1 < phases;
+        out.steps = park ? steps2 : 0u;
+        if (park && ph > 1) k_zero_park<<<1, 1, 0, st>>>(a.ctr, out.which);  // its previous contents are consumed
+#define RC_RES(C, D) (park ? resume_b<C, D, true>(a, in, out, st, block) : resume_b<C, D, false>(a, in, out, st, block))
+        if (count) { if (dec64) RC_RES(true, true); else RC_RES(true, false); }
+        else { if (dec64) RC_RES(false, true); else RC_RES(false, false); }
+#undef RC_RES
+    }
 }
 
 struct ShadowArgs {
@@ -811,6 +1001,7 @@ __global__ void k_zero_counters(DevCounters* ctr) {
     ctr->ext_rays = 0;
     ctr->shadow_rays = 0;
     ctr->ovf_total = 0;
+    ctr->park_total = 0;
     ctr->ovf_count = 0;
     ctr->error = 0;
 }
@@ -837,11 +1028,14 @@ int FrameBuffers::ensure(size_t npaths) {
     RC_TRY(slotW.ensure(npaths * sizeof(float)));
     RC_TRY(ovf_list.ensure(npaths * sizeof(uint32_t)));
     RC_TRY(deep_stack.ensure(rc_deep_stack_bytes()));
+    // parked traversals: 1 / RC_PARK_DIV of the pass's paths at most
+    RC_TRY(park.ensure(std::max<size_t>(npaths / RC_PARK_DIV, 256) * kParkWords * sizeof(uint32_t)));
     return RC_OK;
 }
 
 void FrameBuffers::release() {
-    DevBuf* b[] = {&ray_a, &ray_b, &st_a, &st_b, &hits, &srays, &scontrib, &socc, &slotL, &slotW, &ovf_list, &deep_stack};
+    DevBuf* b[] = {&ray_a, &ray_b, &st_a, &st_b, &hits, &srays, &scontrib, &socc, &slotL, &slotW, &ovf_list,
+                   &deep_stack, &park};
     for (DevBuf* x : b) x->release();
     if (own_counters && counters) (void)hipFree(counters);
     if (own_counters) counters = nullptr;
@@ -961,6 +1155,14 @@ int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene*
     if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
     if (B.slotW.bytes < npaths * sizeof(float) || B.ovf_list.bytes < npaths * sizeof(uint32_t) || !B.counters)
         return rc_host_fail(RC_ERR_INVALID, "rc_render: frame buffers are smaller than the pass");
+    // park regions: three quarters of the records for the extension kernel's
+    // parks, a quarter for the resume kernels' (far fewer rays get that far)
+    const uint32_t park_recs = uint32_t(B.park.bytes / (kParkWords * sizeof(uint32_t)));
+    if (c->tuning.park_steps && park_recs < 4)
+        return rc_host_fail(RC_ERR_INVALID, "rc_render: no park records for the extension kernel");
+    const uint32_t park_cap0 = park_recs / 4 * 3;
+    const ParkIO regions[2] = {{B.park.as<uint32_t>(), park_cap0, 0u, 0u},
+                               {B.park.as<uint32_t>() + size_t(park_cap0) * kParkWords, park_recs - park_cap0, 0u, 1u}};
     const bool timed = st == c->stream;  // profiling events live on the context stream
     PassParams P = base_params(cam, cfg);
     P.stack_limit = c->stack_depth();
@@ -1013,8 +1215,13 @@ int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene*
             k_prep<<<1, 1, 0, st>>>(ctr, cout);
             if (timed) RC_TRY(c->prof_begin(&ev));
             const bool camb = RC_FUSED_CAMERA && k == 0;
-            ExtendArgs ea{P, k, s, rin, sin_, rout, sout, srays, scontrib, L, ovf, ctr, cin, cout, CI, W};
+            ParkIO PK = regions[0];
+            PK.steps = camb ? 0u : c->tuning.park_steps;
+            ExtendArgs ea{P, k, s, rin, sin_, rout, sout, srays, scontrib, L, ovf, ctr, cin, cout, CI, W, PK};
             launch_extend(ea, n, st, c->counting, camb, c->hit_fp64, shape.ext_block);
+            if (PK.steps)
+                launch_resumes(ea, regions, RC_PARK_PHASES, RC_PARK_STEPS2, st, c->counting, c->hit_fp64,
+                               shape.ext_block);
             if (timed) RC_TRY(c->prof_end(ev, PK_EXTEND));
             // cold path: rays whose stack outgrew the on-chip depth
             deep.any = false;
@@ -1133,6 +1340,7 @@ int render_frame(rc_ctx* c, const rc_scene* s, rc_film* f, const rc_camera* cam,
     c->stats.node_records = h.node_records;
     c->stats.tri_tests = h.tri_tests;
     c->stats.overflow_rays = h.ovf_total;
+    c->stats.parked_rays = h.park_total;
     if (cache) {
         c->stats.cache_queries = cache_counts[0];
         c->stats.train_samples = train ? cache_counts[1] : 0;

@@ -433,6 +433,48 @@ def test_deep_stack_path_gives_the_same_image(ctx, decision):
     assert ovf0 < st.overflow_rays
 
 
+@pytest.mark.parametrize("scene_name", ["terrain", "box"])
+def test_parked_traversals_give_the_same_image(ctx, decision, scene_name):
+    """An extension ray still traversing after park_steps loop iterations is
+    parked and finished by the resume kernel: the same steps, the same hit and
+    the same shading, so the film is bit-identical to the unparked render for
+    every budget, also with the deep-stack path in play (two on-chip stack
+    entries) and with the node and triangle counters on (whose totals must not
+    change either)."""
+    sd, s, _ = scene(ctx, scene_name)
+    cfg = cfg_for(sd, 64, 48, 4)
+    ctx.set_tuning(park_steps=0)
+    try:
+        _, sums0 = render_gpu(ctx, s, sd.desc.camera, cfg)
+        ctx.set_counting(True)
+        _, sums0c = render_gpu(ctx, s, sd.desc.camera, cfg)
+        st0 = ctx.stats()
+        ctx.set_counting(False)
+        for steps in (1, 8, 32):
+            ctx.set_tuning(park_steps=steps)
+            _, sums = render_gpu(ctx, s, sd.desc.camera, cfg)
+            parked = ctx.stats().parked_rays
+            assert sums.tobytes() == sums0.tobytes(), steps
+            assert parked > 0 or steps > 8, steps  # the small scenes' rays rarely take 32 steps
+            print(f"{scene_name} park_steps {steps}: {parked} of {ctx.stats().extension_rays} extension rays parked")
+        ctx.set_counting(True)
+        _, sums = render_gpu(ctx, s, sd.desc.camera, cfg)
+        st = ctx.stats()
+        ctx.set_counting(False)
+        assert sums.tobytes() == sums0c.tobytes()
+        assert (st.node_records, st.tri_tests, st.extension_rays) == (st0.node_records, st0.tri_tests,
+                                                                       st0.extension_rays)
+        ctx.set_tuning(park_steps=4, stack_limit=2)
+        _, sums = render_gpu(ctx, s, sd.desc.camera, cfg)
+        assert sums.tobytes() == sums0.tobytes()
+        assert ctx.stats().overflow_rays > 0 and ctx.stats().parked_rays > 0
+    finally:
+        ctx.set_counting(False)
+        ctx.set_tuning()
+    with pytest.raises(ra.RaycerError):
+        ctx.set_tuning(park_steps=1025)
+
+
 def test_interrupt_flag(ctx):
     import ctypes
     sd, s, _ = scene(ctx, "box")
