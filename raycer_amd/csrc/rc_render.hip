@@ -54,6 +54,15 @@ constexpr int kLdsRows = kLdsStack > 0 ? kLdsStack : 1;  // LDS stack entries pe
 #define RC_PARK_STEPS2 20
 #endif
 static_assert(RC_PARK_PHASES >= 2, "a parking extension launch needs a resume kernel");
+// Shadow rays (any hit) park after RC_SHADOW_PARK_STEPS iterations when
+// parking is on (0: they never park), over RC_SHADOW_PARK_PHASES kernels.
+#ifndef RC_SHADOW_PARK_STEPS
+#define RC_SHADOW_PARK_STEPS 24
+#endif
+#ifndef RC_SHADOW_PARK_PHASES
+#define RC_SHADOW_PARK_PHASES 3
+#endif
+static_assert(RC_SHADOW_PARK_PHASES >= 2, "a parking shadow launch needs a resume kernel");
 #ifndef RC_MIN_WAVES_FP64
 #define RC_MIN_WAVES_FP64 1
 #endif
@@ -662,37 +671,127 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
     }
 }
 
+// The result of a finished shadow ray: its occlusion flag (when kept) and,
+// unoccluded, its contribution added to its slot.
+__device__ __forceinline__ void shadow_result(uint32_t i, bool occluded, const float4* __restrict__ scontrib,
+                                              float4* __restrict__ L, uint8_t* __restrict__ occ_out) {
+    if (occ_out) occ_out[i] = occluded ? 1 : 0;
+    if (!occluded) {
+        float4 c = scontrib[i];
+        uint32_t slot = __float_as_uint(c.w);
+        float4 l = L[slot];
+        L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+    }
+}
+
 // Any hit of a shadow ray; an unoccluded ray adds its contribution to its
 // slot.  Overflowing rays are listed for the deep-stack kernel and applied by
-// k_shadow_apply_listed.
-template <bool COUNT, bool DEC64, int BLOCK>
+// k_shadow_apply_listed.  PARK: as the extension kernel, a ray still
+// traversing after PK.steps iterations is parked for k_shadow_resume.
+template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
 __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT* __restrict__ nodes,
                                                        const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
                                                        const float4* __restrict__ scontrib, float4* __restrict__ L,
                                                        uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr,
-                                                       uint8_t* __restrict__ occ_out, int stack_limit) {
+                                                       uint8_t* __restrict__ occ_out, int stack_limit, ParkIO PK) {
     __shared__ uint32_t stack[kLdsRows * BLOCK];
     const uint32_t n = ctr->queue_count[2];
     if (blockIdx.x * uint32_t(BLOCK) >= n) return;
     const uint32_t i = trace_block<BLOCK>(n) * BLOCK + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
-    if (i < n) {
+    if constexpr (PARK) {
+        __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
+        Trav<DEC64> S;
+        bool done = true;
+        if (i < n) {
+            const float4* rp = reinterpret_cast<const float4*>(srays + i);
+            const float4 r0 = rp[0], r1 = rp[1];
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+            uint32_t steps = 0;
+            do {
+                done = trav_step<true, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, stack_limit, n_rec,
+                                                     n_tri);
+            } while (!done && ++steps < PK.steps);
+        }
+        const uint32_t pos = block_append<BLOCK>(!done, &ctr->park_count[PK.which], append_scratch);
+        bool parked = false;
+        if (!done) {
+            if (pos < PK.cap) {
+                trav_park(S, i, stack + threadIdx.x, BLOCK, PK.rec + size_t(pos) * kParkWords);
+                parked = true;
+            } else {
+                while (!trav_step<true, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, stack_limit, n_rec,
+                                                      n_tri)) {
+                }
+            }
+        }
+        if (i < n && !parked) {
+            if (S.R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+            else shadow_result(i, S.R.id != kNoHit, scontrib, L, occ_out);
+        }
+    } else if (i < n) {
         const float4* rp = reinterpret_cast<const float4*>(srays + i);
         float4 r0 = rp[0], r1 = rp[1];
         TraceResult R = traverse<true, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
                                                      BLOCK, stack_limit, n_rec, n_tri);
-        if (R.overflow) {
-            ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
-        } else {
-            if (occ_out) occ_out[i] = R.id == kNoHit ? 0 : 1;
-            if (R.id == kNoHit) {
-                float4 c = scontrib[i];
-                uint32_t slot = __float_as_uint(c.w);
-                float4 l = L[slot];
-                L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+        if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+        else shadow_result(i, R.id != kNoHit, scontrib, L, occ_out);
+    }
+    if (COUNT) {
+        wave_add(&ctr->node_records, n_rec);
+        wave_add(&ctr->tri_tests, n_tri);
+    }
+}
+
+// The parked shadow traversals of region `in`, continued (and with PARK
+// parked again in region `out` after out.steps iterations) and applied.
+template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
+__global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow_resume(
+    const NodeT* __restrict__ nodes, const TriRec* __restrict__ tris, const rc_ray* __restrict__ srays,
+    const float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
+    DevCounters* __restrict__ ctr, uint8_t* __restrict__ occ_out, int stack_limit, ParkIO in, ParkIO out) {
+    __shared__ uint32_t stack[kLdsRows * BLOCK];
+    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
+    const uint32_t n = min(ctr->park_count[in.which], in.cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&ctr->park_total, (unsigned long long)n);
+    if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t q = blockIdx.x * uint32_t(BLOCK) + threadIdx.x;
+    uint32_t n_rec = 0, n_tri = 0;
+    Trav<DEC64> S;
+    uint32_t i = 0;
+    bool done = true;
+    if (q < n) {
+        const uint32_t* rec = in.rec + size_t(q) * kParkWords;
+        i = rec[0];
+        {
+            const float4* rp = reinterpret_cast<const float4*>(srays + i);
+            const float4 r0 = rp[0], r1 = rp[1];
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+        }
+        trav_unpark(S, rec, stack + threadIdx.x, BLOCK);
+        uint32_t steps = 0;
+        do {
+            done = trav_step<true, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, stack_limit, n_rec, n_tri);
+        } while (!done && (!PARK || ++steps < out.steps));
+    }
+    bool parked = false;
+    if constexpr (PARK) {
+        const uint32_t pos = block_append<BLOCK>(!done, &ctr->park_count[out.which], append_scratch);
+        if (!done) {
+            if (pos < out.cap) {
+                trav_park(S, i, stack + threadIdx.x, BLOCK, out.rec + size_t(pos) * kParkWords);
+                parked = true;
+            } else {
+                while (!trav_step<true, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, stack_limit, n_rec,
+                                                      n_tri)) {
+                }
             }
         }
+    }
+    if (q < n && !parked) {
+        if (S.R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
+        else shadow_result(i, S.R.id != kNoHit, scontrib, L, occ_out);
     }
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
@@ -774,6 +873,12 @@ __global__ void k_prep(DevCounters* ctr, uint32_t cout) {
 }
 
 __global__ void k_reset_ovf(DevCounters* ctr) { ctr->ovf_count = 0; }
+// Before a shadow pass: the overflow list and both park regions are free.
+__global__ void k_reset_shadow(DevCounters* ctr) {
+    ctr->ovf_count = 0;
+    ctr->park_count[0] = 0;
+    ctr->park_count[1] = 0;
+}
 
 __device__ __forceinline__ void kahan(float& s, float& c, float v) {
     float y = v - c;
@@ -961,30 +1066,60 @@ struct ShadowArgs {
     DevCounters* ctr;
     uint8_t* occ_out;
     int stack_limit;
+    ParkIO PK;  // PK.steps > 0: long traversals are parked
 };
 
-template <bool COUNT, bool DEC64, int BLOCK>
+template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
 void shadow_t(const ShadowArgs& a, uint32_t n, hipStream_t st) {
-    k_shadow<COUNT, DEC64, BLOCK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
-        a.s->nodes, a.s->tris, a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit);
+    k_shadow<COUNT, DEC64, BLOCK, PARK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
+        a.s->nodes, a.s->tris, a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit, a.PK);
 }
 
-template <bool COUNT, bool DEC64>
+template <bool COUNT, bool DEC64, bool PARK>
 void shadow_b(const ShadowArgs& a, uint32_t n, hipStream_t st, uint32_t block) {
     if constexpr (!COUNT) {
-        if (block == 64) return shadow_t<COUNT, DEC64, 64>(a, n, st);
-        if (block == 256) return shadow_t<COUNT, DEC64, 256>(a, n, st);
+        if (block == 64) return shadow_t<COUNT, DEC64, 64, PARK>(a, n, st);
+        if (block == 256) return shadow_t<COUNT, DEC64, 256, PARK>(a, n, st);
     }
-    shadow_t<COUNT, DEC64, 128>(a, n, st);
+    shadow_t<COUNT, DEC64, 128, PARK>(a, n, st);
 }
 
 void launch_shadow(const ShadowArgs& a, uint32_t n, hipStream_t st, bool count, bool dec64, uint32_t block) {
-    if (count) {
-        if (dec64) shadow_b<true, true>(a, n, st, block);
-        else shadow_b<true, false>(a, n, st, block);
-    } else {
-        if (dec64) shadow_b<false, true>(a, n, st, block);
-        else shadow_b<false, false>(a, n, st, block);
+#define RC_SHD(C, D) (a.PK.steps ? shadow_b<C, D, true>(a, n, st, block) : shadow_b<C, D, false>(a, n, st, block))
+    if (count) { if (dec64) RC_SHD(true, true); else RC_SHD(true, false); }
+    else { if (dec64) RC_SHD(false, true); else RC_SHD(false, false); }
+#undef RC_SHD
+}
+
+template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
+void shadow_resume_t(const ShadowArgs& a, const ParkIO& in, const ParkIO& out, hipStream_t st) {
+    k_shadow_resume<COUNT, DEC64, BLOCK, PARK><<<std::max(1u, (in.cap + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(
+        a.s->nodes, a.s->tris, a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit, in, out);
+}
+
+template <bool COUNT, bool DEC64, bool PARK>
+void shadow_resume_b(const ShadowArgs& a, const ParkIO& in, const ParkIO& out, hipStream_t st, uint32_t block) {
+    if constexpr (!COUNT) {
+        if (block == 64) return shadow_resume_t<COUNT, DEC64, 64, PARK>(a, in, out, st);
+        if (block == 256) return shadow_resume_t<COUNT, DEC64, 256, PARK>(a, in, out, st);
+    }
+    shadow_resume_t<COUNT, DEC64, 128, PARK>(a, in, out, st);
+}
+
+// The resume kernels of a shadow launch that parked into region 0, as
+// launch_resumes for the extension kernel.
+void launch_shadow_resumes(const ShadowArgs& a, const ParkIO regions[2], uint32_t phases, uint32_t steps2,
+                           hipStream_t st, bool count, bool dec64, uint32_t block) {
+    for (uint32_t ph = 1; ph < phases; ++ph) {
+        ParkIO in = regions[(ph - 1) & 1u], out = regions[ph & 1u];
+        const bool park = ph + 1 < phases;
+        out.steps = park ? steps2 : 0u;
+        if (park && ph > 1) k_zero_park<<<1, 1, 0, st>>>(a.ctr, out.which);
+#define RC_SRES(C, D) \
+    (park ? shadow_resume_b<C, D, true>(a, in, out, st, block) : shadow_resume_b<C, D, false>(a, in, out, st, block))
+        if (count) { if (dec64) RC_SRES(true, true); else RC_SRES(true, false); }
+        else { if (dec64) RC_SRES(false, true); else RC_SRES(false, false); }
+#undef RC_SRES
     }
 }
 
@@ -1235,10 +1370,15 @@ int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene*
                 // at the cache vertex the shadow results are kept for the training records
                 const bool nee_fix = CI.cache_vertex && k == CI.cache_vertex;
                 uint8_t* occ_out = nee_fix ? socc : nullptr;
-                k_reset_ovf<<<1, 1, 0, st>>>(ctr);
+                k_reset_shadow<<<1, 1, 0, st>>>(ctr);
                 if (timed) RC_TRY(c->prof_begin(&ev));
-                ShadowArgs sa_{s, srays, scontrib, L, ovf, ctr, occ_out, P.stack_limit};
+                ParkIO SPK = regions[0];
+                SPK.steps = c->tuning.park_steps ? uint32_t(RC_SHADOW_PARK_STEPS) : 0u;
+                ShadowArgs sa_{s, srays, scontrib, L, ovf, ctr, occ_out, P.stack_limit, SPK};
                 launch_shadow(sa_, n, st, c->counting, c->hit_fp64, shape.shadow_block);
+                if (SPK.steps)
+                    launch_shadow_resumes(sa_, regions, RC_SHADOW_PARK_PHASES, RC_PARK_STEPS2, st, c->counting,
+                                          c->hit_fp64, shape.shadow_block);
                 if (timed) RC_TRY(c->prof_end(ev, PK_SHADOW));
                 deep.any = true;
                 deep.rays = srays;
