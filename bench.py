@@ -602,14 +602,16 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_extend_shade<false> (closest-hit traversal fused with shading)",
+            "kernel": "extension step: k_extend_shade<false> (closest-hit traversal fused with shading) and the "
+                      "k_extend_resume kernels that finish its parked traversals; one step per bounce",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "achieved_basis": "rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch (profiles/) over the live launch "
-                              "time" if traffic else "algorithmic bytes (no counter profile found)",
+            "achieved_basis": "rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per extension step (profiles/) over the "
+                              "step's live time (HIP events)" if traffic else "algorithmic bytes (no counter profile "
+                                                                              "found)",
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "achievable_copy_GBs": round(copy_gbs, 1),
             "frac_of_achievable_copy": round(achieved / copy_gbs, 4),

@@ -45,7 +45,7 @@ constexpr int kLdsRows = kLdsStack > 0 ? kLdsStack : 1;  // LDS stack entries pe
 // RC_PARK_PHASES - 1 resume kernels) and the iteration budget of the
 // resume kernels that park again.
 #ifndef RC_PARK_DIV
-#define RC_PARK_DIV 2
+#define RC_PARK_DIV 4
 #endif
 #ifndef RC_PARK_PHASES
 #define RC_PARK_PHASES 3
@@ -311,8 +311,9 @@ __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
 
 struct ShadeOut {
     bool alive, shadow;
+    bool query;  // the path ends into the radiance cache: newo = position, newd = outgoing direction, rec = triangle
     float3 thr, newo, newd, contrib, so;
-    uint32_t rec;  // radiance-cache training record written at this vertex, or kNoRecord
+    uint32_t rec;  // radiance-cache training record written at this vertex, or kNoRecord (or the query's triangle)
 };
 constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 
@@ -337,7 +338,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
                                                  const DevMaterial* __restrict__ mats, float4* __restrict__ L,
                                                  const CacheIO& CI) {
     ShadeOut O;
-    O.alive = O.shadow = false;
+    O.alive = O.shadow = O.query = false;
     O.rec = kNoRecord;
     const bool more = (k + 1) < P.max_len;
     uint32_t slot = __float_as_uint(s.w);
@@ -375,16 +376,14 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
         // vertex (the direct sun is the next-event ray above, for every path).
         const float3 wo = make_float3(-r1.x, -r1.y, -r1.z);
         if (!training_path(P, slot, CI)) {
-            // the path ends here: its indirect radiance comes from the cache
-            // (one reservation per wave for the lanes that end here)
-            uint32_t q = wave_append(true, &CI.counters[0]);
-            if (q < CI.cap) {
-                float4* Q = CI.queries + 4 * size_t(q);
-                Q[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
-                Q[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
-                Q[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.y);
-                Q[3] = make_float4(rho.x, rho.y, rho.z, thr.z);
-            }
+            // the path ends here: its indirect radiance comes from the cache;
+            // the caller appends the query (emit_vertex, one reservation per
+            // block like the ray queues)
+            O.query = true;
+            O.newo = p;
+            O.newd = wo;
+            O.thr = thr;
+            O.rec = uint32_t(tri);
             return O;
         }
         // training vertex: the radiance the path gathers after this vertex's
@@ -423,11 +422,25 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
 // the wave calls this at the same point; with BLOCK > 0 every thread of the
 // block, which reserves once per block, else once per wave).
 template <int BLOCK>
-__device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut& O, uint32_t slot,
+__device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, const ShadeOut& O, uint32_t slot,
                                             rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
                                             DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI,
+                                            const float4* __restrict__ shade, const DevMaterial* __restrict__ mats,
                                             uint32_t* append_scratch = nullptr) {
+    if (CI.cache_vertex > 0 && k == CI.cache_vertex) {  // uniform: paths end into the cache at this vertex
+        const uint32_t qpos = BLOCK ? block_append<BLOCK>(O.query, &CI.counters[0], append_scratch)
+                                    : wave_append(O.query, &CI.counters[0]);
+        if (O.query && qpos < CI.cap) {
+            const float4 sh = shade[O.rec];
+            const float4 rho = mats[__float_as_uint(sh.w)].diffuse;
+            float4* Q = CI.queries + 4 * size_t(qpos);
+            Q[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, __uint_as_float(slot));
+            Q[1] = make_float4(O.newd.x, O.newd.y, O.newd.z, O.thr.x);
+            Q[2] = make_float4(sh.x, sh.y, sh.z, O.thr.y);
+            Q[3] = make_float4(rho.x, rho.y, rho.z, O.thr.z);
+        }
+    }
     // the extension kernel reserves once per block, k_shade_listed once per wave
     uint32_t pos = BLOCK ? block_append<BLOCK>(O.alive, &ctr->queue_count[cout], append_scratch)
                          : wave_append(O.alive, &ctr->queue_count[cout]);
@@ -444,7 +457,7 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, const ShadeOut&
         ro[0] = make_float4(O.so.x, O.so.y, O.so.z, 0.0f);
         ro[1] = make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff());
         scontrib[spos] = make_float4(O.contrib.x, O.contrib.y, O.contrib.z, __uint_as_float(slot));
-        if (O.rec != kNoRecord) CI.records[5 * size_t(O.rec) + 4].w = __uint_as_float(spos);
+        if (!O.query && O.rec != kNoRecord) CI.records[5 * size_t(O.rec) + 4].w = __uint_as_float(spos);
     }
 }
 
@@ -492,7 +505,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
     if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
-    O.alive = O.shadow = false;
+    O.alive = O.shadow = O.query = false;
     O.rec = kNoRecord;
     uint32_t slot = 0;
     if constexpr (PARK) {
@@ -560,7 +573,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
             O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
         }
     }
-    emit_vertex<BLOCK>(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
+    emit_vertex<BLOCK>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
@@ -589,7 +602,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
     const uint32_t q = blockIdx.x * uint32_t(BLOCK) + threadIdx.x;
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
-    O.alive = O.shadow = false;
+    O.alive = O.shadow = O.query = false;
     O.rec = kNoRecord;
     uint32_t slot = 0;
     Trav<DEC64> S;
@@ -634,7 +647,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
                              S.R.t, S.R.id == kNoHit ? -1 : int(S.R.id), shade, mats, L, CI);
         }
     }
-    emit_vertex<BLOCK>(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, append_scratch);
+    emit_vertex<BLOCK>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
@@ -656,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t q = base + threadIdx.x;
         ShadeOut O;
-        O.alive = O.shadow = false;
+        O.alive = O.shadow = O.query = false;
         O.rec = kNoRecord;
         uint32_t slot = 0;
         if (q < n) {
@@ -667,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
             slot = __float_as_uint(s.w);
             O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L, CI);
         }
-        emit_vertex<0>(P, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI);
+        emit_vertex<0>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats);
     }
 }
 
@@ -1290,9 +1303,13 @@ int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene*
     if (npaths > 0x7fffffffull) return rc_host_fail(RC_ERR_INVALID, "rc_render: pass too large");
     if (B.slotW.bytes < npaths * sizeof(float) || B.ovf_list.bytes < npaths * sizeof(uint32_t) || !B.counters)
         return rc_host_fail(RC_ERR_INVALID, "rc_render: frame buffers are smaller than the pass");
-    // park regions: three quarters of the records for the extension kernel's
-    // parks, a quarter for the resume kernels' (far fewer rays get that far)
-    const uint32_t park_recs = uint32_t(B.park.bytes / (kParkWords * sizeof(uint32_t)));
+    // park regions, sized for this pass (the resume kernels' grids cover a
+    // region, and a frame far smaller than the buffers must not launch grids
+    // for the largest): three quarters of the records for the extension
+    // kernel's parks, a quarter for the resume kernels' (far fewer rays get
+    // that far)
+    const uint32_t park_recs = uint32_t(std::min<size_t>(B.park.bytes / (kParkWords * sizeof(uint32_t)),
+                                                         std::max<size_t>(npaths / RC_PARK_DIV, 256)));
     if (c->tuning.park_steps && park_recs < 4)
         return rc_host_fail(RC_ERR_INVALID, "rc_render: no park records for the extension kernel");
     const uint32_t park_cap0 = park_recs / 4 * 3;
