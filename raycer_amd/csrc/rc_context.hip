@@ -600,9 +600,38 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     // a traversal stack holds at most (kWidth - 1) entries per level
     if (depth * (kWidth - 1) >= uint32_t(kDeepStack))
         return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: BVH too deep for the traversal stack");
+    // The top of the tree (breadth first, at most kTopNodes wide nodes) moves
+    // to the front of the node array, so the traversal kernels can keep nodes
+    // [0, ntop) in LDS; the rest keep their order.  Links still point forward.
+    uint32_t ntop = 0;
+    {
+        const size_t nn = nrec / kWidth;
+        std::vector<uint32_t> top;  // breadth-first from the root
+        if (nn) top.push_back(0);
+        for (size_t q = 0; q < top.size() && top.size() < kTopNodes; ++q) {
+            const uint32_t r = top[q] * kWidth;
+            for (int k = 0; k < kWidth && top.size() < kTopNodes; ++k)
+                if (recs[r + k].count == 0) top.push_back(recs[r + k].link / kWidth);
+        }
+        ntop = uint32_t(top.size());
+        std::vector<uint32_t> to(nn, 0xffffffffu);
+        for (uint32_t t = 0; t < ntop; ++t) to[top[t]] = t;
+        uint32_t next = ntop;
+        for (size_t n = 0; n < nn; ++n)
+            if (to[n] == 0xffffffffu) to[n] = next++;
+        std::vector<NodeRec> moved(nrec);
+        for (size_t n = 0; n < nn; ++n)
+            for (int k = 0; k < kWidth; ++k) {
+                NodeRec r = recs[n * kWidth + k];
+                if (r.count == 0) r.link = to[r.link / kWidth] * kWidth;
+                moved[size_t(to[n]) * kWidth + k] = r;
+            }
+        recs.swap(moved);
+    }
 #if RC_QNODE
     std::vector<QNode> qn(nrec / kWidth);
     for (size_t n = 0; n < qn.size(); ++n) quantize_node(&recs[n * kWidth], qn[n]);
+    if (!qn.empty()) qn[0].pad = RC_LDS_TOP ? ntop : 0u;  // the root carries the size of the LDS top
     const void* node_src = qn.data();
     const size_t b_nodes = qn.size() * sizeof(QNode);
 #else

@@ -499,8 +499,10 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
     static_assert(!(PARK && CAM), "the camera bounce does not park");
     __shared__ uint32_t stack[kLdsRows * BLOCK];
     __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
+    __shared__ uint4 top[4 * kTopNodes];
     const uint32_t n = ctr->queue_count[cin];
     if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t ntop = load_top(nodes, top);
     const uint32_t i = trace_block<BLOCK>(n) * BLOCK + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
         if (i < n) {
             const float4* rp = reinterpret_cast<const float4*>(rays + i);
             const float4 r0 = rp[0], r1 = rp[1];
-            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
             uint32_t steps = 0;
             do {
                 done = trav_step<false, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, P.stack_limit,
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
         }
         slot = __float_as_uint(s.w);
         TraceResult R = traverse<false, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
-                                                      BLOCK, P.stack_limit, n_rec, n_tri);
+                                                      BLOCK, P.stack_limit, n_rec, n_tri, top, ntop);
         if (R.overflow) {
             if (CAM) {  // the deep-stack kernel and k_shade_listed read the entry
                 float4* rp = reinterpret_cast<float4*>(cam_rays + i);
@@ -596,9 +598,11 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
     DevCounters* __restrict__ ctr, uint32_t cout, CacheIO CI, ParkIO in, ParkIO out) {
     __shared__ uint32_t stack[kLdsRows * BLOCK];
     __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
+    __shared__ uint4 top[4 * kTopNodes];
     const uint32_t n = min(ctr->park_count[in.which], in.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&ctr->park_total, (unsigned long long)n);
     if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t ntop = load_top(nodes, top);
     const uint32_t q = blockIdx.x * uint32_t(BLOCK) + threadIdx.x;
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
@@ -614,7 +618,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
         {
             const float4* rp = reinterpret_cast<const float4*>(rays + i);
             const float4 r0 = rp[0], r1 = rp[1];
-            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
         }
         trav_unpark(S, rec, stack + threadIdx.x, BLOCK);
         uint32_t steps = 0;
@@ -708,8 +712,10 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT*
                                                        uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr,
                                                        uint8_t* __restrict__ occ_out, int stack_limit, ParkIO PK) {
     __shared__ uint32_t stack[kLdsRows * BLOCK];
+    __shared__ uint4 top[4 * kTopNodes];
     const uint32_t n = ctr->queue_count[2];
     if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t ntop = load_top(nodes, top);
     const uint32_t i = trace_block<BLOCK>(n) * BLOCK + threadIdx.x;
     if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
@@ -720,7 +726,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT*
         if (i < n) {
             const float4* rp = reinterpret_cast<const float4*>(srays + i);
             const float4 r0 = rp[0], r1 = rp[1];
-            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
             uint32_t steps = 0;
             do {
                 done = trav_step<true, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, stack_limit, n_rec,
@@ -747,7 +753,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT*
         const float4* rp = reinterpret_cast<const float4*>(srays + i);
         float4 r0 = rp[0], r1 = rp[1];
         TraceResult R = traverse<true, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
-                                                     BLOCK, stack_limit, n_rec, n_tri);
+                                                     BLOCK, stack_limit, n_rec, n_tri, top, ntop);
         if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         else shadow_result(i, R.id != kNoHit, scontrib, L, occ_out);
     }
@@ -766,9 +772,11 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow_resume(
     DevCounters* __restrict__ ctr, uint8_t* __restrict__ occ_out, int stack_limit, ParkIO in, ParkIO out) {
     __shared__ uint32_t stack[kLdsRows * BLOCK];
     __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
+    __shared__ uint4 top[4 * kTopNodes];
     const uint32_t n = min(ctr->park_count[in.which], in.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&ctr->park_total, (unsigned long long)n);
     if (blockIdx.x * uint32_t(BLOCK) >= n) return;
+    const uint32_t ntop = load_top(nodes, top);
     const uint32_t q = blockIdx.x * uint32_t(BLOCK) + threadIdx.x;
     uint32_t n_rec = 0, n_tri = 0;
     Trav<DEC64> S;
@@ -780,7 +788,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow_resume(
         {
             const float4* rp = reinterpret_cast<const float4*>(srays + i);
             const float4 r0 = rp[0], r1 = rp[1];
-            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w);
+            trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
         }
         trav_unpark(S, rec, stack + threadIdx.x, BLOCK);
         uint32_t steps = 0;

@@ -46,13 +46,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(const NodeT* __restrict__ node
                                                   uint32_t* __restrict__ ovf_list, DevCounters* __restrict__ ctr,
                                                   int max_depth) {
     __shared__ uint32_t stack[(kLdsStack > 0 ? kLdsStack : 1) * kBlock];
+    __shared__ uint4 top[4 * kTopNodes];
+    const uint32_t ntop = load_top(nodes, top);
     const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     uint32_t n_rec = 0, n_tri = 0;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
         float4 r0 = rp[0], r1 = rp[1];
         TraceResult R = traverse<ANY, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
-                                                    kBlock, max_depth, n_rec, n_tri);
+                                                    kBlock, max_depth, n_rec, n_tri, top, ntop);
         if (R.overflow) {
             ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
             continue;
