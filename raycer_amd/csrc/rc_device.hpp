@@ -54,12 +54,14 @@ struct alignas(64) QNode {
 // RC_LDS_TOP = 1 keeps the first kTopNodes wide nodes (the upload puts the
 // tree's breadth-first top there) in LDS for the traversal kernels.  Off: on
 // the 2M-triangle terrain it was 1% slower than fetching those nodes through
-// the caches (1,848 against 1,869 Msamples/s), while the breadth-first
-// layout itself gained 1%.
+// the caches (1,848 against 1,868 Msamples/s).
 #ifndef RC_LDS_TOP
 #define RC_LDS_TOP 0
 #endif
-constexpr uint32_t kTopNodes = 21;  // levels 0..2 of a 4-wide tree, 1,344 bytes
+#ifndef RC_TOP_NODES
+#define RC_TOP_NODES 21
+#endif
+constexpr uint32_t kTopNodes = RC_TOP_NODES;  // 21: levels 0..2 of a 4-wide tree, 1,344 bytes
 static_assert(sizeof(QNode) == 64, "QNode must be 64 bytes");
 #if RC_QNODE
 using NodeT = QNode;
