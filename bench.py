@@ -96,14 +96,26 @@ def cpu_reference(sd, cam, cfg, subset: int = 64, seed: int = 7, ref_spp: int = 
     sp = orc.render_split_tail(ob, sd.tri_material, sd.materials, cam, scfg, ids, threads=threads)
     dt_split = time.perf_counter() - t1
     tail_var = float(np.mean(sp[:, 4:7] / (2.0 * sp[:, 0:1] ** 2)))
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), cpu_model)
+    except OSError:
+        pass
     base = {
         "value": round(samples / dt / 1e6, 4),
         "unit": "Msamples/s",
         "cores": threads,
+        "cpu_model": cpu_model,
         "kind": "port",
         "sample": f"{len(ids)} seeded pixels (1/{subset} of {cfg.width}x{cfg.height}) x {ref_spp} spp "
                   f"(the 4096-spp reference image), {dt:.1f} s, unordered FP64 traversal as the reference CPU path, "
                   f"extension rays/sample {st[:, 11].sum() / samples:.2f}",
+        # the reference's unordered FP64 traversal of the same paths' extension rays
+        "reference_work_per_extension_ray": {
+            "binary_node_visits": round(float(st[:, 12].sum() / st[:, 11].sum()), 2),
+            "triangle_tests": round(float(st[:, 13].sum() / st[:, 11].sum()), 2),
+        },
     }
     return {"baseline": base, "ids": ids, "ref": ref, "tail_var": tail_var, "split_spp": split_spp,
             "split_s": round(dt_split, 1)}
@@ -617,6 +629,8 @@ def main() -> int:
             "frac_of_achievable_copy": round(achieved / copy_gbs, 4),
             "demand": {
                 "bytes_per_ray": round(bytes_per_ray, 1),
+                "work_per_ray": {"wide_node_visits": round(cst.node_records / 4 / ext_rays, 2),
+                                 "triangle_tests": round(cst.tri_tests / ext_rays, 2)},
                 "bytes_per_launch": round(bytes_per_ray * total_ext / max(ext_launches, 1)),
                 "GBs": round(demand, 1),
                 "frac_of_hbm_peak": round(demand / HBM_PEAK_GBS, 4),

@@ -229,14 +229,15 @@ def render_cfg(cfg, traversal: int = 1) -> RenderCfg:
 
 
 def render(bvh: BVH, tri_mat, mats, cam, cfg: RenderCfg, pixel_ids=None, threads: int = 0):
-    """Per-pixel stats (npix, 12): sum_w, sum_w2, sum_wL[3], sum_w2L[3], sum_w2L2[3], rays.
+    """Per-pixel stats (npix, 14): sum_w, sum_w2, sum_wL[3], sum_w2L[3], sum_w2L2[3], rays, and
+    the reference traversal's node visits and triangle tests for those rays (traversal 0).
     Runs in the library the BVH was built with."""
     L = lib(bvh.fast)
     tm = np.ascontiguousarray(tri_mat, np.uint32)
     ma = np.ascontiguousarray(materials_array(mats) if mats.dtype.names else mats, np.float64)
     ids = None if pixel_ids is None else np.ascontiguousarray(pixel_ids, np.uint32)
     npix = cfg.width * cfg.height if ids is None else ids.shape[0]
-    out = np.empty((npix, 12))
+    out = np.empty((npix, 14))
     stop = C.c_int(0)
     ca = camera_array(cam)
     L.or_render(bvh.h, _p(bvh.vertices), bvh.vertices.shape[0], _p(tm), _p(ma), ma.shape[0],

@@ -162,7 +162,7 @@ inline RayD load_ray(const float* r) {
 // t >= 0 only; a hit replaces the current one unless t > current distance, so
 // the last tested triangle wins an exact tie.  any=true is fastOcclusion.
 bool trace_ref(const or_bvh* b, const float* vertices, const RayD& r, bool any, double& best_t, int64_t& best_id,
-               const uint32_t* tri_flags, uint64_t* nodes_visited) {
+               const uint32_t* tri_flags, uint64_t* nodes_visited, uint64_t* tris_tested) {
     best_t = std::numeric_limits<double>::max();
     best_id = -1;
     uint64_t stack[128];
@@ -177,6 +177,7 @@ bool trace_ref(const or_bvh* b, const float* vertices, const RayD& r, bool any, 
             for (uint64_t i = 0; i < n.primitive_count; ++i) {
                 uint64_t id = b->ordered[n.start_offset + i];
                 if (any && tri_flags && (tri_flags[id] & 2u)) continue;  // nonShadowing
+                if (tris_tested) ++*tris_tested;
                 Tri tr = load_tri(vertices, id);
                 double t, u, v, det;
                 if (!mt_intersect(tr, r.o, r.d, r.tmin, r.tmax, t, u, v, det)) continue;

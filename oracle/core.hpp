@@ -228,6 +228,6 @@ struct Diag {
 struct or_bvh;
 namespace orc {
 bool trace_ref(const or_bvh* b, const float* vertices, const RayD& r, bool any, double& best_t, int64_t& best_id,
-               const uint32_t* tri_flags, uint64_t* nodes_visited);
+               const uint32_t* tri_flags, uint64_t* nodes_visited, uint64_t* tris_tested = nullptr);
 void trace_ordered(const or_bvh* b, const float* vertices, const RayD& r, double rel_tie, double band, Diag& D);
 }  // namespace orc

@@ -133,8 +133,10 @@ typedef struct or_render_cfg {
 enum { OR_MAT_STRIDE = 16 };
 
 /* Path-trace the pixels listed in pixel_ids (or all if pixel_ids == NULL).
- * stats_out: per listed pixel 12 doubles:
- *   sum_w, sum_w2, sum_wL[3], sum_w2L[3], sum_w2L2[3], extension_rays
+ * stats_out: per listed pixel 14 doubles:
+ *   sum_w, sum_w2, sum_wL[3], sum_w2L[3], sum_w2L2[3], extension_rays,
+ *   node visits and triangle tests of those rays (the reference's unordered
+ *   traversal, cfg->traversal 0; zero with the ordered one)
  * Returns 0, or nonzero when *interrupt becomes nonzero. */
 int or_render(const or_bvh* b, const float* vertices, size_t ntri, const uint32_t* tri_mat,
               const double* materials, size_t nmat, const double cam[7],

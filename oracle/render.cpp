@@ -213,8 +213,11 @@ struct Scene {
     std::vector<uint32_t> tri_flags;  // material flags per triangle
 };
 
-bool closest(const Scene& s, const RayD& r, uint32_t traversal, double& t, int64_t& id) {
-    if (traversal == 0) return trace_ref(s.bvh, s.verts, r, false, t, id, nullptr, nullptr);
+// nodes / tris: the reference traversal's node visits and triangle tests
+// (the unordered traversal only; the ordered one is not counted).
+bool closest(const Scene& s, const RayD& r, uint32_t traversal, double& t, int64_t& id, uint64_t* nodes = nullptr,
+             uint64_t* tris = nullptr) {
+    if (traversal == 0) return trace_ref(s.bvh, s.verts, r, false, t, id, nullptr, nodes, tris);
     Diag D;
     trace_ordered(s.bvh, s.verts, r, 0.0, 0.0, D);
     t = D.t;
@@ -227,9 +230,12 @@ bool occluded(const Scene& s, const RayD& r) {
     return trace_ref(s.bvh, s.verts, r, true, t, id, s.tri_flags.data(), nullptr);
 }
 
+constexpr int kRenderStats = 14;  // doubles per pixel of or_render
+
 struct PathOut {
     V3 L;
     int rays = 0;
+    uint64_t nodes = 0, tris = 0;  // the reference traversal's work for the path's extension rays
 };
 
 // PathTracer::traceRecursive (PathTracer.cpp:25-75) unrolled into a loop, plus
@@ -257,7 +263,7 @@ PathOut trace_path(const Scene& s, const or_render_cfg& c, uint32_t pixel, uint3
         RayD r = make_ray(o, d, 0.0, std::numeric_limits<double>::max());
         double t;
         int64_t id;
-        bool hit = closest(s, r, c.traversal, t, id);
+        bool hit = closest(s, r, c.traversal, t, id, &P.nodes, &P.tris);
         P.rays++;
         if (trace_out) {
             double* q = trace_out + 8 * k;
@@ -387,8 +393,8 @@ extern "C" int or_render(const or_bvh* b, const float* vertices, size_t ntri, co
     for (int64_t i = 0; i < int64_t(count); ++i) {
         if (interrupt && *interrupt) { stopped = 1; continue; }
         uint32_t pixel = pixel_ids ? pixel_ids[i] : uint32_t(i);
-        double* st = stats + 12 * i;
-        for (int q = 0; q < 12; ++q) st[q] = 0.0;
+        double* st = stats + kRenderStats * i;
+        for (int q = 0; q < kRenderStats; ++q) st[q] = 0.0;
         for (uint32_t smp = 0; smp < spp; ++smp) {
             V3 o, d;
             double w;
@@ -403,6 +409,8 @@ extern "C" int or_render(const or_bvh* b, const float* vertices, size_t ntri, co
                 st[8 + ch] += w * w * L[ch] * L[ch];
             }
             st[11] += P.rays;
+            st[12] += double(P.nodes);
+            st[13] += double(P.tris);
         }
     }
     return stopped;

@@ -218,7 +218,7 @@ void oracle_paths() {
     c.width = 8; c.height = 8; c.spp = 4; c.sampler = 0; c.max_path_length = 6; c.rr_start = 3; c.filter = 1;
     c.estimator = 0; c.traversal = 1; c.filter_radius = 1.0; c.ray_start_offset = 1e-5; c.seed = 1;
     c.cache_vertex = 2;
-    std::vector<double> st(64 * 12), sp(64 * 7);
+    std::vector<double> st(64 * 14), sp(64 * 7);  // or_render: 14 doubles per pixel
     std::vector<uint32_t> ids(64);
     for (uint32_t i = 0; i < 64; ++i) ids[i] = i;
     check(or_render(b, v.data(), d.ntri, tm.data(), mat_arr.data(), d.nmat, cam, &c, nullptr, 0, st.data(), 2,

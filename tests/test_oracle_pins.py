@@ -314,6 +314,31 @@ def test_plane_under_sky_and_sun_closed_form(estimator):
     assert Lhat == pytest.approx(expect, rel=0.01)
 
 
+def test_reference_traversal_work_counts():
+    """or_render's work columns count the reference traversal's node visits
+    and triangle tests per extension ray: two triangles make one leaf (the
+    reference's maxLeafSize is 5), so every extension ray visits exactly the
+    root and tests both triangles; the ordered traversal is not counted."""
+    v, tm, mats = _plane_scene(0.6)
+    b = orc.BVH(v)
+    cfg = orc.RenderCfg()
+    cfg.width, cfg.height, cfg.spp, cfg.sampler = 4, 4, 16, 1
+    cfg.max_path_length, cfg.rr_start, cfg.filter, cfg.estimator, cfg.traversal = 4, 0, 0, 0, 0
+    cfg.filter_radius, cfg.ray_start_offset = 0.5, 1e-5
+    for k in range(3):
+        cfg.sky[k] = 0.5
+    cam = [0, 1, 0, -90.0, 0.0, 0.0, 30.0]
+    st = orc.render(b, tm, mats, cam, cfg, threads=2)
+    rays = st[:, 11]
+    assert rays.min() > 0
+    np.testing.assert_array_equal(st[:, 12], rays)
+    np.testing.assert_array_equal(st[:, 13], 2 * rays)
+    cfg.traversal = 1
+    st1 = orc.render(b, tm, mats, cam, cfg, threads=2)
+    np.testing.assert_array_equal(st1[:, :12], st[:, :12])  # the same paths either way
+    assert not st1[:, 12:].any()
+
+
 def test_russian_roulette_keeps_the_expectation():
     v, tm, mats = _plane_scene(0.9)
     # add a ceiling so paths bounce several times before escaping
