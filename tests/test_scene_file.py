@@ -276,3 +276,31 @@ def test_scene_file_errors(tmp_path):
     assert sf.vertices.shape == (1, 9) and sf.skipped_primitives == 14
     np.testing.assert_array_equal(sf.vertices[0], np.float32([-1, 0, 0, 1, 0, 0, 0, 2, 0]))
     assert sf.material_ids[sf.tri_material[0]] == 2
+
+
+def test_cpp_wrapper_rethrows_c_abi_errors(tmp_path):
+    """The C++ interface turns a nonzero C-ABI return into raycer::RaycerError,
+    as the reference's CLManager::checkError throws: the console runner, given
+    a truncated or hostile scene file or an unknown option value, reports the
+    library's message and exits 1 (no crash; no GPU needed, the scene is read
+    before the tracer starts)."""
+    import subprocess
+
+    from raycer_amd import build
+
+    runner = build.RUNNER
+    if not runner.exists():
+        pytest.skip("the console runner is not built")
+    cases = {
+        "truncated.json": ('{"scene": [[[[', "raycer error 1: scene file: unexpected end"),
+        "deep.json": ("[" * 100000, "nesting deeper than 256 levels"),
+        "deep.xml": ("<a>" * 100000, "nesting deeper than 256 levels"),
+    }
+    for name, (text, message) in cases.items():
+        f = tmp_path / name
+        f.write_text(text)
+        r = subprocess.run([str(runner), "--scene-file", str(f)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and message in r.stderr, (name, r.returncode, r.stderr[-300:])
+    r = subprocess.run([str(runner), "--scene", "box", "--sampler", "bogus"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "unknown sampler bogus" in r.stderr
