@@ -10,6 +10,8 @@ import numpy as np
 import pytest
 
 import oracle as orc
+import raycer_amd as ra
+from raycer_amd import api
 
 
 # ------------------------------------------------------------------ math KATs
@@ -337,6 +339,71 @@ def test_reference_traversal_work_counts():
     st1 = orc.render(b, tm, mats, cam, cfg, threads=2)
     np.testing.assert_array_equal(st1[:, :12], st[:, :12])  # the same paths either way
     assert not st1[:, 12:].any()
+
+
+def _box_split_setup(cache_vertex, max_len=8, spp=64):
+    sd = ra.generate_scene(ra.scene_params(api.RCH_SCENE_BOX))
+    b = orc.BVH(sd.vertices)
+    cfg = orc.render_cfg(sd.desc.cfg, traversal=1)
+    cfg.width, cfg.height, cfg.spp = 16, 16, spp
+    cfg.max_path_length = max_len
+    cfg.cache_vertex = cache_vertex
+    return sd, b, cfg
+
+
+def test_split_tail_first_path_is_the_rendered_path():
+    """or_render_split_tail's first path of each sample is the path or_render
+    traces (same keys up to the cache vertex, and the tail's draws use the
+    frame itself), so its weights and weighted radiance equal or_render's."""
+    sd, b, cfg = _box_split_setup(2)
+    ids = np.arange(cfg.width * cfg.height, dtype=np.uint32)
+    sp = orc.render_split_tail(b, sd.tri_material, sd.materials, sd.desc.camera, cfg, ids, threads=4)
+    st = orc.render(b, sd.tri_material, sd.materials, sd.desc.camera, cfg, threads=4)
+    np.testing.assert_array_equal(sp[:, 0], st[:, 0])
+    np.testing.assert_array_equal(sp[:, 1:4], st[:, 2:5])
+
+
+def test_split_tail_vanishes_without_tail_draws():
+    """No draw after the cache vertex's next-event ray, no tail variance: with
+    max_path_length = cache_vertex + 1 no continuation ray is traced after
+    that vertex, and on a plane under the sky every bounce ray escapes, so the
+    two paths of a sample agree exactly."""
+    sd, b, cfg = _box_split_setup(2, max_len=3)
+    ids = np.arange(cfg.width * cfg.height, dtype=np.uint32)
+    sp = orc.render_split_tail(b, sd.tri_material, sd.materials, sd.desc.camera, cfg, ids, threads=4)
+    assert not sp[:, 4:7].any() and sp[:, 1:4].any()
+    v, tm, mats = _plane_scene(0.6)
+    pb = orc.BVH(v)
+    pc = orc.RenderCfg()
+    pc.width, pc.height, pc.spp, pc.sampler = 4, 4, 64, 1
+    pc.max_path_length, pc.rr_start, pc.filter, pc.estimator, pc.traversal = 4, 0, 0, 0, 1
+    pc.filter_radius, pc.ray_start_offset, pc.cache_vertex = 0.5, 1e-5, 1
+    for k in range(3):
+        pc.sky[k] = 0.5
+    pp = orc.render_split_tail(pb, tm, mats, [0, 1, 0, -90.0, 0.0, 0.0, 30.0], pc, np.arange(16, dtype=np.uint32),
+                               threads=2)
+    assert not pp[:, 4:7].any() and (pp[:, 1:4] > 0).all()
+
+
+def test_split_tail_variance_shrinks_with_a_later_cache_vertex():
+    """The tail after vertex 1 contains the tail after vertex 2, so by the law
+    of total variance E[Var(L | first k vertices)] cannot grow with k: the
+    estimated tail variance at cache vertex 1 exceeds the one at vertex 2
+    (in the closed box most light is indirect, so both are large), and both
+    stay below the pixels' total per-sample variance."""
+    tails = []
+    for cv in (1, 2):
+        sd, b, cfg = _box_split_setup(cv, spp=256)
+        ids = np.arange(cfg.width * cfg.height, dtype=np.uint32)
+        sp = orc.render_split_tail(b, sd.tri_material, sd.materials, sd.desc.camera, cfg, ids, threads=8)
+        tails.append(float(np.mean(sp[:, 4:7].sum(axis=1))))
+    st = orc.render(b, sd.tri_material, sd.materials, sd.desc.camera, cfg, threads=8)
+    sw, sw2 = st[:, 0:1], st[:, 1:2]
+    mu = st[:, 2:5] / sw
+    total = float(np.mean((st[:, 8:11] - 2 * mu * st[:, 5:8] + mu * mu * sw2).sum(axis=1)))  # sum w^2 (L - mu)^2
+    print(f"box tail variance: vertex 1 {tails[0] / 2:.4g}, vertex 2 {tails[1] / 2:.4g}, total {total:.4g}")
+    assert tails[0] > 1.2 * tails[1] > 0
+    assert tails[0] / 2 < total
 
 
 def test_russian_roulette_keeps_the_expectation():
