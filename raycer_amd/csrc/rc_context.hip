@@ -312,7 +312,14 @@ void quantize_node(const NodeRec* r, QNode& q) {
             }
         }
         q.meta |= meta << (8 * k);
-        q.link[k] = c.count == 0 ? c.link / kWidth : c.link;  // interior links become node indices
+        // the traversal's stack entry: an interior child's node index, a leaf
+        // tagged with its count and first triangle, or kEmptyChild
+        if (c.count == kEmptyChild)
+            q.link[k] = kEmptyChild;
+        else if (c.count == 0)
+            q.link[k] = c.link / kWidth;
+        else
+            q.link[k] = kLeafTag | (c.count << 26) | c.link;
     }
 }
 

@@ -49,7 +49,8 @@ struct alignas(64) QNode {
     uint32_t qhi[3];
     uint32_t meta;     // per child, byte k: 0xFF unused, 0 interior, else leaf triangle count (1..31)
     uint32_t pad;      // the root's: the number of top nodes kept in LDS (nodes [0, pad)); others 0
-    uint32_t link[4];  // interior: node index; leaf: first triangle
+    uint32_t link[4];  // child k's stack entry: interior, node index; leaf, kLeafTag | count << 26 | first
+                       // triangle; unused, kEmptyChild
 };
 // RC_LDS_TOP = 1 keeps the first kTopNodes wide nodes (the upload puts the
 // tree's breadth-first top there) in LDS for the traversal kernels.  Off: on
@@ -354,23 +355,23 @@ __device__ __forceinline__ bool mt_hit64(const TriRec& T, float3 of, float3 df, 
 // epsilon.  The bounds are tested without a division: u = U/det and v =
 // V/det lie in [0, 1] with u + v <= 1 exactly when sign(det) U, sign(det) V
 // >= 0 and their sum <= |det|; t = T/det is one IEEE division for the
-// triangles that pass.
-__device__ __forceinline__ bool mt_dec32(const TriRec& T, float3 o, float3 d, float& t, float& u, float& v) {
+// triangles that pass.  u and v are returned undivided (U, V and adet = |det|).
+__device__ __forceinline__ bool mt_dec32(const TriRec& T, float3 o, float3 d, float& t, float& U, float& V,
+                                         float& adet) {
     const float3 v0 = f3(T.v0);
     const float3 e1 = f3(T.v1) - v0, e2 = f3(T.v2) - v0;
     const float3 pv = cross3(d, e2);
     const float det = dot3(e1, pv);
     if (det == 0.0f) return false;
-    const float sg = det < 0.0f ? -1.0f : 1.0f, adet = fabsf(det);
+    const float sg = det < 0.0f ? -1.0f : 1.0f;
+    adet = fabsf(det);
     const float3 tv = o - v0;
-    const float U = sg * dot3(tv, pv);
+    U = sg * dot3(tv, pv);
     if (U < 0.0f || U > adet) return false;
     const float3 qv = cross3(tv, e1);
-    const float V = sg * dot3(d, qv);
+    V = sg * dot3(d, qv);
     if (V < 0.0f || U + V > adet) return false;
     t = dot3(e2, qv) / det;
-    u = U / adet;
-    v = V / adet;
     return true;
 }
 
@@ -409,11 +410,17 @@ __device__ __forceinline__ uint32_t stack_pop(uint32_t* reg, int& sp, const uint
     return x;
 }
 
+// The barycentrics of the hit are u / w and v / w: the FP32 decision keeps
+// them undivided (w = |det|) so only a reader of u and v pays the divisions
+// (bary_u / bary_v); the FP64 decision stores them divided, with w = 1.
 struct TraceResult {
     float t, u, v;
     uint32_t id;
     bool overflow;
+    float w;
 };
+__device__ __forceinline__ float bary_u(const TraceResult& R) { return R.u / R.w; }
+__device__ __forceinline__ float bary_v(const TraceResult& R) { return R.v / R.w; }
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup w runs on XCD
 // w % 8).  This bijection on [0, n) gives XCD x the contiguous range of
@@ -480,12 +487,13 @@ __device__ __forceinline__ bool test_leaf(const TriRec* __restrict__ tris, uint3
                 R.t = float(t64);
                 R.u = float(u64);
                 R.v = float(v64);
+                R.w = 1.0f;
                 R.id = id;
                 cull = R.t * (1.0f + kRefineWindow);
             }
         } else {
-            float t, u, v;
-            if (!mt_dec32(T, o, d, t, u, v)) continue;
+            float t, U, V, adet;
+            if (!mt_dec32(T, o, d, t, U, V, adet)) continue;
             if (t < tmin || t > cull) continue;
             uint32_t id = __float_as_uint(T.v0.w);
             if (ANY) {
@@ -497,8 +505,9 @@ __device__ __forceinline__ bool test_leaf(const TriRec* __restrict__ tris, uint3
             } else if (t < best || (t == best && id < R.id)) {
                 best = t;
                 R.t = t;
-                R.u = u;
-                R.v = v;
+                R.u = U;
+                R.v = V;
+                R.w = adet;
                 R.id = id;
                 cull = t * (1.0f + kTieWindow);
             }
@@ -513,7 +522,17 @@ __device__ __forceinline__ float initial_cull(float tmax) {
     return tmax * (1.0f + (DEC64 ? kRefineWindow : kTieWindow));
 }
 
-// Compare-exchange of two children by entry distance, as selects (no branch).
+// Compare-exchange of two children (entry distance, stack entry) by distance,
+// as selects (no branch).
+__device__ __forceinline__ void order2(float& ta, uint32_t& ea, float& tb, uint32_t& eb) {
+    const bool sw = tb < ta;
+    const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
+    const uint32_t e0 = sw ? eb : ea, e1 = sw ? ea : eb;
+    ta = t0; tb = t1;
+    ea = e0; eb = e1;
+}
+
+// The same with the link and count of a child record carried separately.
 __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, float& tb, uint32_t& lb, uint32_t& cb) {
     const bool sw = tb < ta;
     const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
@@ -528,6 +547,22 @@ __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, fl
 // 2M-triangle terrain than testing leaf children at once (0).
 #ifndef RC_TRAV_IFIF
 #define RC_TRAV_IFIF 1
+#endif
+// 1: any-hit queries visit children nearest first too; 0: in slot order (an
+// any-hit answer does not depend on the order, only its cost does).
+#ifndef RC_ANY_SORT
+#define RC_ANY_SORT 0
+#endif
+// 0: each step tests the leaf a lane holds at once; n > 0: a wave tests
+// leaves only when at least n / 64 of its lanes hold one (or none has a node).
+#ifndef RC_LEAF_BATCH
+#define RC_LEAF_BATCH 0
+#endif
+// 0: off; n > 0: leaves are set aside and tested when at least n / 64 of the
+// wave's lanes hold one (or none has a node), the lanes meanwhile going on
+// with their stacks.
+#ifndef RC_LEAF_POSTPONE
+#define RC_LEAF_POSTPONE 0
 #endif
 // Stack entries of the if-if traversal: an interior child's record index, or
 // a leaf tagged with kLeafTag, its triangle count (1..31) in bits 26..30 and
@@ -549,7 +584,7 @@ __device__ __forceinline__ TraceResult traverse_inline(const NodeRec* __restrict
                                                 float3 o, float3 d, float tmin, float tmax, uint32_t* stack,
                                                 int stride, int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
     const float INF = __builtin_inff();
-    TraceResult R{tmax, 0.0f, 0.0f, kNoHit, false};
+    TraceResult R{tmax, 0.0f, 0.0f, kNoHit, false, 1.0f};
     tmin = fmaxf(tmin, 0.0f);
     typename BestT<DEC64>::T best = tmax;
     float cull = initial_cull<DEC64>(tmax);
@@ -624,6 +659,7 @@ struct Trav {
     uint32_t cur;
     int sp;
     uint32_t reg[kRegStack > 0 ? kRegStack : 1];  // top of the stack
+    uint32_t leaf;     // a leaf set aside (leaf postponing), else kEmptyChild
     const uint4* top;  // LDS copy of nodes [0, ntop), or null
     uint32_t ntop;
 };
@@ -655,9 +691,10 @@ __device__ __forceinline__ void trav_begin(Trav<DEC64>& S, float3 o, float3 d, f
     S.tmin = fmaxf(tmin, 0.0f);
     S.best = tmax;
     S.cull = initial_cull<DEC64>(tmax);
-    S.R = TraceResult{tmax, 0.0f, 0.0f, kNoHit, false};
+    S.R = TraceResult{tmax, 0.0f, 0.0f, kNoHit, false, 1.0f};
     S.cur = 0;  // root: interior record 0
     S.sp = 0;
+    S.leaf = kEmptyChild;
 }
 
 // One step; true when the traversal has ended (S.R holds the result,
@@ -691,29 +728,36 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
         const float bx = (__uint_as_float(q0.x) - S.o.x) * S.inv.x;
         const float by = (__uint_as_float(q0.y) - S.o.y) * S.inv.y;
         const float bz = (__uint_as_float(q0.z) - S.o.z) * S.inv.z;
-#define RC_QSLAB(k)                                                                                            \
+        // Per axis the entry plane is the lower bound when the direction is
+        // positive and the upper one when it is negative (the decode is
+        // monotonic in q): picking the two words by sign once per node takes
+        // the place of a min and a max per axis and child, with the same result.
+        const uint32_t nx = S.inv.x < 0.0f ? q1.w : q1.x, fx = S.inv.x < 0.0f ? q1.x : q1.w;
+        const uint32_t ny = S.inv.y < 0.0f ? q2.x : q1.y, fy = S.inv.y < 0.0f ? q1.y : q2.x;
+        const uint32_t nz = S.inv.z < 0.0f ? q2.y : q1.z, fz = S.inv.z < 0.0f ? q1.z : q2.y;
+#define RC_QSLAB(k, e)                                                                                         \
         [&]() {                                                                                                \
-            const float tx0 = fmaf(float((q1.x >> (8 * k)) & 0xFFu), ax, bx);                                   \
-            const float tx1 = fmaf(float((q1.w >> (8 * k)) & 0xFFu), ax, bx);                                   \
-            const float ty0 = fmaf(float((q1.y >> (8 * k)) & 0xFFu), ay, by);                                   \
-            const float ty1 = fmaf(float((q2.x >> (8 * k)) & 0xFFu), ay, by);                                   \
-            const float tz0 = fmaf(float((q1.z >> (8 * k)) & 0xFFu), az, bz);                                   \
-            const float tz1 = fmaf(float((q2.y >> (8 * k)) & 0xFFu), az, bz);                                   \
-            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), S.tmin));   \
-            const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), S.cull));   \
-            return tn <= tf * 1.000002f ? tn : INF;                                                            \
+            const float tnx = fmaf(float((nx >> (8 * k)) & 0xFFu), ax, bx);                                     \
+            const float tfx = fmaf(float((fx >> (8 * k)) & 0xFFu), ax, bx);                                     \
+            const float tny = fmaf(float((ny >> (8 * k)) & 0xFFu), ay, by);                                     \
+            const float tfy = fmaf(float((fy >> (8 * k)) & 0xFFu), ay, by);                                     \
+            const float tnz = fmaf(float((nz >> (8 * k)) & 0xFFu), az, bz);                                     \
+            const float tfz = fmaf(float((fz >> (8 * k)) & 0xFFu), az, bz);                                     \
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, S.tmin));                                        \
+            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, S.cull));                                        \
+            return tn <= tf * 1.000002f && e != kEmptyChild ? tn : INF;                                        \
         }()
-        const uint32_t m0 = q2.z & 0xFFu, m1 = (q2.z >> 8) & 0xFFu, m2 = (q2.z >> 16) & 0xFFu, m3 = q2.z >> 24;
-        uint32_t l0 = q3.x, c0 = m0 == 0xFFu ? kEmptyChild : m0;
-        uint32_t l1 = q3.y, c1 = m1 == 0xFFu ? kEmptyChild : m1;
-        uint32_t l2 = q3.z, c2 = m2 == 0xFFu ? kEmptyChild : m2;
-        uint32_t l3 = q3.w, c3 = m3 == 0xFFu ? kEmptyChild : m3;
-        float t0 = RC_QSLAB(0), t1 = RC_QSLAB(1), t2 = RC_QSLAB(2), t3 = RC_QSLAB(3);
+        // links are the children's stack entries (kEmptyChild for an unused slot)
+        uint32_t e0 = q3.x, e1 = q3.y, e2 = q3.z, e3 = q3.w;
+        float t0 = RC_QSLAB(0, e0), t1 = RC_QSLAB(1, e1), t2 = RC_QSLAB(2, e2), t3 = RC_QSLAB(3, e3);
 #undef RC_QSLAB
-        t0 = c0 == kEmptyChild ? INF : t0;
-        t1 = c1 == kEmptyChild ? INF : t1;
-        t2 = c2 == kEmptyChild ? INF : t2;
-        t3 = c3 == kEmptyChild ? INF : t3;
+        if (!ANY || RC_ANY_SORT) {
+            order2(t0, e0, t1, e1);
+            order2(t2, e2, t3, e3);
+            order2(t0, e0, t2, e2);
+            order2(t1, e1, t3, e3);
+            order2(t1, e1, t2, e2);
+        }
 #else
         const float4* p = reinterpret_cast<const float4*>(nodes + S.cur);
         float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6], q7 = p[7];
@@ -735,17 +779,18 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
         float t2 = c2 == kEmptyChild ? INF : slab(q4, q5, S.o, S.inv, S.tmin, S.cull);
         float t3 = c3 == kEmptyChild ? INF : slab(q6, q7, S.o, S.inv, S.tmin, S.cull);
 #endif
+        uint32_t e0 = c0 != 0 ? (kLeafTag | (c0 << 26) | l0) : l0, e1 = c1 != 0 ? (kLeafTag | (c1 << 26) | l1) : l1;
+        uint32_t e2 = c2 != 0 ? (kLeafTag | (c2 << 26) | l2) : l2, e3 = c3 != 0 ? (kLeafTag | (c3 << 26) | l3) : l3;
+        order2(t0, e0, t1, e1);
+        order2(t2, e2, t3, e3);
+        order2(t0, e0, t2, e2);
+        order2(t1, e1, t3, e3);
+        order2(t1, e1, t2, e2);
 #endif  // RC_QNODE
-        order2(t0, l0, c0, t1, l1, c1);
-        order2(t2, l2, c2, t3, l3, c3);
-        order2(t0, l0, c0, t2, l2, c2);
-        order2(t1, l1, c1, t3, l3, c3);
-        order2(t1, l1, c1, t2, l2, c2);
         bool have = false;
         uint32_t next = 0;
-#define RC_CHILD(t, l, c)                                                                                      \
+#define RC_CHILD(t, e)                                                                                         \
         if (t != INF && t <= S.cull) {                                                                         \
-            const uint32_t e = c != 0 ? (kLeafTag | (c << 26) | l) : l;                                       \
             if (have && !stack_push(S.reg, S.sp, next, stack, stride, max_depth)) {                            \
                 S.R.overflow = true;                                                                           \
                 return true;                                                                                   \
@@ -753,19 +798,56 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
             next = e;                                                                                          \
             have = true;                                                                                       \
         }
-        RC_CHILD(t3, l3, c3)
-        RC_CHILD(t2, l2, c2)
-        RC_CHILD(t1, l1, c1)
-        RC_CHILD(t0, l0, c0)
+        RC_CHILD(t3, e3)
+        RC_CHILD(t2, e2)
+        RC_CHILD(t1, e1)
+        RC_CHILD(t0, e0)
 #undef RC_CHILD
         if (have) {
             S.cur = next;
+        } else if (S.sp == 0) {
+            if (!RC_LEAF_POSTPONE) return true;
+            S.cur = kEmptyChild;  // nothing left but a leaf set aside
         } else {
-            if (S.sp == 0) return true;
             S.cur = stack_pop(S.reg, S.sp, stack, stride);
         }
     }
+#if RC_LEAF_POSTPONE
+    // Leaf postponing: a lane that reaches a leaf sets it aside (one slot)
+    // and goes on with its stack; the wave tests the leaves set aside when at
+    // least RC_LEAF_POSTPONE / 64 of its lanes hold one, or when no lane has a
+    // node left to visit.  A lane whose next entry is a second leaf waits.
+    if (S.cur != kEmptyChild && (S.cur & kLeafTag) && S.leaf == kEmptyChild) {
+        S.leaf = S.cur;
+        S.cur = S.sp == 0 ? kEmptyChild : stack_pop(S.reg, S.sp, stack, stride);
+    }
+    {
+        const bool held = S.leaf != kEmptyChild;
+        const uint64_t lanes = __ballot(1), visit = __ballot(!(S.cur & kLeafTag)), holding = __ballot(held);
+        const bool run = visit == 0 || uint32_t(__popcll(holding)) * 64u >= uint32_t(__popcll(lanes)) * RC_LEAF_POSTPONE;
+        if (held && run) {
+            const uint32_t first = S.leaf & ((1u << 26) - 1u), count = (S.leaf >> 26) & 31u;
+            if (test_leaf<ANY, COUNT, DEC64>(tris, first, count, S.o, S.d, S.tmin, S.R.t, S.best, S.cull, S.R, n_tri))
+                return true;
+            S.leaf = kEmptyChild;
+        }
+    }
+    return S.cur == kEmptyChild && S.leaf == kEmptyChild;
+#else
+#if RC_LEAF_BATCH
+    // Leaf batching: a wave tests leaves only when at least RC_LEAF_BATCH / 64
+    // of its lanes hold one, or when no lane has a node left to visit; lanes
+    // holding a leaf wait for that step.
+    bool run_leaves;
+    {
+        const bool at_leaf = (S.cur & kLeafTag) != 0;
+        const uint64_t lanes = __ballot(1), leaves = __ballot(at_leaf);
+        run_leaves = leaves == lanes || uint32_t(__popcll(leaves)) * 64u >= uint32_t(__popcll(lanes)) * RC_LEAF_BATCH;
+    }
+    if ((S.cur & kLeafTag) && run_leaves) {
+#else
     if (S.cur & kLeafTag) {
+#endif
         const uint32_t first = S.cur & ((1u << 26) - 1u), count = (S.cur >> 26) & 31u;
         // an any-hit query's R.t holds tmax until the query is answered
         if (test_leaf<ANY, COUNT, DEC64>(tris, first, count, S.o, S.d, S.tmin, S.R.t, S.best, S.cull, S.R, n_tri))
@@ -774,6 +856,7 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
         S.cur = stack_pop(S.reg, S.sp, stack, stride);
     }
     return false;
+#endif  // RC_LEAF_POSTPONE
 }
 
 // A traversal parked mid-way (the extension kernel's step budget ran out):
@@ -782,9 +865,10 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
 // 4..6 its t, u, v, 7 the cull distance, 8..9 the best distance (a double
 // with the FP64 decision, else a float in word 8), 10.. the register stack
 // (top first), then the LDS rows in their order (only the first sp - kRegStack
-// are meaningful).
+// are meaningful); word 30 the best hit's w, word 31 the leaf set aside.
 constexpr int kParkWords = 32;
-static_assert(10 + kRegStack + kLdsStack <= kParkWords, "a parked traversal does not fit its record");
+constexpr int kParkW = kParkWords - 2;
+static_assert(10 + kRegStack + kLdsStack <= kParkW, "a parked traversal does not fit its record");
 
 // Word w of a parked record (compile-time w after unrolling).
 template <bool DEC64>
@@ -806,6 +890,8 @@ __device__ __forceinline__ uint32_t park_word(const Trav<DEC64>& S, uint32_t ent
             return w == 8 ? __float_as_uint(S.best) : 0u;
         }
     }
+    if (w == kParkW) return __float_as_uint(S.R.w);
+    if (w == kParkW + 1) return S.leaf;
     if (w < 10 + kRegStack) return S.reg[w - 10];
     const int j = w - 10 - kRegStack;  // LDS row
     return j < kLdsStack && j < S.sp - kRegStack ? stack[j * stride] : 0u;
@@ -831,6 +917,8 @@ __device__ __forceinline__ void unpark_word(Trav<DEC64>& S, uint32_t* stack, int
     else if (w == 5) S.R.u = __uint_as_float(v);
     else if (w == 6) S.R.v = __uint_as_float(v);
     else if (w == 7) S.cull = __uint_as_float(v);
+    else if (w == kParkW) S.R.w = __uint_as_float(v);
+    else if (w == kParkW + 1) S.leaf = v;
     else if (w == 8) {
         best_lo = v;
         if constexpr (!DEC64) S.best = __uint_as_float(v);

@@ -32,8 +32,8 @@ __device__ __forceinline__ void store_result(bool any, const TraceResult& R, uin
     } else {
         rc_hit h;
         h.t = hit ? R.t : __builtin_inff();
-        h.u = hit ? R.u : 0.0f;
-        h.v = hit ? R.v : 0.0f;
+        h.u = hit ? bary_u(R) : 0.0f;
+        h.v = hit ? bary_v(R) : 0.0f;
         h.tri = hit ? int32_t(R.id) : -1;
         reinterpret_cast<float4*>(hits)[i] = *reinterpret_cast<float4*>(&h);
     }
