@@ -562,7 +562,11 @@ __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, fl
 // wave's lanes hold one (or none has a node), the lanes meanwhile going on
 // with their stacks.
 #ifndef RC_LEAF_POSTPONE
-#define RC_LEAF_POSTPONE 0
+#define RC_LEAF_POSTPONE 16
+#endif
+// The same threshold for any-hit queries.
+#ifndef RC_LEAF_POSTPONE_ANY
+#define RC_LEAF_POSTPONE_ANY 32
 #endif
 // Stack entries of the if-if traversal: an interior child's record index, or
 // a leaf tagged with kLeafTag, its triangle count (1..31) in bits 26..30 and
@@ -813,6 +817,7 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
         }
     }
 #if RC_LEAF_POSTPONE
+    constexpr uint32_t kHold = ANY ? RC_LEAF_POSTPONE_ANY : RC_LEAF_POSTPONE;
     // Leaf postponing: a lane that reaches a leaf sets it aside (one slot)
     // and goes on with its stack; the wave tests the leaves set aside when at
     // least RC_LEAF_POSTPONE / 64 of its lanes hold one, or when no lane has a
@@ -824,7 +829,7 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
     {
         const bool held = S.leaf != kEmptyChild;
         const uint64_t lanes = __ballot(1), visit = __ballot(!(S.cur & kLeafTag)), holding = __ballot(held);
-        const bool run = visit == 0 || uint32_t(__popcll(holding)) * 64u >= uint32_t(__popcll(lanes)) * RC_LEAF_POSTPONE;
+        const bool run = visit == 0 || uint32_t(__popcll(holding)) * 64u >= uint32_t(__popcll(lanes)) * kHold;
         if (held && run) {
             const uint32_t first = S.leaf & ((1u << 26) - 1u), count = (S.leaf >> 26) & 31u;
             if (test_leaf<ANY, COUNT, DEC64>(tris, first, count, S.o, S.d, S.tmin, S.R.t, S.best, S.cull, S.R, n_tri))

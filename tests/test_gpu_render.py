@@ -439,8 +439,10 @@ def test_parked_traversals_give_the_same_image(ctx, decision, scene_name):
     parked and finished by the resume kernel: the same steps, the same hit and
     the same shading, so the film is bit-identical to the unparked render for
     every budget, also with the deep-stack path in play (two on-chip stack
-    entries) and with the node and triangle counters on (whose totals must not
-    change either)."""
+    entries) and with the node and triangle counters on.  Their totals may move
+    a little: a leaf set aside is tested once enough of its wave's lanes hold
+    one, so the nodes a ray visits meanwhile depend on the lanes it shares a
+    wave with, and parking regroups the lanes."""
     sd, s, _ = scene(ctx, scene_name)
     cfg = cfg_for(sd, 64, 48, 4)
     ctx.set_tuning(park_steps=0)
@@ -462,8 +464,9 @@ def test_parked_traversals_give_the_same_image(ctx, decision, scene_name):
         st = ctx.stats()
         ctx.set_counting(False)
         assert sums.tobytes() == sums0c.tobytes()
-        assert (st.node_records, st.tri_tests, st.extension_rays) == (st0.node_records, st0.tri_tests,
-                                                                       st0.extension_rays)
+        assert st.extension_rays == st0.extension_rays
+        assert abs(st.node_records - st0.node_records) <= 0.02 * st0.node_records
+        assert abs(st.tri_tests - st0.tri_tests) <= 0.02 * st0.tri_tests
         ctx.set_tuning(park_steps=4, stack_limit=2)
         _, sums = render_gpu(ctx, s, sd.desc.camera, cfg)
         assert sums.tobytes() == sums0.tobytes()
