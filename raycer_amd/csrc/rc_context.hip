@@ -645,7 +645,7 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     const void* node_src = recs.data();
     const size_t b_nodes = nrec * sizeof(NodeRec);
 #endif
-    std::vector<TriRec> tris(ntri);
+    std::vector<TriRec> tris(ntri), tri_edges(RC_TRI_EDGES ? ntri : 0);
     std::vector<float4> shade(ntri);
     std::vector<DevMaterial> dm(nmat);
     for (size_t i = 0; i < nmat; ++i) {
@@ -688,6 +688,12 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
         tris[j].v0 = make_float4(p[0], p[1], p[2], idf);
         tris[j].v1 = make_float4(p[3], p[4], p[5], flf);
         tris[j].v2 = make_float4(p[6], p[7], p[8], 0.0f);
+        if (RC_TRI_EDGES) {
+            tri_edges[j] = tris[j];
+            // the FP32 differences the kernel would otherwise form (IEEE, round to nearest)
+            tri_edges[j].v1 = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], flf);
+            tri_edges[j].v2 = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.0f);
+        }
     }
     rc_scene* s = new (std::nothrow) rc_scene();
     if (!s) return rc_host_fail(RC_ERR_ALLOC, "rc_scene_upload: out of memory");
@@ -704,11 +710,13 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     // triangle record
     const size_t nalloc = std::max<size_t>(ntri, 1);
     tris.resize(nalloc);
+    if (RC_TRI_EDGES) tri_edges.resize(nalloc);
     shade.resize(nalloc);
     size_t b_tris = nalloc * sizeof(TriRec), b_shade = nalloc * sizeof(float4), b_mats = nmat * sizeof(DevMaterial);
-    s->bytes = b_nodes + b_tris + b_shade + b_mats;
+    s->bytes = b_nodes + (RC_TRI_EDGES ? 2 : 1) * b_tris + b_shade + b_mats;
     auto cleanup = [&](int code, const std::string& msg) {
         if (s->nodes) (void)hipFree(s->nodes);
+        if (s->tris_exact && s->tris_exact != s->tris) (void)hipFree(s->tris_exact);
         if (s->tris) (void)hipFree(s->tris);
         if (s->shade) (void)hipFree(s->shade);
         if (s->mats) (void)hipFree(s->mats);
@@ -716,10 +724,13 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
         return rc_host_fail(code, msg);
     };
     if (hipMalloc(&s->nodes, b_nodes) != hipSuccess || hipMalloc(&s->tris, b_tris) != hipSuccess ||
+        (RC_TRI_EDGES && hipMalloc(&s->tris_exact, b_tris) != hipSuccess) ||
         hipMalloc(&s->shade, b_shade) != hipSuccess || hipMalloc(&s->mats, b_mats) != hipSuccess)
         return cleanup(RC_ERR_ALLOC, "rc_scene_upload: device allocation failed");
+    if (!RC_TRI_EDGES) s->tris_exact = s->tris;
     if (hipMemcpy(s->nodes, node_src, b_nodes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->tris, tris.data(), b_tris, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->tris, (RC_TRI_EDGES ? tri_edges : tris).data(), b_tris, hipMemcpyHostToDevice) != hipSuccess ||
+        (RC_TRI_EDGES && hipMemcpy(s->tris_exact, tris.data(), b_tris, hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(s->shade, shade.data(), b_shade, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->mats, dm.data(), b_mats, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(RC_ERR_HIP, "rc_scene_upload: copy to device failed");
@@ -731,6 +742,7 @@ extern "C" int rc_scene_free(rc_scene* s) {
     if (!s) return RC_OK;
     (void)hipStreamSynchronize(s->ctx->stream);
     (void)hipFree(s->nodes);
+    if (s->tris_exact != s->tris) (void)hipFree(s->tris_exact);
     (void)hipFree(s->tris);
     (void)hipFree(s->shade);
     (void)hipFree(s->mats);

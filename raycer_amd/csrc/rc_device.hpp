@@ -70,10 +70,17 @@ using NodeT = QNode;
 using NodeT = NodeRec;
 #endif
 
+// RC_TRI_EDGES 1: the FP32 decision's records hold v0 and the edges
+// e1 = v1 - v0, e2 = v2 - v0 (rounded in FP32 on the host exactly as the
+// kernel would round them), saving six subtractions per triangle test; the FP64
+// decision keeps a second array with the vertices.
+#ifndef RC_TRI_EDGES
+#define RC_TRI_EDGES 0
+#endif
 struct alignas(16) TriRec {
     float4 v0;  // w: original triangle id (bits)
-    float4 v1;  // w: flags (bit1 nonShadowing)
-    float4 v2;  // w: unused
+    float4 v1;  // w: flags (bit1 nonShadowing); e1 in the FP32 decision's records with RC_TRI_EDGES
+    float4 v2;  // w: unused; e2 likewise
 };
 static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
 
@@ -359,7 +366,7 @@ __device__ __forceinline__ bool mt_hit64(const TriRec& T, float3 of, float3 df, 
 __device__ __forceinline__ bool mt_dec32(const TriRec& T, float3 o, float3 d, float& t, float& U, float& V,
                                          float& adet) {
     const float3 v0 = f3(T.v0);
-    const float3 e1 = f3(T.v1) - v0, e2 = f3(T.v2) - v0;
+    const float3 e1 = RC_TRI_EDGES ? f3(T.v1) : f3(T.v1) - v0, e2 = RC_TRI_EDGES ? f3(T.v2) : f3(T.v2) - v0;
     const float3 pv = cross3(d, e2);
     const float det = dot3(e1, pv);
     if (det == 0.0f) return false;
@@ -413,6 +420,46 @@ __device__ __forceinline__ uint32_t stack_pop(uint32_t* reg, int& sp, const uint
 // The barycentrics of the hit are u / w and v / w: the FP32 decision keeps
 // them undivided (w = |det|) so only a reader of u and v pays the divisions
 // (bary_u / bary_v); the FP64 decision stores them divided, with w = 1.
+// Streamed accesses: queue entries, path states, slot radiance and parked
+// traversals are written once and read once per bounce.  RC_NT_QUEUES 1 makes
+// them nontemporal loads and stores, so they do not displace nodes and
+// triangles from L2 and the Infinity Cache.
+#ifndef RC_NT_QUEUES
+#define RC_NT_QUEUES 0
+#endif
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+    if constexpr (RC_NT_QUEUES) {
+        const v4f_t v = __builtin_nontemporal_load(reinterpret_cast<const v4f_t*>(p));
+        return make_float4(v.x, v.y, v.z, v.w);
+    } else {
+        return *p;
+    }
+}
+__device__ __forceinline__ void st_stream(float4* p, float4 v) {
+    if constexpr (RC_NT_QUEUES) {
+        __builtin_nontemporal_store(v4f_t{v.x, v.y, v.z, v.w}, reinterpret_cast<v4f_t*>(p));
+    } else {
+        *p = v;
+    }
+}
+__device__ __forceinline__ uint4 ld_stream(const uint4* p) {
+    if constexpr (RC_NT_QUEUES) {
+        const v4u_t v = __builtin_nontemporal_load(reinterpret_cast<const v4u_t*>(p));
+        return make_uint4(v.x, v.y, v.z, v.w);
+    } else {
+        return *p;
+    }
+}
+__device__ __forceinline__ void st_stream(uint4* p, uint4 v) {
+    if constexpr (RC_NT_QUEUES) {
+        __builtin_nontemporal_store(v4u_t{v.x, v.y, v.z, v.w}, reinterpret_cast<v4u_t*>(p));
+    } else {
+        *p = v;
+    }
+}
+
 struct TraceResult {
     float t, u, v;
     uint32_t id;
@@ -872,7 +919,7 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
 // (top first), then the LDS rows in their order (only the first sp - kRegStack
 // are meaningful); word 30 the best hit's w, word 31 the leaf set aside.
 constexpr int kParkWords = 32;
-constexpr int kParkW = kParkWords - 2;
+constexpr int kParkW = 30;
 static_assert(10 + kRegStack + kLdsStack <= kParkW, "a parked traversal does not fit its record");
 
 // Word w of a parked record (compile-time w after unrolling).
@@ -908,8 +955,10 @@ __device__ __forceinline__ void trav_park(const Trav<DEC64>& S, uint32_t entry, 
     uint4* r = reinterpret_cast<uint4*>(rec);
 #pragma unroll
     for (int q = 0; q < kParkWords / 4; ++q)
-        r[q] = make_uint4(park_word(S, entry, stack, stride, 4 * q), park_word(S, entry, stack, stride, 4 * q + 1),
-                          park_word(S, entry, stack, stride, 4 * q + 2), park_word(S, entry, stack, stride, 4 * q + 3));
+        st_stream(r + q, make_uint4(park_word(S, entry, stack, stride, 4 * q),
+                                    park_word(S, entry, stack, stride, 4 * q + 1),
+                                    park_word(S, entry, stack, stride, 4 * q + 2),
+                                    park_word(S, entry, stack, stride, 4 * q + 3)));
 }
 
 template <bool DEC64>
@@ -947,7 +996,7 @@ __device__ __forceinline__ void trav_unpark(Trav<DEC64>& S, const uint32_t* __re
     uint32_t best_lo = 0;
 #pragma unroll
     for (int q = 0; q < kParkWords / 4; ++q) {
-        const uint4 v = r[q];
+        const uint4 v = ld_stream(r + q);
         unpark_word(S, stack, stride, 4 * q, v.x, best_lo);
         unpark_word(S, stack, stride, 4 * q + 1, v.y, best_lo);
         unpark_word(S, stack, stride, 4 * q + 2, v.z, best_lo);

@@ -127,7 +127,8 @@ struct rc_scene {
     rc_ctx* ctx = nullptr;
     rcd::NodeT* nodes = nullptr;  // NodeRec records, or QNode nodes when RC_QNODE
     size_t nrec = 0;
-    rcd::TriRec* tris = nullptr;
+    rcd::TriRec* tris = nullptr;        // the FP32 decision's records (edges with RC_TRI_EDGES)
+    rcd::TriRec* tris_exact = nullptr;  // v0, v1, v2 for the FP64 decision (tris itself without RC_TRI_EDGES)
     size_t ntri = 0;
     float4* shade = nullptr;  // normal.xyz, material index bits
     rcd::DevMaterial* mats = nullptr;
@@ -136,6 +137,12 @@ struct rc_scene {
     uint32_t depth = 0;
     size_t bytes = 0;
 };
+
+// The triangle records a traversal with the given decision reads.
+template <bool DEC64>
+inline const rcd::TriRec* tri_array(const rc_scene* s) {
+    return DEC64 ? s->tris_exact : s->tris;
+}
 
 struct rc_film {
     rc_ctx* ctx = nullptr;

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
     rp[0] = r0;
     rp[1] = r1;
     st[slot] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(slot));
-    L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st_stream(&L[slot], make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     W[slot] = w;
 }
 
@@ -392,8 +392,8 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
     uint32_t slot = __float_as_uint(s.w);
     float3 thr = f3(s);
     if (tri < 0) {
-        float4 l = L[slot];
-        L[slot] = make_float4(l.x + thr.x * P.sky.x, l.y + thr.y * P.sky.y, l.z + thr.z * P.sky.z, l.w);
+        float4 l = ld_stream(&L[slot]);
+        st_stream(&L[slot], make_float4(l.x + thr.x * P.sky.x, l.y + thr.y * P.sky.y, l.z + thr.z * P.sky.z, l.w));
         return O;
     }
     float4 sh = shade[tri];
@@ -401,8 +401,8 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
     float4 dif = m.diffuse;
     if (__float_as_uint(dif.w) & RC_MAT_EMISSIVE) {
         float4 e = m.emittance;
-        float4 l = L[slot];
-        L[slot] = make_float4(l.x + thr.x * e.x, l.y + thr.y * e.y, l.z + thr.z * e.z, l.w);
+        float4 l = ld_stream(&L[slot]);
+        st_stream(&L[slot], make_float4(l.x + thr.x * e.x, l.y + thr.y * e.y, l.z + thr.z * e.z, l.w));
         return O;
     }
     float3 nrm = f3(sh);
@@ -441,7 +441,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
         // once it is appended (emit_vertex)
         uint32_t r = wave_append(true, &CI.counters[1]);
         if (r < CI.cap) {
-            float4 lb = L[slot];
+            float4 lb = ld_stream(&L[slot]);
             float4* R = CI.records + 5 * size_t(r);
             R[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
             R[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
@@ -483,10 +483,10 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
             const float4 sh = shade[O.rec];
             const float4 rho = mats[__float_as_uint(sh.w)].diffuse;
             float4* Q = CI.queries + 4 * size_t(qpos);
-            Q[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, __uint_as_float(slot));
-            Q[1] = make_float4(O.newd.x, O.newd.y, O.newd.z, O.thr.x);
-            Q[2] = make_float4(sh.x, sh.y, sh.z, O.thr.y);
-            Q[3] = make_float4(rho.x, rho.y, rho.z, O.thr.z);
+            st_stream(Q + 0, make_float4(O.newo.x, O.newo.y, O.newo.z, __uint_as_float(slot)));
+            st_stream(Q + 1, make_float4(O.newd.x, O.newd.y, O.newd.z, O.thr.x));
+            st_stream(Q + 2, make_float4(sh.x, sh.y, sh.z, O.thr.y));
+            st_stream(Q + 3, make_float4(rho.x, rho.y, rho.z, O.thr.z));
         }
     }
     // the extension kernel reserves once per block, k_shade_listed once per wave
@@ -498,17 +498,17 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
                     : wave_append(O.alive, &ctr->queue_count[cout]);
     if (O.alive) {
         float4* ro = reinterpret_cast<float4*>(rays_out + pos);
-        ro[0] = make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f);
-        ro[1] = make_float4(O.newd.x, O.newd.y, O.newd.z, __builtin_inff());
-        st_out[pos] = make_float4(O.thr.x, O.thr.y, O.thr.z, __uint_as_float(slot));
+        st_stream(ro, make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f));
+        st_stream(ro + 1, make_float4(O.newd.x, O.newd.y, O.newd.z, __builtin_inff()));
+        st_stream(&st_out[pos], make_float4(O.thr.x, O.thr.y, O.thr.z, __uint_as_float(slot)));
     }
     uint32_t spos = BLOCK ? block_append<BLOCK>(O.shadow, &ctr->queue_count[2], append_scratch)
                           : wave_append(O.shadow, &ctr->queue_count[2]);
     if (O.shadow) {
         float4* ro = reinterpret_cast<float4*>(srays + spos);
-        ro[0] = make_float4(O.so.x, O.so.y, O.so.z, 0.0f);
-        ro[1] = make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff());
-        scontrib[spos] = make_float4(O.contrib.x, O.contrib.y, O.contrib.z, __uint_as_float(slot));
+        st_stream(ro, make_float4(O.so.x, O.so.y, O.so.z, 0.0f));
+        st_stream(ro + 1, make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff()));
+        st_stream(&scontrib[spos], make_float4(O.contrib.x, O.contrib.y, O.contrib.z, __uint_as_float(slot)));
         if (!O.query && O.rec != kNoRecord) CI.records[5 * size_t(O.rec) + 4].w = __uint_as_float(spos);
     }
 }
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
         bool done = true;
         if (i < n) {
             const float4* rp = reinterpret_cast<const float4*>(rays + i);
-            const float4 r0 = rp[0], r1 = rp[1];
+            const float4 r0 = ld_stream(rp), r1 = ld_stream(rp + 1);
             trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
             uint32_t steps = 0;
             do {
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
             if (S.R.overflow) {
                 ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
             } else {
-                const float4 s = st[i];
+                const float4 s = ld_stream(&st[i]);
                 slot = __float_as_uint(s.w);
                 O = shade_vertex(P, k, make_float4(S.o.x, S.o.y, S.o.z, 0.0f), make_float4(S.d.x, S.d.y, S.d.z, 0.0f),
                                  s, S.R.t, S.R.id == kNoHit ? -1 : int(S.R.id), shade, mats, L, CI);
@@ -605,12 +605,12 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
         if (CAM) {
             W[i] = camera_ray(P, i, r0, r1);
             s = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(i));
-            L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            st_stream(&L[i], make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         } else {
             const float4* rp = reinterpret_cast<const float4*>(rays + i);
-            r0 = rp[0];
-            r1 = rp[1];
-            s = st[i];
+            r0 = ld_stream(rp);
+            r1 = ld_stream(rp + 1);
+            s = ld_stream(&st[i]);
         }
         slot = __float_as_uint(s.w);
         TraceResult R = traverse<false, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
@@ -669,7 +669,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
         i = rec[0];  // the queue entry (word 0)
         {
             const float4* rp = reinterpret_cast<const float4*>(rays + i);
-            const float4 r0 = rp[0], r1 = rp[1];
+            const float4 r0 = ld_stream(rp), r1 = ld_stream(rp + 1);
             trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
         }
         trav_unpark(S, rec, stack + threadIdx.x, BLOCK);
@@ -697,7 +697,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
         if (S.R.overflow) {
             ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         } else {
-            const float4 s = st[i];
+            const float4 s = ld_stream(&st[i]);
             slot = __float_as_uint(s.w);
             O = shade_vertex(P, k, make_float4(S.o.x, S.o.y, S.o.z, 0.0f), make_float4(S.d.x, S.d.y, S.d.z, 0.0f), s,
                              S.R.t, S.R.id == kNoHit ? -1 : int(S.R.id), shade, mats, L, CI);
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
         if (q < n) {
             uint32_t i = list[q];
             const float4* rp = reinterpret_cast<const float4*>(rays + i);
-            float4 s = st[i];
+            float4 s = ld_stream(&st[i]);
             float4 h = reinterpret_cast<const float4*>(hits)[i];
             slot = __float_as_uint(s.w);
             O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L, CI);
@@ -746,10 +746,10 @@ __device__ __forceinline__ void shadow_result(uint32_t i, bool occluded, const f
                                               float4* __restrict__ L, uint8_t* __restrict__ occ_out) {
     if (occ_out) occ_out[i] = occluded ? 1 : 0;
     if (!occluded) {
-        float4 c = scontrib[i];
+        float4 c = ld_stream(&scontrib[i]);
         uint32_t slot = __float_as_uint(c.w);
-        float4 l = L[slot];
-        L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+        float4 l = ld_stream(&L[slot]);
+        st_stream(&L[slot], make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w));
     }
 }
 
@@ -777,7 +777,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT*
         bool done = true;
         if (i < n) {
             const float4* rp = reinterpret_cast<const float4*>(srays + i);
-            const float4 r0 = rp[0], r1 = rp[1];
+            const float4 r0 = ld_stream(rp), r1 = ld_stream(rp + 1);
             trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
             uint32_t steps = 0;
             do {
@@ -803,7 +803,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT*
         }
     } else if (i < n) {
         const float4* rp = reinterpret_cast<const float4*>(srays + i);
-        float4 r0 = rp[0], r1 = rp[1];
+        float4 r0 = ld_stream(rp), r1 = ld_stream(rp + 1);
         TraceResult R = traverse<true, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
                                                      BLOCK, stack_limit, n_rec, n_tri, top, ntop);
         if (R.overflow) ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow_resume(
         i = rec[0];
         {
             const float4* rp = reinterpret_cast<const float4*>(srays + i);
-            const float4 r0 = rp[0], r1 = rp[1];
+            const float4 r0 = ld_stream(rp), r1 = ld_stream(rp + 1);
             trav_begin(S, f3(r0), f3(r1), r0.w, r1.w, top, ntop);
         }
         trav_unpark(S, rec, stack + threadIdx.x, BLOCK);
@@ -880,10 +880,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow_apply_listed(const float4* __
     for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
         uint32_t i = list[q];
         if (occ[i]) continue;
-        float4 c = scontrib[i];
+        float4 c = ld_stream(&scontrib[i]);
         uint32_t slot = __float_as_uint(c.w);
-        float4 l = L[slot];
-        L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+        float4 l = ld_stream(&L[slot]);
+        st_stream(&L[slot], make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w));
     }
 }
 
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(kBlock) void k_cache_nee_fix(float4* __restrict__ r
         float4 r4 = records[5 * size_t(i) + 4];
         const uint32_t spos = __float_as_uint(r4.w);
         if (spos == kNoRecord || occ[spos]) continue;
-        const float4 c = scontrib[spos];
+        const float4 c = ld_stream(&scontrib[spos]);
         records[5 * size_t(i) + 4] = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, r4.w);
     }
 }
@@ -972,7 +972,7 @@ __global__ __launch_bounds__(kBlock) void k_film_add(PixelOrder order, uint32_t 
     for (uint32_t j = 0; j < spp_pass; ++j) {
         uint32_t slot = RC_PIXEL_MAJOR ? q * spp_pass + j : j * npix + q;
         float w = W[slot];
-        float4 l = L[slot];
+        float4 l = ld_stream(&L[slot]);
         kahan(s.x, c.x, w * l.x);
         kahan(s.y, c.y, w * l.y);
         kahan(s.z, c.z, w * l.z);
@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(kFilmBlock) void k_film_add_pm(PixelOrder order, ui
         for (uint32_t e = threadIdx.x; e < nq * ch; e += kFilmBlock) {
             const uint32_t i = e / ch, k = e - i * ch;
             const size_t slot = size_t(q0 + i) * spp_pass + j0 + k;
-            sl[i * kFilmChunk + k] = L[slot];
+            sl[i * kFilmChunk + k] = ld_stream(&L[slot]);
             sw[i * kFilmChunk + k] = W[slot];
         }
         __syncthreads();
@@ -1064,7 +1064,7 @@ struct ExtendArgs {
 template <bool COUNT, bool CAM, bool DEC64, int BLOCK, bool PARK>
 void extend_t(const ExtendArgs& a, uint32_t n, hipStream_t st) {
     k_extend_shade<COUNT, CAM, DEC64, BLOCK, PARK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
-        a.P, a.k, a.s->nodes, a.s->tris, a.rin, a.sin, a.s->shade, a.s->mats, a.rout, a.sout, a.srays, a.scontrib, a.L,
+        a.P, a.k, a.s->nodes, tri_array<DEC64>(a.s), a.rin, a.sin, a.s->shade, a.s->mats, a.rout, a.sout, a.srays, a.scontrib, a.L,
         a.ovf, a.ctr, a.cin, a.cout, a.CI, a.W, a.rin, a.sin, a.PK);
 }
 
@@ -1101,7 +1101,7 @@ void launch_extend(const ExtendArgs& a, uint32_t n, hipStream_t st, bool count, 
 template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
 void resume_t(const ExtendArgs& a, const ParkIO& in, const ParkIO& out, hipStream_t st) {
     k_extend_resume<COUNT, DEC64, BLOCK, PARK><<<std::max(1u, (in.cap + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(
-        a.P, a.k, a.s->nodes, a.s->tris, a.rin, a.sin, a.s->shade, a.s->mats, a.rout, a.sout, a.srays, a.scontrib, a.L,
+        a.P, a.k, a.s->nodes, tri_array<DEC64>(a.s), a.rin, a.sin, a.s->shade, a.s->mats, a.rout, a.sout, a.srays, a.scontrib, a.L,
         a.ovf, a.ctr, a.cout, a.CI, in, out);
 }
 
@@ -1145,7 +1145,7 @@ struct ShadowArgs {
 template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
 void shadow_t(const ShadowArgs& a, uint32_t n, hipStream_t st) {
     k_shadow<COUNT, DEC64, BLOCK, PARK><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(
-        a.s->nodes, a.s->tris, a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit, a.PK);
+        a.s->nodes, tri_array<DEC64>(a.s), a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit, a.PK);
 }
 
 template <bool COUNT, bool DEC64, bool PARK>
@@ -1167,7 +1167,7 @@ void launch_shadow(const ShadowArgs& a, uint32_t n, hipStream_t st, bool count, 
 template <bool COUNT, bool DEC64, int BLOCK, bool PARK>
 void shadow_resume_t(const ShadowArgs& a, const ParkIO& in, const ParkIO& out, hipStream_t st) {
     k_shadow_resume<COUNT, DEC64, BLOCK, PARK><<<std::max(1u, (in.cap + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(
-        a.s->nodes, a.s->tris, a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit, in, out);
+        a.s->nodes, tri_array<DEC64>(a.s), a.srays, a.scontrib, a.L, a.ovf, a.ctr, a.occ_out, a.stack_limit, in, out);
 }
 
 template <bool COUNT, bool DEC64, bool PARK>

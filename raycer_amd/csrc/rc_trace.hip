@@ -101,14 +101,14 @@ __global__ void k_reset_overflow(DevCounters* ctr) { ctr->ovf_count = 0; }
 
 template <bool ANY, bool COUNT, bool DEC64>
 void launch_pair(const TraceLaunch& L, uint32_t blocks) {
-    k_trace<ANY, COUNT, DEC64><<<blocks, kBlock, 0, L.stream>>>(L.scene->nodes, L.scene->tris, L.rays, L.n_dev,
+    k_trace<ANY, COUNT, DEC64><<<blocks, kBlock, 0, L.stream>>>(L.scene->nodes, tri_array<DEC64>(L.scene), L.rays, L.n_dev,
                                                                 L.n_bound, L.hits, L.occ, L.ovf_list, L.ctr,
                                                                 L.max_depth);
 }
 
 template <bool ANY, bool COUNT, bool DEC64>
 void launch_deep(const TraceLaunch& L) {
-    k_trace_deep<ANY, COUNT, DEC64><<<kDeepThreads / 64, 64, 0, L.stream>>>(L.scene->nodes, L.scene->tris, L.rays,
+    k_trace_deep<ANY, COUNT, DEC64><<<kDeepThreads / 64, 64, 0, L.stream>>>(L.scene->nodes, tri_array<DEC64>(L.scene), L.rays,
                                                                            L.hits, L.occ, L.ovf_list, L.deep_stack,
                                                                            L.ctr);
 }
