@@ -369,12 +369,12 @@ __device__ __forceinline__ bool mt_dec32(const TriRec& T, float3 o, float3 d, fl
     const float3 e1 = RC_TRI_EDGES ? f3(T.v1) : f3(T.v1) - v0, e2 = RC_TRI_EDGES ? f3(T.v2) : f3(T.v2) - v0;
     const float3 pv = cross3(d, e2);
     const float det = dot3(e1, pv);
-    if (det == 0.0f) return false;
     const float sg = det < 0.0f ? -1.0f : 1.0f;
     adet = fabsf(det);
     const float3 tv = o - v0;
     U = sg * dot3(tv, pv);
-    if (U < 0.0f || U > adet) return false;
+    // one branch for both tests, so the load of v0 is not sunk behind the first
+    if (det == 0.0f || U < 0.0f || U > adet) return false;
     const float3 qv = cross3(tv, e1);
     V = sg * dot3(d, qv);
     if (V < 0.0f || U + V > adet) return false;

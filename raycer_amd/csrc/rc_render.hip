@@ -29,8 +29,14 @@ constexpr float kPi = 3.14159265358979323846f;
 // Threads per block of the extension kernel (4 waves per queue reservation:
 // 3% faster than 512 and 128 on the 2M-triangle terrain) and of the shadow
 // kernel, unless a launch shape says otherwise (the batch autotuner).
-constexpr uint32_t kExtBlockDefault = 256;
-constexpr uint32_t kShadowBlockDefault = 128;
+#ifndef RC_EXT_BLOCK
+#define RC_EXT_BLOCK 256
+#endif
+#ifndef RC_SHADOW_BLOCK
+#define RC_SHADOW_BLOCK 128
+#endif
+constexpr uint32_t kExtBlockDefault = RC_EXT_BLOCK;
+constexpr uint32_t kShadowBlockDefault = RC_SHADOW_BLOCK;
 constexpr int kLdsRows = kLdsStack > 0 ? kLdsStack : 1;  // LDS stack entries per thread (at least one row)
 // Minimum waves per SIMD the traversal kernels are compiled for, by triangle
 // decision.  FP32: 8, which caps them at 64 VGPRs with no spills (59-62 used;
