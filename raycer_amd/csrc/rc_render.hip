@@ -63,7 +63,7 @@ static_assert(RC_PARK_PHASES >= 2, "a parking extension launch needs a resume ke
 // Shadow rays (any hit) park after RC_SHADOW_PARK_STEPS iterations when
 // parking is on (0: they never park), over RC_SHADOW_PARK_PHASES kernels.
 #ifndef RC_SHADOW_PARK_STEPS
-#define RC_SHADOW_PARK_STEPS 24
+#define RC_SHADOW_PARK_STEPS 32
 #endif
 #ifndef RC_SHADOW_PARK_PHASES
 #define RC_SHADOW_PARK_PHASES 3
