@@ -46,6 +46,32 @@ def mt_fp64(vertices, tri_ids, rays):
     return np.einsum("ij,ij->i", e2, q) / det
 
 
+def mt_fp64_uv(vertices, tri_ids, rays):
+    """FP64 Möller–Trumbore barycentrics (u, v) of each ray on the given triangle."""
+    v = vertices[tri_ids].astype(np.float64).reshape(-1, 3, 3)
+    o = rays["org"].astype(np.float64)
+    d = rays["dir"].astype(np.float64)
+    e1, e2 = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+    p = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, p)
+    tv = o - v[:, 0]
+    q = np.cross(tv, e1)
+    return np.einsum("ij,ij->i", tv, p) / det, np.einsum("ij,ij->i", d, q) / det
+
+
+def fp32_uv_tolerance(vertices, tri_ids, rays):
+    """Absolute bound on the FP32 decision's barycentrics: FP32_ERR times the
+    largest coordinate or edge over the shorter edge, over cond (the rounding
+    of o - v0 and of the edges, carried through the division by det)."""
+    v = vertices[tri_ids].astype(np.float64).reshape(-1, 3, 3)
+    o = rays["org"].astype(np.float64)
+    e1, e2 = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+    l1, l2 = np.linalg.norm(e1, axis=1), np.linalg.norm(e2, axis=1)
+    big = np.max(np.stack([np.abs(o).max(axis=1), np.abs(v).max(axis=(1, 2)), l1, l2]), axis=0)
+    amp = big / np.maximum(np.minimum(l1, l2), 1e-30)
+    return np.maximum(1e-5, FP32_ERR * amp / np.maximum(tri_cond(vertices, tri_ids, rays), 1e-30))
+
+
 def random_rays(n, seed, lo, hi, tmax=np.inf):
     rng = np.random.default_rng(seed)
     lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
@@ -128,6 +154,14 @@ def check_closest(gpu_hits, vertices, rays, diag, max_listed=0.01, rel_t=1e-5, f
         if fp32:
             tol = fp32_t_tolerance(vertices, gid[hit], rays[hit], t64, rel_t)
         assert (err <= tol).all(), f"max relative t error {err.max():.3g}, worst excess {(err / tol).max():.3g}x"
+        # barycentrics of the reported triangle: inside it, and near the FP64 values
+        u, v = gpu_hits["u"][hit].astype(np.float64), gpu_hits["v"][hit].astype(np.float64)
+        assert (u >= 0).all() and (v >= 0).all() and (u + v <= 1 + 2.0 ** -22).all()
+        u64, v64 = mt_fp64_uv(vertices, gid[hit], rays[hit])
+        tol_uv = fp32_uv_tolerance(vertices, gid[hit], rays[hit]) if fp32 else np.full(u.shape, rel_t)
+        err_uv = np.maximum(np.abs(u - u64), np.abs(v - v64))
+        assert (err_uv <= tol_uv).all(), (
+            f"max barycentric error {err_uv.max():.3g}, worst excess {(err_uv / tol_uv).max():.3g}x")
     agree = hit & ~mismatch
     if agree.any() and not fp32:
         err = np.abs(gpu_hits["t"][agree] - t_o[agree]) / t_o[agree]
