@@ -97,7 +97,7 @@ def build_sanitized(out_dir: Path) -> Path:
     """The host half of the library (csrc/host: OBJ/PLY, scene files, BVH
     builders and files, scene generators) and the FP64 oracle, built with
     AddressSanitizer and UndefinedBehaviorSanitizer (g++, host code only) and
-    linked into the driver tests/native/host_sanitize.cpp."""
+    linked into the test program tests/native/host_sanitize.cpp."""
     out_dir.mkdir(parents=True, exist_ok=True)
     exe = out_dir / "host_sanitize"
     srcs = [ROOT / "tests" / "native" / "host_sanitize.cpp", *sorted((CSRC / "host").glob("*.cpp")),

@@ -1,6 +1,6 @@
 """The host half of the library and the oracle under AddressSanitizer and
 UndefinedBehaviorSanitizer (a host-only g++ build: python -m raycer_amd.build
---sanitize DIR).  The driver, tests/native/host_sanitize.cpp, loads the
+--sanitize DIR).  The test program, tests/native/host_sanitize.cpp, loads the
 reference's meshes and sample scene, round-trips scene and BVH files, feeds the
 scene reader every truncation and seeded mutations of the sample and hostile
 nesting, builds BVHs with both builders and runs the oracle's traversal,
