@@ -600,11 +600,6 @@ __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, fl
 #ifndef RC_ANY_SORT
 #define RC_ANY_SORT 0
 #endif
-// 0: each step tests the leaf a lane holds at once; n > 0: a wave tests
-// leaves only when at least n / 64 of its lanes hold one (or none has a node).
-#ifndef RC_LEAF_BATCH
-#define RC_LEAF_BATCH 0
-#endif
 // 0: off; n > 0: leaves are set aside and tested when at least n / 64 of the
 // wave's lanes hold one (or none has a node), the lanes meanwhile going on
 // with their stacks.
@@ -886,20 +881,7 @@ __device__ __forceinline__ bool trav_step(Trav<DEC64>& S, const NodeT* __restric
     }
     return S.cur == kEmptyChild && S.leaf == kEmptyChild;
 #else
-#if RC_LEAF_BATCH
-    // Leaf batching: a wave tests leaves only when at least RC_LEAF_BATCH / 64
-    // of its lanes hold one, or when no lane has a node left to visit; lanes
-    // holding a leaf wait for that step.
-    bool run_leaves;
-    {
-        const bool at_leaf = (S.cur & kLeafTag) != 0;
-        const uint64_t lanes = __ballot(1), leaves = __ballot(at_leaf);
-        run_leaves = leaves == lanes || uint32_t(__popcll(leaves)) * 64u >= uint32_t(__popcll(lanes)) * RC_LEAF_BATCH;
-    }
-    if ((S.cur & kLeafTag) && run_leaves) {
-#else
     if (S.cur & kLeafTag) {
-#endif
         const uint32_t first = S.cur & ((1u << 26) - 1u), count = (S.cur >> 26) & 31u;
         // an any-hit query's R.t holds tmax until the query is answered
         if (test_leaf<ANY, COUNT, DEC64>(tris, first, count, S.o, S.d, S.tmin, S.R.t, S.best, S.cull, S.R, n_tri))

@@ -302,54 +302,6 @@ __device__ __forceinline__ uint32_t block_append(bool pred, uint32_t* counter, u
     return base + before + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
-// RC_RAY_BINS > 1: the extension kernels write a block's continuation rays
-// grouped by a direction key (the octant with 8 bins), so the waves of the
-// next bounce hold rays of fewer directions; 1: in wave and lane order.
-#ifndef RC_RAY_BINS
-#define RC_RAY_BINS 1
-#endif
-constexpr uint32_t kRayBins = RC_RAY_BINS;
-template <int BLOCK>
-constexpr uint32_t kAppendWords = (BLOCK / 64) * kRayBins + 1;
-
-__device__ __forceinline__ uint32_t ray_bin(float3 d) {
-    if (kRayBins == 1) return 0u;
-    const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
-    return oct % kRayBins;
-}
-
-// block_append with the block's entries grouped by key (< kRayBins): the
-// reserved range holds bin 0's entries first (in wave, then lane order), then
-// bin 1's, and so on.  scratch: kAppendWords<BLOCK> words.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t block_append_binned(bool pred, uint32_t key, uint32_t* counter,
-                                                        uint32_t* scratch) {
-    constexpr uint32_t kW = BLOCK / 64, kN = kW * kRayBins;
-    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t rank = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < kRayBins; ++b) {
-        const uint64_t m = __ballot(pred && key == b);
-        if (lane == 0) scratch[b * kW + wave] = uint32_t(__popcll(m));
-        rank = key == b ? uint32_t(__popcll(m & below)) : rank;
-    }
-    __syncthreads();
-    const uint32_t mine = key * kW + wave;
-    uint32_t before = 0, total = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kN; ++j) {
-        const uint32_t c = scratch[j];
-        before += j < mine ? c : 0u;
-        total += c;
-    }
-    if (threadIdx.x == 0) scratch[kN] = total ? atomicAdd(counter, total) : 0u;
-    __syncthreads();
-    const uint32_t base = scratch[kN];
-    __syncthreads();  // the scratch words may be reused by the next call
-    return base + before + rank;
-}
-
 // mapToHemisphere with distribution 1 (Sampler.cpp:50-64) in the ONB of a
 // triangle without texcoords (Triangle.cpp:40-44: T = n x ALMOST_UP, B = T x n).
 __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
@@ -496,12 +448,8 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
         }
     }
     // the extension kernel reserves once per block, k_shade_listed once per wave
-    uint32_t pos;
-    if constexpr (BLOCK > 0 && kRayBins > 1)
-        pos = block_append_binned<BLOCK>(O.alive, ray_bin(O.newd), &ctr->queue_count[cout], append_scratch);
-    else
-        pos = BLOCK ? block_append<BLOCK>(O.alive, &ctr->queue_count[cout], append_scratch)
-                    : wave_append(O.alive, &ctr->queue_count[cout]);
+    uint32_t pos = BLOCK ? block_append<BLOCK>(O.alive, &ctr->queue_count[cout], append_scratch)
+                         : wave_append(O.alive, &ctr->queue_count[cout]);
     if (O.alive) {
         float4* ro = reinterpret_cast<float4*>(rays_out + pos);
         st_stream(ro, make_float4(O.newo.x, O.newo.y, O.newo.z, 0.0f));
@@ -556,7 +504,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
     rc_ray* __restrict__ cam_rays, float4* __restrict__ cam_st, ParkIO PK) {
     static_assert(!(PARK && CAM), "the camera bounce does not park");
     __shared__ uint32_t stack[kLdsRows * BLOCK];
-    __shared__ uint32_t append_scratch[kAppendWords<BLOCK>];
+    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
     __shared__ uint4 top[4 * kTopNodes];
     const uint32_t n = ctr->queue_count[cin];
     if (blockIdx.x * uint32_t(BLOCK) >= n) return;
@@ -655,7 +603,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
     rc_ray* __restrict__ srays, float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
     DevCounters* __restrict__ ctr, uint32_t cout, CacheIO CI, ParkIO in, ParkIO out) {
     __shared__ uint32_t stack[kLdsRows * BLOCK];
-    __shared__ uint32_t append_scratch[kAppendWords<BLOCK>];
+    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
     __shared__ uint4 top[4 * kTopNodes];
     const uint32_t n = min(ctr->park_count[in.which], in.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&ctr->park_total, (unsigned long long)n);
@@ -778,7 +726,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT*
     if (i == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     if constexpr (PARK) {
-        __shared__ uint32_t append_scratch[kAppendWords<BLOCK>];
+        __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
         Trav<DEC64> S;
         bool done = true;
         if (i < n) {
@@ -829,7 +777,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow_resume(
     const float4* __restrict__ scontrib, float4* __restrict__ L, uint32_t* __restrict__ ovf_list,
     DevCounters* __restrict__ ctr, uint8_t* __restrict__ occ_out, int stack_limit, ParkIO in, ParkIO out) {
     __shared__ uint32_t stack[kLdsRows * BLOCK];
-    __shared__ uint32_t append_scratch[kAppendWords<BLOCK>];
+    __shared__ uint32_t append_scratch[BLOCK / 64 + 1];
     __shared__ uint4 top[4 * kTopNodes];
     const uint32_t n = min(ctr->park_count[in.which], in.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&ctr->park_total, (unsigned long long)n);
