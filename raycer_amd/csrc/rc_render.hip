@@ -260,6 +260,25 @@ __global__ void k_pass_start(DevCounters* ctr, uint32_t n) {
     ctr->queue_count[2] = 0;
 }
 
+// RC_PARK_KEEP > 0: a wave whose budget is spent keeps traversing while at
+// least RC_PARK_KEEP of its lanes still are (up to twice the budget), so
+// well-filled waves do not park; 0: every traversal parks at its budget.
+#ifndef RC_PARK_KEEP
+#define RC_PARK_KEEP 8
+#endif
+#ifndef RC_PARK_KEEP_ANY
+#define RC_PARK_KEEP_ANY 0
+#endif
+// Whether a parking traversal (ANY: a shadow ray's) goes on after `steps`
+// iterations of `budget`.
+template <bool ANY>
+__device__ __forceinline__ bool keep_going(bool done, uint32_t steps, uint32_t budget) {
+    constexpr uint32_t keep = ANY ? RC_PARK_KEEP_ANY : RC_PARK_KEEP;
+    if (keep == 0) return !done && steps < budget;
+    const uint32_t busy = uint32_t(__popcll(__ballot(!done)));
+    return !done && (steps < budget || (steps < 2 * budget && busy >= keep));
+}
+
 // Reserves `pred ? 1 : 0` entries of a queue: one atomic per wave, positions
 // from the ballot prefix.
 __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
@@ -529,7 +548,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
             do {
                 done = trav_step<false, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, P.stack_limit,
                                                       n_rec, n_tri);
-            } while (!done && ++steps < PK.steps);
+            } while (keep_going<false>(done, ++steps, PK.steps));
         }
         // every thread of the block reserves (one atomic per block)
         const uint32_t pos = block_append<BLOCK>(!done, &ctr->park_count[PK.which], append_scratch);
@@ -631,7 +650,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
         do {
             done = trav_step<false, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, P.stack_limit, n_rec,
                                                   n_tri);
-        } while (!done && (!PARK || ++steps < out.steps));
+        } while (PARK ? keep_going<false>(done, ++steps, out.steps) : !done);
     }
     bool parked = false;
     if constexpr (PARK) {
@@ -737,7 +756,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow(const NodeT*
             do {
                 done = trav_step<true, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, stack_limit, n_rec,
                                                      n_tri);
-            } while (!done && ++steps < PK.steps);
+            } while (keep_going<true>(done, ++steps, PK.steps));
         }
         const uint32_t pos = block_append<BLOCK>(!done, &ctr->park_count[PK.which], append_scratch);
         bool parked = false;
@@ -800,7 +819,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_shadow_resume(
         uint32_t steps = 0;
         do {
             done = trav_step<true, COUNT, DEC64>(S, nodes, tris, stack + threadIdx.x, BLOCK, stack_limit, n_rec, n_tri);
-        } while (!done && (!PARK || ++steps < out.steps));
+        } while (PARK ? keep_going<true>(done, ++steps, out.steps) : !done);
     }
     bool parked = false;
     if constexpr (PARK) {

@@ -214,6 +214,28 @@ def test_determinism_and_counting(ctx):
     print("records/ray", st.node_records / len(rays), "tris/ray", st.tri_tests / len(rays))
 
 
+def test_answers_do_not_depend_on_the_lanes_a_ray_shares_a_wave_with(ctx, decision):
+    """Leaf postponing makes the order in which a ray's leaves are tested depend
+    on the other rays of its wave; the closest hit (exact ties to the lower
+    id) and the any-hit answer must not.  The same rays in a shuffled order
+    (camera rays mixed with random ones, so waves mix long and short
+    traversals) give bit-identical results."""
+    sd, s, _ = scene(ctx, "terrain128")
+    cam = camera_rays(sd.desc.camera, 160, 120)
+    lo, hi = sd.vertices.reshape(-1, 3).min(axis=0), sd.vertices.reshape(-1, 3).max(axis=0)
+    rays = np.concatenate([cam, random_rays(len(cam), 5, lo, hi)])
+    perm = np.random.default_rng(3).permutation(len(rays))
+    a = ctx.trace_closest(s, rays)
+    b = ctx.trace_closest(s, rays[perm])
+    back = np.empty_like(b)
+    back[perm] = b
+    assert a.tobytes() == back.tobytes()
+    assert (a["tri"] >= 0).mean() > 0.3
+    occ_a = ctx.trace_any(s, rays)
+    occ_b = ctx.trace_any(s, rays[perm])
+    assert (occ_a == occ_b[np.argsort(perm)]).all()
+
+
 def test_upload_rejects_corrupt_bvh(ctx):
     sd, _, _ = scene(ctx, "box")
     nodes, ordered, _, _ = ra.build_bvh(sd.vertices)
