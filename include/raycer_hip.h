@@ -181,7 +181,7 @@ typedef struct rc_point_light {
  * of a bounce before it is parked and finished by a second kernel whose
  * waves hold only such long traversals (0: never park; 1..1024; the results
  * are the same either way).  Defaults 256, 0, 64, 0, RC_PARK_STEPS_DEFAULT. */
-#define RC_PARK_STEPS_DEFAULT 28
+#define RC_PARK_STEPS_DEFAULT 20
 typedef struct rc_tuning {
     uint32_t block;
     uint32_t spp_per_pass;

@@ -159,7 +159,7 @@ class PointLight(C.Structure):
                 ("ambient_intensity", C.c_float)]
 
 
-PARK_STEPS_DEFAULT = 28  # RC_PARK_STEPS_DEFAULT in include/raycer_hip.h
+PARK_STEPS_DEFAULT = 20  # RC_PARK_STEPS_DEFAULT in include/raycer_hip.h
 
 
 class Tuning(C.Structure):
