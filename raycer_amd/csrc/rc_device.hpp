@@ -604,7 +604,7 @@ __device__ __forceinline__ void order2(float& ta, uint32_t& la, uint32_t& ca, fl
 // wave's lanes hold one (or none has a node), the lanes meanwhile going on
 // with their stacks.
 #ifndef RC_LEAF_POSTPONE
-#define RC_LEAF_POSTPONE 16
+#define RC_LEAF_POSTPONE 20
 #endif
 // The same threshold for any-hit queries.
 #ifndef RC_LEAF_POSTPONE_ANY
