@@ -179,7 +179,8 @@ typedef struct rc_point_light {
  * more rays down the deep-stack path and changes no result), and the
  * traversal steps a path tracer's extension ray takes in the first kernel
  * of a bounce before it is parked and finished by a second kernel whose
- * waves hold only such long traversals (0: never park; 1..1024; the results
+ * waves hold only such long traversals (a wave with at least 8 lanes still
+ * traversing goes on, up to twice that; 0: never park; 1..1024; the results
  * are the same either way).  Defaults 256, 0, 64, 0, RC_PARK_STEPS_DEFAULT. */
 #define RC_PARK_STEPS_DEFAULT 20
 typedef struct rc_tuning {
