@@ -337,8 +337,10 @@ __device__ __forceinline__ float3 hemisphere(float3 n, float u1, float u2) {
 struct ShadeOut {
     bool alive, shadow;
     bool query;  // the path ends into the radiance cache: newo = position, newd = outgoing direction, rec = triangle
+    bool train;  // a training vertex: tr holds the first four words of its record (emit_vertex writes it)
     float3 thr, newo, newd, contrib, so;
-    uint32_t rec;  // radiance-cache training record written at this vertex, or kNoRecord (or the query's triangle)
+    uint32_t rec;  // the query's triangle, else kNoRecord
+    float4 tr[4];
 };
 constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 
@@ -363,7 +365,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
                                                  const DevMaterial* __restrict__ mats, float4* __restrict__ L,
                                                  const CacheIO& CI) {
     ShadeOut O;
-    O.alive = O.shadow = O.query = false;
+    O.alive = O.shadow = O.query = O.train = false;
     O.rec = kNoRecord;
     const bool more = (k + 1) < P.max_len;
     uint32_t slot = __float_as_uint(s.w);
@@ -413,20 +415,14 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
         }
         // training vertex: the radiance the path gathers after this vertex's
         // next-event ray, over the throughput arriving here, is the target;
-        // the record's "before" value gets that ray's contribution added once
-        // its occlusion is known (k_cache_nee_fix), and its queue position
-        // once it is appended (emit_vertex)
-        uint32_t r = wave_append(true, &CI.counters[1]);
-        if (r < CI.cap) {
-            float4 lb = ld_stream(&L[slot]);
-            float4* R = CI.records + 5 * size_t(r);
-            R[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
-            R[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
-            R[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.y);
-            R[3] = make_float4(rho.x, rho.y, rho.z, thr.z);
-            R[4] = make_float4(lb.x, lb.y, lb.z, __uint_as_float(kNoRecord));
-            O.rec = r;
-        }
+        // emit_vertex reserves and writes the record (one reservation per
+        // block), whose "before" value gets that ray's contribution added once
+        // its occlusion is known (k_cache_nee_fix)
+        O.train = true;
+        O.tr[0] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+        O.tr[1] = make_float4(wo.x, wo.y, wo.z, thr.x);
+        O.tr[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.y);
+        O.tr[3] = make_float4(rho.x, rho.y, rho.z, thr.z);
     }
     float3 w = hemisphere(nrm, u.x, u.y);
     if (P.estimator == RC_EST_REFERENCE) thr = mul3(thr, rho * (2.0f * fabsf(dot3(w, nrm))));
@@ -452,8 +448,21 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
                                             DevCounters* __restrict__ ctr, uint32_t cout, const CacheIO& CI,
                                             const float4* __restrict__ shade, const DevMaterial* __restrict__ mats,
-                                            uint32_t* append_scratch = nullptr) {
+                                            const float4* __restrict__ L, uint32_t* append_scratch = nullptr) {
+    uint32_t rec = kNoRecord;  // this vertex's training record
     if (CI.cache_vertex > 0 && k == CI.cache_vertex) {  // uniform: paths end into the cache at this vertex
+        const uint32_t r = BLOCK ? block_append<BLOCK>(O.train, &CI.counters[1], append_scratch)
+                                 : wave_append(O.train, &CI.counters[1]);
+        if (O.train && r < CI.cap) {
+            const float4 lb = ld_stream(&L[slot]);
+            float4* R = CI.records + 5 * size_t(r);
+            R[0] = O.tr[0];
+            R[1] = O.tr[1];
+            R[2] = O.tr[2];
+            R[3] = O.tr[3];
+            R[4] = make_float4(lb.x, lb.y, lb.z, __uint_as_float(kNoRecord));
+            rec = r;
+        }
         const uint32_t qpos = BLOCK ? block_append<BLOCK>(O.query, &CI.counters[0], append_scratch)
                                     : wave_append(O.query, &CI.counters[0]);
         if (O.query && qpos < CI.cap) {
@@ -482,7 +491,7 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
         st_stream(ro, make_float4(O.so.x, O.so.y, O.so.z, 0.0f));
         st_stream(ro + 1, make_float4(P.sun.x, P.sun.y, P.sun.z, __builtin_inff()));
         st_stream(&scontrib[spos], make_float4(O.contrib.x, O.contrib.y, O.contrib.z, __uint_as_float(slot)));
-        if (!O.query && O.rec != kNoRecord) CI.records[5 * size_t(O.rec) + 4].w = __uint_as_float(spos);
+        if (rec != kNoRecord) CI.records[5 * size_t(rec) + 4].w = __uint_as_float(spos);
     }
 }
 
@@ -532,7 +541,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
     if (i == 0) atomicAdd(&ctr->ext_rays, (unsigned long long)n);
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
-    O.alive = O.shadow = O.query = false;
+    O.alive = O.shadow = O.query = O.train = false;
     O.rec = kNoRecord;
     uint32_t slot = 0;
     if constexpr (PARK) {
@@ -600,7 +609,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
             O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
         }
     }
-    emit_vertex<BLOCK>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, append_scratch);
+    emit_vertex<BLOCK>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, L, append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
@@ -631,7 +640,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
     const uint32_t q = blockIdx.x * uint32_t(BLOCK) + threadIdx.x;
     uint32_t n_rec = 0, n_tri = 0;
     ShadeOut O;
-    O.alive = O.shadow = O.query = false;
+    O.alive = O.shadow = O.query = O.train = false;
     O.rec = kNoRecord;
     uint32_t slot = 0;
     Trav<DEC64> S;
@@ -676,7 +685,7 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_resume(
                              S.R.t, S.R.id == kNoHit ? -1 : int(S.R.id), shade, mats, L, CI);
         }
     }
-    emit_vertex<BLOCK>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, append_scratch);
+    emit_vertex<BLOCK>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, L, append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
@@ -698,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t q = base + threadIdx.x;
         ShadeOut O;
-        O.alive = O.shadow = O.query = false;
+        O.alive = O.shadow = O.query = O.train = false;
         O.rec = kNoRecord;
         uint32_t slot = 0;
         if (q < n) {
@@ -709,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_listed(PassParams P, uint32_t 
             slot = __float_as_uint(s.w);
             O = shade_vertex(P, k, rp[0], rp[1], s, h.x, __float_as_int(h.w), shade, mats, L, CI);
         }
-        emit_vertex<0>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats);
+        emit_vertex<0>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, L);
     }
 }
 
