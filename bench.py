@@ -248,6 +248,16 @@ def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2
                     "ms_per_frame": round(ms / frames, 2), "train_samples_last_frame": int(st.train_samples),
                     "cache_queries_last_frame": int(st.cache_queries)}, lin()
 
+        # one untimed cached frame with a throwaway cache first: the context's
+        # query and record buffers are allocated on first use, which is not
+        # what a cold (untrained) cache costs per frame
+        scratch = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, lr=1e-3, seed=2, max_batch=65536)
+        try:
+            ccfg.frame = 150000
+            scratch.render(scene, film, cam, ccfg, train=train_steps, want_loss=False)
+            ctx.synchronize()
+        finally:
+            scratch.close()
         cold = api.RadianceCache(ctx, sd.desc.bounds_min, sd.desc.bounds_max, lr=1e-3, seed=1, max_batch=65536)
         try:
             out["cached_cold"], images["cached_cold"] = run(cold)
