@@ -359,7 +359,9 @@ __device__ __forceinline__ bool training_path(const PassParams& P, uint32_t slot
 // One path vertex: PathTracer::traceRecursive (PathTracer.cpp:25-75) for the
 // hit (t, tri) of ray (r0, r1), plus sky on a miss, sun next-event estimation
 // and Russian roulette.  Radiance goes to L[slot]; the continuation and the
-// shadow ray are returned for the caller to append.
+// shadow ray are returned for the caller to append.  CACHE false: the caller
+// is the camera bounce, whose vertex is never the cache vertex.
+template <bool CACHE = true>
 __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k, float4 r0, float4 r1, float4 s,
                                                  float t, int tri, const float4* __restrict__ shade,
                                                  const DevMaterial* __restrict__ mats, float4* __restrict__ L,
@@ -398,7 +400,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
             O.shadow = true;
         }
     }
-    if (CI.cache_vertex > 0 && k == CI.cache_vertex) {
+    if (CACHE && CI.cache_vertex > 0 && k == CI.cache_vertex) {
         // The cache holds the indirect part of the radiance leaving this
         // vertex (the direct sun is the next-event ray above, for every path).
         const float3 wo = make_float3(-r1.x, -r1.y, -r1.z);
@@ -442,7 +444,7 @@ __device__ __forceinline__ ShadeOut shade_vertex(const PassParams& P, uint32_t k
 // Appends the continuation and shadow ray of a shaded vertex (every lane of
 // the wave calls this at the same point; with BLOCK > 0 every thread of the
 // block, which reserves once per block, else once per wave).
-template <int BLOCK>
+template <int BLOCK, bool CACHE = true>
 __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, const ShadeOut& O, uint32_t slot,
                                             rc_ray* __restrict__ rays_out, float4* __restrict__ st_out,
                                             rc_ray* __restrict__ srays, float4* __restrict__ scontrib,
@@ -450,7 +452,7 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
                                             const float4* __restrict__ shade, const DevMaterial* __restrict__ mats,
                                             const float4* __restrict__ L, uint32_t* append_scratch = nullptr) {
     uint32_t rec = kNoRecord;  // this vertex's training record
-    if (CI.cache_vertex > 0 && k == CI.cache_vertex) {  // uniform: paths end into the cache at this vertex
+    if (CACHE && CI.cache_vertex > 0 && k == CI.cache_vertex) {  // uniform: paths end into the cache at this vertex
         const uint32_t r = BLOCK ? block_append<BLOCK>(O.train, &CI.counters[1], append_scratch)
                                  : wave_append(O.train, &CI.counters[1]);
         if (O.train && r < CI.cap) {
@@ -606,10 +608,11 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
             }
             ovf_list[atomicAdd(&ctr->ovf_count, 1u)] = i;
         } else {
-            O = shade_vertex(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
+            O = shade_vertex<!CAM>(P, k, r0, r1, s, R.t, R.id == kNoHit ? -1 : int(R.id), shade, mats, L, CI);
         }
     }
-    emit_vertex<BLOCK>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, L, append_scratch);
+    emit_vertex<BLOCK, !CAM>(P, k, O, slot, rays_out, st_out, srays, scontrib, ctr, cout, CI, shade, mats, L,
+                             append_scratch);
     if (COUNT) {
         wave_add(&ctr->node_records, n_rec);
         wave_add(&ctr->tri_tests, n_tri);
