@@ -198,10 +198,24 @@ int rc_launch_film_add(rc_ctx* c, rc_film* f, const rc_render_cfg* cfg, uint32_t
 void rc_whitted_release(rc_ctx* c);
 
 // Radiance-cache plumbing of a frame (all null / 0 when the cache is off).
+// Radiance-cache device counters (uint32 words).  Words that kernels hit at
+// the same time sit one 128-byte line apart: every block's reservation
+// atomics on the query and the record counter, and every wave's read of the
+// training stride (sharing a line with those atomics made the cache-vertex
+// bounce several times slower).
+constexpr uint32_t kCcQueries = 0;        // queries of the pass
+constexpr uint32_t kCcRecords = 32;       // training records of the pass
+constexpr uint32_t kCcSamples = 64;       // training samples of the frame
+constexpr uint32_t kCcFrameQueries = 65;  // cache queries of the frame (kCcSamples + 1: copied out together)
+constexpr uint32_t kCcLastSamples = 66;   // training samples of the last finished frame
+constexpr uint32_t kCcSteps = 67;         // Adam steps taken
+constexpr uint32_t kCcStride = 96;        // adaptive training-path stride
+constexpr uint32_t kCacheCounters = 128;
+
 struct CacheIO {
     float4* queries;        // 4 float4 per query: (p, slot), (w_o, thr.r), (n, thr.g), (albedo, thr.b)
     float4* records;        // 5 float4 per training vertex: the query layout, then (L before, 0)
-    uint32_t* counters;     // [0] queries, [1] records of the pass; [2] training samples, [3] queries of the frame
+    uint32_t* counters;     // kCacheCounters words (kCc*)
     float* batch_q;         // 12 floats per training sample
     float* batch_t;         // 3 floats per training sample
     uint32_t batch_cap;

@@ -507,16 +507,16 @@ __global__ void k_cache_step_done(uint32_t n_host, const uint32_t* __restrict__ 
 }
 
 // End of a rendered frame: the training-sample count is frozen for the
-// optimizer steps (counters[4]; the next frame clears counters[2]) and, when
+// optimizer steps (kCcLastSamples; the next frame clears kCcSamples) and, when
 // adaptive, the training-path stride is steered towards 0.9 max_batch
-// samples per frame (counters[5]).
+// samples per frame (kCcStride).
 __global__ void k_cache_frame_end(uint32_t* __restrict__ ctr, uint32_t max_batch, uint32_t adaptive) {
-    const uint32_t got = ctr[2];
-    ctr[4] = min(got, max_batch);
+    const uint32_t got = ctr[kCcSamples];
+    ctr[kCcLastSamples] = min(got, max_batch);
     if (adaptive) {
-        const float cur = float(ctr[5]);
+        const float cur = float(ctr[kCcStride]);
         const float next = got ? cur * float(got) / (0.9f * float(max_batch)) : cur * 0.5f;
-        ctr[5] = uint32_t(fminf(fmaxf(next, 1.0f), float(1u << 20)));
+        ctr[kCcStride] = uint32_t(fminf(fmaxf(next, 1.0f), float(1u << 20)));
     }
 }
 
@@ -533,11 +533,7 @@ __global__ void k_zero_u(uint32_t* p, uint32_t n) {
 
 }  // namespace
 
-// Device counters of a cache (uint32): [0] queries and [1] training records of
-// the current pass, [2] training samples and [3] cache queries of the current
-// frame, [4] training samples of the last finished frame, [5] adaptive
-// training-path stride, [6] Adam steps taken.
-constexpr int kCacheCounters = 8;
+// Device counters of a cache: kCacheCounters words laid out as kCc* says.
 
 struct rc_cache {
     rc_ctx* ctx = nullptr;
@@ -551,7 +547,7 @@ struct rc_cache {
     v4s* bwd = nullptr;
     float* loss = nullptr;
     uint32_t* counters = nullptr;  // kCacheCounters words, read and written by the kernels
-    bool stride_set = false;       // counters[5] holds an adaptive stride
+    bool stride_set = false;       // counters[kCcStride] holds an adaptive stride
     DevBuf act, dz, partial, queries, targets, out;
     DevBuf pass_q, pass_r;  // queries and training records of one render pass
     // The optimizer steps after a rendered frame run on their own stream,
@@ -609,8 +605,8 @@ int adam_device(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t n_host, const u
                 uint32_t j) {
     k_cache_adam<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, st>>>(
         k->params, k->m, k->v, k->grad, k->cfg.lr, k->cfg.beta1, k->cfg.beta2, k->cfg.adam_eps, n_host, n_dev, steps, j,
-        k->counters + 6);
-    k_cache_step_done<<<1, 1, 0, st>>>(n_host, n_dev, steps, j, k->counters + 6);
+        k->counters + kCcSteps);
+    k_cache_step_done<<<1, 1, 0, st>>>(n_host, n_dev, steps, j, k->counters + kCcSteps);
     const uint32_t n = (kFwdTiles + kBwdTiles) * 64;
     k_cache_pack<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>(k->params, k->fwd, k->bwd);
     RC_TRY_HIP(hipGetLastError());
@@ -650,15 +646,15 @@ int rc_cache_frame_io(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t cache_ver
     io->batch_cap = k->cfg.max_batch;
     io->cache_vertex = cache_vertex;
     io->train_stride = k->cfg.train_stride;
-    io->stride_dev = k->counters + 5;
+    io->stride_dev = k->counters + kCcStride;
     io->cap = uint32_t(pass_paths);
-    k_zero_u<<<1, 64, 0, st>>>(k->counters, 4);
+    k_zero_u<<<1, 64, 0, st>>>(k->counters, kCcFrameQueries + 1);  // the pass and frame counts
     if (k->cfg.train_stride == 0 && !k->stride_set) {
         // adaptive: the first frame as if every training path reached the
         // cache vertex; later frames steer the count (k_cache_frame_end)
         const uint32_t first =
             uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(frame_samples / k->cfg.max_batch, 1u << 20)));
-        k_set_u32<<<1, 1, 0, st>>>(k->counters + 5, first);
+        k_set_u32<<<1, 1, 0, st>>>(k->counters + kCcStride, first);
         k->stride_set = true;
     }
     RC_TRY_HIP(hipGetLastError());
@@ -670,7 +666,7 @@ rc_ctx* rc_cache_owner(const rc_cache* k) { return k->ctx; }
 int rc_cache_frame_end(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t steps, uint32_t counts[2]) {
     k_cache_frame_end<<<1, 1, 0, st>>>(k->counters, k->cfg.max_batch, k->cfg.train_stride == 0 ? 1u : 0u);
     uint32_t h[2];  // training samples, cache queries of the frame
-    RC_TRY_HIP(hipMemcpyAsync(h, k->counters + 2, sizeof(h), hipMemcpyDeviceToHost, st));
+    RC_TRY_HIP(hipMemcpyAsync(h, k->counters + kCcSamples, sizeof(h), hipMemcpyDeviceToHost, st));
     k->last_steps = steps;
     if (steps > 0) {
         // the optimizer steps wait for the frame's samples on the cache's stream
@@ -680,8 +676,8 @@ int rc_cache_frame_end(rc_ctx* c, hipStream_t st, rc_cache* k, uint32_t steps, u
         k_zero_f<<<1, kThreads, 0, k->train_stream>>>(k->loss, 1);
         for (uint32_t j = 0; j < steps; ++j) {
             RC_TRY(grad_device(c, k->train_stream, k, k->queries.as<float>(), k->targets.as<float>(), k->cfg.max_batch,
-                               k->counters + 4, steps, j));
-            RC_TRY(adam_device(c, k->train_stream, k, k->cfg.max_batch, k->counters + 4, steps, j));
+                               k->counters + kCcLastSamples, steps, j));
+            RC_TRY(adam_device(c, k->train_stream, k, k->cfg.max_batch, k->counters + kCcLastSamples, steps, j));
         }
         RC_TRY_HIP(hipEventRecord(k->ev_trained, k->train_stream));
         k->pending = true;
@@ -791,7 +787,7 @@ extern "C" int rc_cache_set_params(rc_cache* k, const float* params) {
     RC_TRY_HIP(hipMemcpyAsync(k->params, params, kParams * 4, hipMemcpyHostToDevice, c->stream));
     k_zero_f<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, c->stream>>>(k->m, kParams);
     k_zero_f<<<(kParams + kThreads - 1) / kThreads, kThreads, 0, c->stream>>>(k->v, kParams);
-    k_set_u32<<<1, 1, 0, c->stream>>>(k->counters + 6, 0u);  // Adam step count
+    k_set_u32<<<1, 1, 0, c->stream>>>(k->counters + kCcSteps, 0u);  // Adam step count
     RC_TRY(cache_pack(c, k));
     RC_TRY_HIP(hipStreamSynchronize(c->stream));
     return RC_OK;

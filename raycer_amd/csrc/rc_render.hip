@@ -453,8 +453,8 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
                                             const float4* __restrict__ L, uint32_t* append_scratch = nullptr) {
     uint32_t rec = kNoRecord;  // this vertex's training record
     if (CACHE && CI.cache_vertex > 0 && k == CI.cache_vertex) {  // uniform: paths end into the cache at this vertex
-        const uint32_t r = BLOCK ? block_append<BLOCK>(O.train, &CI.counters[1], append_scratch)
-                                 : wave_append(O.train, &CI.counters[1]);
+        const uint32_t r = BLOCK ? block_append<BLOCK>(O.train, &CI.counters[kCcRecords], append_scratch)
+                                 : wave_append(O.train, &CI.counters[kCcRecords]);
         if (O.train && r < CI.cap) {
             const float4 lb = ld_stream(&L[slot]);
             float4* R = CI.records + 5 * size_t(r);
@@ -465,8 +465,8 @@ __device__ __forceinline__ void emit_vertex(const PassParams& P, uint32_t k, con
             R[4] = make_float4(lb.x, lb.y, lb.z, __uint_as_float(kNoRecord));
             rec = r;
         }
-        const uint32_t qpos = BLOCK ? block_append<BLOCK>(O.query, &CI.counters[0], append_scratch)
-                                    : wave_append(O.query, &CI.counters[0]);
+        const uint32_t qpos = BLOCK ? block_append<BLOCK>(O.query, &CI.counters[kCcQueries], append_scratch)
+                                    : wave_append(O.query, &CI.counters[kCcQueries]);
         if (O.query && qpos < CI.cap) {
             const float4 sh = shade[O.rec];
             const float4 rho = mats[__float_as_uint(sh.w)].diffuse;
@@ -874,20 +874,20 @@ __global__ __launch_bounds__(kBlock) void k_shadow_apply_listed(const float4* __
 
 // Training samples of a pass: query + target (L_final - L_before) / T for the
 // recorded training vertices, appended to the frame's batch (capped); also
-// adds the pass's query count to the frame total counters[3].
+// adds the pass's query count to the frame total (kCcFrameQueries).
 __global__ __launch_bounds__(kBlock) void k_cache_targets(const float4* __restrict__ records,
                                                           uint32_t* __restrict__ counters, uint32_t pass_cap,
                                                           const float4* __restrict__ L, float* __restrict__ batch_q,
                                                           float* __restrict__ batch_t, uint32_t batch_cap) {
-    const uint32_t n = min(counters[1], pass_cap);
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[3] += min(counters[0], pass_cap);
+    const uint32_t n = min(counters[kCcRecords], pass_cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kCcFrameQueries] += min(counters[kCcQueries], pass_cap);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const float4* R = records + 5 * size_t(i);
         float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], r4 = R[4];
         float4 lf = L[__float_as_uint(r0.w)];
         float T[3] = {r1.w, r2.w, r3.w};
         float d[3] = {lf.x - r4.x, lf.y - r4.y, lf.z - r4.z};
-        uint32_t pos = atomicAdd(&counters[2], 1u);
+        uint32_t pos = atomicAdd(&counters[kCcSamples], 1u);
         if (pos >= batch_cap) continue;
         float* q = batch_q + size_t(pos) * 12;
         q[0] = r0.x; q[1] = r0.y; q[2] = r0.z;
@@ -905,7 +905,7 @@ __global__ __launch_bounds__(kBlock) void k_cache_nee_fix(float4* __restrict__ r
                                                           const uint32_t* __restrict__ counters, uint32_t pass_cap,
                                                           const float4* __restrict__ scontrib,
                                                           const uint8_t* __restrict__ occ) {
-    const uint32_t n = min(counters[1], pass_cap);
+    const uint32_t n = min(counters[kCcRecords], pass_cap);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         float4 r4 = records[5 * size_t(i) + 4];
         const uint32_t spos = __float_as_uint(r4.w);
@@ -916,9 +916,9 @@ __global__ __launch_bounds__(kBlock) void k_cache_nee_fix(float4* __restrict__ r
 }
 
 __global__ void k_cache_reset(uint32_t* counters, uint32_t which) {
-    counters[0] = 0;
-    counters[1] = 0;
-    if (which) counters[2] = 0;
+    counters[kCcQueries] = 0;
+    counters[kCcRecords] = 0;
+    if (which) counters[kCcSamples] = 0;
 }
 
 // Zeroes the queue the next shade writes, the shadow queue and the overflow list.
@@ -1457,7 +1457,7 @@ int rc_enqueue_frame(rc_ctx* c, FrameBuffers& B, hipStream_t st, const rc_scene*
         if (CI.cache_vertex) {
             // paths that stopped at the cache vertex: add thr * cache(x, w_o)
             if (timed) RC_TRY(c->prof_begin(&ev));
-            RC_TRY(rc_cache_apply_device(c, st, cache, CI.queries, &CI.counters[0], CI.cap, L));
+            RC_TRY(rc_cache_apply_device(c, st, cache, CI.queries, &CI.counters[kCcQueries], CI.cap, L));
             if (timed) RC_TRY(c->prof_end(ev, PK_CACHE));
             k_cache_targets<<<grid_small, kBlock, 0, st>>>(CI.records, CI.counters, CI.cap, L, CI.batch_q, CI.batch_t,
                                                           CI.batch_cap);
