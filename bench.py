@@ -63,16 +63,17 @@ def log(msg: str) -> None:
 
 
 def cpu_reference(sd, cam, cfg, subset: int = 64, seed: int = 7, ref_spp: int = 4096, split_spp: int = 64,
-                  cache_vertex: int = 2) -> dict:
+                  cache_vertices=(1, 2)) -> dict:
     """The CPU leg: the FP64 oracle (restatement of the reference's OpenMP CPU
     path, built -O3 -ffast-math, unordered traversal as the reference) renders
     a seeded 1/subset of the pixels at ref_spp samples (n x n jittered, n =
     sqrt(ref_spp)).  Timed, that is
     the CPU baseline; the image is the reference the GPU images are scored
-    against.  Then the split-tail estimate on the same pixels at split_spp:
-    the share of a split_spp-sample pixel estimate's variance that the light
-    gathered after vertex cache_vertex's next-event ray carries, which is
-    what a perfect radiance cache at that vertex would remove."""
+    against.  Then, for each of cache_vertices, the split-tail estimate on the
+    same pixels at split_spp: the share of a split_spp-sample pixel
+    estimate's variance that the light gathered after that vertex's
+    next-event ray carries, which is what a perfect radiance cache at that
+    vertex would remove."""
     import oracle as orc
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -88,14 +89,16 @@ def cpu_reference(sd, cam, cfg, subset: int = 64, seed: int = 7, ref_spp: int = 
     dt = time.perf_counter() - t0
     samples = len(ids) * ref_spp
     ref = st[:, 2:5] / st[:, 0:1]
-    scfg = orc.render_cfg(cfg, traversal=0)
-    scfg.spp = split_spp
-    scfg.frame = 600000
-    scfg.cache_vertex = cache_vertex
+    tail_var = {}
     t1 = time.perf_counter()
-    sp = orc.render_split_tail(ob, sd.tri_material, sd.materials, cam, scfg, ids, threads=threads)
+    for cv in cache_vertices:
+        scfg = orc.render_cfg(cfg, traversal=0)
+        scfg.spp = split_spp
+        scfg.frame = 600000
+        scfg.cache_vertex = cv
+        sp = orc.render_split_tail(ob, sd.tri_material, sd.materials, cam, scfg, ids, threads=threads)
+        tail_var[cv] = float(np.mean(sp[:, 4:7] / (2.0 * sp[:, 0:1] ** 2)))
     dt_split = time.perf_counter() - t1
-    tail_var = float(np.mean(sp[:, 4:7] / (2.0 * sp[:, 0:1] ** 2)))
     cpu_model = "unknown"
     try:
         with open("/proc/cpuinfo") as fh:
@@ -183,10 +186,10 @@ BF16_PEAK_TFLOPS = 2500.0
 
 def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2, plain_spp: int = 64,
                  ref_spp: int = 4096, pretrain_frames: int = 128, pretrain_spp: int = 4, pretrain_steps: int = 4,
-                 train_steps: int = 1) -> dict:
+                 train_steps: int = 1, cached_spp: int = 0, gpu_ref=None) -> dict:
     """The plain renderer and the radiance-cached renderer given the same wall
     time.  The budget is one plain frame of plain_spp jittered samples.  A
-    cached frame is the same jittered plain_spp-sample pattern as one
+    cached frame is a jittered cached_spp-sample pattern (0: plain_spp) as one
     wavefront pass whose paths end into the cache at vertex cache_vertex (after
     two bounces for 2), followed by train_steps Adam steps on its training
     samples, which overlap the next frame's traversal.  Cached frames are
@@ -194,7 +197,8 @@ def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2
     within the budget (at least one frame).  Warm: the cache was trained on
     pretrain_frames earlier frames (as in an animation); cold: a new cache.
     Returns the linear images (scored against the CPU reference later), the
-    times, and a full-image MSE against a ref_spp GPU render."""
+    times, and a full-image MSE against a ref_spp GPU render (gpu_ref: that
+    image from an earlier call, not rendered again)."""
     from raycer_amd import api
 
     cfg = api.RenderCfg.from_buffer_copy(base_cfg)
@@ -210,11 +214,13 @@ def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2
            "train_steps_per_frame": train_steps}
     images = {}
     try:
-        film.clear()
-        for f in range(ref_spp // plain_spp):
-            cfg.frame = 100000 + f
-            ctx.render(scene, film, cam, cfg)
-        images["gpu_ref"] = lin()
+        if gpu_ref is None:
+            film.clear()
+            for f in range(ref_spp // plain_spp):
+                cfg.frame = 100000 + f
+                ctx.render(scene, film, cam, cfg)
+            gpu_ref = lin()
+        images["gpu_ref"] = gpu_ref
         film.clear()
         cfg.frame = 7
         ctx.synchronize()
@@ -226,6 +232,8 @@ def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2
         images["plain"] = lin()
         ccfg = api.RenderCfg.from_buffer_copy(cfg)
         ccfg.cache_vertex = cache_vertex
+        ccfg.spp = cached_spp or plain_spp
+        out["cached_spp_per_frame"] = int(ccfg.spp)
         frame = [200000]
 
         def run(cache):
@@ -244,7 +252,7 @@ def cached_forms(ctx, scene, sd, base_cfg, scale: int = 1, cache_vertex: int = 2
             ctx.synchronize()
             ms = (time.perf_counter() - t0) * 1e3
             st = ctx.stats()
-            return {"frames": frames, "spp": frames * plain_spp, "ms": round(ms, 2),
+            return {"frames": frames, "spp": frames * int(ccfg.spp), "ms": round(ms, 2),
                     "ms_per_frame": round(ms / frames, 2), "train_samples_last_frame": int(st.train_samples),
                     "cache_queries_last_frame": int(st.cache_queries)}, lin()
 
@@ -328,7 +336,8 @@ def score_against_cpu(forms: dict, cpu: dict) -> None:
         "ratio_warm": round(m["cached_warm"] / plain, 4),
         "ratio_cold": round(m["cached_cold"] / plain, 4),
     }
-    tail = cpu["tail_var"] * cpu["split_spp"] / out["plain_spp"]  # the same per-sample variance at plain_spp
+    # the same per-sample variance at plain_spp
+    tail = cpu["tail_var"][out["cache_vertex"]] * cpu["split_spp"] / out["plain_spp"]
     warm = out["cached_warm"]
     time_ratio = warm["ms_per_frame"] / out["budget_ms"]
     out["variance_bound"] = {
@@ -676,15 +685,19 @@ def main() -> int:
         log("64-scene batch (autotuned plans and branches, one captured graph, relaunched)")
         out["batch_64_scenes"] = batch_c4(ctx)
         log(f"batch: {json.dumps(out['batch_64_scenes'])}")
-    forms = None
+    forms = forms2 = None
     if a.mse != "off":
         log(f"plain and radiance-cached forms at equal time ({a.mse})")
         scale = 1 if a.mse == "full" else 4
-        forms = cached_forms(ctx, scene, sd, cfg, scale=scale)
+        # the cache at vertex 1 (the first bounce's hit), and beside it at
+        # vertex 2, which the counter profile of the learned kernels covers
+        forms = cached_forms(ctx, scene, sd, cfg, scale=scale, cache_vertex=1)
+        forms2 = cached_forms(ctx, scene, sd, cfg, scale=scale, cache_vertex=2, gpu_ref=forms["images"]["gpu_ref"])
         out["equal_time_mse"] = forms["summary"]
+        out["equal_time_mse"]["cache_vertex_2"] = forms2["summary"]
         if PROFILE_MFMA.exists():
             try:  # the MFMA counters of the same kernels (rocprofv3, profiles/)
-                out["equal_time_mse"]["mlp_kernels"]["mfma_counters"] = json.loads(PROFILE_MFMA.read_text())["kernels"]
+                forms2["summary"]["mlp_kernels"]["mfma_counters"] = json.loads(PROFILE_MFMA.read_text())["kernels"]
             except (OSError, ValueError, KeyError):
                 pass
         log(f"cached forms: {json.dumps(forms['summary'])}")
@@ -711,9 +724,11 @@ def main() -> int:
         cpu = cpu_reference(sd, cam, ocfg)
         out["cpu_baseline"] = cpu["baseline"]
         if forms is not None:
-            score_against_cpu(forms, cpu)
-            log(f"vs CPU reference: {json.dumps(forms['summary']['mse_vs_cpu_ref'])}")
-            log(f"variance bound: {json.dumps(forms['summary']['variance_bound'])}")
+            for f in (forms, forms2):
+                score_against_cpu(f, cpu)
+                log(f"cache vertex {f['summary']['cache_vertex']} vs CPU reference: "
+                    f"{json.dumps(f['summary']['mse_vs_cpu_ref'])}")
+                log(f"variance bound: {json.dumps(f['summary']['variance_bound'])}")
         out["c1_mse_vs_cpu_ref"] = c1_vs_cpu(*c1)
         log(f"C1 vs CPU reference: {json.dumps(out['c1_mse_vs_cpu_ref'])}")
     else:

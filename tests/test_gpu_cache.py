@@ -128,10 +128,11 @@ def terrain_cfg(sd, w=96, h=96, spp=16, **kw):
     return cfg_for(sd, w, h, spp, **kw)
 
 
-def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx):
-    """A cache that returns exactly 0: every path ends after its vertex-2
-    next-event ray, which (without sky) is what a plain render with at most 4
-    rays per path gathers — bit for bit."""
+@pytest.mark.parametrize("cv", [1, 2])
+def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx, cv):
+    """A cache that returns exactly 0: every path ends after its vertex-cv
+    next-event ray, which (without sky) is what a plain render with at most
+    cv + 2 rays per path gathers — bit for bit."""
     sd, s, _ = scene(ctx, "terrain")
     b0, b1 = np.asarray(sd.desc.bounds_min), np.asarray(sd.desc.bounds_max)
     cache = api.RadianceCache(ctx, b0, b1, train_stride=0xFFFFFFFF)
@@ -139,7 +140,6 @@ def test_cached_render_with_a_zero_cache_equals_the_truncated_render(ctx):
     p[cr.OFF_W[4]:cr.OFF_B[4]] = 0.0
     p[cr.OFF_B[4]:] = -200.0  # exp underflows to exactly 0 in FP32
     cache.set_params(p)
-    cv = 2
     cfg = terrain_cfg(sd, cache_vertex=cv)
     cfg.sky[0] = cfg.sky[1] = cfg.sky[2] = 0.0
     film = ctx.film(cfg.width, cfg.height)
