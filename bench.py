@@ -689,15 +689,15 @@ def main() -> int:
     if a.mse != "off":
         log(f"plain and radiance-cached forms at equal time ({a.mse})")
         scale = 1 if a.mse == "full" else 4
-        # the cache at vertex 1 (the first bounce's hit), and beside it at
-        # vertex 2, which the counter profile of the learned kernels covers
-        forms = cached_forms(ctx, scene, sd, cfg, scale=scale, cache_vertex=1)
-        forms2 = cached_forms(ctx, scene, sd, cfg, scale=scale, cache_vertex=2, gpu_ref=forms["images"]["gpu_ref"])
+        # paths end into the cache after two bounces (vertex 2); beside it the
+        # same cache one bounce earlier (vertex 1, the first bounce's hit)
+        forms = cached_forms(ctx, scene, sd, cfg, scale=scale, cache_vertex=2)
+        forms2 = cached_forms(ctx, scene, sd, cfg, scale=scale, cache_vertex=1, gpu_ref=forms["images"]["gpu_ref"])
         out["equal_time_mse"] = forms["summary"]
-        out["equal_time_mse"]["cache_vertex_2"] = forms2["summary"]
+        out["equal_time_mse"]["cache_vertex_1"] = forms2["summary"]
         if PROFILE_MFMA.exists():
             try:  # the MFMA counters of the same kernels (rocprofv3, profiles/)
-                forms2["summary"]["mlp_kernels"]["mfma_counters"] = json.loads(PROFILE_MFMA.read_text())["kernels"]
+                forms["summary"]["mlp_kernels"]["mfma_counters"] = json.loads(PROFILE_MFMA.read_text())["kernels"]
             except (OSError, ValueError, KeyError):
                 pass
         log(f"cached forms: {json.dumps(forms['summary'])}")
