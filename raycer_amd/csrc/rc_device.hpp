@@ -1017,4 +1017,116 @@ __device__ __forceinline__ TraceResult traverse(const NodeT* __restrict__ nodes,
 #endif
 }
 
+#if RC_QNODE
+// Closest hit of a wave of nearly coherent rays traversed as one packet (the
+// camera bounce, whose waves each hold one pixel's samples).  The wave
+// follows one stack and visits a child when any of its lanes enters it, so
+// every lane visits a superset of the nodes its own traversal would; the
+// exact-tie rule of test_leaf makes each answer independent of the order and
+// of the extra visits, so the hits are those of traverse().  Children are
+// taken nearest first by the first active lane's entry distances.  The
+// stack entries are wave-uniform; each lane keeps its own copy in its LDS
+// row (entry k at stack[k * stride]), up to max_depth entries, beyond which
+// every lane of the wave reports an overflow.  Every active lane of the wave
+// calls this (the loop and its exit are uniform).
+template <bool COUNT>
+__device__ __forceinline__ TraceResult traverse_packet(const QNode* __restrict__ nodes,
+                                                       const TriRec* __restrict__ tris, float3 o, float3 d,
+                                                       float tmin, float tmax, uint32_t* stack, int stride,
+                                                       int max_depth, uint32_t& n_rec, uint32_t& n_tri) {
+    const float INF = __builtin_inff();
+    const float3 inv = safe_inv(d);
+    tmin = fmaxf(tmin, 0.0f);
+    float best = tmax;
+    float cull = initial_cull<false>(tmax);
+    TraceResult R{tmax, 0.0f, 0.0f, kNoHit, false, 1.0f};
+    uint32_t cur = 0;  // root: interior record 0 (wave-uniform, as sp)
+    int sp = 0;
+    for (;;) {
+        if (!(cur & kLeafTag)) {
+            const uint4* p = reinterpret_cast<const uint4*>(nodes + cur);
+            const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+            if (COUNT) n_rec += kWidth;
+            const float ax = __uint_as_float((q0.w & 0xFFu) << 23) * inv.x;
+            const float ay = __uint_as_float(((q0.w >> 8) & 0xFFu) << 23) * inv.y;
+            const float az = __uint_as_float(((q0.w >> 16) & 0xFFu) << 23) * inv.z;
+            const float bx = (__uint_as_float(q0.x) - o.x) * inv.x;
+            const float by = (__uint_as_float(q0.y) - o.y) * inv.y;
+            const float bz = (__uint_as_float(q0.z) - o.z) * inv.z;
+            const uint32_t nx = inv.x < 0.0f ? q1.w : q1.x, fx = inv.x < 0.0f ? q1.x : q1.w;
+            const uint32_t ny = inv.y < 0.0f ? q2.x : q1.y, fy = inv.y < 0.0f ? q1.y : q2.x;
+            const uint32_t nz = inv.z < 0.0f ? q2.y : q1.z, fz = inv.z < 0.0f ? q1.z : q2.y;
+            auto slab_q = [&](int k, uint32_t e) {
+                const float tnx = fmaf(float((nx >> (8 * k)) & 0xFFu), ax, bx);
+                const float tfx = fmaf(float((fx >> (8 * k)) & 0xFFu), ax, bx);
+                const float tny = fmaf(float((ny >> (8 * k)) & 0xFFu), ay, by);
+                const float tfy = fmaf(float((fy >> (8 * k)) & 0xFFu), ay, by);
+                const float tnz = fmaf(float((nz >> (8 * k)) & 0xFFu), az, bz);
+                const float tfz = fmaf(float((fz >> (8 * k)) & 0xFFu), az, bz);
+                const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+                const float tf = fminf(fminf(tfx, tfy), fminf(tfz, cull));
+                return tn <= tf * 1.000002f && e != kEmptyChild ? tn : INF;
+            };
+            const float t0 = slab_q(0, q3.x), t1 = slab_q(1, q3.y), t2 = slab_q(2, q3.z), t3 = slab_q(3, q3.w);
+            // per child: entered by any lane (m), the first active lane's
+            // distance (r) and the entry (e), all wave-uniform
+            uint32_t m0 = __ballot(t0 < INF) != 0, m1 = __ballot(t1 < INF) != 0;
+            uint32_t m2 = __ballot(t2 < INF) != 0, m3 = __ballot(t3 < INF) != 0;
+            float r0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(t0)));
+            float r1 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(t1)));
+            float r2 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(t2)));
+            float r3 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(t3)));
+            // the entry of an entered child, else kEmptyChild (which sorts
+            // with the misses)
+            uint32_t e0 = m0 ? __builtin_amdgcn_readfirstlane(q3.x) : kEmptyChild;
+            uint32_t e1 = m1 ? __builtin_amdgcn_readfirstlane(q3.y) : kEmptyChild;
+            uint32_t e2 = m2 ? __builtin_amdgcn_readfirstlane(q3.z) : kEmptyChild;
+            uint32_t e3 = m3 ? __builtin_amdgcn_readfirstlane(q3.w) : kEmptyChild;
+            order2(r0, e0, r1, e1);
+            order2(r2, e2, r3, e3);
+            order2(r0, e0, r2, e2);
+            order2(r1, e1, r3, e3);
+            order2(r1, e1, r2, e2);
+            // far to near: the nearest entered child is visited next, the
+            // others pushed farthest first
+            uint32_t next = kEmptyChild;
+            bool overflow = false;
+            auto take = [&](uint32_t e) {
+                if (e == kEmptyChild) return;
+                if (next != kEmptyChild) {
+                    if (sp >= max_depth) {
+                        overflow = true;
+                        return;
+                    }
+                    stack[sp * stride] = next;
+                    ++sp;
+                }
+                next = e;
+            };
+            take(e3);
+            take(e2);
+            take(e1);
+            take(e0);
+            if (overflow) {
+                R.overflow = true;
+                return R;
+            }
+            if (next != kEmptyChild) {
+                cur = next;
+                continue;
+            }
+        } else {
+            // every active lane tests the leaf: a lane whose ray misses the
+            // leaf's box finds no hit there it would not also find through
+            // its own traversal
+            test_leaf<false, COUNT, false>(tris, cur & 0x03FFFFFFu, (cur >> 26) & 0x1Fu, o, d, tmin, 0.0f, best, cull,
+                                           R, n_tri);
+        }
+        if (sp == 0) return R;
+        --sp;
+        cur = __builtin_amdgcn_readfirstlane(stack[sp * stride]);
+    }
+}
+#endif
+
 }  // namespace rcd

@@ -249,6 +249,15 @@ __global__ __launch_bounds__(kBlock) void k_camera(PassParams P, rc_ray* __restr
 
 // 1: the camera bounce makes its camera rays in the extension kernel (no
 // k_camera pass over the queues).
+// RC_PACKET_CAMERA = 1: the camera bounce's waves (one pixel's samples each,
+// with pixel-major slots) traverse as packets (traverse_packet); 0 (the
+// default): each lane on its own, as the other bounces.  On the 2M-triangle
+// terrain at 64 samples per pass the packets made the camera bounce 1.6 ms
+// slower (12.5 against 14.1 ms): the union of a pixel's 64 traversals costs
+// more visits than the per-lane traversal's leaf postponing and culling.
+#ifndef RC_PACKET_CAMERA
+#define RC_PACKET_CAMERA 0
+#endif
 #ifndef RC_FUSED_CAMERA
 #define RC_FUSED_CAMERA 1
 #endif
@@ -597,8 +606,17 @@ __global__ __launch_bounds__(BLOCK, kMinWaves<DEC64>) void k_extend_shade(
             s = ld_stream(&st[i]);
         }
         slot = __float_as_uint(s.w);
-        TraceResult R = traverse<false, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x,
-                                                      BLOCK, P.stack_limit, n_rec, n_tri, top, ntop);
+        TraceResult R;
+#if RC_QNODE && RC_PACKET_CAMERA
+        if constexpr (CAM && !DEC64) {  // a wave holds one pixel's samples: one packet
+            R = traverse_packet<COUNT>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x, BLOCK,
+                                       min(P.stack_limit, kLdsRows), n_rec, n_tri);
+        } else
+#endif
+        {
+            R = traverse<false, COUNT, DEC64>(nodes, tris, f3(r0), f3(r1), r0.w, r1.w, stack + threadIdx.x, BLOCK,
+                                              P.stack_limit, n_rec, n_tri, top, ntop);
+        }
         if (R.overflow) {
             if (CAM) {  // the deep-stack kernel and k_shade_listed read the entry
                 float4* rp = reinterpret_cast<float4*>(cam_rays + i);
