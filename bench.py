@@ -545,7 +545,9 @@ def main() -> int:
     ctx.set_tuning(spp_per_pass=a.spp_per_pass, park_steps=a.park_steps)
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
     probes = {"l2_4MiB": ctx.read_bandwidth(4 << 20, 512), "infinity_cache_128MiB": ctx.read_bandwidth(128 << 20, 16),
-              "hbm_4GiB": ctx.read_bandwidth(4 << 30, 1)}
+              "hbm_4GiB": ctx.read_bandwidth(4 << 30, 1),
+              # 64-byte records at distinct random addresses of 4 GiB (node-sized, no reuse)
+              "gather_64B_4GiB": ctx.gather_bandwidth(4 << 30, 1 << 24, 3)}
     log(f"streaming copy {copy_gbs:.0f} GB/s, reads {json.dumps({k: round(v) for k, v in probes.items()})} GB/s")
 
     # untimed counting frame: node/triangle fetches of the same rays
@@ -646,6 +648,10 @@ def main() -> int:
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "achievable_copy_GBs": round(copy_gbs, 1),
             "frac_of_achievable_copy": round(achieved / copy_gbs, 4),
+            # every HBM read is a 128-byte line request (rocprofv3 TCC_EA0_RDREQ_128B, profiles/), so
+            # the scattered-read probe's 64-byte records move two bytes of lines per byte requested
+            "achievable_gather_lines_GBs": round(2.0 * probes["gather_64B_4GiB"], 1),
+            "frac_of_achievable_gather": round(achieved / (2.0 * probes["gather_64B_4GiB"]), 4),
             "demand": {
                 "bytes_per_ray": round(bytes_per_ray, 1),
                 "work_per_ray": {"wide_node_visits": round(cst.node_records / 4 / ext_rays, 2),

@@ -246,6 +246,13 @@ int rc_copy_bandwidth(rc_ctx* ctx, size_t bytes, int iters, double* gbps);
  * Infinity Cache (256 MiB) is served from it; returns the best read GB/s over
  * three grid sizes. */
 int rc_read_bandwidth(rc_ctx* ctx, size_t bytes, int iters, double* gbps);
+/* Scattered-read probe (no reference counterpart): `records` threads each
+ * read one 64-byte record (a wide BVH node's size) at distinct pseudo-random
+ * records of a buffer of the largest power-of-two record count within
+ * `bytes`, `iters` launches; returns the requested GB/s.  Its rocprofv3
+ * FETCH_SIZE against the records' 64 bytes shows how the counter scales for
+ * scattered 64-byte reads. */
+int rc_gather_bandwidth(rc_ctx* ctx, size_t bytes, size_t records, int iters, double* gbps);
 
 /* ------------------------------------------------------------ scene */
 /* CLTracer::initializeBuffers / CLScene::readSceneFull (CLScene.cpp:72-263):

@@ -289,6 +289,7 @@ def lib() -> C.CDLL:
             "rc_autotune_batch_plan": [vp, vp, vp, vp, sz, C.c_char_p, vp, C.POINTER(u32)],
             "rc_copy_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
             "rc_read_bandwidth": [vp, sz, i32, C.POINTER(C.c_double)],
+            "rc_gather_bandwidth": [vp, sz, sz, i32, C.POINTER(C.c_double)],
             "rc_render_progress": [vp, vp, vp, C.POINTER(Camera), C.POINTER(RenderCfg), vp, vp],
             "rc_cache_create": [vp, C.POINTER(CacheCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(vp)],
             "rc_cache_free": [vp],
@@ -633,6 +634,12 @@ class Context:
     def copy_bandwidth(self, nbytes: int = 1 << 30, iters: int = 20) -> float:
         g = C.c_double()
         check(lib().rc_copy_bandwidth(self.h, nbytes, iters, C.byref(g)))
+        return g.value
+
+    def gather_bandwidth(self, nbytes: int = 4 << 30, records: int = 1 << 24, iters: int = 3) -> float:
+        """rc_gather_bandwidth: GB/s of 64-byte reads at distinct random records."""
+        g = C.c_double()
+        check(lib().rc_gather_bandwidth(self.h, nbytes, records, iters, C.byref(g)))
         return g.value
 
     def read_bandwidth(self, nbytes: int, iters: int) -> float:

@@ -848,6 +848,21 @@ __global__ __launch_bounds__(256) void k_read(const float4* __restrict__ src, si
     if (acc == -1.0f) sink[blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
+// Reads one 64-byte record per thread at a distinct pseudo-random record of
+// a 2^log2m-record buffer (thread t reads record t * 0x9E3779B1 mod 2^log2m,
+// a bijection of the record indices, so no record is read twice): the access
+// pattern of a traversal's node fetches, with no reuse for a cache to serve.
+__global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ src, uint32_t records, uint32_t log2m,
+                                                float* __restrict__ sink) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= records) return;
+    const uint64_t idx = (uint64_t(t) * 0x9E3779B1ull) & ((uint64_t(1) << log2m) - 1u);
+    const float4* r = src + 4 * idx;
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+    const float acc = (a.x + b.y) + (c.z + d.w);
+    if (acc == -1.0f) sink[t] = acc;
+}
+
 __global__ void k_fill(float4* p, size_t n) {
     for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256)
         p[i] = make_float4(1.0f, 2.0f, 3.0f, 4.0f);
@@ -932,5 +947,32 @@ extern "C" int rc_read_bandwidth(rc_ctx* c, size_t bytes, int iters, double* gbp
     sink.release();
     if (rc != RC_OK) return rc;
     *gbps = best;
+    return RC_OK;
+}
+
+extern "C" int rc_gather_bandwidth(rc_ctx* c, size_t bytes, size_t records, int iters, double* gbps) {
+    RC_TRY(rc_check_ctx(c, "rc_gather_bandwidth"));
+    if (!gbps || bytes < 4096 || records == 0 || records > (size_t(1) << 31) || iters <= 0 || iters > 4096)
+        return rc_host_fail(RC_ERR_INVALID, "rc_gather_bandwidth: bad argument");
+    uint32_t log2m = 0;
+    while ((size_t(64) << (log2m + 1)) <= bytes) ++log2m;  // the largest power-of-two record count that fits
+    if ((size_t(1) << log2m) < records)
+        return rc_host_fail(RC_ERR_INVALID, "rc_gather_bandwidth: more records than the buffer holds");
+    RC_TRY_HIP(hipSetDevice(c->device));
+    const size_t n = (size_t(64) << log2m) / sizeof(float4);
+    DevBuf a, sink;
+    RC_TRY(a.ensure(n * sizeof(float4)));
+    int rc = sink.ensure(records * sizeof(float));
+    if (rc != RC_OK) { a.release(); return rc; }
+    k_fill<<<unsigned(c->cu_count) * 32u, 256, 0, c->stream>>>(a.as<float4>(), n);
+    float ms = 0.0f;
+    const unsigned grid = unsigned((records + 255) / 256);
+    rc = time_launches(
+        c, iters, [&]() { k_gather<<<grid, 256, 0, c->stream>>>(a.as<float4>(), uint32_t(records), log2m, sink.as<float>()); },
+        &ms);
+    a.release();
+    sink.release();
+    if (rc != RC_OK) return rc;
+    *gbps = double(records) * 64.0 * iters / (double(ms) * 1e-3) / 1e9;
     return RC_OK;
 }

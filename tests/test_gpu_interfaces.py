@@ -268,3 +268,17 @@ def test_console_runner_scene_file_round_trip(ctx, tmp_path, ext):
     assert ia.max() > 0 and np.array_equal(ia, ib)
     sf = api.load_scene_file(scene_file)
     assert sf.nodes is not None and len(sf.vertices) == len(sf.tri_material) > 0 and len(sf.lights) == 1
+
+
+def test_bandwidth_probes_run_and_reject_bad_arguments(ctx):
+    """The achievable-bandwidth probes the roofline is priced against: the
+    streaming copy, a cache-resident read and the scattered 64-byte read give
+    positive rates; the scattered read refuses more records than its buffer
+    holds (64 MiB holds 2^20 records)."""
+    assert ctx.copy_bandwidth(64 << 20, 2) > 0.0
+    assert ctx.read_bandwidth(4 << 20, 4) > 0.0
+    assert ctx.gather_bandwidth(64 << 20, 1 << 20, 2) > 0.0
+    with pytest.raises(RuntimeError):
+        ctx.gather_bandwidth(64 << 20, (1 << 20) + 1, 2)
+    with pytest.raises(RuntimeError):
+        ctx.gather_bandwidth(64 << 20, 0, 2)
