@@ -607,6 +607,41 @@ extern "C" int rc_scene_upload(rc_ctx* c, const float* v, size_t ntri, const uin
     // a traversal stack holds at most (kWidth - 1) entries per level
     if (depth * (kWidth - 1) >= uint32_t(kDeepStack))
         return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: BVH too deep for the traversal stack");
+#if RC_NODE_ORDER
+    // Each wide node's interior children side by side (one block per node),
+    // the blocks in depth-first order: siblings a ray visits one after the
+    // other share 128-byte lines, and a subtree stays in one region.
+    {
+        const size_t nn = nrec / kWidth;
+        std::vector<uint32_t> order, st;
+        order.reserve(nn);
+        if (nn) {
+            order.push_back(0);
+            st.push_back(0);
+        }
+        while (!st.empty()) {
+            const uint32_t n = st.back();
+            st.pop_back();
+            uint32_t kids[kWidth];
+            int nk = 0;
+            for (int k = 0; k < kWidth; ++k)
+                if (recs[size_t(n) * kWidth + k].count == 0) kids[nk++] = recs[size_t(n) * kWidth + k].link / kWidth;
+            for (int k = 0; k < nk; ++k) order.push_back(kids[k]);
+            for (int k = nk - 1; k >= 0; --k) st.push_back(kids[k]);
+        }
+        if (order.size() != nn) return rc_host_fail(RC_ERR_INVALID, "rc_scene_upload: wide nodes not a tree");
+        std::vector<uint32_t> to(nn);
+        for (uint32_t t = 0; t < nn; ++t) to[order[t]] = t;
+        std::vector<NodeRec> moved(nrec);
+        for (size_t n = 0; n < nn; ++n)
+            for (int k = 0; k < kWidth; ++k) {
+                NodeRec r = recs[n * kWidth + k];
+                if (r.count == 0) r.link = to[r.link / kWidth] * kWidth;
+                moved[size_t(to[n]) * kWidth + k] = r;
+            }
+        recs.swap(moved);
+    }
+#endif
     // The top of the tree (breadth first, at most kTopNodes wide nodes) moves
     // to the front of the node array, so the traversal kernels can keep nodes
     // [0, ntop) in LDS; the rest keep their order.  Links still point forward.

@@ -56,6 +56,12 @@ struct alignas(64) QNode {
 // tree's breadth-first top there) in LDS for the traversal kernels.  Off: on
 // the 2M-triangle terrain it was 1% slower than fetching those nodes through
 // the caches (1,848 against 1,868 Msamples/s).
+// RC_NODE_ORDER = 1 stores each wide node's interior children side by side
+// (blocks in depth-first order); 0 (the default) keeps the collapse's
+// preorder, a node followed by its first child's subtree.
+#ifndef RC_NODE_ORDER
+#define RC_NODE_ORDER 0
+#endif
 #ifndef RC_LDS_TOP
 #define RC_LDS_TOP 0
 #endif

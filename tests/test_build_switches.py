@@ -1,10 +1,11 @@
 """Every compile-time switch of the kernels still compiles in its non-default
 setting.  The switches keep the measured alternatives of the traversal, the
-queues and the launch shapes (node format, stack split, hit-test records,
-child order, leaf postponing, parking rule, camera fusion and packets, XCD
-remap); the library is built with the defaults only, so without this check a
-switched-off path could stop compiling unnoticed.  hipcc runs in syntax-only mode for
-gfx950 (templates are instantiated and checked, no code is generated)."""
+queues and the launch shapes (node format and order, stack split, hit-test
+records, child order, leaf postponing, parking rule, camera fusion and
+packets, XCD remap); the library is built with the defaults only, so without
+this check a switched-off path could stop compiling unnoticed.  hipcc runs in
+syntax-only mode for gfx950 (templates are instantiated and checked, no code
+is generated)."""
 from __future__ import annotations
 
 import os
@@ -19,6 +20,7 @@ SWITCHES = [
     "RC_QNODE=0", "RC_LDS_TOP=1", "RC_TRI_EDGES=1", "RC_REG_STACK=0", "RC_REG_STACK=2", "RC_PIXEL_MAJOR=0",
     "RC_FAST_RCP=0", "RC_SLAB_SELECT=0", "RC_NT_QUEUES=1", "RC_TRAV_IFIF=0", "RC_ANY_SORT=1",
     "RC_LEAF_POSTPONE=0", "RC_FUSED_CAMERA=0", "RC_XCD_REMAP=1", "RC_PARK_KEEP=0", "RC_PACKET_CAMERA=1",
+    "RC_NODE_ORDER=1",
 ]
 
 
